@@ -1,0 +1,204 @@
+/*
+ * wavebem_bem.h -- C-ABI of the MI355X (gfx950) collocation-BEM library.
+ *
+ * The entry points replace the arithmetic bodies of WaveBEM's BEMProblem<3>
+ * (include/bem_problem.h:87-180, source/bem_problem.cc) behind its unchanged class
+ * interface.  Each declaration names the reference member it stands in for.  Plain
+ * pointers and sizes only; every function returns a wb_status and never throws.
+ *
+ * Index conventions: dof indices and vectors are in the caller's (deal.II) dof
+ * numbering.  Matrices returned by wb_get_matrices are row-major n x n like deal.II's
+ * FullMatrix.  Host pointers unless a function name ends in _dev.
+ *
+ * Threading: one wb_ctx per host thread; each context owns one HIP stream.  There is no
+ * global mutable state (the reference keeps static locals in compute_alpha, vmult,
+ * compute_rhs and assemble_preconditioner; here that state lives in the context).
+ */
+#ifndef WAVEBEM_BEM_H
+#define WAVEBEM_BEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    WB_OK = 0,
+    WB_E_ARG = 1,       /* invalid argument / inconsistent sizes */
+    WB_E_OOM = 2,       /* device or host allocation failed */
+    WB_E_NOCONV = 3,    /* SolverControl::NoConvergence (max steps reached / NaN) */
+    WB_E_HIP = 4,       /* HIP runtime error */
+    WB_E_STATE = 5,     /* call out of order (e.g. solve before set_mesh) */
+    WB_E_UNSUPPORTED = 6
+} wb_status;
+
+typedef enum {
+    WB_PREC_FP64 = 0,    /* G, H stored and applied in FP64 */
+    WB_PREC_FP32_IR = 1  /* FP64 quadrature; G, H stored as FP32 + bf16 correction;
+                            FP32 GMRES inside FP64 iterative refinement */
+} wb_precision;
+
+typedef enum {
+    WB_PRECOND_NONE = 0,
+    WB_PRECOND_BLOCK_JACOBI = 1  /* stands in for the banded UMFPACK LU of
+                                    source/bem_problem.cc:1107-1149 */
+} wb_preconditioner;
+
+/* Parameters the reference reads from its .prm file:
+ * "Quadrature rules" (source/computational_domain.cc:84-91), "Solver" SolverControl
+ * and GMRES AdditionalData(100) (source/bem_problem.cc:83-85,826-827). */
+typedef struct {
+    int device;                    /* HIP device ordinal */
+    int quadrature_order;          /* QGauss<2>(q): 4 */
+    int singular_quadrature_order; /* QGaussOneOverR<2>(s): 5 */
+    double solver_tolerance;       /* absolute, preconditioned residual: 1e-10 */
+    int solver_max_steps;          /* total GMRES iterations: 1000 */
+    int gmres_max_tmp_vectors;     /* AdditionalData(100): Krylov dim = value - 2 */
+    int precision;                 /* wb_precision */
+    int preconditioner;            /* wb_preconditioner */
+    int precond_block;             /* block size for block Jacobi (<= 128) */
+    int ir_max_sweeps;             /* FP32_IR: outer refinement sweeps */
+    double ir_inner_reduction;     /* FP32_IR: inner relative residual reduction */
+} wb_config;
+
+typedef struct {
+    int iterations;         /* GMRES iterations (FP32_IR: inner iterations, all sweeps) */
+    int restarts;
+    int ir_sweeps;          /* FP32_IR outer sweeps (0 for FP64) */
+    double residual;        /* last preconditioned residual (convergence test value) */
+    double true_rel_residual; /* ||b - A x|| / ||b|| of the constrained system, FP64 */
+    double t_alpha_ms, t_rhs_ms, t_precond_ms, t_gmres_ms;
+} wb_solve_stats;
+
+typedef struct wb_ctx wb_ctx;
+
+/* Defaults equal to the reference's parameter defaults (prm-files/default.prm:217-229)
+ * and deal.II's GMRES AdditionalData(100). */
+void wb_config_default(wb_config *cfg);
+
+/* BEMProblem(ComputationalDomain&, BEMFMA&) (include/bem_problem.h:94) and
+ * parse_parameters (:107).  Creates the context on cfg->device. */
+wb_status wb_create(wb_ctx **ctx, const wb_config *cfg);
+void wb_destroy(wb_ctx *ctx);
+const char *wb_last_error(const wb_ctx *ctx);
+const char *wb_version(void);
+
+/* BEMProblem::reinit (source/bem_problem.cc:55-71) together with the mesh state it reads
+ * from ComputationalDomain: support points (map_dofs_to_support_points,
+ * :168-169), cells with their dofs in deal.II vertex order and direction flag, the
+ * double_nodes_set (source/computational_domain.cc:257-307; CSR, each set contains the
+ * dof itself; pass NULL to have it computed with the same 1e-8 coincidence rule), and
+ * an optional per-cell material id (cell->material_id(), NULL allowed) used only to
+ * group dofs in the device ordering.  Host buffers are copied. */
+wb_status wb_set_mesh(wb_ctx *ctx, int n_dofs, int n_cells, const double *support_points,
+                      const int32_t *cell_dofs, const int8_t *cell_orient,
+                      const int32_t *cell_material, const int32_t *dn_ptr,
+                      const int32_t *dn_idx);
+
+/* comp_dom.surface_nodes / other_nodes (source/numerical_towing_tank.cc:1799-1807,
+ * source/free_surface.cc:8756-8772): 1 = Dirichlet (phi known) / Neumann. */
+wb_status wb_set_masks(wb_ctx *ctx, const double *surface_nodes, const double *other_nodes);
+
+/* BEMProblem::assemble_system (source/bem_problem.cc:106-590): G (dirichlet_matrix) and
+ * H (neumann_matrix) on the device, plus the solid-angle row sums. */
+wb_status wb_assemble(wb_ctx *ctx);
+
+/* Copies G and H back (row-major, n x n, caller dof order), for parity checks; either
+ * pointer may be NULL. */
+wb_status wb_get_matrices(wb_ctx *ctx, double *G, double *H);
+
+/* BEMProblem::compute_alpha (source/bem_problem.cc:594-618): alpha = H * (-1). */
+wb_status wb_compute_alpha(wb_ctx *ctx, double *alpha);
+
+/* BEMProblem::vmult (source/bem_problem.cc:621-670), unconstrained. */
+wb_status wb_vmult(wb_ctx *ctx, double *dst, const double *src);
+
+/* BEMProblem::compute_rhs (source/bem_problem.cc:673-717), unconstrained. */
+wb_status wb_compute_rhs(wb_ctx *ctx, double *dst, const double *src);
+
+/* The ConstraintMatrix used by ConstrainedOperator (include/constrained_matrix.h:73-96),
+ * as closed CSR lines: row line_rows[l] reads x_row - sum_p w[p] x[col[p]] and its
+ * right-hand side is inhom[l].  Replaces any previous constraints. */
+wb_status wb_set_constraints(wb_ctx *ctx, int n_lines, const int32_t *line_rows,
+                             const int32_t *line_ptr, const int32_t *ent_col,
+                             const double *ent_w, const double *inhom);
+
+/* BEMProblem::compute_constraints (source/bem_problem.cc:990-1105) for the current masks
+ * and double nodes, from tmp_rhs.  node_normals (n x 3, from
+ * ComputationalDomain::compute_normals) is needed only when two Dirichlet dofs
+ * coincide, node_surface_gradients (n x 3, compute_surface_gradients) only on sharp
+ * Dirichlet-Dirichlet edges; both may be NULL otherwise.  Installs the result as with
+ * wb_set_constraints and returns the number of lines in *n_lines (may be NULL). */
+wb_status wb_compute_constraints(wb_ctx *ctx, const double *tmp_rhs, const double *node_normals,
+                                 const double *node_surface_gradients, int *n_lines);
+
+/* ConstrainedOperator<Vector,BEMProblem>::vmult: dst = A src with constrained rows
+ * replaced. */
+wb_status wb_constrained_vmult(wb_ctx *ctx, double *dst, const double *src);
+
+/* BEMProblem::solve_system (source/bem_problem.cc:821-895): alpha, rhs from tmp_rhs,
+ * distribute_rhs, preconditioned GMRES (sol = 0 start), then phi(i) = sol(i) where
+ * surface_nodes(i) == 0 else dphi_dn(i) = sol(i).  Uses the installed constraints.
+ * On WB_E_NOCONV phi/dphi_dn are left unchanged, as the reference throws before the
+ * scatter.  stats may be NULL. */
+wb_status wb_solve_system(wb_ctx *ctx, const double *tmp_rhs, double *phi, double *dphi_dn,
+                          wb_solve_stats *stats);
+
+/* Same with device pointers (n doubles each, internal stream ordering respected). */
+wb_status wb_solve_system_dev(wb_ctx *ctx, const double *tmp_rhs_dev, double *phi_dev,
+                              double *dphi_dn_dev, wb_solve_stats *stats);
+
+/* BEMProblem::solve (source/bem_problem.cc:969-987), Direct branch: assemble_system +
+ * solve_system. */
+wb_status wb_solve(wb_ctx *ctx, const double *tmp_rhs, double *phi, double *dphi_dn,
+                   wb_solve_stats *stats);
+
+/* BEMProblem::residual (source/bem_problem.cc:903-961), Direct branch, with the
+ * installed constraints. */
+wb_status wb_residual(wb_ctx *ctx, double *res, const double *phi, const double *dphi_dn);
+
+/* Multi right-hand side form of solve_system: k boundary-data vectors (column j at
+ * tmp_rhs + j*n), solved together with one batched (MFMA) matrix product per GMRES
+ * step.  Constraint inhomogeneities are taken per right-hand side from inhom_k
+ * (n_lines x k, column-major; NULL = the installed ones for every column). */
+wb_status wb_solve_system_multi(wb_ctx *ctx, int k, const double *tmp_rhs, double *phi,
+                                double *dphi_dn, const double *inhom_k,
+                                wb_solve_stats *stats);
+
+/* ---- host-only utilities (no device needed) ---------------------------------------- */
+
+/* The quadrature tables the device path uses: QGauss<2>(q) (q^2 points, xi fastest,
+ * reg_pts[2q+{0,1}]) and QGaussOneOverR<2>(s, vertex v, factor_out=true) for v = 0..3
+ * (2 s^2 points each, vertex-major: sing_pts[2(v*2s^2+k)+{0,1}]); as requested at
+ * source/bem_problem.cc:116-120,125-137.  Any pointer may be NULL. */
+int wb_host_quadrature(int q_order, int s_order, double *reg_pts, double *reg_w,
+                       double *sing_pts, double *sing_w);
+
+/* The device dof ordering (perm[dof] = device index), the double_nodes_set computed with
+ * the 1e-8 coincidence rule of source/computational_domain.cc:257-307 (CSR), and the
+ * sizes of the assembly plan.  Any output pointer may be NULL. */
+int wb_host_mesh_plan(int n_dofs, int n_cells, const double *support_points,
+                      const int32_t *cell_dofs, const int32_t *cell_material, int32_t *perm,
+                      int32_t *dn_ptr, int32_t *dn_idx, int dn_cap, int32_t *n_tiles,
+                      int32_t *n_tile_cells, int32_t *n_singular_pairs);
+
+/* BEMProblem::compute_constraints (source/bem_problem.cc:990-1105) rules, closed as by
+ * ConstraintMatrix::close; returns the number of lines, or minus a wb_status. */
+int wb_host_compute_constraints(int n, const int32_t *dn_ptr, const int32_t *dn_idx,
+                                const double *surface, const double *node_normals,
+                                const double *node_surface_gradients, const double *tmp_rhs,
+                                int cap_lines, int cap_nnz, int32_t *line_rows, int32_t *line_ptr,
+                                int32_t *ent_col, double *ent_w, double *inhom);
+
+/* Device memory used by the context in bytes. */
+size_t wb_device_bytes(const wb_ctx *ctx);
+
+/* Synchronizes the context stream. */
+wb_status wb_synchronize(wb_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

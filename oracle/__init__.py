@@ -1,0 +1,230 @@
+"""Python face of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg import
+this package, as the checker.  The product library never touches it.
+
+Every function wraps the C restatement in ``bem_oracle.c``; see that file for the
+WaveBEM file:line each routine follows.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_bp = ctypes.POINTER(ctypes.c_byte)
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "liboracle.so"],
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def qgauss_1d(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    assert lib().orc_qgauss_1d(n, _d(x), _d(w)) == 0
+    return x, w
+
+
+def qgauss_2d(n):
+    p = np.zeros((n * n, 2))
+    w = np.zeros(n * n)
+    assert lib().orc_qgauss_2d(n, _d(p), _d(w)) == 0
+    return p, w
+
+
+def qgauss_one_over_r(n, vertex, factor_out=True):
+    p = np.zeros((2 * n * n, 2))
+    w = np.zeros(2 * n * n)
+    assert lib().orc_qgauss_one_over_r(n, vertex, int(factor_out), _d(p), _d(w)) == 0
+    return p, w
+
+
+def fe_values(vertices, orient, pts, w):
+    v = _f64(vertices).reshape(12)
+    pts = _f64(pts)
+    w = _f64(w)
+    nq = w.size
+    qp = np.zeros((nq, 3))
+    nr = np.zeros((nq, 3))
+    jxw = np.zeros(nq)
+    sh = np.zeros((nq, 4))
+    lib().orc_fe_values(_d(v), int(orient), nq, _d(pts), _d(w), _d(qp), _d(nr), _d(jxw), _d(sh))
+    return qp, nr, jxw, sh
+
+
+def assemble(support, cell_dofs, cell_orient, dn_ptr, dn_idx, q_order=4, sing_order=5,
+             n_threads=0):
+    """Returns (H, G) = (neumann_matrix, dirichlet_matrix)."""
+    support = _f64(support)
+    cd = _i32(cell_dofs)
+    orient = np.ascontiguousarray(cell_orient, dtype=np.int8)
+    dp = _i32(dn_ptr)
+    di = _i32(dn_idx)
+    n = support.shape[0]
+    nc = cd.shape[0]
+    H = np.zeros((n, n))
+    G = np.zeros((n, n))
+    rc = lib().orc_assemble(n, nc, _i(cd), orient.ctypes.data_as(_bp), _d(support), _i(dp),
+                            _i(di), q_order, sing_order, _d(H), _d(G), int(n_threads))
+    if rc != 0:
+        raise RuntimeError(f"orc_assemble failed: {rc}")
+    return H, G
+
+
+def compute_alpha(H):
+    H = _f64(H)
+    n = H.shape[0]
+    a = np.zeros(n)
+    lib().orc_compute_alpha(n, _d(H), _d(a))
+    return a
+
+
+def vmult(G, H, alpha, surface, other, src):
+    n = G.shape[0]
+    dst = np.zeros(n)
+    lib().orc_vmult(n, _d(_f64(G)), _d(_f64(H)), _d(_f64(alpha)), _d(_f64(surface)),
+                    _d(_f64(other)), _d(_f64(src)), _d(dst))
+    return dst
+
+
+def compute_rhs(G, H, alpha, surface, other, src):
+    n = G.shape[0]
+    dst = np.zeros(n)
+    lib().orc_compute_rhs(n, _d(_f64(G)), _d(_f64(H)), _d(_f64(alpha)), _d(_f64(surface)),
+                          _d(_f64(other)), _d(_f64(src)), _d(dst))
+    return dst
+
+
+@dataclass
+class Constraints:
+    rows: np.ndarray
+    ptr: np.ndarray
+    cols: np.ndarray
+    weights: np.ndarray
+    inhom: np.ndarray
+
+    @property
+    def n_lines(self):
+        return int(self.rows.size)
+
+
+def compute_constraints(dn_ptr, dn_idx, surface, tmp_rhs, normals=None, grads=None):
+    dp = _i32(dn_ptr)
+    di = _i32(dn_idx)
+    n = dp.size - 1
+    cap_l = n
+    cap_nnz = 8 * n + 16
+    rows = np.zeros(cap_l, dtype=np.int32)
+    ptr = np.zeros(cap_l + 1, dtype=np.int32)
+    cols = np.zeros(cap_nnz, dtype=np.int32)
+    w = np.zeros(cap_nnz)
+    inh = np.zeros(cap_l)
+    nrm = _f64(normals) if normals is not None else None
+    grd = _f64(grads) if grads is not None else None
+    nl = lib().orc_compute_constraints(
+        n, _i(dp), _i(di), _d(_f64(surface)), _d(nrm) if nrm is not None else None,
+        _d(grd) if grd is not None else None, _d(_f64(tmp_rhs)), cap_l, cap_nnz, _i(rows),
+        _i(ptr), _i(cols), _d(w), _d(inh))
+    if nl < 0:
+        raise RuntimeError(f"orc_compute_constraints failed: {nl}")
+    nnz = int(ptr[nl])
+    return Constraints(rows[:nl].copy(), ptr[:nl + 1].copy(), cols[:nnz].copy(), w[:nnz].copy(),
+                       inh[:nl].copy())
+
+
+def constrained_vmult(G, H, alpha, surface, other, cons: Constraints, src):
+    n = G.shape[0]
+    dst = np.zeros(n)
+    lib().orc_constrained_vmult(n, _d(_f64(G)), _d(_f64(H)), _d(_f64(alpha)), _d(_f64(surface)),
+                                _d(_f64(other)), cons.n_lines, _i(cons.rows), _i(cons.ptr),
+                                _i(cons.cols), _d(cons.weights), _d(_f64(src)), _d(dst))
+    return dst
+
+
+@dataclass
+class SolveResult:
+    phi: np.ndarray
+    dphi_dn: np.ndarray
+    alpha: np.ndarray
+    sol: np.ndarray
+    iterations: int
+    residual: float
+    status: int
+
+
+def solve_system(G, H, surface, other, cons: Constraints, tmp_rhs, phi=None, dphi_dn=None,
+                 tol=1e-10, max_steps=1000, restart_vectors=100, precond=1, band=100):
+    n = G.shape[0]
+    phi = np.zeros(n) if phi is None else _f64(phi).copy()
+    dphi = np.zeros(n) if dphi_dn is None else _f64(dphi_dn).copy()
+    alpha = np.zeros(n)
+    sol = np.zeros(n)
+    it = ctypes.c_int(0)
+    res = ctypes.c_double(0)
+    rc = lib().orc_solve_system(
+        n, _d(_f64(G)), _d(_f64(H)), _d(_f64(surface)), _d(_f64(other)), cons.n_lines,
+        _i(cons.rows), _i(cons.ptr), _i(cons.cols), _d(cons.weights), _d(cons.inhom),
+        _d(_f64(tmp_rhs)), ctypes.c_double(tol), int(max_steps), int(restart_vectors),
+        int(precond), int(band), _d(phi), _d(dphi), _d(alpha), _d(sol), ctypes.byref(it),
+        ctypes.byref(res))
+    if rc < 0:
+        raise RuntimeError(f"orc_solve_system failed: {rc}")
+    return SolveResult(phi, dphi, alpha, sol, it.value, res.value, rc)
+
+
+def residual(G, H, surface, other, cons: Constraints, phi, dphi_dn):
+    n = G.shape[0]
+    res = np.zeros(n)
+    lib().orc_residual(n, _d(_f64(G)), _d(_f64(H)), _d(_f64(surface)), _d(_f64(other)),
+                       cons.n_lines, _i(cons.rows), _i(cons.ptr), _i(cons.cols),
+                       _d(cons.weights), _d(cons.inhom), _d(_f64(phi)), _d(_f64(dphi_dn)),
+                       _d(res))
+    return res
+
+
+def _setup_argtypes():
+    L = lib()
+    L.orc_solve_system.argtypes = None  # varargs-free; ctypes converts per call
+
+
+if __name__ == "__main__":
+    build(force=True)

@@ -1,0 +1,159 @@
+"""GPU parity: the HIP path against the CPU oracle on the same seeded inputs.
+
+Bars (from the hot-path contract): G and H entries within 1e-10 of the oracle relative to
+the row scale, alpha within 1e-12, operator products within 1e-12, solutions (phi and
+dphi/dn) within 1e-8 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests import helpers as T
+from wavebem_amd import mesh as M
+from wavebem_amd.bem import BEMProblem, Config, NoConvergence
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(m, ptr, idx, **cfg):
+    p = BEMProblem(m, Config(**cfg), double_nodes=(ptr, idx))
+    p.assemble_system()
+    return p
+
+
+@pytest.mark.parametrize("kind", ["wigley_small", "box6", "sphere4"])
+def test_assembly_matches_oracle(gpu_available, kind):
+    m, ptr, idx, H, G = T.oracle_assembly(kind)
+    print(f"[assembly parity] {kind} n={m.n_dofs}", flush=True)
+    with _problem(m, ptr, idx) as p:
+        Gd, Hd = p.matrices()
+        alpha = p.compute_alpha()
+    eg = T.row_scaled_err(Gd, G)
+    eh = T.row_scaled_err(Hd, H)
+    ea = np.abs(alpha - O.compute_alpha(H)).max()
+    print(f"  G {eg:.2e} H {eh:.2e} alpha {ea:.2e}")
+    assert eg < 1e-10
+    assert eh < 1e-10
+    assert ea < 1e-12
+
+
+def test_assembly_fp32_ir_storage(gpu_available):
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    with _problem(m, ptr, idx, precision="fp32_ir") as p:
+        Gd, Hd = p.matrices()
+        alpha = p.compute_alpha()
+    # hi (FP32) + lo (bf16) keeps about 2^-33 relative per entry
+    assert T.row_scaled_err(Gd, G) < 1e-9
+    assert T.row_scaled_err(Hd, H) < 1e-9
+    assert np.abs(alpha - O.compute_alpha(H)).max() < 1e-12
+
+
+def test_vmult_rhs_match_oracle(gpu_available):
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    rng = np.random.default_rng(3)
+    src = rng.standard_normal(m.n_dofs)
+    with _problem(m, ptr, idx) as p:
+        alpha = O.compute_alpha(H)
+        for name, fd, fo in (("vmult", p.vmult, O.vmult), ("rhs", p.compute_rhs, O.compute_rhs)):
+            d = fd(src)
+            o = fo(G, H, alpha, m.surface_nodes, m.other_nodes, src)
+            err = np.abs(d - o).max() / np.abs(o).max()
+            print(f"  {name} rel err {err:.2e}")
+            assert err < 1e-12
+
+
+def test_constrained_vmult_matches_oracle(gpu_available):
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    bc, _ = M.wigley_bc(m)
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, bc)
+    rng = np.random.default_rng(4)
+    src = rng.standard_normal(m.n_dofs)
+    with _problem(m, ptr, idx) as p:
+        nl = p.compute_constraints(bc)
+        assert nl == cons.n_lines
+        d = p.constrained_vmult(src)
+    o = O.constrained_vmult(G, H, O.compute_alpha(H), m.surface_nodes, m.other_nodes, cons, src)
+    assert np.abs(d - o).max() / np.abs(o).max() < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_solve_system_matches_dense_oracle(gpu_available, precision):
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    bc, _ = M.wigley_bc(m, seed=0)
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, bc)
+    x = T.dense_solution(H, G, m.surface_nodes, m.other_nodes, cons, bc)
+    phi = bc * m.surface_nodes
+    dphi = bc * m.other_nodes
+    with _problem(m, ptr, idx, precision=precision, solver_tolerance=1e-13) as p:
+        p.compute_constraints(bc)
+        st = p.solve_system(phi, dphi, bc)
+    print(f"  [{precision}] stats {st}")
+    N = m.other_nodes > 0
+    S = ~N
+    e_phi = np.abs(phi[N] - x[N]).max() / np.abs(x[N]).max()
+    e_dn = np.abs(dphi[S] - x[S]).max() / np.abs(x[S]).max()
+    print(f"  phi err {e_phi:.2e} dphi_dn err {e_dn:.2e}")
+    assert e_phi < 1e-8
+    assert e_dn < 1e-8
+    # known entries untouched
+    assert np.array_equal(phi[S], bc[S])
+    assert np.array_equal(dphi[N], bc[N])
+
+
+def test_solve_matches_oracle_gmres(gpu_available):
+    """Same converged solution as the oracle's reference-preconditioned GMRES."""
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    bc, _ = M.wigley_bc(m, seed=1)
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, bc)
+    r = O.solve_system(G, H, m.surface_nodes, m.other_nodes, cons, bc, phi=bc * m.surface_nodes,
+                       dphi_dn=bc * m.other_nodes, tol=1e-14, precond=1)
+    assert r.status == 0
+    phi = bc * m.surface_nodes
+    dphi = bc * m.other_nodes
+    with _problem(m, ptr, idx, solver_tolerance=1e-14) as p:
+        p.compute_constraints(bc)
+        p.solve_system(phi, dphi, bc)
+    assert np.abs(phi - r.phi).max() / np.abs(r.phi).max() < 1e-8
+    assert np.abs(dphi - r.dphi_dn).max() / np.abs(r.dphi_dn).max() < 1e-8
+
+
+def test_residual_matches_oracle(gpu_available):
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    bc, _ = M.wigley_bc(m, seed=2)
+    rng = np.random.default_rng(5)
+    phi = rng.standard_normal(m.n_dofs)
+    dphi = rng.standard_normal(m.n_dofs)
+    tmp = dphi * m.other_nodes + phi * m.surface_nodes
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, tmp)
+    o = O.residual(G, H, m.surface_nodes, m.other_nodes, cons, phi, dphi)
+    with _problem(m, ptr, idx) as p:
+        p.compute_constraints(tmp)
+        d = p.residual(phi, dphi)
+    assert np.abs(d - o).max() / np.abs(o).max() < 1e-12
+
+
+def test_no_convergence_raises_and_leaves_outputs(gpu_available):
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    bc, _ = M.wigley_bc(m)
+    phi = bc * m.surface_nodes
+    dphi = bc * m.other_nodes
+    phi0, dphi0 = phi.copy(), dphi.copy()
+    with _problem(m, ptr, idx, solver_tolerance=0.0, solver_max_steps=2) as p:
+        p.compute_constraints(bc)
+        with pytest.raises(NoConvergence):
+            p.solve_system(phi, dphi, bc)
+    assert np.array_equal(phi, phi0) and np.array_equal(dphi, dphi0)
+
+
+def test_pure_neumann_vmult(gpu_available):
+    """All-Neumann masks trigger dst.add(-||dst||) (source/bem_problem.cc:667-668)."""
+    m, ptr, idx, H, G = T.oracle_assembly("box6")
+    surf = np.zeros(m.n_dofs)
+    oth = np.ones(m.n_dofs)
+    rng = np.random.default_rng(6)
+    src = rng.standard_normal(m.n_dofs)
+    with _problem(m, ptr, idx) as p:
+        p.set_masks(surf, oth)
+        d = p.vmult(src)
+    o = O.vmult(G, H, O.compute_alpha(H), surf, oth, src)
+    assert np.abs(d - o).max() / np.abs(o).max() < 1e-12

@@ -1,0 +1,309 @@
+"""Python mirror of WaveBEM's ``BEMProblem<3>`` over the MI355X C-ABI.
+
+Same method names, argument meaning and error behaviour as the reference class
+(include/bem_problem.h:87-180): vectors are numpy float64 arrays in the caller's dof
+numbering, ``solve``/``solve_system`` overwrite only the unknown entries of ``phi`` and
+``dphi_dn`` (source/bem_problem.cc:869-879), and GMRES non-convergence raises
+``NoConvergence`` like deal.II's ``SolverControl::NoConvergence``.
+
+Every call goes through ``libwavebem_bem.so``; nothing here computes BEM arithmetic.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from .mesh import SurfaceMesh
+
+
+class WaveBemError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{L.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class NoConvergence(WaveBemError):
+    pass
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} entries, got {a.size}")
+    return a
+
+
+@dataclass
+class Config:
+    """Parameters of ``BEMProblem::parse_parameters`` / ``ComputationalDomain``
+    ("Quadrature rules", "Solver"), plus the device-side choices."""
+    device: int = 0
+    quadrature_order: int = 4
+    singular_quadrature_order: int = 5
+    solver_tolerance: float = 1e-10
+    solver_max_steps: int = 1000
+    gmres_max_tmp_vectors: int = 100
+    precision: str = "fp64"  # "fp64" | "fp32_ir"
+    preconditioner: str = "block_jacobi"  # "block_jacobi" | "none"
+    precond_block: int = 64
+    ir_max_sweeps: int = 10
+    ir_inner_reduction: float = 1e-5
+
+    def to_c(self) -> L.WbConfig:
+        c = L.WbConfig()
+        L.lib().wb_config_default(ctypes.byref(c))
+        c.device = self.device
+        c.quadrature_order = self.quadrature_order
+        c.singular_quadrature_order = self.singular_quadrature_order
+        c.solver_tolerance = self.solver_tolerance
+        c.solver_max_steps = self.solver_max_steps
+        c.gmres_max_tmp_vectors = self.gmres_max_tmp_vectors
+        c.precision = {"fp64": L.WB_PREC_FP64, "fp32_ir": L.WB_PREC_FP32_IR}[self.precision]
+        c.preconditioner = {"none": L.WB_PRECOND_NONE,
+                            "block_jacobi": L.WB_PRECOND_BLOCK_JACOBI}[self.preconditioner]
+        c.precond_block = self.precond_block
+        c.ir_max_sweeps = self.ir_max_sweeps
+        c.ir_inner_reduction = self.ir_inner_reduction
+        return c
+
+
+class BEMProblem:
+    """``BEMProblem<3>`` with the Direct solution method, on one GPU."""
+
+    def __init__(self, mesh: SurfaceMesh | None = None, config: Config | None = None,
+                 double_nodes=None):
+        self.config = config or Config()
+        self._ctx = ctypes.c_void_p()
+        cfg = self.config.to_c()
+        self._check(L.lib().wb_create(ctypes.byref(self._ctx), ctypes.byref(cfg)), "wb_create")
+        self.n = 0
+        self.last_stats = None
+        if mesh is not None:
+            self.reinit(mesh, double_nodes)
+
+    # ---------------------------------------------------------------- plumbing
+    def _check(self, status, what):
+        if status == L.WB_OK:
+            return
+        msg = what
+        if self._ctx:
+            err = L.lib().wb_last_error(self._ctx)
+            if err:
+                msg = f"{what}: {err.decode()}"
+        if status == L.WB_E_NOCONV:
+            raise NoConvergence(status, msg)
+        raise WaveBemError(status, msg)
+
+    def close(self):
+        if self._ctx:
+            L.lib().wb_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---------------------------------------------------------------- reference API
+    def reinit(self, mesh: SurfaceMesh, double_nodes=None):
+        """``BEMProblem::reinit`` plus the ComputationalDomain state it reads."""
+        n, nc = mesh.n_dofs, mesh.n_cells
+        X = _f64(mesh.support_points).reshape(-1)
+        cd = np.ascontiguousarray(mesh.cell_dofs, dtype=np.int32).reshape(-1)
+        orient = np.ascontiguousarray(mesh.cell_orient, dtype=np.int8)
+        mat = np.ascontiguousarray(mesh.cell_material, dtype=np.int32)
+        if double_nodes is not None:
+            dp = np.ascontiguousarray(double_nodes[0], dtype=np.int32)
+            di = np.ascontiguousarray(double_nodes[1], dtype=np.int32)
+            dpp, dip = _i(dp), _i(di)
+        else:
+            dpp = dip = None
+        self._check(L.lib().wb_set_mesh(self._ctx, n, nc, _d(X), _i(cd),
+                                        orient.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                                        _i(mat), dpp, dip), "wb_set_mesh")
+        self.n = n
+        self.mesh = mesh
+        self.set_masks(mesh.surface_nodes, mesh.other_nodes)
+
+    def set_masks(self, surface_nodes, other_nodes):
+        s = _f64(surface_nodes, self.n)
+        o = _f64(other_nodes, self.n)
+        self._check(L.lib().wb_set_masks(self._ctx, _d(s), _d(o)), "wb_set_masks")
+        self.surface_nodes, self.other_nodes = s.copy(), o.copy()
+
+    def assemble_system(self):
+        self._check(L.lib().wb_assemble(self._ctx), "wb_assemble")
+
+    def matrices(self):
+        """(G, H) = (dirichlet_matrix, neumann_matrix), caller numbering, float64."""
+        G = np.empty((self.n, self.n))
+        H = np.empty((self.n, self.n))
+        self._check(L.lib().wb_get_matrices(self._ctx, _d(G), _d(H)), "wb_get_matrices")
+        return G, H
+
+    @property
+    def dirichlet_matrix(self):
+        return self.matrices()[0]
+
+    @property
+    def neumann_matrix(self):
+        return self.matrices()[1]
+
+    def compute_alpha(self):
+        a = np.empty(self.n)
+        self._check(L.lib().wb_compute_alpha(self._ctx, _d(a)), "wb_compute_alpha")
+        self.alpha = a
+        return a
+
+    def vmult(self, src):
+        src = _f64(src, self.n)
+        dst = np.empty(self.n)
+        self._check(L.lib().wb_vmult(self._ctx, _d(dst), _d(src)), "wb_vmult")
+        return dst
+
+    def compute_rhs(self, src):
+        src = _f64(src, self.n)
+        dst = np.empty(self.n)
+        self._check(L.lib().wb_compute_rhs(self._ctx, _d(dst), _d(src)), "wb_compute_rhs")
+        return dst
+
+    def compute_constraints(self, tmp_rhs, node_normals=None, node_surface_gradients=None):
+        t = _f64(tmp_rhs, self.n)
+        nn = _f64(node_normals).reshape(-1) if node_normals is not None else None
+        gg = _f64(node_surface_gradients).reshape(-1) if node_surface_gradients is not None else None
+        nl = ctypes.c_int(0)
+        self._check(L.lib().wb_compute_constraints(
+            self._ctx, _d(t), _d(nn) if nn is not None else None,
+            _d(gg) if gg is not None else None, ctypes.byref(nl)), "wb_compute_constraints")
+        return nl.value
+
+    def set_constraints(self, rows, ptr, cols, weights, inhom):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        w = _f64(weights)
+        inh = _f64(inhom)
+        self._check(L.lib().wb_set_constraints(self._ctx, rows.size, _i(rows), _i(ptr), _i(cols),
+                                               _d(w), _d(inh)), "wb_set_constraints")
+
+    def constrained_vmult(self, src):
+        src = _f64(src, self.n)
+        dst = np.empty(self.n)
+        self._check(L.lib().wb_constrained_vmult(self._ctx, _d(dst), _d(src)),
+                    "wb_constrained_vmult")
+        return dst
+
+    def solve_system(self, phi, dphi_dn, tmp_rhs):
+        """Overwrites the unknown entries of phi / dphi_dn in place."""
+        t = _f64(tmp_rhs, self.n)
+        self._inplace_ok(phi)
+        self._inplace_ok(dphi_dn)
+        st = L.WbSolveStats()
+        status = L.lib().wb_solve_system(self._ctx, _d(t), _d(phi), _d(dphi_dn), ctypes.byref(st))
+        self.last_stats = st.as_dict()
+        self._check(status, "wb_solve_system")
+        return self.last_stats
+
+    def solve(self, phi, dphi_dn, tmp_rhs):
+        self.assemble_system()
+        return self.solve_system(phi, dphi_dn, tmp_rhs)
+
+    def residual(self, phi, dphi_dn):
+        p = _f64(phi, self.n)
+        d = _f64(dphi_dn, self.n)
+        res = np.empty(self.n)
+        self._check(L.lib().wb_residual(self._ctx, _d(res), _d(p), _d(d)), "wb_residual")
+        return res
+
+    def solve_system_multi(self, phi, dphi_dn, tmp_rhs):
+        """k right-hand sides as (k, n) arrays; phi/dphi_dn updated in place."""
+        t = _f64(tmp_rhs)
+        k = t.size // self.n
+        for a in (phi, dphi_dn):
+            self._inplace_ok(a)
+            if a.size != k * self.n:
+                raise ValueError("phi/dphi_dn must have shape (k, n)")
+        st = L.WbSolveStats()
+        status = L.lib().wb_solve_system_multi(self._ctx, k, _d(t), _d(phi), _d(dphi_dn), None,
+                                               ctypes.byref(st))
+        self.last_stats = st.as_dict()
+        self._check(status, "wb_solve_system_multi")
+        return self.last_stats
+
+    def device_bytes(self):
+        return int(L.lib().wb_device_bytes(self._ctx))
+
+    @staticmethod
+    def _inplace_ok(a):
+        if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+            raise TypeError("phi / dphi_dn must be C-contiguous float64 numpy arrays")
+
+
+# ---------------------------------------------------------------- host utilities
+def host_quadrature(q_order=4, s_order=5):
+    nq, ns = q_order * q_order, 2 * s_order * s_order
+    rp, rw = np.zeros((nq, 2)), np.zeros(nq)
+    sp, sw = np.zeros((4, ns, 2)), np.zeros((4, ns))
+    st = L.lib().wb_host_quadrature(q_order, s_order, _d(rp), _d(rw), _d(sp), _d(sw))
+    if st != L.WB_OK:
+        raise WaveBemError(st, "wb_host_quadrature")
+    return rp, rw, sp, sw
+
+
+def host_mesh_plan(mesh: SurfaceMesh):
+    n, nc = mesh.n_dofs, mesh.n_cells
+    X = _f64(mesh.support_points).reshape(-1)
+    cd = np.ascontiguousarray(mesh.cell_dofs, dtype=np.int32).reshape(-1)
+    mat = np.ascontiguousarray(mesh.cell_material, dtype=np.int32)
+    perm = np.zeros(n, dtype=np.int32)
+    dp = np.zeros(n + 1, dtype=np.int32)
+    cap = 8 * n
+    di = np.zeros(cap, dtype=np.int32)
+    nt, ntc, nsp = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    st = L.lib().wb_host_mesh_plan(n, nc, _d(X), _i(cd), _i(mat), _i(perm), _i(dp), _i(di), cap,
+                                   ctypes.byref(nt), ctypes.byref(ntc), ctypes.byref(nsp))
+    if st != L.WB_OK:
+        raise WaveBemError(st, "wb_host_mesh_plan")
+    return dict(perm=perm, dn_ptr=dp, dn_idx=di[:dp[-1]].copy(), n_tiles=nt.value,
+                n_tile_cells=ntc.value, n_singular_pairs=nsp.value)
+
+
+def host_compute_constraints(dn_ptr, dn_idx, surface, tmp_rhs, normals=None, grads=None):
+    dp = np.ascontiguousarray(dn_ptr, dtype=np.int32)
+    di = np.ascontiguousarray(dn_idx, dtype=np.int32)
+    n = dp.size - 1
+    cap_l, cap_nnz = n, 8 * n + 16
+    rows = np.zeros(cap_l, dtype=np.int32)
+    ptr = np.zeros(cap_l + 1, dtype=np.int32)
+    cols = np.zeros(cap_nnz, dtype=np.int32)
+    w = np.zeros(cap_nnz)
+    inh = np.zeros(cap_l)
+    nn = _f64(normals).reshape(-1) if normals is not None else None
+    gg = _f64(grads).reshape(-1) if grads is not None else None
+    nl = L.lib().wb_host_compute_constraints(
+        n, _i(dp), _i(di), _d(_f64(surface, n)), _d(nn) if nn is not None else None,
+        _d(gg) if gg is not None else None, _d(_f64(tmp_rhs, n)), cap_l, cap_nnz, _i(rows),
+        _i(ptr), _i(cols), _d(w), _d(inh))
+    if nl < 0:
+        raise WaveBemError(-nl, "wb_host_compute_constraints")
+    nnz = int(ptr[nl])
+    return dict(rows=rows[:nl].copy(), ptr=ptr[:nl + 1].copy(), cols=cols[:nnz].copy(),
+                weights=w[:nnz].copy(), inhom=inh[:nl].copy())

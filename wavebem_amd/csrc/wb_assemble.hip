@@ -1,0 +1,397 @@
+// wb_assemble.hip -- dense single-layer G and double-layer H assembly on gfx950.
+//
+// Replaces the loops of BEMProblem::assemble_system (source/bem_problem.cc:106-590):
+//   G_ij = sum_{c ni j} sum_q  phi_{j,c}(q) JxW_cq / (4 pi |y_cq - x_i|)
+//   H_ij = sum_{c ni j} sum_q  phi_{j,c}(q) (x_i - y_cq).n_cq JxW_cq / (4 pi |y_cq - x_i|^3)
+// (LaplaceKernel::kernels, include/laplace_kernel.h:45-64), with the regular QGauss
+// rule on cells that do not touch x_i and QGaussOneOverR on the cells that do
+// (double-node membership test, source/bem_problem.cc:214-230).
+//
+// Regular cells: one wavefront owns 64 collocation rows (lane = row) and a column tile
+// of TILE_COLS consecutive device-ordered dofs; it sweeps the cells touching the tile.
+// Cell quadrature data is wave-uniform and arrives through scalar loads; each lane keeps
+// the 4+4 local G/H values of the current cell in registers and adds them into an LDS
+// accumulator [column][row]; the finished tile is written once, coalesced, row by row.
+// Singular cells: a second kernel adds their 2 s^2-point contributions per row.
+#include "wb_internal.h"
+
+namespace wb {
+
+// 1/sqrt(r2) in FP64: hardware estimate refined by two Newton steps
+// y <- y + y (1/2 - (r2/2) y^2); each step squares the relative error.
+__device__ __forceinline__ double rsqrt_f64(double r2)
+{
+    double y = __builtin_amdgcn_rsq(r2);
+    const double h = 0.5 * r2;
+    double t = y * y;
+    double e = __builtin_fma(-h, t, 0.5);
+    y = __builtin_fma(y, e, y);
+    t = y * y;
+    e = __builtin_fma(-h, t, 0.5);
+    y = __builtin_fma(y, e, y);
+    return y;
+}
+
+__device__ __forceinline__ void split_hilo(double v, float &hi, uint16_t &lo)
+{
+    hi = (float)v;
+    const float r = (float)(v - (double)hi);
+    uint32_t b = __float_as_uint(r);
+    b += 0x7fffu + ((b >> 16) & 1u);  // round to nearest even on the upper 16 bits
+    lo = (uint16_t)(b >> 16);
+}
+
+__device__ __forceinline__ double join_hilo(float hi, uint16_t lo)
+{
+    return (double)hi + (double)__uint_as_float((uint32_t)lo << 16);
+}
+
+// ------------------------------------------------------------------ cell geometry
+// deal.II FEValues<2,3> with a Q1 mapping: x = sum phi_k v_k, a = dx/dxi, b = dx/deta,
+// JxW = sqrt(det(J^T J)) w, n = orient (a x b)/|a x b|.  Record: y(3), n*JxW(3), JxW, 0.
+__global__ void k_cell_geometry(const double *__restrict__ X, const int *__restrict__ cell_int,
+                                const int8_t *__restrict__ orient, int nc, int nq,
+                                const double *__restrict__ qpts, const double *__restrict__ qw,
+                                double *__restrict__ cellq)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nc * nq) return;
+    const int c = gid / nq, q = gid % nq;
+    const double xi = qpts[2 * q], eta = qpts[2 * q + 1];
+    const double phi[4] = {(1 - xi) * (1 - eta), xi * (1 - eta), (1 - xi) * eta, xi * eta};
+    const double dxi[4] = {-(1 - eta), (1 - eta), -eta, eta};
+    const double deta[4] = {-(1 - xi), -xi, (1 - xi), xi};
+    double x[3] = {0, 0, 0}, a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        const int v = cell_int[4 * c + k];
+        for (int d = 0; d < 3; ++d) {
+            const double p = X[3 * v + d];
+            x[d] += phi[k] * p;
+            a[d] += dxi[k] * p;
+            b[d] += deta[k] * p;
+        }
+    }
+    const double g00 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    const double g01 = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    const double g11 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+    const double jxw = sqrt(g00 * g11 - g01 * g01) * qw[q];
+    double nn[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double nl = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+    const double s = (orient[c] < 0 ? -1.0 : 1.0) / nl;
+    double *r = cellq + (size_t)gid * 8;
+    r[0] = x[0];
+    r[1] = x[1];
+    r[2] = x[2];
+    r[3] = nn[0] * s * jxw;
+    r[4] = nn[1] * s * jxw;
+    r[5] = nn[2] * s * jxw;
+    r[6] = jxw;
+    r[7] = 0.0;
+}
+
+void launch_cell_geometry(const double *X, const int *cell_int, const int8_t *orient, int nc,
+                          int nq, const double *qpts, const double *qw, double *cellq,
+                          hipStream_t s)
+{
+    const int total = nc * nq;
+    hipLaunchKernelGGL(k_cell_geometry, dim3((total + 255) / 256), dim3(256), 0, s, X, cell_int,
+                       orient, nc, nq, qpts, qw, cellq);
+}
+
+// ------------------------------------------------------------------ regular cells
+constexpr int LDS_STRIDE = TILE_ROWS + 1;  // [column][row], padded against bank conflicts
+
+__device__ __forceinline__ void lds_add(double *p, double v)
+{
+    // fire-and-forget ds_add_f64; each address is owned by one lane of one wavefront
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Q = points per direction of the tensor rule (NQ = Q*Q, point q = i + Q*j, i fastest).
+// The Q1 shape values factor as phi_k(q) = a_k(xi_i) b_k(eta_j), so the four local sums
+// are formed per rule row j from S0 = sum_i (1-xi_i) t_ij and S1 = sum_i xi_i t_ij:
+// only the Q Gauss nodes are needed (kept in SGPRs).
+template <int Q, int OUT>
+__global__ __launch_bounds__(TILE_ROWS) void k_assemble_tiles(
+    const double *__restrict__ xrow, const int *__restrict__ row_cls,
+    const double *__restrict__ cellq, const int4 *__restrict__ cell_cls,
+    const int *__restrict__ tile_ptr, const int *__restrict__ tile_cell,
+    const int *__restrict__ tile_cols, int n, int n_tiles, size_t ld, PhiTable nodes,
+    void *__restrict__ Gv, void *__restrict__ Hv, uint16_t *__restrict__ Glo,
+    uint16_t *__restrict__ Hlo, double *__restrict__ hsum_part)
+{
+    constexpr int NQ = Q * Q;
+    __shared__ double accG[TILE_COLS * LDS_STRIDE];
+    __shared__ double accH[TILE_COLS * LDS_STRIDE];
+    const int lane = threadIdx.x;
+    const int rb = blockIdx.x;
+    const int tg = blockIdx.y;
+    const int row = rb * TILE_ROWS + lane;
+    const int rr = row < n ? row : n - 1;
+    const double x0 = xrow[3 * rr], x1 = xrow[3 * rr + 1], x2 = xrow[3 * rr + 2];
+    const int rc = row_cls[rr];
+    double hs_total = 0.0;
+    const int t_begin = tg * TILES_PER_WG;
+    const int t_end = min(n_tiles, t_begin + TILES_PER_WG);
+    for (int tile = t_begin; tile < t_end; ++tile) {
+#pragma unroll
+        for (int c = 0; c < TILE_COLS; ++c) {
+            accG[c * LDS_STRIDE + lane] = 0.0;
+            accH[c * LDS_STRIDE + lane] = 0.0;
+        }
+        double hs = 0.0;
+        const int p0 = tile_ptr[tile], p1 = tile_ptr[tile + 1];
+        for (int p = p0; p < p1; ++p) {
+            const int cell = __builtin_amdgcn_readfirstlane(tile_cell[p]);
+            const int cols = __builtin_amdgcn_readfirstlane(tile_cols[p]);
+            const int4 cc = cell_cls[cell];
+            const double *rec = cellq + (size_t)cell * (NQ * 8);
+            double g0 = 0, g1 = 0, g2 = 0, g3 = 0, h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+#pragma unroll
+            for (int j = 0; j < Q; ++j) {
+                double sg = 0, sg1 = 0, sh = 0, sh1 = 0;
+#pragma unroll
+                for (int i = 0; i < Q; ++i) {
+                    const double *r = rec + (i + Q * j) * 8;
+                    const double R0 = r[0] - x0, R1 = r[1] - x1, R2 = r[2] - x2;
+                    const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
+                    const double ri = rsqrt_f64(r2);
+                    const double tgv = ri * r[6];
+                    const double dn = __builtin_fma(R0, r[3], __builtin_fma(R1, r[4], R2 * r[5]));
+                    const double thv = dn * (ri * ri * ri);
+                    sg += tgv;
+                    sh += thv;
+                    sg1 = __builtin_fma(nodes.phi[i], tgv, sg1);
+                    sh1 = __builtin_fma(nodes.phi[i], thv, sh1);
+                }
+                const double sg0 = sg - sg1, sh0 = sh - sh1;  // sum (1 - xi_i) t
+                const double eta = nodes.phi[j], ome = 1.0 - eta;
+                g0 = __builtin_fma(ome, sg0, g0);
+                g1 = __builtin_fma(ome, sg1, g1);
+                g2 = __builtin_fma(eta, sg0, g2);
+                g3 = __builtin_fma(eta, sg1, g3);
+                h0 = __builtin_fma(ome, sh0, h0);
+                h1 = __builtin_fma(ome, sh1, h1);
+                h2 = __builtin_fma(eta, sh0, h2);
+                h3 = __builtin_fma(eta, sh1, h3);
+            }
+            const bool sing = (rc == cc.x) | (rc == cc.y) | (rc == cc.z) | (rc == cc.w);
+            if (!sing) {
+                const int c0 = (int)(int8_t)(cols & 0xff);
+                const int c1 = (int)(int8_t)((cols >> 8) & 0xff);
+                const int c2 = (int)(int8_t)((cols >> 16) & 0xff);
+                const int c3 = (int)(int8_t)((cols >> 24) & 0xff);
+                if (c0 >= 0) {
+                    lds_add(&accG[c0 * LDS_STRIDE + lane], g0);
+                    lds_add(&accH[c0 * LDS_STRIDE + lane], h0);
+                    hs += h0;
+                }
+                if (c1 >= 0) {
+                    lds_add(&accG[c1 * LDS_STRIDE + lane], g1);
+                    lds_add(&accH[c1 * LDS_STRIDE + lane], h1);
+                    hs += h1;
+                }
+                if (c2 >= 0) {
+                    lds_add(&accG[c2 * LDS_STRIDE + lane], g2);
+                    lds_add(&accH[c2 * LDS_STRIDE + lane], h2);
+                    hs += h2;
+                }
+                if (c3 >= 0) {
+                    lds_add(&accG[c3 * LDS_STRIDE + lane], g3);
+                    lds_add(&accH[c3 * LDS_STRIDE + lane], h3);
+                    hs += h3;
+                }
+            }
+        }
+        __syncthreads();
+        constexpr int RPI = TILE_ROWS / TILE_COLS;  // rows written per iteration
+        const int cl = lane % TILE_COLS;
+        const int gcol = tile * TILE_COLS + cl;
+#pragma unroll 4
+        for (int r0 = 0; r0 < TILE_ROWS; r0 += RPI) {
+            const int rl = r0 + lane / TILE_COLS;
+            const int grow = rb * TILE_ROWS + rl;
+            if (grow < n && gcol < n) {
+                const double gv = accG[cl * LDS_STRIDE + rl] * INV4PI;
+                const double hv = accH[cl * LDS_STRIDE + rl] * (-INV4PI);
+                const size_t off = (size_t)grow * ld + gcol;
+                if (OUT == 0) {
+                    __builtin_nontemporal_store(gv, (double *)Gv + off);
+                    __builtin_nontemporal_store(hv, (double *)Hv + off);
+                } else {
+                    float gh, hh;
+                    uint16_t gl, hl;
+                    split_hilo(gv, gh, gl);
+                    split_hilo(hv, hh, hl);
+                    __builtin_nontemporal_store(gh, (float *)Gv + off);
+                    __builtin_nontemporal_store(hh, (float *)Hv + off);
+                    __builtin_nontemporal_store(gl, Glo + off);
+                    __builtin_nontemporal_store(hl, Hlo + off);
+                }
+            }
+        }
+        __syncthreads();
+        hs_total += hs;
+    }
+    if (row < n) hsum_part[(size_t)tg * n + row] = hs_total * (-INV4PI);
+}
+
+template <int Q>
+static void launch_tiles_q(int precision, const double *xrow, const int *row_cls,
+                           const double *cellq, const int *cell_cls, const int *tile_ptr,
+                           const int *tile_cell, const int *tile_cols, int n, int n_tiles,
+                           size_t ld, const PhiTable &nodes, void *G, void *H, void *Glo,
+                           void *Hlo, double *hsum_part, hipStream_t s)
+{
+    dim3 grid((n + TILE_ROWS - 1) / TILE_ROWS, (n_tiles + TILES_PER_WG - 1) / TILES_PER_WG);
+    if (precision == WB_PREC_FP64)
+        hipLaunchKernelGGL((k_assemble_tiles<Q, 0>), grid, dim3(TILE_ROWS), 0, s, xrow, row_cls,
+                           cellq, (const int4 *)cell_cls, tile_ptr, tile_cell, tile_cols, n,
+                           n_tiles, ld, nodes, G, H, (uint16_t *)Glo, (uint16_t *)Hlo, hsum_part);
+    else
+        hipLaunchKernelGGL((k_assemble_tiles<Q, 1>), grid, dim3(TILE_ROWS), 0, s, xrow, row_cls,
+                           cellq, (const int4 *)cell_cls, tile_ptr, tile_cell, tile_cols, n,
+                           n_tiles, ld, nodes, G, H, (uint16_t *)Glo, (uint16_t *)Hlo, hsum_part);
+}
+
+// nodes.phi[0..q_order-1] = 1-D Gauss nodes on [0,1] of the regular rule
+void launch_assemble_tiles(int precision, int q_order, const double *xrow, const int *row_cls,
+                           const double *cellq, const int *cell_cls, const int *tile_ptr,
+                           const int *tile_cell, const int *tile_cols, int n, int n_tiles,
+                           size_t ld, const PhiTable &nodes, void *G, void *H, void *Glo,
+                           void *Hlo, double *hsum_part, hipStream_t s)
+{
+#define WB_TILES(QV)                                                                           \
+    case QV:                                                                                   \
+        launch_tiles_q<QV>(precision, xrow, row_cls, cellq, cell_cls, tile_ptr, tile_cell,     \
+                           tile_cols, n, n_tiles, ld, nodes, G, H, Glo, Hlo, hsum_part, s);    \
+        break;
+    switch (q_order) {
+        WB_TILES(1)
+        WB_TILES(2)
+        WB_TILES(3)
+        WB_TILES(4)
+        WB_TILES(5)
+        WB_TILES(6)
+    default: break;
+    }
+#undef WB_TILES
+}
+
+// ------------------------------------------------------------------ singular cells
+// One thread per row: for each cell touching x_i (first coincident vertex j*), evaluate
+// the QGaussOneOverR rule of vertex j* with full FEValues geometry at every point
+// (source/bem_problem.cc:475-525) and add the 4 local values into row i.
+template <int OUT>
+__global__ __launch_bounds__(64) void k_assemble_singular(
+    const double *__restrict__ X, const int *__restrict__ cell_int,
+    const int8_t *__restrict__ orient, const int *__restrict__ sing_ptr,
+    const int *__restrict__ sing_cell, const int *__restrict__ sing_vtx, int n, int ns,
+    const double *__restrict__ rule, size_t ld, void *__restrict__ Gv, void *__restrict__ Hv,
+    uint16_t *__restrict__ Glo, uint16_t *__restrict__ Hlo, double *__restrict__ hsum_sing)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x0 = X[3 * i], x1 = X[3 * i + 1], x2 = X[3 * i + 2];
+    double hs = 0.0;
+    for (int p = sing_ptr[i]; p < sing_ptr[i + 1]; ++p) {
+        const int c = sing_cell[p], vtx = sing_vtx[p];
+        int vid[4];
+        double v[4][3];
+        for (int k = 0; k < 4; ++k) {
+            vid[k] = cell_int[4 * c + k];
+            for (int d = 0; d < 3; ++d) v[k][d] = X[3 * vid[k] + d];
+        }
+        const double sgn = orient[c] < 0 ? -1.0 : 1.0;
+        const double *ru = rule + (size_t)vtx * ns * 3;
+        double lg[4] = {0, 0, 0, 0}, lh[4] = {0, 0, 0, 0};
+        for (int q = 0; q < ns; ++q) {
+            const double xi = ru[3 * q], eta = ru[3 * q + 1], w = ru[3 * q + 2];
+            const double phi[4] = {(1 - xi) * (1 - eta), xi * (1 - eta), (1 - xi) * eta, xi * eta};
+            const double dxi[4] = {-(1 - eta), (1 - eta), -eta, eta};
+            const double deta[4] = {-(1 - xi), -xi, (1 - xi), xi};
+            double y[3] = {0, 0, 0}, a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+            for (int k = 0; k < 4; ++k)
+                for (int d = 0; d < 3; ++d) {
+                    y[d] += phi[k] * v[k][d];
+                    a[d] += dxi[k] * v[k][d];
+                    b[d] += deta[k] * v[k][d];
+                }
+            const double g00 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+            const double g01 = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+            const double g11 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+            const double jxw = sqrt(g00 * g11 - g01 * g01) * w;
+            double nn[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                            a[0] * b[1] - a[1] * b[0]};
+            const double nl = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            const double R0 = y[0] - x0, R1 = y[1] - x1, R2 = y[2] - x2;
+            const double r2 = R0 * R0 + R1 * R1 + R2 * R2;
+            const double ri = rsqrt_f64(r2);
+            const double dn = (R0 * nn[0] + R1 * nn[1] + R2 * nn[2]) * (sgn / nl);
+            const double tg = ri * jxw;
+            const double th = dn * ri * ri * ri * jxw;
+            for (int k = 0; k < 4; ++k) {
+                lg[k] = __builtin_fma(tg, phi[k], lg[k]);
+                lh[k] = __builtin_fma(th, phi[k], lh[k]);
+            }
+        }
+        for (int k = 0; k < 4; ++k) {
+            const double gadd = lg[k] * INV4PI, hadd = lh[k] * (-INV4PI);
+            const size_t off = (size_t)i * ld + vid[k];
+            if (OUT == 0) {
+                ((double *)Gv)[off] += gadd;
+                ((double *)Hv)[off] += hadd;
+            } else {
+                float gh, hh;
+                uint16_t gl, hl;
+                const double gv = join_hilo(((float *)Gv)[off], Glo[off]) + gadd;
+                const double hv = join_hilo(((float *)Hv)[off], Hlo[off]) + hadd;
+                split_hilo(gv, gh, gl);
+                split_hilo(hv, hh, hl);
+                ((float *)Gv)[off] = gh;
+                ((float *)Hv)[off] = hh;
+                Glo[off] = gl;
+                Hlo[off] = hl;
+            }
+            hs += hadd;
+        }
+    }
+    hsum_sing[i] = hs;
+}
+
+void launch_assemble_singular(int precision, const double *X, const int *cell_int,
+                              const int8_t *orient, const int *sing_ptr, const int *sing_cell,
+                              const int *sing_vtx, int n, int ns, const double *rule, size_t ld,
+                              void *G, void *H, void *Glo, void *Hlo, double *hsum_sing,
+                              hipStream_t s)
+{
+    dim3 grid((n + 63) / 64);
+    if (precision == WB_PREC_FP64)
+        hipLaunchKernelGGL(k_assemble_singular<0>, grid, dim3(64), 0, s, X, cell_int, orient,
+                           sing_ptr, sing_cell, sing_vtx, n, ns, rule, ld, G, H, (uint16_t *)Glo,
+                           (uint16_t *)Hlo, hsum_sing);
+    else
+        hipLaunchKernelGGL(k_assemble_singular<1>, grid, dim3(64), 0, s, X, cell_int, orient,
+                           sing_ptr, sing_cell, sing_vtx, n, ns, rule, ld, G, H, (uint16_t *)Glo,
+                           (uint16_t *)Hlo, hsum_sing);
+}
+
+// alpha_i = -(sum over tile groups of the regular H row sums + singular H row sum)
+__global__ void k_alpha_reduce(const double *__restrict__ part, int n_parts,
+                               const double *__restrict__ sing, int n, double *__restrict__ alpha)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int p = 0; p < n_parts; ++p) s += part[(size_t)p * n + i];
+    alpha[i] = -(s + sing[i]);
+}
+
+void launch_alpha_reduce(const double *hsum_part, int n_parts, const double *hsum_sing, int n,
+                         double *alpha, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_alpha_reduce, dim3((n + 255) / 256), dim3(256), 0, s, hsum_part, n_parts,
+                       hsum_sing, n, alpha);
+}
+
+}  // namespace wb

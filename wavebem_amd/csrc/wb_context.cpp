@@ -1,0 +1,1103 @@
+// wb_context.cpp -- C-ABI entry points, device state and the GMRES driver.
+//
+// Stands in for the method bodies of BEMProblem<3> (source/bem_problem.cc); each entry
+// point's reference counterpart is named in include/wavebem_bem.h.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "wb_internal.h"
+
+using namespace wb;
+
+namespace {
+
+struct WbError : std::runtime_error {
+    wb_status code;
+    WbError(wb_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw WbError(e_ == hipErrorOutOfMemory ? WB_E_OOM : WB_E_HIP,                   \
+                          std::string(#expr) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t b)
+    {
+        if (b <= bytes && p) return;
+        release();
+        if (b == 0) return;
+        HIPCHK(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <typename T> T *as() const { return (T *)p; }
+};
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+}  // namespace
+
+struct wb_ctx {
+    wb_config cfg{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    Quadrature quad;
+    MeshPlan plan;
+    bool have_mesh = false, have_masks = false, assembled = false;
+    int n = 0;
+    size_t ld = 0;
+    std::vector<double> surface, other;  // caller order
+    bool pure_neumann = false;
+    // constraints (caller numbering) and whether they are rebuilt from tmp_rhs
+    ConstraintSet cons;
+    bool cons_auto = false;
+    std::vector<double> cons_normals, cons_grads;
+    bool have_cons = false;
+    // versions for the preconditioner cache
+    long assembly_ver = 0, mask_ver = 0, cons_struct_ver = 0;
+    long pc_assembly_ver = -1, pc_mask_ver = -1, pc_cons_ver = -1;
+    int pc_bs = 0;
+
+    // device: mesh
+    DevBuf X, cell_int, orient, cell_cls, row_cls, tile_ptr, tile_cell, tile_cols, sing_ptr,
+        sing_cell, sing_vtx, qpts, qw, srule, cellq, perm_d, iperm_d;
+    // device: operator
+    DevBuf G, H, Glo, Hlo, hsum_part, hsum_sing, alpha;
+    DevBuf surf_d, gmask, diag_op, diag_rhs;
+    DevBuf con_rows, con_ptr, con_cols, con_w, con_inhom, constrained, free_rows;
+    int n_lines = 0, n_free = 0;
+    DevBuf binv, binv32;
+    // device: solver work
+    DevBuf V, z, w, P1, P2, Pn, hbuf, vx, vb, vr, vt, vs;   // double / raw
+    DevBuf V32, z32, w32, r32, d32;
+    double *hpin = nullptr;  // pinned host copy of hbuf
+    size_t vlen = 0;         // padded vector length
+};
+
+static void set_err(wb_ctx *c, const std::string &m)
+{
+    if (c) c->err = m;
+}
+
+template <typename F>
+static wb_status guarded(wb_ctx *c, F &&f)
+{
+    if (!c) return WB_E_ARG;
+    try {
+        c->err.clear();
+        f();
+        return WB_OK;
+    } catch (const WbError &e) {
+        set_err(c, e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_err(c, "host allocation failed");
+        return WB_E_OOM;
+    } catch (const std::exception &e) {
+        set_err(c, e.what());
+        return WB_E_HIP;
+    }
+}
+
+static void require(bool cond, wb_status code, const char *msg)
+{
+    if (!cond) throw WbError(code, msg);
+}
+
+template <typename T>
+static void upload(DevBuf &b, const T *src, size_t count, hipStream_t s)
+{
+    b.ensure(sizeof(T) * std::max<size_t>(count, 1));
+    if (count) HIPCHK(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+}
+
+static size_t elem_bytes(const wb_ctx *c) { return c->cfg.precision == WB_PREC_FP64 ? 8 : 4; }
+
+// ------------------------------------------------------------------ vector plumbing
+static void to_internal(const wb_ctx *c, const double *src, std::vector<double> &dst)
+{
+    dst.assign(c->vlen, 0.0);
+    for (int d = 0; d < c->n; ++d) dst[c->plan.perm[d]] = src[d];
+}
+
+static void to_caller(const wb_ctx *c, const std::vector<double> &src, double *dst)
+{
+    for (int d = 0; d < c->n; ++d) dst[d] = src[c->plan.perm[d]];
+}
+
+static void upload_vec(wb_ctx *c, DevBuf &b, const double *caller)
+{
+    std::vector<double> tmp;
+    to_internal(c, caller, tmp);
+    b.ensure(sizeof(double) * c->vlen);
+    HIPCHK(hipMemcpyAsync(b.p, tmp.data(), sizeof(double) * c->vlen, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+static void download_vec(wb_ctx *c, const DevBuf &b, double *caller)
+{
+    std::vector<double> tmp(c->vlen);
+    HIPCHK(hipMemcpyAsync(tmp.data(), b.p, sizeof(double) * c->vlen, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    to_caller(c, tmp, caller);
+}
+
+static void ensure_work(wb_ctx *c)
+{
+    const size_t vb = sizeof(double) * c->vlen;
+    for (DevBuf *b : {&c->z, &c->w, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs}) {
+        if (b->bytes < vb) {
+            b->ensure(vb);
+            HIPCHK(hipMemsetAsync(b->p, 0, vb, c->stream));
+        }
+    }
+    const int parts = krylov_partials_blocks(c->n);
+    c->P1.ensure(sizeof(double) * parts * 128);
+    c->P2.ensure(sizeof(double) * parts * 128);
+    c->Pn.ensure(sizeof(double) * parts);
+    c->hbuf.ensure(sizeof(double) * 132);
+    if (!c->hpin) HIPCHK(hipHostMalloc((void **)&c->hpin, sizeof(double) * 132, 0));
+}
+
+static double device_norm(wb_ctx *c, const void *x, int is_float)
+{
+    const int parts = krylov_partials_blocks(c->n);
+    launch_norm2_partials(is_float, x, c->n, c->Pn.as<double>(), c->stream);
+    std::vector<double> p(parts);
+    HIPCHK(hipMemcpyAsync(p.data(), c->Pn.p, sizeof(double) * parts, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (double v : p) s += v;
+    return std::sqrt(s);
+}
+
+// ------------------------------------------------------------------ operator pieces
+// which: 0 = BEMProblem::vmult selection, 1 = compute_rhs selection
+static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, void *y,
+                 bool free_rows_only)
+{
+    const double *diag = which == 0 ? c->diag_op.as<double>() : c->diag_rhs.as<double>();
+    const double s_sel = which == 0 ? 1.0 : -1.0, s_uns = which == 0 ? -1.0 : 1.0;
+    const int *rows = free_rows_only ? c->free_rows.as<int>() : nullptr;
+    const int nr = free_rows_only ? c->n_free : c->n;
+    launch_gemv_sel(mode, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld, c->n, c->gmask.as<uint8_t>(),
+                    which, s_sel, s_uns, x, diag, xd, rows, nr, y, c->stream);
+}
+
+// y = ConstrainedOperator(A) x in FP64 (mode 0 or 2) or FP32 (mode 1)
+static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
+{
+    const int is_float = mode == 1;
+    if (c->pure_neumann) {
+        gemv(c, mode, 0, x, x, y, false);
+        const double nrm = device_norm(c, y, is_float);
+        // dst.add(-dst.l2_norm()): add -||dst|| to every entry
+        std::vector<char> tmp;
+        if (is_float) {
+            std::vector<float> v(c->n);
+            HIPCHK(hipMemcpy(v.data(), y, sizeof(float) * c->n, hipMemcpyDeviceToHost));
+            for (auto &e : v) e = (float)((double)e - nrm);
+            HIPCHK(hipMemcpy(y, v.data(), sizeof(float) * c->n, hipMemcpyHostToDevice));
+        } else {
+            std::vector<double> v(c->n);
+            HIPCHK(hipMemcpy(v.data(), y, sizeof(double) * c->n, hipMemcpyDeviceToHost));
+            for (auto &e : v) e -= nrm;
+            HIPCHK(hipMemcpy(y, v.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
+        }
+    } else {
+        gemv(c, mode, 0, x, x, y, true);
+    }
+    launch_constraint_rows(is_float, c->n_lines, c->con_rows.as<int>(), c->con_ptr.as<int>(),
+                           c->con_cols.as<int>(), c->con_w.as<double>(),
+                           c->con_inhom.as<double>(), 0, x, y, c->stream);
+}
+
+static void apply_precond(wb_ctx *c, int is_float, const void *x, void *y)
+{
+    if (c->cfg.preconditioner == WB_PRECOND_BLOCK_JACOBI) {
+        launch_block_jacobi_apply(is_float, c->n, c->pc_bs, is_float ? c->binv32.p : c->binv.p, x,
+                                  y, 1, c->vlen, c->stream);
+    } else {
+        HIPCHK(hipMemcpyAsync(y, x, (is_float ? 4 : 8) * (size_t)c->n, hipMemcpyDeviceToDevice,
+                              c->stream));
+    }
+}
+
+static void refresh_diag(wb_ctx *c)
+{
+    // diag_op = alpha .* other, diag_rhs = -alpha .* surface (internal order)
+    std::vector<double> so, ss;
+    to_internal(c, c->other.data(), so);
+    to_internal(c, c->surface.data(), ss);
+    for (auto &v : ss) v = -v;
+    c->vt.ensure(sizeof(double) * c->vlen);
+    upload(c->vt, so.data(), c->vlen, c->stream);
+    c->diag_op.ensure(sizeof(double) * c->vlen);
+    c->diag_rhs.ensure(sizeof(double) * c->vlen);
+    launch_vec_ops(4, c->n, c->alpha.p, c->vt.p, c->diag_op.p, 0, c->stream);
+    upload(c->vt, ss.data(), c->vlen, c->stream);
+    launch_vec_ops(4, c->n, c->alpha.p, c->vt.p, c->diag_rhs.p, 0, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+static void install_constraints(wb_ctx *c, const ConstraintSet &cs)
+{
+    const int n = c->n;
+    const int nl = (int)cs.rows.size();
+    std::vector<int> rows(nl), ptr(cs.ptr), cols(cs.cols.size());
+    std::vector<uint8_t> flag(n, 0);
+    for (int l = 0; l < nl; ++l) {
+        require(cs.rows[l] >= 0 && cs.rows[l] < n, WB_E_ARG, "constraint row out of range");
+        rows[l] = c->plan.perm[cs.rows[l]];
+        require(!flag[rows[l]], WB_E_ARG, "duplicate constraint line");
+        flag[rows[l]] = 1;
+    }
+    for (size_t p = 0; p < cs.cols.size(); ++p) {
+        require(cs.cols[p] >= 0 && cs.cols[p] < n, WB_E_ARG, "constraint column out of range");
+        cols[p] = c->plan.perm[cs.cols[p]];
+    }
+    bool same_struct = c->have_cons && (int)c->cons.rows.size() == nl && c->cons.rows == cs.rows &&
+                       c->cons.ptr == cs.ptr && c->cons.cols == cs.cols && c->cons.w == cs.w;
+    c->cons = cs;
+    c->have_cons = true;
+    c->n_lines = nl;
+    upload(c->con_rows, rows.data(), rows.size(), c->stream);
+    upload(c->con_ptr, ptr.data(), ptr.size(), c->stream);
+    upload(c->con_cols, cols.data(), cols.size(), c->stream);
+    upload(c->con_w, cs.w.data(), cs.w.size(), c->stream);
+    upload(c->con_inhom, cs.inhom.data(), cs.inhom.size(), c->stream);
+    upload(c->constrained, flag.data(), flag.size(), c->stream);
+    std::vector<int> fr;
+    fr.reserve(n);
+    for (int i = 0; i < n; ++i)
+        if (!flag[i]) fr.push_back(i);
+    c->n_free = (int)fr.size();
+    upload(c->free_rows, fr.data(), fr.size(), c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!same_struct) c->cons_struct_ver++;
+}
+
+static void ensure_precond(wb_ctx *c)
+{
+    if (c->cfg.preconditioner != WB_PRECOND_BLOCK_JACOBI) return;
+    if (c->pc_assembly_ver == c->assembly_ver && c->pc_mask_ver == c->mask_ver &&
+        c->pc_cons_ver == c->cons_struct_ver)
+        return;
+    const int bs = c->pc_bs;
+    const int nb = (c->n + bs - 1) / bs;
+    c->binv.ensure(sizeof(double) * (size_t)nb * bs * bs);
+    c->binv32.ensure(sizeof(float) * (size_t)nb * bs * bs);
+    launch_block_jacobi_build(c->n, bs, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->cfg.precision,
+                              c->ld, c->alpha.as<double>(), c->surf_d.as<double>(),
+                              c->constrained.as<uint8_t>(), c->binv.as<double>(),
+                              c->binv32.as<float>(), c->stream);
+    HIPCHK(hipGetLastError());
+    c->pc_assembly_ver = c->assembly_ver;
+    c->pc_mask_ver = c->mask_ver;
+    c->pc_cons_ver = c->cons_struct_ver;
+}
+
+// ------------------------------------------------------------------ GMRES
+// deal.II SolverGMRES semantics (left preconditioning, restart after max_tmp-2 inner
+// steps, SolverControl check on |gamma| with an absolute tolerance, x0 given), with the
+// Arnoldi step done as classical Gram-Schmidt with one re-orthogonalization pass.
+// is_float: vectors and operator in FP32 (inner solver of the refinement), else FP64.
+// b and x are device vectors of the same type.  Returns 1 converged, -1 failed.
+struct GmresResult {
+    int status = 0, iterations = 0, restarts = 0;
+    double residual = 0.0, first_residual = 0.0;
+};
+
+static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double abs_tol,
+                         double rel_tol, int max_steps, bool x_is_zero)
+{
+    const int n = c->n;
+    const size_t ldv = c->vlen;
+    const int m = std::max(1, c->cfg.gmres_max_tmp_vectors - 2);
+    const size_t es = is_float ? 4 : 8;
+    DevBuf &V = is_float ? c->V32 : c->V;
+    if (V.bytes < es * ldv * (m + 1)) {
+        V.ensure(es * ldv * (m + 1));
+        HIPCHK(hipMemsetAsync(V.p, 0, V.bytes, c->stream));
+    }
+    void *z = is_float ? c->z32.p : c->z.p;
+    void *w = is_float ? c->w32.p : c->w.p;
+    const int mode = is_float ? 1 : (c->cfg.precision == WB_PREC_FP64 ? 0 : 2);
+    const int parts = krylov_partials_blocks(n);
+    auto vec = [&](int k) { return (void *)((char *)V.p + es * ldv * k); };
+    std::vector<double> h(m + 2), gamma(m + 2), ci(m + 1), si(m + 1), Hh((size_t)(m + 1) * m),
+        y(m + 1);
+    GmresResult R;
+    int acc = 0;
+    int state = 0;
+    bool first = true;
+    double tol = abs_tol;
+    do {
+        // v0 = P^-1 (b - A x)
+        if (x_is_zero && first) {
+            apply_precond(c, is_float, b, z);
+        } else {
+            apply_constrained(c, mode, x, w);
+            launch_vec_ops(is_float ? 9 : 8, n, w, b, w, 0, c->stream);
+            apply_precond(c, is_float, w, z);
+        }
+        launch_norm2_partials(is_float, z, n, c->Pn.as<double>(), c->stream);
+        launch_finalize(is_float, 0, c->P1.as<double>(), nullptr, c->Pn.as<double>(), parts, z, n,
+                        vec(0), c->hbuf.as<double>(), c->stream);
+        HIPCHK(hipMemcpyAsync(c->hpin, c->hbuf.p, sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double rho = c->hpin[0];
+        if (first) {
+            R.first_residual = rho;
+            if (rel_tol > 0) tol = std::max(abs_tol, rel_tol * rho);
+        }
+        first = false;
+        R.residual = rho;
+        // SolverControl::check
+        if (rho <= tol) state = 1;
+        else if (std::isnan(rho) || acc >= max_steps) state = -1;
+        if (state != 0) break;
+        gamma[0] = rho;
+        int dim = 0;
+        for (int inner = 0; inner < m && state == 0; ++inner) {
+            ++acc;
+            apply_constrained(c, mode, vec(inner), w);
+            apply_precond(c, is_float, w, z);
+            dim = inner + 1;
+            launch_dot_partials(is_float, V.p, ldv, dim, z, n, c->P1.as<double>(), c->stream);
+            launch_update_dot(is_float, V.p, ldv, dim, c->P1.as<double>(), parts, z, n,
+                              c->P2.as<double>(), nullptr, 0, c->stream);
+            launch_update_dot(is_float, V.p, ldv, dim, c->P2.as<double>(), parts, z, n, nullptr,
+                              c->Pn.as<double>(), 1, c->stream);
+            launch_finalize(is_float, dim, c->P1.as<double>(), c->P2.as<double>(),
+                            c->Pn.as<double>(), parts, z, n, vec(dim), c->hbuf.as<double>(),
+                            c->stream);
+            HIPCHK(hipMemcpyAsync(c->hpin, c->hbuf.p, sizeof(double) * (dim + 1),
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int i = 0; i <= dim; ++i) h[i] = c->hpin[i];
+            for (int i = 0; i < inner; ++i) {
+                const double s = si[i], cc = ci[i], d0 = h[i];
+                h[i] = cc * d0 + s * h[i + 1];
+                h[i + 1] = -s * d0 + cc * h[i + 1];
+            }
+            const double r = 1. / std::sqrt(h[inner] * h[inner] + h[inner + 1] * h[inner + 1]);
+            si[inner] = h[inner + 1] * r;
+            ci[inner] = h[inner] * r;
+            h[inner] = ci[inner] * h[inner] + si[inner] * h[inner + 1];
+            gamma[inner + 1] = -si[inner] * gamma[inner];
+            gamma[inner] *= ci[inner];
+            for (int i = 0; i < dim; ++i) Hh[(size_t)i * m + inner] = h[i];
+            rho = std::fabs(gamma[dim]);
+            R.residual = rho;
+            if (rho <= tol) state = 1;
+            else if (std::isnan(rho) || acc >= max_steps) state = -1;
+        }
+        for (int i = dim - 1; i >= 0; --i) {
+            double s = gamma[i];
+            for (int j = i + 1; j < dim; ++j) s -= Hh[(size_t)i * m + j] * y[j];
+            y[i] = s / Hh[(size_t)i * m + i];
+        }
+        if (dim > 0) {
+            HIPCHK(hipMemcpyAsync(c->hbuf.p, y.data(), sizeof(double) * dim,
+                                  hipMemcpyHostToDevice, c->stream));
+            launch_axpy_basis(is_float, V.p, ldv, dim, c->hbuf.as<double>(), x, is_float, n,
+                              c->stream);
+        }
+        if (state == 0) R.restarts++;
+    } while (state == 0);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    R.iterations = acc;
+    R.status = state;
+    return R;
+}
+
+// ------------------------------------------------------------------ solve_system
+// b (device, internal order, vlen) must already hold the distributed right-hand side;
+// the solution lands in c->vx.
+static void run_solver(wb_ctx *c, wb_solve_stats *st)
+{
+    ensure_work(c);
+    const int n = c->n;
+    double t0 = now_ms();
+    ensure_precond(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double t1 = now_ms();
+    if (st) st->t_precond_ms = t1 - t0;
+    launch_vec_ops(6, n, nullptr, nullptr, c->vx.p, 0, c->stream);
+    wb_solve_stats s{};
+    if (c->cfg.precision == WB_PREC_FP64) {
+        GmresResult R = gmres(c, 0, c->vb.p, c->vx.p, c->cfg.solver_tolerance, 0.0,
+                              c->cfg.solver_max_steps, true);
+        s.iterations = R.iterations;
+        s.restarts = R.restarts;
+        s.residual = R.residual;
+        if (R.status != 1) {
+            if (st) *st = s;
+            throw WbError(WB_E_NOCONV, "GMRES did not converge (SolverControl::NoConvergence)");
+        }
+    } else {
+        // FP64 iterative refinement around an FP32 GMRES on the FP32 (hi) operator.
+        const size_t vb4 = sizeof(float) * c->vlen;
+        for (DevBuf *b : {&c->z32, &c->w32, &c->r32, &c->d32}) {
+            if (b->bytes < vb4) {
+                b->ensure(vb4);
+                HIPCHK(hipMemsetAsync(b->p, 0, vb4, c->stream));
+            }
+        }
+        // r = b (x = 0)
+        HIPCHK(hipMemcpyAsync(c->vr.p, c->vb.p, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                              c->stream));
+        int total_inner = 0, sweeps = 0;
+        double rho = 0.0;
+        bool converged = false;
+        for (;;) {
+            apply_precond(c, 0, c->vr.p, c->vt.p);
+            rho = device_norm(c, c->vt.p, 0);
+            s.residual = rho;
+            if (rho <= c->cfg.solver_tolerance) {
+                converged = true;
+                break;
+            }
+            if (std::isnan(rho) || sweeps >= c->cfg.ir_max_sweeps ||
+                total_inner >= c->cfg.solver_max_steps)
+                break;
+            // inner: A32 d = r, from d = 0, relative reduction ir_inner_reduction
+            launch_vec_ops(1, n, c->vr.p, nullptr, c->r32.p, 0, c->stream);
+            launch_vec_ops(7, n, nullptr, nullptr, c->d32.p, 0, c->stream);
+            GmresResult R = gmres(c, 1, c->r32.p, c->d32.p, 0.0, c->cfg.ir_inner_reduction,
+                                  c->cfg.solver_max_steps - total_inner, true);
+            total_inner += R.iterations;
+            s.restarts += R.restarts;
+            launch_vec_ops(2, n, c->d32.p, nullptr, c->vx.p, 0, c->stream);
+            // r = b - A x  (FP64-accurate operator: hi + lo)
+            apply_constrained(c, 2, c->vx.p, c->w.p);
+            launch_vec_ops(8, n, c->w.p, c->vb.p, c->vr.p, 0, c->stream);
+            sweeps++;
+        }
+        s.iterations = total_inner;
+        s.ir_sweeps = sweeps;
+        if (!converged) {
+            if (st) *st = s;
+            throw WbError(WB_E_NOCONV, "iterative refinement did not converge");
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double t2 = now_ms();
+    s.t_precond_ms = t1 - t0;
+    s.t_gmres_ms = t2 - t1;
+    // true relative residual of the constrained system, FP64 operator
+    {
+        const int mode = c->cfg.precision == WB_PREC_FP64 ? 0 : 2;
+        apply_constrained(c, mode, c->vx.p, c->w.p);
+        launch_vec_ops(8, n, c->w.p, c->vb.p, c->vt.p, 0, c->stream);
+        const double rn = device_norm(c, c->vt.p, 0);
+        const double bn = device_norm(c, c->vb.p, 0);
+        s.true_rel_residual = bn > 0 ? rn / bn : rn;
+    }
+    if (st) {
+        s.t_alpha_ms = st->t_alpha_ms;
+        s.t_rhs_ms = st->t_rhs_ms;
+        *st = s;
+    }
+}
+
+// rhs = compute_rhs(tmp) then distribute_rhs (constrained rows = inhomogeneity), in vb
+static void build_rhs(wb_ctx *c, const void *tmp_dev)
+{
+    gemv(c, c->cfg.precision == WB_PREC_FP64 ? 0 : 2, 1, tmp_dev, tmp_dev, c->vb.p, true);
+    launch_constraint_rows(0, c->n_lines, c->con_rows.as<int>(), c->con_ptr.as<int>(),
+                           c->con_cols.as<int>(), c->con_w.as<double>(), c->con_inhom.as<double>(),
+                           1, nullptr, c->vb.p, c->stream);
+}
+
+static void refresh_auto_constraints(wb_ctx *c, const double *tmp_rhs_caller)
+{
+    if (!c->cons_auto) return;
+    ConstraintSet cs;
+    std::string e = build_constraints(
+        c->n, c->plan.dn_ptr.data(), c->plan.dn_idx.data(), c->surface.data(),
+        c->cons_normals.empty() ? nullptr : c->cons_normals.data(),
+        c->cons_grads.empty() ? nullptr : c->cons_grads.data(), tmp_rhs_caller, cs);
+    require(e.empty(), WB_E_UNSUPPORTED, e.c_str());
+    install_constraints(c, cs);
+}
+
+static void check_ready(wb_ctx *c, bool need_assembly)
+{
+    require(c->have_mesh, WB_E_STATE, "wb_set_mesh has not been called");
+    require(c->have_masks, WB_E_STATE, "wb_set_masks has not been called");
+    if (need_assembly) require(c->assembled, WB_E_STATE, "wb_assemble has not been called");
+}
+
+static void solve_system_dev(wb_ctx *c, const double *tmp_dev_internal, wb_solve_stats *st)
+{
+    double t0 = now_ms();
+    build_rhs(c, tmp_dev_internal);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (st) {
+        st->t_alpha_ms = 0.0;  // alpha comes out of the assembly kernels
+        st->t_rhs_ms = now_ms() - t0;
+    }
+    run_solver(c, st);
+}
+
+// ==================================================================== C-ABI
+extern "C" {
+
+void wb_config_default(wb_config *cfg)
+{
+    if (!cfg) return;
+    cfg->device = 0;
+    cfg->quadrature_order = 4;
+    cfg->singular_quadrature_order = 5;
+    cfg->solver_tolerance = 1e-10;
+    cfg->solver_max_steps = 1000;
+    cfg->gmres_max_tmp_vectors = 100;
+    cfg->precision = WB_PREC_FP64;
+    cfg->preconditioner = WB_PRECOND_BLOCK_JACOBI;
+    cfg->precond_block = 64;
+    cfg->ir_max_sweeps = 10;
+    cfg->ir_inner_reduction = 1e-5;
+}
+
+const char *wb_version(void) { return "wavebem_amd 0.1 (gfx950)"; }
+
+wb_status wb_create(wb_ctx **out, const wb_config *cfg)
+{
+    if (!out) return WB_E_ARG;
+    *out = nullptr;
+    wb_ctx *c = new (std::nothrow) wb_ctx();
+    if (!c) return WB_E_OOM;
+    if (cfg)
+        c->cfg = *cfg;
+    else
+        wb_config_default(&c->cfg);
+    const wb_config &k = c->cfg;
+    if (k.quadrature_order < 1 || k.quadrature_order > MAX_NQ1 ||
+        k.singular_quadrature_order < 1 || k.singular_quadrature_order > MAX_SING ||
+        k.gmres_max_tmp_vectors < 3 || k.gmres_max_tmp_vectors > 129 ||
+        (k.precision != WB_PREC_FP64 && k.precision != WB_PREC_FP32_IR) ||
+        k.precond_block < 1 || k.precond_block > 128 || k.solver_max_steps < 0 ||
+        !(k.solver_tolerance >= 0)) {
+        delete c;
+        return WB_E_ARG;
+    }
+    wb_status st = guarded(c, [&] {
+        HIPCHK(hipSetDevice(k.device));
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        build_quadrature(k.quadrature_order, k.singular_quadrature_order, c->quad);
+    });
+    if (st != WB_OK) {
+        delete c;
+        return st;
+    }
+    c->pc_bs = k.precond_block;
+    *out = c;
+    return WB_OK;
+}
+
+void wb_destroy(wb_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DevBuf *bufs[] = {&c->X, &c->cell_int, &c->orient, &c->cell_cls, &c->row_cls, &c->tile_ptr,
+                      &c->tile_cell, &c->tile_cols, &c->sing_ptr, &c->sing_cell, &c->sing_vtx,
+                      &c->qpts, &c->qw, &c->srule, &c->cellq, &c->perm_d, &c->iperm_d, &c->G,
+                      &c->H, &c->Glo, &c->Hlo, &c->hsum_part, &c->hsum_sing, &c->alpha,
+                      &c->surf_d, &c->gmask, &c->diag_op, &c->diag_rhs, &c->con_rows,
+                      &c->con_ptr, &c->con_cols, &c->con_w, &c->con_inhom, &c->constrained,
+                      &c->free_rows, &c->binv, &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2,
+                      &c->Pn, &c->hbuf, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
+                      &c->z32, &c->w32, &c->r32, &c->d32};
+    for (DevBuf *b : bufs) b->release();
+    if (c->hpin) (void)hipHostFree(c->hpin);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *wb_last_error(const wb_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int32_t *cell_dofs,
+                      const int8_t *cell_orient, const int32_t *cell_material,
+                      const int32_t *dn_ptr, const int32_t *dn_idx)
+{
+    return guarded(c, [&] {
+        require(n > 0 && nc > 0 && support && cell_dofs && cell_orient, WB_E_ARG,
+                "wb_set_mesh: bad sizes or null pointers");
+        require((dn_ptr == nullptr) == (dn_idx == nullptr), WB_E_ARG,
+                "wb_set_mesh: dn_ptr and dn_idx must both be given or both NULL");
+        for (int k = 0; k < nc; ++k)
+            require(cell_orient[k] == 1 || cell_orient[k] == -1, WB_E_ARG,
+                    "wb_set_mesh: cell_orient must be +-1");
+        HIPCHK(hipSetDevice(c->cfg.device));
+        MeshPlan P;
+        std::string e = plan_mesh(n, nc, support, cell_dofs, cell_material, dn_ptr, dn_idx, P);
+        require(e.empty(), WB_E_ARG, e.c_str());
+        c->have_mesh = c->have_masks = c->assembled = c->have_cons = false;
+        c->plan = std::move(P);
+        c->n = n;
+        c->ld = c->plan.ld;
+        c->vlen = c->ld;
+        const MeshPlan &p = c->plan;
+        std::vector<double> Xi(3 * (size_t)n);
+        for (int d = 0; d < n; ++d)
+            for (int k = 0; k < 3; ++k) Xi[3 * p.perm[d] + k] = support[3 * d + k];
+        hipStream_t s = c->stream;
+        upload(c->X, Xi.data(), Xi.size(), s);
+        upload(c->cell_int, p.cell_int.data(), p.cell_int.size(), s);
+        upload(c->orient, cell_orient, nc, s);
+        upload(c->cell_cls, p.cell_cls.data(), p.cell_cls.size(), s);
+        upload(c->row_cls, p.cls.data(), p.cls.size(), s);
+        upload(c->tile_ptr, p.tile_ptr.data(), p.tile_ptr.size(), s);
+        upload(c->tile_cell, p.tile_cell.data(), p.tile_cell.size(), s);
+        upload(c->tile_cols, p.tile_cols.data(), p.tile_cols.size(), s);
+        upload(c->sing_ptr, p.sing_ptr.data(), p.sing_ptr.size(), s);
+        upload(c->sing_cell, p.sing_cell.data(), p.sing_cell.size(), s);
+        upload(c->sing_vtx, p.sing_vtx.data(), p.sing_vtx.size(), s);
+        upload(c->perm_d, p.perm.data(), p.perm.size(), s);
+        upload(c->iperm_d, p.iperm.data(), p.iperm.size(), s);
+        const Quadrature &q = c->quad;
+        upload(c->qpts, q.reg_pts.data(), q.reg_pts.size(), s);
+        upload(c->qw, q.reg_w.data(), q.reg_w.size(), s);
+        const int ns = 2 * q.s_order * q.s_order;
+        std::vector<double> rule(4 * (size_t)ns * 3);
+        for (int v = 0; v < 4; ++v)
+            for (int k = 0; k < ns; ++k) {
+                rule[3 * (v * ns + k)] = q.sing_pts[2 * (v * ns + k)];
+                rule[3 * (v * ns + k) + 1] = q.sing_pts[2 * (v * ns + k) + 1];
+                rule[3 * (v * ns + k) + 2] = q.sing_w[v * ns + k];
+            }
+        upload(c->srule, rule.data(), rule.size(), s);
+        const int nq = q.q_order * q.q_order;
+        c->cellq.ensure(sizeof(double) * 8 * (size_t)nq * nc);
+        launch_cell_geometry(c->X.as<double>(), c->cell_int.as<int>(), c->orient.as<int8_t>(), nc,
+                             nq, c->qpts.as<double>(), c->qw.as<double>(), c->cellq.as<double>(),
+                             s);
+        HIPCHK(hipGetLastError());
+        // matrices, zero-filled so the padding columns read as 0
+        const size_t entries = (size_t)n * c->ld;
+        c->G.ensure(entries * elem_bytes(c));
+        c->H.ensure(entries * elem_bytes(c));
+        HIPCHK(hipMemsetAsync(c->G.p, 0, c->G.bytes, s));
+        HIPCHK(hipMemsetAsync(c->H.p, 0, c->H.bytes, s));
+        if (c->cfg.precision == WB_PREC_FP32_IR) {
+            c->Glo.ensure(entries * 2);
+            c->Hlo.ensure(entries * 2);
+            HIPCHK(hipMemsetAsync(c->Glo.p, 0, c->Glo.bytes, s));
+            HIPCHK(hipMemsetAsync(c->Hlo.p, 0, c->Hlo.bytes, s));
+        } else {
+            c->Glo.release();
+            c->Hlo.release();
+        }
+        const int ntg = (p.n_tiles + TILES_PER_WG - 1) / TILES_PER_WG;
+        c->hsum_part.ensure(sizeof(double) * (size_t)ntg * n);
+        c->hsum_sing.ensure(sizeof(double) * n);
+        c->alpha.ensure(sizeof(double) * c->vlen);
+        HIPCHK(hipMemsetAsync(c->alpha.p, 0, c->alpha.bytes, s));
+        c->surf_d.ensure(sizeof(double) * c->vlen);
+        c->gmask.ensure((c->vlen + 3) / 4);
+        // default: no constraints
+        c->n_lines = 0;
+        std::vector<int> fr(n);
+        for (int i = 0; i < n; ++i) fr[i] = i;
+        c->n_free = n;
+        upload(c->free_rows, fr.data(), fr.size(), s);
+        std::vector<uint8_t> zf(n, 0);
+        upload(c->constrained, zf.data(), zf.size(), s);
+        c->con_rows.ensure(4);
+        c->con_ptr.ensure(4);
+        c->con_cols.ensure(4);
+        c->con_w.ensure(8);
+        c->con_inhom.ensure(8);
+        c->cons = ConstraintSet();
+        c->cons.ptr.push_back(0);
+        c->cons_auto = false;
+        c->cons_struct_ver++;
+        HIPCHK(hipStreamSynchronize(s));
+        c->have_mesh = true;
+    });
+}
+
+wb_status wb_set_masks(wb_ctx *c, const double *surface, const double *other)
+{
+    return guarded(c, [&] {
+        require(c->have_mesh, WB_E_STATE, "wb_set_masks before wb_set_mesh");
+        require(surface && other, WB_E_ARG, "null mask");
+        const int n = c->n;
+        // vmult/compute_rhs use the masks as multipliers; they must be complementary 0/1
+        for (int i = 0; i < n; ++i)
+            require((surface[i] == 0.0 || surface[i] == 1.0) && other[i] == 1.0 - surface[i],
+                    WB_E_UNSUPPORTED, "masks must be complementary 0/1 vectors");
+        c->surface.assign(surface, surface + n);
+        c->other.assign(other, other + n);
+        double linf = 0;
+        for (int i = 0; i < n; ++i) linf = std::max(linf, std::fabs(surface[i]));
+        c->pure_neumann = linf < 1e-10;
+        std::vector<double> si;
+        to_internal(c, surface, si);
+        upload(c->surf_d, si.data(), c->vlen, c->stream);
+        // column selection: bit set -> column comes from H in vmult (other_nodes != 0)
+        std::vector<double> oi;
+        to_internal(c, other, oi);
+        const size_t ng = (c->vlen + 3) / 4;
+        std::vector<uint8_t> gm(ng, 0);
+        for (int j = 0; j < n; ++j)
+            if (oi[j] != 0.0) gm[j / 4] |= (uint8_t)(1u << (j % 4));
+        upload(c->gmask, gm.data(), gm.size(), c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->have_masks = true;
+        c->mask_ver++;
+        if (c->assembled) refresh_diag(c);
+    });
+}
+
+wb_status wb_assemble(wb_ctx *c)
+{
+    return guarded(c, [&] {
+        require(c->have_mesh, WB_E_STATE, "wb_assemble before wb_set_mesh");
+        HIPCHK(hipSetDevice(c->cfg.device));
+        const MeshPlan &p = c->plan;
+        PhiTable nodes{};
+        for (int i = 0; i < c->quad.q_order; ++i) nodes.phi[i] = c->quad.reg_pts[2 * i];
+        launch_assemble_tiles(c->cfg.precision, c->quad.q_order, c->X.as<double>(),
+                              c->row_cls.as<int>(),
+                              c->cellq.as<double>(), c->cell_cls.as<int>(), c->tile_ptr.as<int>(),
+                              c->tile_cell.as<int>(), c->tile_cols.as<int>(), c->n, p.n_tiles,
+                              c->ld, nodes, c->G.p, c->H.p, c->Glo.p, c->Hlo.p,
+                              c->hsum_part.as<double>(), c->stream);
+        HIPCHK(hipGetLastError());
+        const int ns = 2 * c->quad.s_order * c->quad.s_order;
+        launch_assemble_singular(c->cfg.precision, c->X.as<double>(), c->cell_int.as<int>(),
+                                 c->orient.as<int8_t>(), c->sing_ptr.as<int>(),
+                                 c->sing_cell.as<int>(), c->sing_vtx.as<int>(), c->n, ns,
+                                 c->srule.as<double>(), c->ld, c->G.p, c->H.p, c->Glo.p, c->Hlo.p,
+                                 c->hsum_sing.as<double>(), c->stream);
+        HIPCHK(hipGetLastError());
+        const int ntg = (p.n_tiles + TILES_PER_WG - 1) / TILES_PER_WG;
+        launch_alpha_reduce(c->hsum_part.as<double>(), ntg, c->hsum_sing.as<double>(), c->n,
+                            c->alpha.as<double>(), c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->assembled = true;
+        c->assembly_ver++;
+        if (c->have_masks) refresh_diag(c);
+    });
+}
+
+wb_status wb_get_matrices(wb_ctx *c, double *G, double *H)
+{
+    return guarded(c, [&] {
+        require(c->assembled, WB_E_STATE, "wb_get_matrices before wb_assemble");
+        const int n = c->n;
+        const size_t ld = c->ld;
+        for (int which = 0; which < 2; ++which) {
+            double *out = which == 0 ? G : H;
+            if (!out) continue;
+            const DevBuf &hi = which == 0 ? c->G : c->H;
+            const DevBuf &lo = which == 0 ? c->Glo : c->Hlo;
+            std::vector<double> row(ld);
+            std::vector<float> rowf(ld);
+            std::vector<uint16_t> rowl(ld);
+            for (int d = 0; d < n; ++d) {
+                const int i = c->plan.perm[d];
+                if (c->cfg.precision == WB_PREC_FP64) {
+                    HIPCHK(hipMemcpy(row.data(), (char *)hi.p + sizeof(double) * ld * i,
+                                     sizeof(double) * ld, hipMemcpyDeviceToHost));
+                } else {
+                    HIPCHK(hipMemcpy(rowf.data(), (char *)hi.p + sizeof(float) * ld * i,
+                                     sizeof(float) * ld, hipMemcpyDeviceToHost));
+                    HIPCHK(hipMemcpy(rowl.data(), (char *)lo.p + sizeof(uint16_t) * ld * i,
+                                     sizeof(uint16_t) * ld, hipMemcpyDeviceToHost));
+                    for (size_t j = 0; j < ld; ++j) {
+                        uint32_t b = (uint32_t)rowl[j] << 16;
+                        float lf;
+                        std::memcpy(&lf, &b, 4);
+                        row[j] = (double)rowf[j] + (double)lf;
+                    }
+                }
+                double *o = out + (size_t)d * n;
+                for (int e = 0; e < n; ++e) o[e] = row[c->plan.perm[e]];
+            }
+        }
+    });
+}
+
+wb_status wb_compute_alpha(wb_ctx *c, double *alpha)
+{
+    return guarded(c, [&] {
+        require(c->assembled, WB_E_STATE, "wb_compute_alpha before wb_assemble");
+        require(alpha != nullptr, WB_E_ARG, "null alpha");
+        download_vec(c, c->alpha, alpha);
+    });
+}
+
+static void vmult_like(wb_ctx *c, int which, double *dst, const double *src)
+{
+    check_ready(c, true);
+    require(dst && src, WB_E_ARG, "null vector");
+    ensure_work(c);
+    upload_vec(c, c->vs, src);
+    const int mode = c->cfg.precision == WB_PREC_FP64 ? 0 : 2;
+    gemv(c, mode, which, c->vs.p, c->vs.p, c->vr.p, false);
+    if (which == 0 && c->pure_neumann) {
+        const double nrm = device_norm(c, c->vr.p, 0);
+        std::vector<double> v(c->vlen);
+        HIPCHK(hipMemcpy(v.data(), c->vr.p, sizeof(double) * c->vlen, hipMemcpyDeviceToHost));
+        for (int i = 0; i < c->n; ++i) v[i] -= nrm;
+        to_caller(c, v, dst);
+        return;
+    }
+    download_vec(c, c->vr, dst);
+}
+
+wb_status wb_vmult(wb_ctx *c, double *dst, const double *src)
+{
+    return guarded(c, [&] { vmult_like(c, 0, dst, src); });
+}
+
+wb_status wb_compute_rhs(wb_ctx *c, double *dst, const double *src)
+{
+    return guarded(c, [&] { vmult_like(c, 1, dst, src); });
+}
+
+wb_status wb_set_constraints(wb_ctx *c, int n_lines, const int32_t *rows, const int32_t *ptr,
+                             const int32_t *cols, const double *w, const double *inhom)
+{
+    return guarded(c, [&] {
+        require(c->have_mesh, WB_E_STATE, "wb_set_constraints before wb_set_mesh");
+        require(n_lines >= 0, WB_E_ARG, "negative n_lines");
+        ConstraintSet cs;
+        cs.ptr.push_back(0);
+        if (n_lines > 0) {
+            require(rows && ptr && inhom, WB_E_ARG, "null constraint arrays");
+            require(ptr[0] == 0, WB_E_ARG, "line_ptr[0] must be 0");
+            const int nnz = ptr[n_lines];
+            require(nnz >= 0 && (nnz == 0 || (cols && w)), WB_E_ARG, "bad constraint entries");
+            cs.rows.assign(rows, rows + n_lines);
+            cs.ptr.assign(ptr, ptr + n_lines + 1);
+            for (int l = 0; l < n_lines; ++l)
+                require(cs.ptr[l + 1] >= cs.ptr[l], WB_E_ARG, "line_ptr decreasing");
+            cs.cols.assign(cols, cols + nnz);
+            cs.w.assign(w, w + nnz);
+            cs.inhom.assign(inhom, inhom + n_lines);
+        }
+        c->cons_auto = false;
+        install_constraints(c, cs);
+    });
+}
+
+wb_status wb_compute_constraints(wb_ctx *c, const double *tmp_rhs, const double *normals,
+                                 const double *grads, int *n_lines)
+{
+    return guarded(c, [&] {
+        check_ready(c, false);
+        require(tmp_rhs != nullptr, WB_E_ARG, "null tmp_rhs");
+        c->cons_normals.clear();
+        c->cons_grads.clear();
+        if (normals) c->cons_normals.assign(normals, normals + 3 * (size_t)c->n);
+        if (grads) c->cons_grads.assign(grads, grads + 3 * (size_t)c->n);
+        c->cons_auto = true;
+        refresh_auto_constraints(c, tmp_rhs);
+        if (n_lines) *n_lines = c->n_lines;
+    });
+}
+
+wb_status wb_constrained_vmult(wb_ctx *c, double *dst, const double *src)
+{
+    return guarded(c, [&] {
+        check_ready(c, true);
+        require(dst && src, WB_E_ARG, "null vector");
+        ensure_work(c);
+        upload_vec(c, c->vs, src);
+        apply_constrained(c, c->cfg.precision == WB_PREC_FP64 ? 0 : 2, c->vs.p, c->vr.p);
+        download_vec(c, c->vr, dst);
+    });
+}
+
+wb_status wb_solve_system(wb_ctx *c, const double *tmp_rhs, double *phi, double *dphi_dn,
+                          wb_solve_stats *st)
+{
+    return guarded(c, [&] {
+        check_ready(c, true);
+        require(tmp_rhs && phi && dphi_dn, WB_E_ARG, "null vector");
+        refresh_auto_constraints(c, tmp_rhs);
+        ensure_work(c);
+        upload_vec(c, c->vs, tmp_rhs);
+        solve_system_dev(c, c->vs.as<double>(), st);
+        std::vector<double> sol(c->n);
+        download_vec(c, c->vx, sol.data());
+        for (int i = 0; i < c->n; ++i) {
+            if (c->surface[i] == 0)
+                phi[i] = sol[i];
+            else
+                dphi_dn[i] = sol[i];
+        }
+    });
+}
+
+wb_status wb_solve_system_dev(wb_ctx *c, const double *tmp_dev, double *phi_dev,
+                              double *dphi_dev, wb_solve_stats *st)
+{
+    return guarded(c, [&] {
+        check_ready(c, true);
+        require(tmp_dev && phi_dev && dphi_dev, WB_E_ARG, "null vector");
+        ensure_work(c);
+        // caller order -> internal order on the device
+        launch_permute(c->n, c->iperm_d.as<int>(), tmp_dev, c->vs.as<double>(), 1, c->stream);
+        if (c->cons_auto) {
+            std::vector<double> h(c->n);
+            HIPCHK(hipMemcpyAsync(h.data(), tmp_dev, sizeof(double) * c->n,
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            refresh_auto_constraints(c, h.data());
+        }
+        solve_system_dev(c, c->vs.as<double>(), st);
+        // scatter by mask: internal sol -> caller phi / dphi_dn
+        launch_permute(c->n, c->perm_d.as<int>(), c->vx.as<double>(), c->vt.as<double>(), 1,
+                       c->stream);  // vt[d] = sol[perm[d]] (caller order)
+        std::vector<double> sol(c->n);
+        HIPCHK(hipMemcpyAsync(sol.data(), c->vt.p, sizeof(double) * c->n, hipMemcpyDeviceToHost,
+                              c->stream));
+        std::vector<double> ph(c->n), dp(c->n);
+        HIPCHK(hipMemcpyAsync(ph.data(), phi_dev, sizeof(double) * c->n, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(dp.data(), dphi_dev, sizeof(double) * c->n, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < c->n; ++i) {
+            if (c->surface[i] == 0)
+                ph[i] = sol[i];
+            else
+                dp[i] = sol[i];
+        }
+        HIPCHK(hipMemcpyAsync(phi_dev, ph.data(), sizeof(double) * c->n, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(dphi_dev, dp.data(), sizeof(double) * c->n, hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+wb_status wb_solve(wb_ctx *c, const double *tmp_rhs, double *phi, double *dphi_dn,
+                   wb_solve_stats *st)
+{
+    wb_status s = wb_assemble(c);
+    if (s != WB_OK) return s;
+    return wb_solve_system(c, tmp_rhs, phi, dphi_dn, st);
+}
+
+wb_status wb_residual(wb_ctx *c, double *res, const double *phi, const double *dphi_dn)
+{
+    return guarded(c, [&] {
+        check_ready(c, true);
+        require(res && phi && dphi_dn, WB_E_ARG, "null vector");
+        const int n = c->n;
+        std::vector<double> tmp(n), sol(n);
+        for (int i = 0; i < n; ++i) {
+            tmp[i] = 0. + dphi_dn[i] * c->other[i] + phi[i] * c->surface[i];
+            sol[i] = 0. + dphi_dn[i] * c->surface[i] + phi[i] * c->other[i];
+        }
+        refresh_auto_constraints(c, tmp.data());
+        ensure_work(c);
+        upload_vec(c, c->vs, tmp.data());
+        build_rhs(c, c->vs.p);  // rrhs in vb
+        upload_vec(c, c->vs, sol.data());
+        apply_constrained(c, c->cfg.precision == WB_PREC_FP64 ? 0 : 2, c->vs.p, c->vr.p);
+        launch_vec_ops(0, n, c->vr.p, c->vb.p, c->vt.p, 0, c->stream);  // res = A sol - rrhs
+        download_vec(c, c->vt, res);
+    });
+}
+
+wb_status wb_solve_system_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi,
+                                double *dphi_dn, const double *inhom_k, wb_solve_stats *st)
+{
+    return guarded(c, [&] {
+        check_ready(c, true);
+        require(k >= 1 && tmp_rhs && phi && dphi_dn, WB_E_ARG, "bad multi-rhs arguments");
+        // one solve per right-hand side (the batched MFMA form is added separately)
+        wb_solve_stats acc{};
+        for (int j = 0; j < k; ++j) {
+            const size_t off = (size_t)j * c->n;
+            if (inhom_k && !c->cons_auto) {
+                ConstraintSet cs = c->cons;
+                for (size_t l = 0; l < cs.rows.size(); ++l) cs.inhom[l] = inhom_k[j * cs.rows.size() + l];
+                install_constraints(c, cs);
+            }
+            refresh_auto_constraints(c, tmp_rhs + off);
+            ensure_work(c);
+            upload_vec(c, c->vs, tmp_rhs + off);
+            wb_solve_stats s1{};
+            solve_system_dev(c, c->vs.as<double>(), &s1);
+            std::vector<double> sol(c->n);
+            download_vec(c, c->vx, sol.data());
+            for (int i = 0; i < c->n; ++i) {
+                if (c->surface[i] == 0)
+                    phi[off + i] = sol[i];
+                else
+                    dphi_dn[off + i] = sol[i];
+            }
+            acc.iterations += s1.iterations;
+            acc.restarts += s1.restarts;
+            acc.ir_sweeps += s1.ir_sweeps;
+            acc.residual = std::max(acc.residual, s1.residual);
+            acc.true_rel_residual = std::max(acc.true_rel_residual, s1.true_rel_residual);
+            acc.t_rhs_ms += s1.t_rhs_ms;
+            acc.t_precond_ms += s1.t_precond_ms;
+            acc.t_gmres_ms += s1.t_gmres_ms;
+        }
+        if (st) *st = acc;
+    });
+}
+
+size_t wb_device_bytes(const wb_ctx *c)
+{
+    if (!c) return 0;
+    size_t t = 0;
+    const DevBuf *bufs[] = {&c->X, &c->cell_int, &c->orient, &c->cell_cls, &c->row_cls,
+                            &c->tile_ptr, &c->tile_cell, &c->tile_cols, &c->sing_ptr,
+                            &c->sing_cell, &c->sing_vtx, &c->qpts, &c->qw, &c->srule, &c->cellq,
+                            &c->perm_d, &c->iperm_d, &c->G, &c->H, &c->Glo, &c->Hlo,
+                            &c->hsum_part, &c->hsum_sing, &c->alpha, &c->surf_d, &c->gmask,
+                            &c->diag_op, &c->diag_rhs, &c->con_rows, &c->con_ptr, &c->con_cols,
+                            &c->con_w, &c->con_inhom, &c->constrained, &c->free_rows, &c->binv,
+                            &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2, &c->Pn, &c->hbuf,
+                            &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32, &c->z32, &c->w32,
+                            &c->r32, &c->d32};
+    for (const DevBuf *b : bufs) t += b->bytes;
+    return t;
+}
+
+wb_status wb_synchronize(wb_ctx *c)
+{
+    return guarded(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// wb_host.cpp -- host-only C-ABI utilities (no device needed).
+//
+// They expose the library's own quadrature tables, double-node sets and constraint rules
+// so that a caller (or a test without a GPU) can inspect exactly what the device path uses.
+#include <algorithm>
+#include <cstring>
+
+#include "wb_internal.h"
+
+extern "C" {
+
+int wb_host_quadrature(int q_order, int s_order, double *reg_pts, double *reg_w,
+                       double *sing_pts, double *sing_w)
+{
+    if (q_order < 1 || q_order > wb::MAX_NQ1 || s_order < 1 || s_order > wb::MAX_SING)
+        return WB_E_ARG;
+    wb::Quadrature Q;
+    wb::build_quadrature(q_order, s_order, Q);
+    if (reg_pts) std::memcpy(reg_pts, Q.reg_pts.data(), sizeof(double) * Q.reg_pts.size());
+    if (reg_w) std::memcpy(reg_w, Q.reg_w.data(), sizeof(double) * Q.reg_w.size());
+    if (sing_pts) std::memcpy(sing_pts, Q.sing_pts.data(), sizeof(double) * Q.sing_pts.size());
+    if (sing_w) std::memcpy(sing_w, Q.sing_w.data(), sizeof(double) * Q.sing_w.size());
+    return WB_OK;
+}
+
+int wb_host_mesh_plan(int n, int nc, const double *support, const int32_t *cell_dofs,
+                      const int32_t *cell_material, int32_t *perm, int32_t *dn_ptr,
+                      int32_t *dn_idx, int dn_cap, int32_t *n_tiles, int32_t *n_tile_cells,
+                      int32_t *n_singular_pairs)
+{
+    if (n <= 0 || nc <= 0 || !support || !cell_dofs) return WB_E_ARG;
+    wb::MeshPlan P;
+    std::string e = wb::plan_mesh(n, nc, support, cell_dofs, cell_material, nullptr, nullptr, P);
+    if (!e.empty()) return WB_E_ARG;
+    if (perm) std::copy(P.perm.begin(), P.perm.end(), perm);
+    if (dn_ptr) std::copy(P.dn_ptr.begin(), P.dn_ptr.end(), dn_ptr);
+    if (dn_idx) {
+        if ((int)P.dn_idx.size() > dn_cap) return WB_E_ARG;
+        std::copy(P.dn_idx.begin(), P.dn_idx.end(), dn_idx);
+    }
+    if (n_tiles) *n_tiles = P.n_tiles;
+    if (n_tile_cells) *n_tile_cells = (int32_t)P.tile_cell.size();
+    if (n_singular_pairs) *n_singular_pairs = (int32_t)P.sing_cell.size();
+    return WB_OK;
+}
+
+int wb_host_compute_constraints(int n, const int32_t *dn_ptr, const int32_t *dn_idx,
+                                const double *surface, const double *normals,
+                                const double *grads, const double *tmp_rhs, int cap_lines,
+                                int cap_nnz, int32_t *line_rows, int32_t *line_ptr,
+                                int32_t *ent_col, double *ent_w, double *inhom)
+{
+    if (n <= 0 || !dn_ptr || !dn_idx || !surface || !tmp_rhs) return -WB_E_ARG;
+    wb::ConstraintSet cs;
+    std::string e =
+        wb::build_constraints(n, dn_ptr, dn_idx, surface, normals, grads, tmp_rhs, cs);
+    if (!e.empty()) return -WB_E_UNSUPPORTED;
+    const int nl = (int)cs.rows.size();
+    if (nl > cap_lines || (int)cs.cols.size() > cap_nnz) return -WB_E_ARG;
+    std::copy(cs.rows.begin(), cs.rows.end(), line_rows);
+    std::copy(cs.ptr.begin(), cs.ptr.end(), line_ptr);
+    std::copy(cs.cols.begin(), cs.cols.end(), ent_col);
+    std::copy(cs.w.begin(), cs.w.end(), ent_w);
+    std::copy(cs.inhom.begin(), cs.inhom.end(), inhom);
+    return nl;
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// wb_internal.h -- internal state and kernel launchers of libwavebem_bem.
+//
+// Device layout (all in the internal dof order chosen by wb_setup.cpp: dofs grouped by
+// cell material, then Morton order of the support point, so that consecutive internal
+// indices are spatial neighbours and patches with one boundary-condition type are
+// contiguous):
+//   G, H        n x ld row-major (ld = n rounded up to 64).  FP64 mode: double.
+//               FP32_IR mode: hi = float, lo = bf16 bits of (A - hi) (uint16).
+//   cellq       [n_cells][NQ][8] doubles: quadrature point y(3), n*JxW(3), JxW, pad.
+//   column tiles of TILE_COLS consecutive internal columns; for each tile the list of
+//               cells having a vertex dof in it, with the 4 tile-local columns packed
+//               as int8 (-1 = vertex dof outside the tile).
+//   singular pairs (row, cell, vertex) CSR by row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/wavebem_bem.h"
+
+namespace wb {
+
+constexpr int TILE_ROWS = 64;   // rows per assembly workgroup (one wavefront, lane = row)
+constexpr int TILE_COLS = 32;   // columns per assembly tile
+constexpr int TILES_PER_WG = 8; // column tiles swept by one workgroup
+constexpr int MAX_NQ1 = 6;      // supported regular QGauss orders 1..6
+constexpr int MAX_SING = 8;     // supported singular QGaussOneOverR orders 1..8
+constexpr double INV4PI = 0.07957747154594767;  // 1/(4 pi)
+
+struct PhiTable {
+    double phi[MAX_NQ1 * MAX_NQ1 * 4];  // phi[q*4+k], regular rule shape values
+};
+
+struct Quadrature {
+    int q_order = 0, s_order = 0;
+    std::vector<double> reg_pts, reg_w;    // nq x 2, nq
+    std::vector<double> sing_pts, sing_w;  // 4 x ns x 2, 4 x ns (vertex-major)
+};
+
+// Host quadrature tables (deal.II QGauss<2> and QGaussOneOverR<2> semantics).
+void build_quadrature(int q_order, int s_order, Quadrature &q);
+void gauss_legendre_01(int n, double *x, double *w);
+
+struct MeshPlan {
+    int n = 0, nc = 0;
+    size_t ld = 0;
+    std::vector<int> perm, iperm;          // dof -> internal, internal -> dof
+    std::vector<int> cls;                  // per internal index: double-node class id
+    std::vector<int> cell_cls;             // nc x 4 class ids of cell vertices
+    std::vector<int> cell_int;             // nc x 4 internal vertex indices
+    std::vector<int> tile_ptr, tile_cell, tile_cols;
+    std::vector<int> sing_ptr, sing_cell, sing_vtx;  // by internal row
+    int n_tiles = 0;
+    std::vector<int> dn_ptr, dn_idx;       // caller-numbered double node sets
+};
+
+std::string plan_mesh(int n, int nc, const double *support, const int32_t *cell_dofs,
+                      const int32_t *cell_material, const int32_t *dn_ptr, const int32_t *dn_idx,
+                      MeshPlan &plan);
+
+// Constraint rules of BEMProblem::compute_constraints, caller numbering, closed.
+struct ConstraintSet {
+    std::vector<int> rows, ptr, cols;
+    std::vector<double> w, inhom;
+};
+std::string build_constraints(int n, const int *dn_ptr, const int *dn_idx, const double *surface,
+                              const double *normals, const double *grads, const double *tmp_rhs,
+                              ConstraintSet &out);
+
+// ---------------------------------------------------------------- kernel launchers
+void launch_cell_geometry(const double *support_int, const int *cell_int, const int8_t *orient,
+                          int nc, int nq, const double *qpts, const double *qw, double *cellq,
+                          hipStream_t s);
+void launch_assemble_tiles(int precision, int q_order, const double *xrow, const int *row_cls,
+                           const double *cellq, const int *cell_cls, const int *tile_ptr,
+                           const int *tile_cell, const int *tile_cols, int n, int n_tiles,
+                           size_t ld, const PhiTable &phi, void *G, void *H, void *Glo,
+                           void *Hlo, double *hsum_part, hipStream_t s);
+void launch_assemble_singular(int precision, const double *support_int, const int *cell_int,
+                              const int8_t *orient, const int *sing_ptr, const int *sing_cell,
+                              const int *sing_vtx, int n, int ns, const double *sing_rule,
+                              size_t ld, void *G, void *H, void *Glo, void *Hlo,
+                              double *hsum_sing, hipStream_t s);
+void launch_alpha_reduce(const double *hsum_part, int n_parts, const double *hsum_sing, int n,
+                         double *alpha, hipStream_t s);
+
+// y[r] = sum_j M_sel(j)[r,j] * sign_sel(j) * x[j] + diag[r] * xd[r] for r in rows (or all)
+// sel bit per column pair in pairflag (0: G,G 1: H,H 2: G,H 3: H,G), flipped by `flip`.
+// mode: 0 = FP64 matrices, 1 = hi only (FP32 accumulate), 2 = hi+lo (FP64 accumulate)
+void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, const void *Hlo,
+                     size_t ld, int n, const uint8_t *pairflag, int flip, double s_sel,
+                     double s_unsel, const void *x, const double *diag, const void *xd,
+                     const int *rows, int n_rows, void *y, hipStream_t s);
+void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *Glo,
+                           const void *Hlo, size_t ld, int n, const uint8_t *pairflag, int flip,
+                           double s_sel, double s_unsel, int k, const void *X, size_t ldx,
+                           const double *diag, const void *Xd, const int *rows, int n_rows,
+                           void *Y, hipStream_t s);
+// constrained rows: y[row] = x[row] - sum w x[col]  (mode_inhom=0)  or y[row] = inhom (1)
+void launch_constraint_rows(int is_float, int n_lines, const int *rows, const int *ptr,
+                            const int *cols, const double *w, const double *inhom, int mode_inhom,
+                            const void *x, void *y, hipStream_t s);
+void launch_block_jacobi_build(int n, int bs, const void *G, const void *H, const void *Glo,
+                               const void *Hlo, int precision, size_t ld, const double *alpha,
+                               const double *surface,
+                               const uint8_t *constrained, double *binv, float *binv32,
+                               hipStream_t s);
+void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, const void *x,
+                               void *y, int k, size_t ldx, hipStream_t s);
+
+// Krylov helpers (is_float selects float vectors)
+int krylov_partials_blocks(int n);
+void launch_dot_partials(int is_float, const void *V, size_t ldv, int kcount, const void *z,
+                         int n, double *P, hipStream_t s);
+void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, const double *Pin,
+                       int nparts, void *z, int n, double *Pout, double *hacc, int want_norm,
+                       hipStream_t s);
+void launch_finalize(int is_float, int kcount, const double *P1, const double *P2,
+                     const double *Pn, int nparts, void *z, int n, void *vout, double *hout,
+                     hipStream_t s);
+void launch_axpy_basis(int is_float, const void *V, size_t ldv, int kcount, const double *y,
+                       void *x_acc, int x_is_float, int n, hipStream_t s);
+void launch_norm2_partials(int is_float, const void *x, int n, double *P, hipStream_t s);
+void launch_vec_ops(int op, int n, const void *a, const void *b, void *c, double alpha,
+                    hipStream_t s);
+void launch_permute(int n, const int *idx, const double *src, double *dst, int gather,
+                    hipStream_t s);
+
+}  // namespace wb

@@ -1,0 +1,431 @@
+// wb_setup.cpp -- host-side preparation for the device BEM path.
+//
+//  * quadrature tables with the semantics of deal.II QGauss<2> and
+//    QGaussOneOverR<2>(n, vertex, true) as requested by BEMProblem::assemble_system
+//    (source/bem_problem.cc:116-137);
+//  * mesh planning: double-node classes (source/computational_domain.cc:257-307),
+//    the device dof ordering, column tiles and the singular (row, cell) pairs of
+//    source/bem_problem.cc:214-230;
+//  * the constraint rules of BEMProblem::compute_constraints
+//    (source/bem_problem.cc:990-1105) closed like deal.II's ConstraintMatrix::close.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <unordered_map>
+
+#include "wb_internal.h"
+
+namespace wb {
+
+// Gauss-Legendre nodes on [0,1], ascending: Newton's method on the three-term Legendre
+// recurrence in long double, started from the Tricomi-type estimate
+// cos(pi (i - 1/4)/(n + 1/2)); weights 1/((1 - z^2) P_n'(z)^2) scaled to [0,1].  This is
+// the algorithm deal.II uses for QGauss<1>, so nodes agree to the last bit or two.
+void gauss_legendre_01(int n, double *x, double *w)
+{
+    const int m = (n + 1) / 2;
+    const long double tol = std::max((long double)__DBL_EPSILON__ / 100,
+                                     (long double)__LDBL_EPSILON__ * 5);
+    for (int i = 1; i <= m; ++i) {
+        long double z = std::cos((long double)M_PI * (i - 0.25L) / (n + 0.5L));
+        long double dp = 0, pn = 0;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1.0L, pm = 0.0L;
+            for (int j = 0; j < n; ++j) {
+                const long double pj = ((2.0L * j + 1.0L) * z * p0 - j * pm) / (j + 1);
+                pm = p0;
+                p0 = pj;
+            }
+            pn = p0;
+            dp = n * (z * pn - pm) / (z * z - 1.0L);
+            const long double dz = pn / dp;
+            z -= dz;
+            if (std::fabs(dz) <= tol) break;
+        }
+        const double half = (double)(0.5L * z);
+        const double wt = (double)(1.0L / ((1.0L - z * z) * dp * dp));
+        x[i - 1] = 0.5 - half;
+        x[n - i] = 0.5 + half;
+        w[i - 1] = wt;
+        w[n - i] = wt;
+    }
+}
+
+void build_quadrature(int q_order, int s_order, Quadrature &Q)
+{
+    Q.q_order = q_order;
+    Q.s_order = s_order;
+    double gx[64], gw[64];
+    gauss_legendre_01(q_order, gx, gw);
+    const int nq = q_order * q_order;
+    Q.reg_pts.resize(2 * nq);
+    Q.reg_w.resize(nq);
+    for (int j = 0; j < q_order; ++j)        // second coordinate slowest
+        for (int i = 0; i < q_order; ++i) {  // first coordinate fastest
+            const int q = i + q_order * j;
+            Q.reg_pts[2 * q] = gx[i];
+            Q.reg_pts[2 * q + 1] = gx[j];
+            Q.reg_w[q] = gw[i] * gw[j];
+        }
+    // singular rule for vertex 0: polar-type map of QGauss<2>(s) to the triangle under
+    // the diagonal, mirrored across it; weight carries |p| (1/R factored out).
+    gauss_legendre_01(s_order, gx, gw);
+    const int ng = s_order * s_order, ns = 2 * ng;
+    std::vector<double> p0(2 * ns), w0(ns);
+    const double pi4 = M_PI / 4;
+    for (int j = 0; j < s_order; ++j)
+        for (int i = 0; i < s_order; ++i) {
+            const int q = i + s_order * j;
+            const double u = gx[i], v = gx[j];
+            const double a = u, b = u * std::tan(pi4 * v);
+            double wq = gw[i] * gw[j] * pi4 / std::cos(pi4 * v);
+            wq *= std::sqrt(a * a + b * b);
+            p0[2 * q] = a;
+            p0[2 * q + 1] = b;
+            w0[q] = wq;
+            p0[2 * (ng + q)] = b;
+            p0[2 * (ng + q) + 1] = a;
+            w0[ng + q] = wq;
+        }
+    Q.sing_pts.resize(4 * 2 * ns);
+    Q.sing_w.resize(4 * ns);
+    const double theta[4] = {0.0, M_PI / 2, -M_PI / 2, M_PI};
+    for (int v = 0; v < 4; ++v) {
+        const double c = std::cos(theta[v]), s = std::sin(theta[v]);
+        for (int q = 0; q < ns; ++q) {
+            double x = p0[2 * q], y = p0[2 * q + 1];
+            if (v != 0) {  // rotate about (1/2, 1/2)
+                const double xx = x - 0.5, yy = y - 0.5;
+                x = c * xx - s * yy + 0.5;
+                y = s * xx + c * yy + 0.5;
+            }
+            Q.sing_pts[2 * (v * ns + q)] = x;
+            Q.sing_pts[2 * (v * ns + q) + 1] = y;
+            Q.sing_w[v * ns + q] = w0[q];
+        }
+    }
+}
+
+static uint64_t spread3(uint64_t v)
+{
+    v &= 0x1fffffULL;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+struct UnionFind {
+    std::vector<int> p;
+    explicit UnionFind(int n) : p(n) { std::iota(p.begin(), p.end(), 0); }
+    int find(int x)
+    {
+        while (p[x] != x) {
+            p[x] = p[p[x]];
+            x = p[x];
+        }
+        return x;
+    }
+    void unite(int a, int b)
+    {
+        a = find(a);
+        b = find(b);
+        if (a != b) p[std::max(a, b)] = std::min(a, b);
+    }
+};
+
+// Geometric coincidence (distance < 1e-8) via a hash grid of cell size 1e-8: coincident
+// points differ by at most one cell per axis.  Same sets as the reference's O(n^2) scan.
+static void double_nodes(int n, const double *X, std::vector<int> &ptr, std::vector<int> &idx)
+{
+    const double tol = 1e-8;
+    std::unordered_map<uint64_t, std::vector<int>> grid;
+    grid.reserve(2 * n);
+    auto key = [](int64_t a, int64_t b, int64_t c) {
+        return (uint64_t)(a * 73856093LL) ^ (uint64_t)(b * 19349663LL) ^ (uint64_t)(c * 83492791LL);
+    };
+    std::vector<int64_t> q(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d) q[3 * i + d] = (int64_t)std::floor(X[3 * i + d] / tol);
+        grid[key(q[3 * i], q[3 * i + 1], q[3 * i + 2])].push_back(i);
+    }
+    std::vector<std::vector<int>> sets(n);
+    for (int i = 0; i < n; ++i) {
+        std::vector<int> &s = sets[i];
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dz = -1; dz <= 1; ++dz) {
+                    auto it = grid.find(key(q[3 * i] + dx, q[3 * i + 1] + dy, q[3 * i + 2] + dz));
+                    if (it == grid.end()) continue;
+                    for (int j : it->second) {
+                        const double a = X[3 * i] - X[3 * j], b = X[3 * i + 1] - X[3 * j + 1],
+                                     c = X[3 * i + 2] - X[3 * j + 2];
+                        if (std::sqrt(a * a + b * b + c * c) < tol) s.push_back(j);
+                    }
+                }
+        std::sort(s.begin(), s.end());
+        s.erase(std::unique(s.begin(), s.end()), s.end());
+    }
+    ptr.assign(n + 1, 0);
+    idx.clear();
+    for (int i = 0; i < n; ++i) {
+        idx.insert(idx.end(), sets[i].begin(), sets[i].end());
+        ptr[i + 1] = (int)idx.size();
+    }
+}
+
+std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const int32_t *cmat,
+                      const int32_t *dnp, const int32_t *dni, MeshPlan &P)
+{
+    P = MeshPlan();
+    P.n = n;
+    P.nc = nc;
+    for (int c = 0; c < nc; ++c)
+        for (int k = 0; k < 4; ++k)
+            if (cd[4 * c + k] < 0 || cd[4 * c + k] >= n) return "cell dof index out of range";
+    for (int i = 0; i < 3 * n; ++i)
+        if (!std::isfinite(X[i])) return "non-finite support point";
+    if (dnp && dni) {
+        P.dn_ptr.assign(dnp, dnp + n + 1);
+        if (P.dn_ptr[0] != 0) return "double node CSR must start at 0";
+        for (int i = 0; i < n; ++i)
+            if (P.dn_ptr[i + 1] < P.dn_ptr[i]) return "double node CSR pointer decreasing";
+        P.dn_idx.assign(dni, dni + P.dn_ptr[n]);
+        for (int v : P.dn_idx)
+            if (v < 0 || v >= n) return "double node index out of range";
+    } else {
+        double_nodes(n, X, P.dn_ptr, P.dn_idx);
+    }
+    // classes = connected components; every set must be exactly its class
+    UnionFind uf(n);
+    for (int i = 0; i < n; ++i)
+        for (int p = P.dn_ptr[i]; p < P.dn_ptr[i + 1]; ++p) uf.unite(i, P.dn_idx[p]);
+    std::vector<int> csize(n, 0);
+    for (int i = 0; i < n; ++i) csize[uf.find(i)]++;
+    for (int i = 0; i < n; ++i) {
+        bool self = false;
+        for (int p = P.dn_ptr[i]; p < P.dn_ptr[i + 1]; ++p) self |= P.dn_idx[p] == i;
+        if (!self) return "double_nodes_set[i] must contain i";
+        if (P.dn_ptr[i + 1] - P.dn_ptr[i] != csize[uf.find(i)])
+            return "double node sets are not an equivalence relation";
+    }
+    // dof material: first cell containing the dof
+    std::vector<int> dmat(n, 0);
+    if (cmat) {
+        std::vector<char> seen(n, 0);
+        for (int c = 0; c < nc; ++c)
+            for (int k = 0; k < 4; ++k) {
+                const int d = cd[4 * c + k];
+                if (!seen[d]) {
+                    seen[d] = 1;
+                    dmat[d] = cmat[c];
+                }
+            }
+    }
+    double lo[3] = {X[0], X[1], X[2]}, hi[3] = {X[0], X[1], X[2]};
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], X[3 * i + d]);
+            hi[d] = std::max(hi[d], X[3 * i + d]);
+        }
+    double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (ext <= 0) ext = 1;
+    std::vector<uint64_t> key(n);
+    for (int i = 0; i < n; ++i) {
+        uint64_t m = 0;
+        for (int d = 0; d < 3; ++d) {
+            double t = (X[3 * i + d] - lo[d]) / ext;
+            t = std::min(std::max(t, 0.0), 1.0);
+            m |= spread3((uint64_t)(t * 2097151.0)) << d;
+        }
+        key[i] = m;
+    }
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (dmat[a] != dmat[b]) return dmat[a] < dmat[b];
+        return key[a] < key[b];
+    });
+    P.iperm = order;
+    P.perm.assign(n, 0);
+    for (int i = 0; i < n; ++i) P.perm[order[i]] = i;
+    P.ld = ((size_t)n + 63) / 64 * 64;
+    P.cls.resize(n);
+    for (int i = 0; i < n; ++i) P.cls[i] = P.perm[uf.find(P.iperm[i])];
+    P.cell_int.resize(4 * (size_t)nc);
+    P.cell_cls.resize(4 * (size_t)nc);
+    for (int c = 0; c < nc; ++c)
+        for (int k = 0; k < 4; ++k) {
+            const int d = cd[4 * c + k];
+            P.cell_int[4 * c + k] = P.perm[d];
+            P.cell_cls[4 * c + k] = P.perm[uf.find(d)];
+        }
+    // column tiles
+    P.n_tiles = (n + TILE_COLS - 1) / TILE_COLS;
+    std::vector<std::vector<int>> tl(P.n_tiles), tc(P.n_tiles);
+    for (int c = 0; c < nc; ++c) {
+        int tiles[4];
+        for (int k = 0; k < 4; ++k) tiles[k] = P.cell_int[4 * c + k] / TILE_COLS;
+        for (int k = 0; k < 4; ++k) {
+            bool dup = false;
+            for (int j = 0; j < k; ++j) dup |= tiles[j] == tiles[k];
+            if (dup) continue;
+            const int t = tiles[k];
+            uint32_t packed = 0;
+            for (int j = 0; j < 4; ++j) {
+                const int v = P.cell_int[4 * c + j];
+                const int local = (v / TILE_COLS == t) ? v - t * TILE_COLS : -1;
+                packed |= (uint32_t)(uint8_t)(int8_t)local << (8 * j);
+            }
+            tl[t].push_back(c);
+            tc[t].push_back((int)packed);
+        }
+    }
+    P.tile_ptr.assign(P.n_tiles + 1, 0);
+    for (int t = 0; t < P.n_tiles; ++t) {
+        P.tile_ptr[t + 1] = P.tile_ptr[t] + (int)tl[t].size();
+        P.tile_cell.insert(P.tile_cell.end(), tl[t].begin(), tl[t].end());
+        P.tile_cols.insert(P.tile_cols.end(), tc[t].begin(), tc[t].end());
+    }
+    // singular pairs: cells with a vertex in the row's class, first such vertex
+    std::vector<std::vector<int>> class_cells(n);
+    for (int c = 0; c < nc; ++c)
+        for (int k = 0; k < 4; ++k) {
+            auto &v = class_cells[P.cell_cls[4 * c + k]];
+            if (v.empty() || v.back() != c) v.push_back(c);
+        }
+    P.sing_ptr.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        auto &v = class_cells[P.cls[i]];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        for (int c : v) {
+            int jstar = -1;
+            for (int k = 0; k < 4; ++k)
+                if (P.cell_cls[4 * c + k] == P.cls[i]) {
+                    jstar = k;
+                    break;
+                }
+            P.sing_cell.push_back(c);
+            P.sing_vtx.push_back(jstar);
+        }
+        P.sing_ptr[i + 1] = (int)P.sing_cell.size();
+    }
+    return "";
+}
+
+// ------------------------------------------------------------------- constraints
+namespace {
+struct Line {
+    std::vector<std::pair<int, double>> e;
+    double inhom = 0;
+};
+}  // namespace
+
+std::string build_constraints(int n, const int *dnp, const int *dni, const double *surface,
+                              const double *normals, const double *grads, const double *tmp_rhs,
+                              ConstraintSet &out)
+{
+    std::map<int, Line> lines;
+    auto add_line = [&](int r) { lines.emplace(r, Line()); };  // existing line kept
+    auto add_entry = [&](int r, int c, double w) {
+        Line &L = lines[r];
+        for (auto &p : L.e)
+            if (p.first == c) return;
+        L.e.emplace_back(c, w);
+    };
+    for (int i = 0; i < n; ++i) {
+        const int p0 = dnp[i], p1 = dnp[i + 1];
+        int first = dni[p0];
+        for (int p = p0; p < p1; ++p)
+            if (surface[dni[p]] == 1) {
+                first = dni[p];
+                break;
+            }
+        if (i != first) continue;
+        if (surface[i] == 1) {
+            for (int p = p0; p < p1; ++p) {
+                const int k = dni[p];
+                if (k == i) continue;
+                if (surface[k] == 1) {
+                    if (!normals) return "Dirichlet-Dirichlet double node needs node normals";
+                    const double *ni = normals + 3 * i, *nk = normals + 3 * k;
+                    const double dx = nk[0] - ni[0], dy = nk[1] - ni[1], dz = nk[2] - ni[2];
+                    if (std::sqrt(dx * dx + dy * dy + dz * dz) < 1e-4) {
+                        add_line(k);
+                        add_entry(k, i, 1.0);
+                    } else {
+                        if (!grads) return "sharp Dirichlet-Dirichlet edge needs surface gradients";
+                        const double *gi = grads + 3 * i, *gk = grads + 3 * k;
+                        const double c = ni[0] * nk[0] + ni[1] * nk[1] + ni[2] * nk[2];
+                        const double gkni = gk[0] * ni[0] + gk[1] * ni[1] + gk[2] * ni[2];
+                        const double gink = gi[0] * nk[0] + gi[1] * nk[1] + gi[2] * nk[2];
+                        const double f = 1.0 / (1.0 - c * c);
+                        add_line(k);
+                        add_line(i);
+                        lines[i].inhom = f * (gkni + gink * c);
+                        lines[k].inhom = f * (gink + gkni * c);
+                    }
+                } else {
+                    add_line(k);
+                    lines[k].inhom = tmp_rhs[i];
+                }
+            }
+        } else {
+            for (int p = p0; p < p1; ++p) {
+                const int k = dni[p];
+                if (k == i) continue;
+                add_line(k);
+                add_entry(k, i, 1.0);
+            }
+        }
+    }
+    // close: resolve entries on constrained dofs (chains), merge, sort
+    for (int pass = 0;; ++pass) {
+        if (pass > 64) return "constraint chain does not terminate";
+        bool changed = false;
+        for (auto &kv : lines) {
+            Line &L = kv.second;
+            std::vector<std::pair<int, double>> ne;
+            for (auto &p : L.e) {
+                auto it = lines.find(p.first);
+                if (it == lines.end()) {
+                    ne.push_back(p);
+                    continue;
+                }
+                if (it->first == kv.first) return "self-referencing constraint";
+                changed = true;
+                L.inhom += p.second * it->second.inhom;
+                for (auto &q : it->second.e) ne.emplace_back(q.first, p.second * q.second);
+            }
+            std::sort(ne.begin(), ne.end());
+            std::vector<std::pair<int, double>> merged;
+            for (auto &p : ne) {
+                if (!merged.empty() && merged.back().first == p.first)
+                    merged.back().second += p.second;
+                else
+                    merged.push_back(p);
+            }
+            L.e.swap(merged);
+        }
+        if (!changed) break;
+    }
+    out = ConstraintSet();
+    out.ptr.push_back(0);
+    for (auto &kv : lines) {
+        out.rows.push_back(kv.first);
+        out.inhom.push_back(kv.second.inhom);
+        for (auto &p : kv.second.e) {
+            out.cols.push_back(p.first);
+            out.w.push_back(p.second);
+        }
+        out.ptr.push_back((int)out.cols.size());
+    }
+    return "";
+}
+
+}  // namespace wb
