@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""BEM step time on one MI355X: FP32 + FP64 iterative refinement vs plain FP64.
+
+One step = the hot path of BEMProblem::solve (source/bem_problem.cc:969-987) on the
+synthetic closed Wigley half-box: assemble_system (dense G, H and the solid angles) +
+solve_system (rhs, constraints, preconditioner, GMRES, scatter) with the boundary data
+already resident in HBM.  Both precisions run back to back on the same mesh and boundary
+data; the FP32-IR solution is compared with the FP64 one.
+
+Prints ONE JSON line (rank 0).  value = FP32-IR milliseconds per step.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP64_VALU_PEAK_TFLOPS = 78.6
+ASM_FLOPS_PER_ROW_QP = 36  # algorithmic flops per (collocation row, quadrature point)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_precision(mesh, bc, precision, steps, warmup, tol, ir_inner=1e-6, tile=16):
+    import torch
+
+    from wavebem_amd.bem import BEMProblem, Config
+
+    n = mesh.n_dofs
+    cfg = Config(precision=precision, solver_tolerance=tol, solver_max_steps=5000,
+                 ir_inner_reduction=ir_inner, assembly_tile=tile)
+    p = BEMProblem(mesh, cfg)
+    p.compute_constraints(bc)
+    bc_d = torch.tensor(bc, dtype=torch.float64, device="cuda")
+    phi_d = torch.tensor(bc * mesh.surface_nodes, dtype=torch.float64, device="cuda")
+    dphi_d = torch.tensor(bc * mesh.other_nodes, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for w in range(warmup):
+        p.assemble_system()
+        st = p.solve_system_dev(phi_d.data_ptr(), dphi_d.data_ptr(), bc_d.data_ptr())
+        log(f"[{precision}] warmup {w}: {st['iterations']} it, sweeps {st['ir_sweeps']}, "
+            f"true rel res {st['true_rel_residual']:.2e}")
+    torch.cuda.synchronize()
+    p.set_profiling(True)
+    t_asm = 0.0
+    stats = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ta = time.perf_counter()
+        p.assemble_system()
+        t_asm += time.perf_counter() - ta
+        stats.append(p.solve_system_dev(phi_d.data_ptr(), dphi_d.data_ptr(), bc_d.data_ptr()))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ks = p.kernel_stats()
+    p.set_profiling(False)
+    ms = (t1 - t0) * 1e3 / steps
+    log(f"[{precision}] {ms:.1f} ms/step (assembly {t_asm * 1e3 / steps:.1f} ms), "
+        f"iterations {stats[-1]['iterations']}, sweeps {stats[-1]['ir_sweeps']}")
+    out = dict(ms_per_step=ms, assembly_ms=t_asm * 1e3 / steps, stats=stats[-1], kernels=ks,
+               phi=phi_d.cpu().numpy(), dphi=dphi_d.cpu().numpy(), device_bytes=p.device_bytes())
+    p.close()
+    del bc_d, phi_d, dphi_d
+    torch.cuda.empty_cache()
+    return out
+
+
+def kernel_avg_ms(ks, name):
+    k = ks[name]
+    return k["total_ms"] / k["launches"] if k["launches"] else float("nan")
+
+
+def cpu_baseline(mesh, iters64, threads, rows_sample, gemv_n):
+    """Oracle (CPU restatement of the reference dense path) on a bounded sample."""
+    import oracle as O
+    from wavebem_amd.mesh import double_nodes_csr
+
+    n = mesh.n_dofs
+    ptr, idx = double_nodes_csr(mesh.support_points)
+    rows = min(rows_sample, n)
+    r0 = (n - rows) // 2
+    t0 = time.perf_counter()
+    O.assemble_rows(mesh.support_points, mesh.cell_dofs, mesh.cell_orient, ptr, idx, r0,
+                    r0 + rows, n_threads=threads)
+    t_rows = time.perf_counter() - t0
+    t_asm = t_rows * n / rows
+    gn = min(gemv_n, n)
+    t_gemv = O.gemv_time(gn, reps=3, n_threads=threads) * (n / gn) ** 2
+    # the reference's GMRES operator reads both G and H (two FullMatrix::vmult per step)
+    t_step = t_asm + iters64 * 2 * t_gemv
+    return dict(value=t_step * 1e3, unit="ms", cores=threads, kind="port",
+                sample=(f"oracle assemble_system rows [{r0},{r0 + rows}) of {n} ({t_rows:.1f} s, "
+                        f"scaled by n/rows) + FullMatrix::vmult timed at n={gn} "
+                        f"(scaled by (n/{gn})^2) x 2 per GMRES step x {iters64} steps "
+                        f"(the FP64 GPU iteration count); preconditioner not counted"),
+                assembly_ms=t_asm * 1e3, gemv_ms=t_gemv * 1e3)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="40k", help="Wigley preset: 2.5k 10k 20k 40k 60k")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-rows", type=int, default=2048)
+    ap.add_argument("--cpu-gemv-n", type=int, default=8192)
+    ap.add_argument("--only", choices=["fp64", "fp32_ir"], default=None)
+    ap.add_argument("--ir-inner", type=float, default=1e-6,
+                    help="FP32-IR inner GMRES relative residual reduction per sweep")
+    ap.add_argument("--tile", type=int, default=16, choices=[16, 32])
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(row-sharded design described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    from wavebem_amd import mesh as M
+
+    t = time.perf_counter()
+    mesh = M.wigley_preset(args.workload)
+    bc, U = M.wigley_bc(mesh, froude=0.3, seed=0)
+    log(f"mesh {args.workload}: n={mesh.n_dofs} cells={mesh.n_cells} "
+        f"({time.perf_counter() - t:.1f} s)")
+
+    res = {}
+    for prec in ("fp64", "fp32_ir"):
+        if args.only and prec != args.only:
+            continue
+        res[prec] = run_precision(mesh, bc, prec, args.steps, args.warmup, args.tol,
+                                  args.ir_inner, args.tile)
+
+    n = mesh.n_dofs
+    N = mesh.other_nodes > 0
+    S = ~N
+    head = res.get("fp32_ir") or res["fp64"]
+    line = {
+        "metric": "BEM step time (assembly+GMRES) at 40k dofs: FP32-IR vs FP64, at 1e-8 rel err",
+        "value": round(head["ms_per_step"], 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64-ir" if "fp32_ir" in res else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"wigley_closed_halfbox_{args.workload}", "n_dofs": n,
+                   "n_cells": mesh.n_cells, "quadrature": "QGauss 4x4 + QGaussOneOverR(5)",
+                   "gmres_tol_abs_preconditioned": args.tol, "gmres_restart": 98,
+                   "preconditioner": "block_jacobi_64", "froude": 0.3},
+    }
+    if "fp64" in res:
+        line["fp64_ms_per_step"] = round(res["fp64"]["ms_per_step"], 3)
+        line["fp64_iterations"] = res["fp64"]["stats"]["iterations"]
+        line["fp64_true_rel_residual"] = res["fp64"]["stats"]["true_rel_residual"]
+    if "fp32_ir" in res:
+        st = res["fp32_ir"]["stats"]
+        line["fp32_ir_inner_iterations"] = st["iterations"]
+        line["fp32_ir_sweeps"] = st["ir_sweeps"]
+        line["fp32_ir_true_rel_residual"] = st["true_rel_residual"]
+    if "fp64" in res and "fp32_ir" in res:
+        a, b = res["fp32_ir"], res["fp64"]
+        line["speedup_vs_fp64"] = round(b["ms_per_step"] / a["ms_per_step"], 3)
+        line["phi_rel_err_vs_fp64"] = float(np.abs(a["phi"][N] - b["phi"][N]).max()
+                                            / np.abs(b["phi"][N]).max())
+        line["dphidn_rel_err_vs_fp64"] = float(np.abs(a["dphi"][S] - b["dphi"][S]).max()
+                                               / np.abs(b["dphi"][S]).max())
+    line["phase_ms"] = {k: {"assembly": round(v["assembly_ms"], 3),
+                            "solve": round(v["ms_per_step"] - v["assembly_ms"], 3)}
+                        for k, v in res.items()}
+
+    # roofline of the dominant kernel of the headline path (column-selected GEMV)
+    ks = head["kernels"]
+    gname = "gemv32" if "fp32_ir" in res else "gemv64"
+    esz = 4 if gname == "gemv32" else 8
+    g = ks[gname]
+    avg_ms = kernel_avg_ms(ks, gname)
+    rows_per_launch = g["rows"] / max(g["launches"], 1)
+    bytes_per_launch = rows_per_launch * n * esz + 2 * n * esz
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    line["roofline"] = {"kernel": f"k_gemv_sel ({gname})", "bound": "hbm",
+                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "avg_launch_ms": round(avg_ms, 4), "launches": g["launches"],
+                        "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    kern = {}
+    for prec, r in res.items():
+        for name, k in r["kernels"].items():
+            if k["launches"]:
+                kern[f"{prec}:{name}"] = {"launches": k["launches"],
+                                          "avg_ms": round(k["total_ms"] / k["launches"], 4)}
+        a = r["kernels"]["assemble"]
+        if a["launches"]:
+            t_a = a["total_ms"] / a["launches"] * 1e-3
+            flops = ASM_FLOPS_PER_ROW_QP * n * 16 * mesh.n_cells
+            kern[f"{prec}:assemble"]["tflops_algorithmic"] = round(flops / t_a / 1e12, 2)
+            kern[f"{prec}:assemble"]["valu_frac_fp64_peak"] = round(
+                flops / t_a / 1e12 / FP64_VALU_PEAK_TFLOPS, 4)
+        for gk, es in (("gemv64", 8), ("gemvhl", 6), ("gemv32", 4)):
+            k = r["kernels"][gk]
+            if k["launches"]:
+                rpl = k["rows"] / k["launches"]
+                kern[f"{prec}:{gk}"]["GBps_algorithmic"] = round(
+                    rpl * n * es / (k["total_ms"] / k["launches"] * 1e-3) / 1e9, 1)
+    line["kernels"] = kern
+
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        iters = res["fp64"]["stats"]["iterations"] if "fp64" in res else 100
+        t = time.perf_counter()
+        line["cpu_baseline"] = cpu_baseline(mesh, iters, threads, args.cpu_rows, args.cpu_gemv_n)
+        log(f"cpu baseline {time.perf_counter() - t:.1f} s")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

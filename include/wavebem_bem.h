@@ -61,6 +61,7 @@ typedef struct {
     int precond_block;             /* block size for block Jacobi (<= 128) */
     int ir_max_sweeps;             /* FP32_IR: outer refinement sweeps */
     double ir_inner_reduction;     /* FP32_IR: inner relative residual reduction */
+    int assembly_tile;             /* columns per register tile of the assembly (16|32) */
 } wb_config;
 
 typedef struct {
@@ -166,6 +167,27 @@ wb_status wb_residual(wb_ctx *ctx, double *res, const double *phi, const double 
 wb_status wb_solve_system_multi(wb_ctx *ctx, int k, const double *tmp_rhs, double *phi,
                                 double *dphi_dn, const double *inhom_k,
                                 wb_solve_stats *stats);
+
+/* ---- kernel timing (HIP events on the context stream around each hot launch) ------- */
+typedef enum {
+    WB_K_ASSEMBLE = 0,  /* regular-cell G/H tile kernel */
+    WB_K_SINGULAR = 1,  /* singular-cell kernel */
+    WB_K_GEMV64 = 2,    /* column-selected GEMV, FP64 matrices */
+    WB_K_GEMV32 = 3,    /* column-selected GEMV, FP32 (hi) matrices, inner solver */
+    WB_K_GEMVHL = 4,    /* column-selected GEMV, FP32 hi + bf16 lo, FP64 accumulation */
+    WB_K_COUNT = 5
+} wb_kernel_kind;
+
+typedef struct {
+    int launches[WB_K_COUNT];
+    double total_ms[WB_K_COUNT];
+    long long rows[WB_K_COUNT]; /* GEMV: matrix rows read, summed over launches */
+} wb_kernel_stats;
+
+/* enable != 0 starts (and resets) event timing of the hot kernels; 0 stops it. */
+wb_status wb_set_profiling(wb_ctx *ctx, int enable);
+/* Synchronizes, folds all recorded events into *out (accumulated since enabling). */
+wb_status wb_get_kernel_stats(wb_ctx *ctx, wb_kernel_stats *out);
 
 /* ---- host-only utilities (no device needed) ---------------------------------------- */
 

@@ -110,6 +110,36 @@ def assemble(support, cell_dofs, cell_orient, dn_ptr, dn_idx, q_order=4, sing_or
     return H, G
 
 
+def assemble_rows(support, cell_dofs, cell_orient, dn_ptr, dn_idx, row0, row1, q_order=4,
+                  sing_order=5, n_threads=0):
+    """Rows [row0, row1) of (H, G)."""
+    support = _f64(support)
+    cd = _i32(cell_dofs)
+    orient = np.ascontiguousarray(cell_orient, dtype=np.int8)
+    dp = _i32(dn_ptr)
+    di = _i32(dn_idx)
+    n = support.shape[0]
+    nc = cd.shape[0]
+    H = np.zeros((row1 - row0, n))
+    G = np.zeros((row1 - row0, n))
+    rc = lib().orc_assemble_rows(n, nc, _i(cd), orient.ctypes.data_as(_bp), _d(support), _i(dp),
+                                 _i(di), q_order, sing_order, int(row0), int(row1), _d(H), _d(G),
+                                 int(n_threads))
+    if rc != 0:
+        raise RuntimeError(f"orc_assemble_rows failed: {rc}")
+    return H, G
+
+
+def gemv_time(n, reps=3, n_threads=0):
+    """Seconds per dense FP64 row-major mat-vec of size n (FullMatrix::vmult loop)."""
+    import time
+    M = np.random.default_rng(0).standard_normal((n, n))
+    x = np.ones(n)
+    y = np.zeros(n)
+    lib().orc_gemv_timed.restype = ctypes.c_double
+    return lib().orc_gemv_timed(n, _d(M), _d(x), _d(y), int(reps), int(n_threads))
+
+
 def compute_alpha(H):
     H = _f64(H)
     n = H.shape[0]

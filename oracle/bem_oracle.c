@@ -206,8 +206,17 @@ int orc_assemble(int n, int nc, const int *cell_dofs, const signed char *orient,
                  const double *support, const int *dn_ptr, const int *dn_idx, int q_order,
                  int sing_order, double *H, double *G, int n_threads)
 {
+    return orc_assemble_rows(n, nc, cell_dofs, orient, support, dn_ptr, dn_idx, q_order,
+                             sing_order, 0, n, H, G, n_threads);
+}
+
+/* Rows [row0, row1) of assemble_system into H, G of shape (row1-row0) x n. */
+int orc_assemble_rows(int n, int nc, const int *cell_dofs, const signed char *orient,
+                      const double *support, const int *dn_ptr, const int *dn_idx, int q_order,
+                      int sing_order, int row0, int row1, double *H, double *G, int n_threads)
+{
     if (n <= 0 || nc <= 0 || q_order <= 0 || q_order > 16 || sing_order <= 0 ||
-        sing_order > 16)
+        sing_order > 16 || row0 < 0 || row1 > n || row1 < row0)
         return -1;
     const int nq = q_order * q_order;
     const int ns = 2 * sing_order * sing_order;
@@ -237,16 +246,16 @@ int orc_assemble(int n, int nc, const int *cell_dofs, const signed char *orient,
                           s + 6 * ns, s + 7 * ns);
         }
     }
-    memset(H, 0, sizeof(double) * (size_t)n * n);
-    memset(G, 0, sizeof(double) * (size_t)n * n);
+    memset(H, 0, sizeof(double) * (size_t)(row1 - row0) * n);
+    memset(G, 0, sizeof(double) * (size_t)(row1 - row0) * n);
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
 #pragma omp parallel for schedule(dynamic, 16)
 #endif
-    for (int i = 0; i < n; ++i) {
+    for (int i = row0; i < row1; ++i) {
         const double *xi = support + 3 * i;
-        double *Hi = H + (size_t)i * n;
-        double *Gi = G + (size_t)i * n;
+        double *Hi = H + (size_t)(i - row0) * n;
+        double *Gi = G + (size_t)(i - row0) * n;
         for (int c = 0; c < nc; ++c) {
             const int *dofs = cell_dofs + 4 * c;
             double lH[4] = {0, 0, 0, 0}, lG[4] = {0, 0, 0, 0};
@@ -302,6 +311,32 @@ static void fm_vmult(int n, const double *M, const double *src, double *dst, int
         for (int j = 0; j < n; ++j) s += src[j] * row[j];
         dst[i] = s;
     }
+}
+
+/* Seconds per FullMatrix::vmult of an n x n matrix, rows split over n_threads (the
+ * per-row arithmetic is that of fm_vmult); used only to time the CPU baseline. */
+double orc_gemv_timed(int n, const double *M, const double *x, double *y, int reps,
+                      int n_threads)
+{
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+    double t0 = omp_get_wtime();
+#pragma omp parallel
+    for (int r = 0; r < reps; ++r) {
+#pragma omp for schedule(static)
+        for (int i = 0; i < n; ++i) {
+            const double *row = M + (size_t)i * n;
+            double s = 0.;
+            for (int j = 0; j < n; ++j) s += x[j] * row[j];
+            y[i] = s;
+        }
+    }
+    return (omp_get_wtime() - t0) / (reps > 0 ? reps : 1);
+#else
+    (void)n_threads;
+    for (int r = 0; r < reps; ++r) fm_vmult(n, M, x, y, 0);
+    return -1.0;
+#endif
 }
 
 /* compute_alpha (source/bem_problem.cc:594-618): alpha = H * (-1) */

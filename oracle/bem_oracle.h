@@ -40,7 +40,15 @@ int orc_assemble(int n_dofs, int n_cells, const int *cell_dofs, const signed cha
                  const double *support, const int *dn_ptr, const int *dn_idx,
                  int q_order, int sing_order, double *H, double *G, int n_threads);
 
+/* rows [row0,row1) only (H, G are (row1-row0) x n); same arithmetic per entry */
+int orc_assemble_rows(int n_dofs, int n_cells, const int *cell_dofs,
+                      const signed char *cell_orient, const double *support, const int *dn_ptr,
+                      const int *dn_idx, int q_order, int sing_order, int row0, int row1,
+                      double *H, double *G, int n_threads);
+
 void orc_compute_alpha(int n, const double *H, double *alpha);
+double orc_gemv_timed(int n, const double *M, const double *x, double *y, int reps,
+                      int n_threads);
 void orc_vmult(int n, const double *G, const double *H, const double *alpha,
                const double *surface, const double *other, const double *src, double *dst);
 void orc_compute_rhs(int n, const double *G, const double *H, const double *alpha,

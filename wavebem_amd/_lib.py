@@ -9,7 +9,7 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libwavebem_bem.so")
+LIB_PATH = os.environ.get("WAVEBEM_LIB") or os.path.join(HERE, "libwavebem_bem.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "wavebem_bem.h")
 
 WB_OK, WB_E_ARG, WB_E_OOM, WB_E_NOCONV, WB_E_HIP, WB_E_STATE, WB_E_UNSUPPORTED = range(7)
@@ -33,6 +33,7 @@ class WbConfig(ctypes.Structure):
         ("precond_block", ctypes.c_int),
         ("ir_max_sweeps", ctypes.c_int),
         ("ir_inner_reduction", ctypes.c_double),
+        ("assembly_tile", ctypes.c_int),
     ]
 
 
@@ -51,6 +52,22 @@ class WbSolveStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+WB_K_NAMES = ["assemble", "singular", "gemv64", "gemv32", "gemvhl"]
+
+
+class WbKernelStats(ctypes.Structure):
+    _fields_ = [
+        ("launches", ctypes.c_int * 5),
+        ("total_ms", ctypes.c_double * 5),
+        ("rows", ctypes.c_longlong * 5),
+    ]
+
+    def as_dict(self):
+        return {name: dict(launches=self.launches[k], total_ms=self.total_ms[k],
+                           rows=self.rows[k])
+                for k, name in enumerate(WB_K_NAMES)}
 
 
 _lib = None
@@ -89,6 +106,8 @@ _SIGS = {
     "wb_host_compute_constraints": (ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _dp, _dp, _dp,
                                                    _dp, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
                                                    _i32p, _dp, _dp]),
+    "wb_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "wb_get_kernel_stats": (ctypes.c_int, [_vp, ctypes.POINTER(WbKernelStats)]),
     "wb_device_bytes": (ctypes.c_size_t, [_vp]),
     "wb_synchronize": (ctypes.c_int, [_vp]),
 }

@@ -58,7 +58,8 @@ class Config:
     preconditioner: str = "block_jacobi"  # "block_jacobi" | "none"
     precond_block: int = 64
     ir_max_sweeps: int = 10
-    ir_inner_reduction: float = 1e-5
+    ir_inner_reduction: float = 1e-6
+    assembly_tile: int = 16
 
     def to_c(self) -> L.WbConfig:
         c = L.WbConfig()
@@ -75,6 +76,7 @@ class Config:
         c.precond_block = self.precond_block
         c.ir_max_sweeps = self.ir_max_sweeps
         c.ir_inner_reduction = self.ir_inner_reduction
+        c.assembly_tile = self.assembly_tile
         return c
 
 
@@ -247,6 +249,25 @@ class BEMProblem:
         self.last_stats = st.as_dict()
         self._check(status, "wb_solve_system_multi")
         return self.last_stats
+
+    def solve_system_dev(self, phi_ptr, dphi_dn_ptr, tmp_rhs_ptr):
+        """Device-pointer form (caller dof order, n float64 each)."""
+        st = L.WbSolveStats()
+        status = L.lib().wb_solve_system_dev(self._ctx, ctypes.c_void_p(tmp_rhs_ptr),
+                                             ctypes.c_void_p(phi_ptr),
+                                             ctypes.c_void_p(dphi_dn_ptr), ctypes.byref(st))
+        self.last_stats = st.as_dict()
+        self._check(status, "wb_solve_system_dev")
+        return self.last_stats
+
+    def set_profiling(self, enable=True):
+        self._check(L.lib().wb_set_profiling(self._ctx, int(enable)), "wb_set_profiling")
+
+    def kernel_stats(self):
+        ks = L.WbKernelStats()
+        self._check(L.lib().wb_get_kernel_stats(self._ctx, ctypes.byref(ks)),
+                    "wb_get_kernel_stats")
+        return ks.as_dict()
 
     def device_bytes(self):
         return int(L.lib().wb_device_bytes(self._ctx))

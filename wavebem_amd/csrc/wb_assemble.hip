@@ -99,30 +99,34 @@ void launch_cell_geometry(const double *X, const int *cell_int, const int8_t *or
 }
 
 // ------------------------------------------------------------------ regular cells
-constexpr int LDS_STRIDE = TILE_ROWS + 1;  // [column][row], padded against bank conflicts
-
-__device__ __forceinline__ void lds_add(double *p, double v)
-{
-    // fire-and-forget ds_add_f64; each address is owned by one lane of one wavefront
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
 // Q = points per direction of the tensor rule (NQ = Q*Q, point q = i + Q*j, i fastest).
 // The Q1 shape values factor as phi_k(q) = a_k(xi_i) b_k(eta_j), so the four local sums
 // are formed per rule row j from S0 = sum_i (1-xi_i) t_ij and S1 = sum_i xi_i t_ij:
 // only the Q Gauss nodes are needed (kept in SGPRs).
-template <int Q, int OUT>
-__global__ __launch_bounds__(TILE_ROWS) void k_assemble_tiles(
+//
+// The TC tile columns of each lane's row live in registers (a TC-wide vector per matrix);
+// a cell's four tile-local columns are wave-uniform, so the compiler updates them with
+// s_set_gpr_idx-indexed register moves, no LDS and no atomics.
+#ifndef WB_ASM_WAVES
+#define WB_ASM_WAVES 3  // requested waves per SIMD for the regular-cell kernel
+#endif
+template <int Q, int TC, int OUT>
+__global__ __launch_bounds__(TILE_ROWS) __attribute__((amdgpu_waves_per_eu(WB_ASM_WAVES)))
+void k_assemble_tiles(
     const double *__restrict__ xrow, const int *__restrict__ row_cls,
     const double *__restrict__ cellq, const int4 *__restrict__ cell_cls,
     const int *__restrict__ tile_ptr, const int *__restrict__ tile_cell,
-    const int *__restrict__ tile_cols, int n, int n_tiles, size_t ld, PhiTable nodes,
+    const int *__restrict__ tile_cols, const int *__restrict__ tile_start, int n, int n_tiles,
+    size_t ld, PhiTable nodes,
     void *__restrict__ Gv, void *__restrict__ Hv, uint16_t *__restrict__ Glo,
     uint16_t *__restrict__ Hlo, double *__restrict__ hsum_part)
 {
     constexpr int NQ = Q * Q;
-    __shared__ double accG[TILE_COLS * LDS_STRIDE];
-    __shared__ double accH[TILE_COLS * LDS_STRIDE];
+    static_assert(TC == 16 || TC == 32, "tile width");
+    // 16-wide register vectors (wider ones are lowered to scratch by the compiler)
+    typedef double accv __attribute__((ext_vector_type(16)));
+    constexpr int STAGE_LD = TC + 1;
+    __shared__ double stage[TILE_ROWS * STAGE_LD];
     const int lane = threadIdx.x;
     const int rb = blockIdx.x;
     const int tg = blockIdx.y;
@@ -134,11 +138,17 @@ __global__ __launch_bounds__(TILE_ROWS) void k_assemble_tiles(
     const int t_begin = tg * TILES_PER_WG;
     const int t_end = min(n_tiles, t_begin + TILES_PER_WG);
     for (int tile = t_begin; tile < t_end; ++tile) {
-#pragma unroll
-        for (int c = 0; c < TILE_COLS; ++c) {
-            accG[c * LDS_STRIDE + lane] = 0.0;
-            accH[c * LDS_STRIDE + lane] = 0.0;
-        }
+        accv aG0 = 0.0, aH0 = 0.0, aG1 = 0.0, aH1 = 0.0;
+        // c is wave-uniform: the branch is scalar and the vector index an s_set_gpr_idx
+        auto upd = [&](int c, double g, double h) {
+            if (TC == 16 || c < 16) {
+                aG0[c & 15] += g;
+                aH0[c & 15] += h;
+            } else {
+                aG1[c & 15] += g;
+                aH1[c & 15] += h;
+            }
+        };
         double hs = 0.0;
         const int p0 = tile_ptr[tile], p1 = tile_ptr[tile + 1];
         for (int p = p0; p < p1; ++p) {
@@ -176,95 +186,119 @@ __global__ __launch_bounds__(TILE_ROWS) void k_assemble_tiles(
                 h3 = __builtin_fma(eta, sh1, h3);
             }
             const bool sing = (rc == cc.x) | (rc == cc.y) | (rc == cc.z) | (rc == cc.w);
-            if (!sing) {
-                const int c0 = (int)(int8_t)(cols & 0xff);
-                const int c1 = (int)(int8_t)((cols >> 8) & 0xff);
-                const int c2 = (int)(int8_t)((cols >> 16) & 0xff);
-                const int c3 = (int)(int8_t)((cols >> 24) & 0xff);
-                if (c0 >= 0) {
-                    lds_add(&accG[c0 * LDS_STRIDE + lane], g0);
-                    lds_add(&accH[c0 * LDS_STRIDE + lane], h0);
-                    hs += h0;
-                }
-                if (c1 >= 0) {
-                    lds_add(&accG[c1 * LDS_STRIDE + lane], g1);
-                    lds_add(&accH[c1 * LDS_STRIDE + lane], h1);
-                    hs += h1;
-                }
-                if (c2 >= 0) {
-                    lds_add(&accG[c2 * LDS_STRIDE + lane], g2);
-                    lds_add(&accH[c2 * LDS_STRIDE + lane], h2);
-                    hs += h2;
-                }
-                if (c3 >= 0) {
-                    lds_add(&accG[c3 * LDS_STRIDE + lane], g3);
-                    lds_add(&accH[c3 * LDS_STRIDE + lane], h3);
-                    hs += h3;
+            if (sing) {
+                g0 = g1 = g2 = g3 = 0.0;
+                h0 = h1 = h2 = h3 = 0.0;
+            }
+            const int c0 = (int)(int8_t)(cols & 0xff);
+            const int c1 = (int)(int8_t)((cols >> 8) & 0xff);
+            const int c2 = (int)(int8_t)((cols >> 16) & 0xff);
+            const int c3 = (int)(int8_t)((cols >> 24) & 0xff);
+            if (c0 >= 0) {
+                upd(c0, g0, h0);
+                hs += h0;
+            }
+            if (c1 >= 0) {
+                upd(c1, g1, h1);
+                hs += h1;
+            }
+            if (c2 >= 0) {
+                upd(c2, g2, h2);
+                hs += h2;
+            }
+            if (c3 >= 0) {
+                upd(c3, g3, h3);
+                hs += h3;
+            }
+        }
+        // Write-out through LDS: each lane parks its row, then lanes are remapped to
+        // (row, column) so every store instruction writes whole contiguous row segments.
+        const int c_begin = tile_start[tile];
+        const int ncols = min(TC, tile_start[tile + 1] - c_begin);
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const double scale = m == 0 ? INV4PI : -INV4PI;
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < TC; ++c) {
+                const double v = m == 0 ? (c < 16 ? aG0[c & 15] : aG1[c & 15])
+                                        : (c < 16 ? aH0[c & 15] : aH1[c & 15]);
+                stage[lane * STAGE_LD + c] = v * scale;
+            }
+            __syncthreads();
+            constexpr int RPI = TILE_ROWS / TC;  // rows per store instruction
+            const int cl = lane % TC, rsub = lane / TC;
+#pragma unroll
+            for (int r0 = 0; r0 < TILE_ROWS; r0 += RPI) {
+                const int rl = r0 + rsub;
+                const int grow = rb * TILE_ROWS + rl;
+                if (grow < n && cl < ncols) {
+                    const double v = stage[rl * STAGE_LD + cl];
+                    const size_t off = (size_t)grow * ld + c_begin + cl;
+                    if (OUT == 0) {
+                        ((double *)(m == 0 ? Gv : Hv))[off] = v;
+                    } else {
+                        float hi;
+                        uint16_t lo;
+                        split_hilo(v, hi, lo);
+                        ((float *)(m == 0 ? Gv : Hv))[off] = hi;
+                        (m == 0 ? Glo : Hlo)[off] = lo;
+                    }
                 }
             }
         }
-        __syncthreads();
-        constexpr int RPI = TILE_ROWS / TILE_COLS;  // rows written per iteration
-        const int cl = lane % TILE_COLS;
-        const int gcol = tile * TILE_COLS + cl;
-#pragma unroll 4
-        for (int r0 = 0; r0 < TILE_ROWS; r0 += RPI) {
-            const int rl = r0 + lane / TILE_COLS;
-            const int grow = rb * TILE_ROWS + rl;
-            if (grow < n && gcol < n) {
-                const double gv = accG[cl * LDS_STRIDE + rl] * INV4PI;
-                const double hv = accH[cl * LDS_STRIDE + rl] * (-INV4PI);
-                const size_t off = (size_t)grow * ld + gcol;
-                if (OUT == 0) {
-                    __builtin_nontemporal_store(gv, (double *)Gv + off);
-                    __builtin_nontemporal_store(hv, (double *)Hv + off);
-                } else {
-                    float gh, hh;
-                    uint16_t gl, hl;
-                    split_hilo(gv, gh, gl);
-                    split_hilo(hv, hh, hl);
-                    __builtin_nontemporal_store(gh, (float *)Gv + off);
-                    __builtin_nontemporal_store(hh, (float *)Hv + off);
-                    __builtin_nontemporal_store(gl, Glo + off);
-                    __builtin_nontemporal_store(hl, Hlo + off);
-                }
-            }
-        }
-        __syncthreads();
         hs_total += hs;
     }
     if (row < n) hsum_part[(size_t)tg * n + row] = hs_total * (-INV4PI);
 }
 
-template <int Q>
-static void launch_tiles_q(int precision, const double *xrow, const int *row_cls,
-                           const double *cellq, const int *cell_cls, const int *tile_ptr,
-                           const int *tile_cell, const int *tile_cols, int n, int n_tiles,
-                           size_t ld, const PhiTable &nodes, void *G, void *H, void *Glo,
-                           void *Hlo, double *hsum_part, hipStream_t s)
+template <int Q, int TC>
+static void launch_tiles_qt(int precision, const double *xrow, const int *row_cls,
+                            const double *cellq, const int *cell_cls, const int *tile_ptr,
+                            const int *tile_cell, const int *tile_cols, const int *tile_start, int n, int n_tiles,
+                            size_t ld, const PhiTable &nodes, void *G, void *H, void *Glo,
+                            void *Hlo, double *hsum_part, hipStream_t s)
 {
     dim3 grid((n + TILE_ROWS - 1) / TILE_ROWS, (n_tiles + TILES_PER_WG - 1) / TILES_PER_WG);
     if (precision == WB_PREC_FP64)
-        hipLaunchKernelGGL((k_assemble_tiles<Q, 0>), grid, dim3(TILE_ROWS), 0, s, xrow, row_cls,
-                           cellq, (const int4 *)cell_cls, tile_ptr, tile_cell, tile_cols, n,
-                           n_tiles, ld, nodes, G, H, (uint16_t *)Glo, (uint16_t *)Hlo, hsum_part);
+        hipLaunchKernelGGL((k_assemble_tiles<Q, TC, 0>), grid, dim3(TILE_ROWS), 0, s, xrow,
+                           row_cls, cellq, (const int4 *)cell_cls, tile_ptr, tile_cell, tile_cols,
+                           tile_start, n, n_tiles, ld, nodes, G, H, (uint16_t *)Glo, (uint16_t *)Hlo,
+                           hsum_part);
     else
-        hipLaunchKernelGGL((k_assemble_tiles<Q, 1>), grid, dim3(TILE_ROWS), 0, s, xrow, row_cls,
-                           cellq, (const int4 *)cell_cls, tile_ptr, tile_cell, tile_cols, n,
-                           n_tiles, ld, nodes, G, H, (uint16_t *)Glo, (uint16_t *)Hlo, hsum_part);
+        hipLaunchKernelGGL((k_assemble_tiles<Q, TC, 1>), grid, dim3(TILE_ROWS), 0, s, xrow,
+                           row_cls, cellq, (const int4 *)cell_cls, tile_ptr, tile_cell, tile_cols,
+                           tile_start, n, n_tiles, ld, nodes, G, H, (uint16_t *)Glo, (uint16_t *)Hlo,
+                           hsum_part);
+}
+
+template <int Q>
+static void launch_tiles_q(int precision, int tc, const double *xrow, const int *row_cls,
+                           const double *cellq, const int *cell_cls, const int *tile_ptr,
+                           const int *tile_cell, const int *tile_cols, const int *tile_start, int n, int n_tiles,
+                           size_t ld, const PhiTable &nodes, void *G, void *H, void *Glo,
+                           void *Hlo, double *hsum_part, hipStream_t s)
+{
+    if (tc == 16)
+        launch_tiles_qt<Q, 16>(precision, xrow, row_cls, cellq, cell_cls, tile_ptr, tile_cell,
+                               tile_cols, tile_start, n, n_tiles, ld, nodes, G, H, Glo, Hlo, hsum_part, s);
+    else
+        launch_tiles_qt<Q, 32>(precision, xrow, row_cls, cellq, cell_cls, tile_ptr, tile_cell,
+                               tile_cols, tile_start, n, n_tiles, ld, nodes, G, H, Glo, Hlo, hsum_part, s);
 }
 
 // nodes.phi[0..q_order-1] = 1-D Gauss nodes on [0,1] of the regular rule
-void launch_assemble_tiles(int precision, int q_order, const double *xrow, const int *row_cls,
+void launch_assemble_tiles(int precision, int q_order, int tc, const double *xrow,
+                           const int *row_cls,
                            const double *cellq, const int *cell_cls, const int *tile_ptr,
-                           const int *tile_cell, const int *tile_cols, int n, int n_tiles,
+                           const int *tile_cell, const int *tile_cols, const int *tile_start, int n, int n_tiles,
                            size_t ld, const PhiTable &nodes, void *G, void *H, void *Glo,
                            void *Hlo, double *hsum_part, hipStream_t s)
 {
 #define WB_TILES(QV)                                                                           \
     case QV:                                                                                   \
-        launch_tiles_q<QV>(precision, xrow, row_cls, cellq, cell_cls, tile_ptr, tile_cell,     \
-                           tile_cols, n, n_tiles, ld, nodes, G, H, Glo, Hlo, hsum_part, s);    \
+        launch_tiles_q<QV>(precision, tc, xrow, row_cls, cellq, cell_cls, tile_ptr, tile_cell, \
+                           tile_cols, tile_start, n, n_tiles, ld, nodes, G, H, Glo, Hlo, hsum_part, s);    \
         break;
     switch (q_order) {
         WB_TILES(1)

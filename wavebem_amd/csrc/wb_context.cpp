@@ -81,6 +81,7 @@ struct wb_ctx {
     int pc_bs = 0;
 
     // device: mesh
+    DevBuf tile_start;
     DevBuf X, cell_int, orient, cell_cls, row_cls, tile_ptr, tile_cell, tile_cols, sing_ptr,
         sing_cell, sing_vtx, qpts, qw, srule, cellq, perm_d, iperm_d;
     // device: operator
@@ -94,7 +95,60 @@ struct wb_ctx {
     DevBuf V32, z32, w32, r32, d32;
     double *hpin = nullptr;  // pinned host copy of hbuf
     size_t vlen = 0;         // padded vector length
+
+    // kernel timing
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    struct Pending {
+        int kind;
+        size_t ev;
+        long long rows;
+    };
+    std::vector<Pending> pending;
+    wb_kernel_stats kstats{};
 };
+
+// Brackets one launch with a pair of events when profiling is on.
+struct KernelScope {
+    wb_ctx *c;
+    int kind;
+    long long rows;
+    size_t ev = 0;
+    KernelScope(wb_ctx *c_, int kind_, long long rows_ = 0) : c(c_), kind(kind_), rows(rows_)
+    {
+        if (!c->profiling) return;
+        while (c->ev_pool.size() < c->ev_used + 2) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c->ev_pool.push_back(e);
+        }
+        ev = c->ev_used;
+        c->ev_used += 2;
+        HIPCHK(hipEventRecord(c->ev_pool[ev], c->stream));
+    }
+    ~KernelScope()
+    {
+        if (!c->profiling) return;
+        if (hipEventRecord(c->ev_pool[ev + 1], c->stream) == hipSuccess)
+            c->pending.push_back({kind, ev, rows});
+    }
+};
+
+static void fold_kernel_stats(wb_ctx *c)
+{
+    if (c->pending.empty()) return;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (const auto &p : c->pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[p.ev], c->ev_pool[p.ev + 1]));
+        c->kstats.launches[p.kind] += 1;
+        c->kstats.total_ms[p.kind] += ms;
+        c->kstats.rows[p.kind] += p.rows;
+    }
+    c->pending.clear();
+    c->ev_used = 0;
+}
 
 static void set_err(wb_ctx *c, const std::string &m)
 {
@@ -205,6 +259,7 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
     const double s_sel = which == 0 ? 1.0 : -1.0, s_uns = which == 0 ? -1.0 : 1.0;
     const int *rows = free_rows_only ? c->free_rows.as<int>() : nullptr;
     const int nr = free_rows_only ? c->n_free : c->n;
+    KernelScope ks(c, mode == 0 ? WB_K_GEMV64 : (mode == 1 ? WB_K_GEMV32 : WB_K_GEMVHL), nr);
     launch_gemv_sel(mode, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld, c->n, c->gmask.as<uint8_t>(),
                     which, s_sel, s_uns, x, diag, xd, rows, nr, y, c->stream);
 }
@@ -512,12 +567,17 @@ static void run_solver(wb_ctx *c, wb_solve_stats *st)
     double t2 = now_ms();
     s.t_precond_ms = t1 - t0;
     s.t_gmres_ms = t2 - t1;
-    // true relative residual of the constrained system, FP64 operator
+    // true relative residual of the constrained system with the FP64-accurate operator
+    // (the refinement loop already holds it in vr)
     {
-        const int mode = c->cfg.precision == WB_PREC_FP64 ? 0 : 2;
-        apply_constrained(c, mode, c->vx.p, c->w.p);
-        launch_vec_ops(8, n, c->w.p, c->vb.p, c->vt.p, 0, c->stream);
-        const double rn = device_norm(c, c->vt.p, 0);
+        double rn;
+        if (c->cfg.precision == WB_PREC_FP64) {
+            apply_constrained(c, 0, c->vx.p, c->w.p);
+            launch_vec_ops(8, n, c->w.p, c->vb.p, c->vt.p, 0, c->stream);
+            rn = device_norm(c, c->vt.p, 0);
+        } else {
+            rn = device_norm(c, c->vr.p, 0);
+        }
         const double bn = device_norm(c, c->vb.p, 0);
         s.true_rel_residual = bn > 0 ? rn / bn : rn;
     }
@@ -584,7 +644,8 @@ void wb_config_default(wb_config *cfg)
     cfg->preconditioner = WB_PRECOND_BLOCK_JACOBI;
     cfg->precond_block = 64;
     cfg->ir_max_sweeps = 10;
-    cfg->ir_inner_reduction = 1e-5;
+    cfg->ir_inner_reduction = 1e-6;
+    cfg->assembly_tile = 16;
 }
 
 const char *wb_version(void) { return "wavebem_amd 0.1 (gfx950)"; }
@@ -605,6 +666,7 @@ wb_status wb_create(wb_ctx **out, const wb_config *cfg)
         k.gmres_max_tmp_vectors < 3 || k.gmres_max_tmp_vectors > 129 ||
         (k.precision != WB_PREC_FP64 && k.precision != WB_PREC_FP32_IR) ||
         k.precond_block < 1 || k.precond_block > 128 || k.solver_max_steps < 0 ||
+        (k.assembly_tile != 16 && k.assembly_tile != 32) ||
         !(k.solver_tolerance >= 0)) {
         delete c;
         return WB_E_ARG;
@@ -629,7 +691,8 @@ void wb_destroy(wb_ctx *c)
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->X, &c->cell_int, &c->orient, &c->cell_cls, &c->row_cls, &c->tile_ptr,
-                      &c->tile_cell, &c->tile_cols, &c->sing_ptr, &c->sing_cell, &c->sing_vtx,
+                      &c->tile_cell, &c->tile_cols, &c->tile_start, &c->sing_ptr, &c->sing_cell,
+                      &c->sing_vtx,
                       &c->qpts, &c->qw, &c->srule, &c->cellq, &c->perm_d, &c->iperm_d, &c->G,
                       &c->H, &c->Glo, &c->Hlo, &c->hsum_part, &c->hsum_sing, &c->alpha,
                       &c->surf_d, &c->gmask, &c->diag_op, &c->diag_rhs, &c->con_rows,
@@ -638,6 +701,7 @@ void wb_destroy(wb_ctx *c)
                       &c->Pn, &c->hbuf, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
                       &c->z32, &c->w32, &c->r32, &c->d32};
     for (DevBuf *b : bufs) b->release();
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->hpin) (void)hipHostFree(c->hpin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -659,7 +723,8 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
                     "wb_set_mesh: cell_orient must be +-1");
         HIPCHK(hipSetDevice(c->cfg.device));
         MeshPlan P;
-        std::string e = plan_mesh(n, nc, support, cell_dofs, cell_material, dn_ptr, dn_idx, P);
+        std::string e = plan_mesh(n, nc, support, cell_dofs, cell_material, dn_ptr, dn_idx,
+                                  c->cfg.assembly_tile, P);
         require(e.empty(), WB_E_ARG, e.c_str());
         c->have_mesh = c->have_masks = c->assembled = c->have_cons = false;
         c->plan = std::move(P);
@@ -679,6 +744,7 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         upload(c->tile_ptr, p.tile_ptr.data(), p.tile_ptr.size(), s);
         upload(c->tile_cell, p.tile_cell.data(), p.tile_cell.size(), s);
         upload(c->tile_cols, p.tile_cols.data(), p.tile_cols.size(), s);
+        upload(c->tile_start, p.tile_start.data(), p.tile_start.size(), s);
         upload(c->sing_ptr, p.sing_ptr.data(), p.sing_ptr.size(), s);
         upload(c->sing_cell, p.sing_cell.data(), p.sing_cell.size(), s);
         upload(c->sing_vtx, p.sing_vtx.data(), p.sing_vtx.size(), s);
@@ -787,20 +853,27 @@ wb_status wb_assemble(wb_ctx *c)
         const MeshPlan &p = c->plan;
         PhiTable nodes{};
         for (int i = 0; i < c->quad.q_order; ++i) nodes.phi[i] = c->quad.reg_pts[2 * i];
-        launch_assemble_tiles(c->cfg.precision, c->quad.q_order, c->X.as<double>(),
-                              c->row_cls.as<int>(),
-                              c->cellq.as<double>(), c->cell_cls.as<int>(), c->tile_ptr.as<int>(),
-                              c->tile_cell.as<int>(), c->tile_cols.as<int>(), c->n, p.n_tiles,
-                              c->ld, nodes, c->G.p, c->H.p, c->Glo.p, c->Hlo.p,
-                              c->hsum_part.as<double>(), c->stream);
-        HIPCHK(hipGetLastError());
+        {
+            KernelScope ks(c, WB_K_ASSEMBLE, c->n);
+            launch_assemble_tiles(c->cfg.precision, c->quad.q_order, p.tc, c->X.as<double>(),
+                                  c->row_cls.as<int>(), c->cellq.as<double>(),
+                                  c->cell_cls.as<int>(), c->tile_ptr.as<int>(),
+                                  c->tile_cell.as<int>(), c->tile_cols.as<int>(),
+                                  c->tile_start.as<int>(), c->n,
+                                  p.n_tiles, c->ld, nodes, c->G.p, c->H.p, c->Glo.p, c->Hlo.p,
+                                  c->hsum_part.as<double>(), c->stream);
+            HIPCHK(hipGetLastError());
+        }
         const int ns = 2 * c->quad.s_order * c->quad.s_order;
-        launch_assemble_singular(c->cfg.precision, c->X.as<double>(), c->cell_int.as<int>(),
-                                 c->orient.as<int8_t>(), c->sing_ptr.as<int>(),
-                                 c->sing_cell.as<int>(), c->sing_vtx.as<int>(), c->n, ns,
-                                 c->srule.as<double>(), c->ld, c->G.p, c->H.p, c->Glo.p, c->Hlo.p,
-                                 c->hsum_sing.as<double>(), c->stream);
-        HIPCHK(hipGetLastError());
+        {
+            KernelScope ks(c, WB_K_SINGULAR, c->n);
+            launch_assemble_singular(c->cfg.precision, c->X.as<double>(), c->cell_int.as<int>(),
+                                     c->orient.as<int8_t>(), c->sing_ptr.as<int>(),
+                                     c->sing_cell.as<int>(), c->sing_vtx.as<int>(), c->n, ns,
+                                     c->srule.as<double>(), c->ld, c->G.p, c->H.p, c->Glo.p,
+                                     c->Hlo.p, c->hsum_sing.as<double>(), c->stream);
+            HIPCHK(hipGetLastError());
+        }
         const int ntg = (p.n_tiles + TILES_PER_WG - 1) / TILES_PER_WG;
         launch_alpha_reduce(c->hsum_part.as<double>(), ntg, c->hsum_sing.as<double>(), c->n,
                             c->alpha.as<double>(), c->stream);
@@ -1082,7 +1155,8 @@ size_t wb_device_bytes(const wb_ctx *c)
     if (!c) return 0;
     size_t t = 0;
     const DevBuf *bufs[] = {&c->X, &c->cell_int, &c->orient, &c->cell_cls, &c->row_cls,
-                            &c->tile_ptr, &c->tile_cell, &c->tile_cols, &c->sing_ptr,
+                            &c->tile_ptr, &c->tile_cell, &c->tile_cols, &c->tile_start,
+                            &c->sing_ptr,
                             &c->sing_cell, &c->sing_vtx, &c->qpts, &c->qw, &c->srule, &c->cellq,
                             &c->perm_d, &c->iperm_d, &c->G, &c->H, &c->Glo, &c->Hlo,
                             &c->hsum_part, &c->hsum_sing, &c->alpha, &c->surf_d, &c->gmask,
@@ -1098,6 +1172,24 @@ size_t wb_device_bytes(const wb_ctx *c)
 wb_status wb_synchronize(wb_ctx *c)
 {
     return guarded(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+}
+
+wb_status wb_set_profiling(wb_ctx *c, int enable)
+{
+    return guarded(c, [&] {
+        fold_kernel_stats(c);
+        c->kstats = wb_kernel_stats{};
+        c->profiling = enable != 0;
+    });
+}
+
+wb_status wb_get_kernel_stats(wb_ctx *c, wb_kernel_stats *out)
+{
+    return guarded(c, [&] {
+        require(out != nullptr, WB_E_ARG, "null stats");
+        fold_kernel_stats(c);
+        *out = c->kstats;
+    });
 }
 
 }  // extern "C"

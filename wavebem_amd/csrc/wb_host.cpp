@@ -30,7 +30,7 @@ int wb_host_mesh_plan(int n, int nc, const double *support, const int32_t *cell_
 {
     if (n <= 0 || nc <= 0 || !support || !cell_dofs) return WB_E_ARG;
     wb::MeshPlan P;
-    std::string e = wb::plan_mesh(n, nc, support, cell_dofs, cell_material, nullptr, nullptr, P);
+    std::string e = wb::plan_mesh(n, nc, support, cell_dofs, cell_material, nullptr, nullptr, wb::TILE_COLS, P);
     if (!e.empty()) return WB_E_ARG;
     if (perm) std::copy(P.perm.begin(), P.perm.end(), perm);
     if (dn_ptr) std::copy(P.dn_ptr.begin(), P.dn_ptr.end(), dn_ptr);

@@ -24,7 +24,7 @@
 namespace wb {
 
 constexpr int TILE_ROWS = 64;   // rows per assembly workgroup (one wavefront, lane = row)
-constexpr int TILE_COLS = 32;   // columns per assembly tile
+constexpr int TILE_COLS = 16;   // default columns per assembly tile (16 or 32)
 constexpr int TILES_PER_WG = 8; // column tiles swept by one workgroup
 constexpr int MAX_NQ1 = 6;      // supported regular QGauss orders 1..6
 constexpr int MAX_SING = 8;     // supported singular QGaussOneOverR orders 1..8
@@ -51,15 +51,16 @@ struct MeshPlan {
     std::vector<int> cls;                  // per internal index: double-node class id
     std::vector<int> cell_cls;             // nc x 4 class ids of cell vertices
     std::vector<int> cell_int;             // nc x 4 internal vertex indices
-    std::vector<int> tile_ptr, tile_cell, tile_cols;
+    std::vector<int> tile_ptr, tile_cell, tile_cols, tile_start;
     std::vector<int> sing_ptr, sing_cell, sing_vtx;  // by internal row
     int n_tiles = 0;
+    int tc = TILE_COLS;                    // columns per tile
     std::vector<int> dn_ptr, dn_idx;       // caller-numbered double node sets
 };
 
 std::string plan_mesh(int n, int nc, const double *support, const int32_t *cell_dofs,
                       const int32_t *cell_material, const int32_t *dn_ptr, const int32_t *dn_idx,
-                      MeshPlan &plan);
+                      int tile_cols, MeshPlan &plan);
 
 // Constraint rules of BEMProblem::compute_constraints, caller numbering, closed.
 struct ConstraintSet {
@@ -74,9 +75,11 @@ std::string build_constraints(int n, const int *dn_ptr, const int *dn_idx, const
 void launch_cell_geometry(const double *support_int, const int *cell_int, const int8_t *orient,
                           int nc, int nq, const double *qpts, const double *qw, double *cellq,
                           hipStream_t s);
-void launch_assemble_tiles(int precision, int q_order, const double *xrow, const int *row_cls,
+void launch_assemble_tiles(int precision, int q_order, int tc, const double *xrow,
+                           const int *row_cls,
                            const double *cellq, const int *cell_cls, const int *tile_ptr,
-                           const int *tile_cell, const int *tile_cols, int n, int n_tiles,
+                           const int *tile_cell, const int *tile_cols, const int *tile_start,
+                           int n, int n_tiles,
                            size_t ld, const PhiTable &phi, void *G, void *H, void *Glo,
                            void *Hlo, double *hsum_part, hipStream_t s);
 void launch_assemble_singular(int precision, const double *support_int, const int *cell_int,

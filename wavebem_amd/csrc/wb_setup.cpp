@@ -179,9 +179,11 @@ static void double_nodes(int n, const double *X, std::vector<int> &ptr, std::vec
 }
 
 std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const int32_t *cmat,
-                      const int32_t *dnp, const int32_t *dni, MeshPlan &P)
+                      const int32_t *dnp, const int32_t *dni, int tc, MeshPlan &P)
 {
     P = MeshPlan();
+    if (tc != 16 && tc != 32) return "tile width must be 16 or 32";
+    P.tc = tc;
     P.n = n;
     P.nc = nc;
     for (int c = 0; c < nc; ++c)
@@ -244,12 +246,116 @@ std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const i
         }
         key[i] = m;
     }
-    std::vector<int> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    std::vector<int> seeds(n);
+    std::iota(seeds.begin(), seeds.end(), 0);
+    std::stable_sort(seeds.begin(), seeds.end(), [&](int a, int b) {
         if (dmat[a] != dmat[b]) return dmat[a] < dmat[b];
         return key[a] < key[b];
     });
+    // Column tiles by greedy region growing on the mesh graph: a tile starts next to the
+    // previous one (or at the first free dof in material/Morton order) and repeatedly
+    // takes the neighbouring dof that brings the fewest new cells into the tile.  Cells
+    // touching a tile are evaluated once per tile, so compact tiles mean less redundant
+    // kernel work.  The device order is the concatenation of the tiles.
+    std::vector<int> dc_ptr(n + 1, 0), dc_idx(4 * (size_t)nc);
+    for (int c = 0; c < nc; ++c)
+        for (int k = 0; k < 4; ++k) dc_ptr[cd[4 * c + k] + 1]++;
+    for (int i = 0; i < n; ++i) dc_ptr[i + 1] += dc_ptr[i];
+    {
+        std::vector<int> fill(dc_ptr.begin(), dc_ptr.end() - 1);
+        for (int c = 0; c < nc; ++c)
+            for (int k = 0; k < 4; ++k) dc_idx[fill[cd[4 * c + k]]++] = c;
+    }
+    std::vector<char> assigned(n, 0);
+    std::vector<int> stamp(nc, -1);
+    std::vector<int> order;
+    order.reserve(n);
+    std::vector<int> tstart{0};
+    std::vector<int> prev, tile, front;
+    size_t seed_ptr = 0;
+    auto free_degree = [&](int v) {
+        int d = 0;
+        for (int p = dc_ptr[v]; p < dc_ptr[v + 1]; ++p)
+            for (int k = 0; k < 4; ++k) d += !assigned[cd[4 * dc_idx[p] + k]];
+        return d;
+    };
+    auto push_front = [&](int v) {
+        for (int p = dc_ptr[v]; p < dc_ptr[v + 1]; ++p)
+            for (int k = 0; k < 4; ++k) {
+                const int w = cd[4 * dc_idx[p] + k];
+                if (!assigned[w] && std::find(front.begin(), front.end(), w) == front.end())
+                    front.push_back(w);
+            }
+    };
+    while ((int)order.size() < n) {
+        const int tid = (int)tstart.size() - 1;
+        int seed = -1;
+        long best = 0;
+        for (int t : prev)
+            for (int p = dc_ptr[t]; p < dc_ptr[t + 1]; ++p)
+                for (int k = 0; k < 4; ++k) {
+                    const int w = cd[4 * dc_idx[p] + k];
+                    if (assigned[w]) continue;
+                    const long s = (long)free_degree(w);
+                    if (seed < 0 || s < best || (s == best && key[w] < key[seed])) {
+                        seed = w;
+                        best = s;
+                    }
+                }
+        if (seed < 0) {
+            while (assigned[seeds[seed_ptr]]) ++seed_ptr;
+            int looked = 0;
+            for (size_t q = seed_ptr; q < seeds.size() && looked < 32; ++q) {
+                const int w = seeds[q];
+                if (assigned[w]) continue;
+                if (dmat[w] != dmat[seeds[seed_ptr]]) break;
+                ++looked;
+                const long s = (long)free_degree(w);
+                if (seed < 0 || s < best) {
+                    seed = w;
+                    best = s;
+                }
+            }
+        }
+        tile.assign(1, seed);
+        assigned[seed] = 1;
+        for (int p = dc_ptr[seed]; p < dc_ptr[seed + 1]; ++p) stamp[dc_idx[p]] = tid;
+        front.clear();
+        push_front(seed);
+        while ((int)tile.size() < tc && !front.empty()) {
+            int bi = -1, bnew = 0, bin = 0;
+            for (int f = 0; f < (int)front.size(); ++f) {
+                const int v = front[f];
+                int nw = 0, in = 0;
+                for (int p = dc_ptr[v]; p < dc_ptr[v + 1]; ++p) {
+                    const int c = dc_idx[p];
+                    if (stamp[c] != tid) ++nw;
+                    for (int k = 0; k < 4; ++k) {
+                        const int w = cd[4 * c + k];
+                        in += (w != v && assigned[w] &&
+                               std::find(tile.begin(), tile.end(), w) != tile.end());
+                    }
+                }
+                if (bi < 0 || nw < bnew || (nw == bnew && in > bin) ||
+                    (nw == bnew && in == bin && key[v] < key[front[bi]])) {
+                    bi = f;
+                    bnew = nw;
+                    bin = in;
+                }
+            }
+            const int v = front[bi];
+            front.erase(front.begin() + bi);
+            tile.push_back(v);
+            assigned[v] = 1;
+            for (int p = dc_ptr[v]; p < dc_ptr[v + 1]; ++p) stamp[dc_idx[p]] = tid;
+            push_front(v);
+            front.erase(std::remove_if(front.begin(), front.end(), [&](int w) { return assigned[w]; }),
+                        front.end());
+        }
+        order.insert(order.end(), tile.begin(), tile.end());
+        tstart.push_back((int)order.size());
+        prev = tile;
+    }
     P.iperm = order;
     P.perm.assign(n, 0);
     for (int i = 0; i < n; ++i) P.perm[order[i]] = i;
@@ -264,12 +370,16 @@ std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const i
             P.cell_int[4 * c + k] = P.perm[d];
             P.cell_cls[4 * c + k] = P.perm[uf.find(d)];
         }
-    // column tiles
-    P.n_tiles = (n + TILE_COLS - 1) / TILE_COLS;
-    std::vector<std::vector<int>> tl(P.n_tiles), tc(P.n_tiles);
+    // column tiles: [tile_start[t], tile_start[t+1]) in the device order, <= tc wide
+    P.n_tiles = (int)tstart.size() - 1;
+    P.tile_start = tstart;
+    std::vector<int> tile_of(n);
+    for (int t = 0; t < P.n_tiles; ++t)
+        for (int i = tstart[t]; i < tstart[t + 1]; ++i) tile_of[i] = t;
+    std::vector<std::vector<int>> tl(P.n_tiles), tpk(P.n_tiles);
     for (int c = 0; c < nc; ++c) {
         int tiles[4];
-        for (int k = 0; k < 4; ++k) tiles[k] = P.cell_int[4 * c + k] / TILE_COLS;
+        for (int k = 0; k < 4; ++k) tiles[k] = tile_of[P.cell_int[4 * c + k]];
         for (int k = 0; k < 4; ++k) {
             bool dup = false;
             for (int j = 0; j < k; ++j) dup |= tiles[j] == tiles[k];
@@ -278,18 +388,18 @@ std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const i
             uint32_t packed = 0;
             for (int j = 0; j < 4; ++j) {
                 const int v = P.cell_int[4 * c + j];
-                const int local = (v / TILE_COLS == t) ? v - t * TILE_COLS : -1;
+                const int local = (tile_of[v] == t) ? v - tstart[t] : -1;
                 packed |= (uint32_t)(uint8_t)(int8_t)local << (8 * j);
             }
             tl[t].push_back(c);
-            tc[t].push_back((int)packed);
+            tpk[t].push_back((int)packed);
         }
     }
     P.tile_ptr.assign(P.n_tiles + 1, 0);
     for (int t = 0; t < P.n_tiles; ++t) {
         P.tile_ptr[t + 1] = P.tile_ptr[t] + (int)tl[t].size();
         P.tile_cell.insert(P.tile_cell.end(), tl[t].begin(), tl[t].end());
-        P.tile_cols.insert(P.tile_cols.end(), tc[t].begin(), tc[t].end());
+        P.tile_cols.insert(P.tile_cols.end(), tpk[t].begin(), tpk[t].end());
     }
     // singular pairs: cells with a vertex in the row's class, first such vertex
     std::vector<std::vector<int>> class_cells(n);
