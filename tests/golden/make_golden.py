@@ -2,10 +2,13 @@
 
 The reference ships no known-answer data for this path, so these fixtures come from the
 mathematics: the Laplace kernels of include/laplace_kernel.h:45-64 integrated against
-the Q1 shape functions over one bilinear (iso-parametric) cell, evaluated with mpmath at
-30 digits (tanh-sinh; for a collocation point at a cell vertex the two Duffy triangles
-(xi,eta)=(u,uv) and (uv,u) remove the 1/r singularity first).  The oracle's Gauss 4x4 /
-QGaussOneOverR(5) values must reproduce them to quadrature accuracy.
+the Q1 shape functions over one bilinear (iso-parametric) cell.  Cells away from the
+collocation point: mpmath tanh-sinh at 30 digits.  Collocation point at a cell vertex: a
+panelled Gauss-Legendre product rule in polar coordinates about that vertex (rho times
+the integrand is smooth there), in float64, checked against a coarser panelling to 1e-12;
+adaptive tanh-sinh was tried first and stops early on the thin hull cell, whose polar
+integrand is sharply peaked in theta.  The oracle's Gauss 4x4 / QGaussOneOverR(5) values
+must reproduce these to quadrature accuracy, and its high-order rules closely.
 
 Entries: G_k = int phi_k / (4 pi |y-x|) dA,  H_k = int phi_k (y-x).n / (-4 pi |y-x|^3) dA,
 with n = orient (a x b)/|a x b| (a = dy/dxi, b = dy/deta), k in deal.II vertex order.
@@ -67,9 +70,51 @@ def integrate(v, orient, x, k, which, singular_vertex=None):
         xi, eta = to_ref(s, t)
         return integrand(v, orient, x, xi, eta, k, which)
 
-    lower = mp.quad(lambda u, w: u * f(u, u * w), [0, 1], [0, 1])
-    upper = mp.quad(lambda u, w: u * f(u * w, u), [0, 1], [0, 1])
-    return lower + upper
+    raise NotImplementedError("singular cells use polar_gauss_legendre")
+
+
+def polar_gauss_legendre(v, orient, sv, k, which, panels=16, npts=48):
+    """Singular-cell integral in polar coordinates (rho, theta) about vertex sv of the
+    reference square: rho * integrand is smooth, so a panelled Gauss-Legendre product rule
+    converges fast.  (Adaptive tanh-sinh stops early here: on thin cells the integrand is
+    sharply peaked in theta.)  numpy float64; convergence is checked by the caller."""
+    import numpy as np
+
+    V = np.array(v, dtype=np.float64)
+    g, w = np.polynomial.legendre.leggauss(npts)
+    total = 0.0
+    for t0, t1, rmax in ((0.0, np.pi / 4, lambda t: 1 / np.cos(t)),
+                         (np.pi / 4, np.pi / 2, lambda t: 1 / np.sin(t))):
+        edges = np.linspace(t0, t1, panels + 1)
+        for a, b in zip(edges[:-1], edges[1:]):
+            th = 0.5 * (b - a) * (g + 1) + a
+            wt = 0.5 * (b - a) * w
+            for ti, wti in zip(th, wt):
+                R = rmax(ti)
+                # graded rho panels towards the vertex
+                rb = R * np.array([0.0, 1e-3, 1e-2, 0.1, 0.4, 1.0])
+                for r0, r1 in zip(rb[:-1], rb[1:]):
+                    rho = 0.5 * (r1 - r0) * (g + 1) + r0
+                    wr = 0.5 * (r1 - r0) * w
+                    s, t = rho * np.cos(ti), rho * np.sin(ti)
+                    xi = s if sv in (0, 2) else 1 - s
+                    eta = t if sv in (0, 1) else 1 - t
+                    ph = np.stack([(1 - xi) * (1 - eta), xi * (1 - eta), (1 - xi) * eta, xi * eta])
+                    dxi = np.stack([-(1 - eta), (1 - eta), -eta, eta])
+                    de = np.stack([-(1 - xi), -xi, (1 - xi), xi])
+                    # y - x_sv = sum_k phi_k (v_k - v_sv): no cancellation near the vertex
+                    D = V - V[sv]
+                    Rv = np.einsum("kn,kd->nd", ph, D)
+                    a_ = np.einsum("kn,kd->nd", dxi, V)
+                    b_ = np.einsum("kn,kd->nd", de, V)
+                    axb = np.cross(a_, b_)
+                    r = np.linalg.norm(Rv, axis=1)
+                    if which == "G":
+                        val = ph[k] * np.linalg.norm(axb, axis=1) / (4 * np.pi * r)
+                    else:
+                        val = ph[k] * orient * (Rv * axb).sum(1) / (-4 * np.pi * r ** 3)
+                    total += wti * (val * rho * wr).sum()
+    return total
 
 
 def cases():
@@ -102,9 +147,21 @@ def main():
     res = []
     for c in cases():
         vals = {}
+        sv = c["singular_vertex"]
         for which in ("G", "H"):
-            vals[which] = [float(integrate(c["vertices"], c["orient"], c["x"], k, which,
-                                           c["singular_vertex"])) for k in range(4)]
+            if sv is None:
+                vals[which] = [float(integrate(c["vertices"], c["orient"], c["x"], k, which))
+                               for k in range(4)]
+                c["method"] = "mpmath tanh-sinh, 30 digits"
+            else:
+                fine = [polar_gauss_legendre(c["vertices"], c["orient"], sv, k, which, 16, 48)
+                        for k in range(4)]
+                coarse = [polar_gauss_legendre(c["vertices"], c["orient"], sv, k, which, 8, 32)
+                          for k in range(4)]
+                scale = max(abs(t) for t in fine)
+                assert max(abs(a - b) for a, b in zip(fine, coarse)) < 1e-12 * scale, (which, c)
+                vals[which] = fine
+                c["method"] = "polar Gauss-Legendre (16 panels x 48, graded rho), float64"
         c = dict(c)
         c.update(vals)
         print(c["name"], vals, file=sys.stderr, flush=True)

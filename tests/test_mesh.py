@@ -1,0 +1,84 @@
+"""Synthetic geometry: closed surfaces, outward (out-of-fluid) orientation, volumes,
+double-node structure and boundary data."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from wavebem_amd import mesh as M
+
+
+def _surface_integrals(m):
+    p, w = O.qgauss_2d(4)
+    area = 0.0
+    nA = np.zeros(3)
+    vol = 0.0
+    for c in range(m.n_cells):
+        V = m.support_points[m.cell_dofs[c]]
+        qp, nr, jxw, _ = O.fe_values(V, m.cell_orient[c], p, w)
+        area += jxw.sum()
+        nA += (nr * jxw[:, None]).sum(0)
+        vol += ((qp * nr).sum(1) * jxw).sum() / 3.0
+    return area, nA, vol
+
+
+def test_box_is_closed_with_outward_normals():
+    m = M.box_mesh(n=3, lo=(0, 0, 0), hi=(1, 2, 3))
+    area, nA, vol = _surface_integrals(m)
+    assert abs(area - 2 * (2 + 3 + 6)) < 1e-12
+    assert np.abs(nA).max() < 1e-12
+    assert abs(vol - 6.0) < 1e-12
+
+
+def test_wigley_tank_closed_and_volume():
+    m = M.wigley_tank(nh=16, nt=6, nu=8, nd=10, ny=10, nzd=4)
+    area, nA, vol = _surface_integrals(m)
+    info = m.info
+    assert np.abs(nA).max() < 1e-10
+    box = (info["x1"] - info["x0"]) * info["Y"] * info["D"]
+    hull_half = 2 * info["B"] * info["L"] * info["T"] / 9  # int (B/2)(1-(2x/L)^2)(1-(z/T)^2)
+    assert abs(vol - (box - hull_half)) / hull_half < 0.02
+
+
+def test_wigley_patches_and_double_nodes():
+    m = M.wigley_preset("2.5k")
+    assert set(np.unique(m.dof_material)) == {M.FREE_SURFACE, M.HULL, M.CENTREPLANE, M.FAR_SIDE,
+                                              M.BOTTOM, M.INFLOW, M.OUTFLOW}
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    cnt = np.diff(ptr)
+    assert cnt.max() == 3  # box corners and the stem/stern waterline points
+    # every free-surface dof on the waterline has a coincident hull dof
+    fs = np.nonzero(m.dof_material == M.FREE_SURFACE)[0]
+    X = m.support_points
+    wl = fs[(np.abs(X[fs, 0]) < 0.5 - 1e-9) & (X[fs, 1] < 0.051) & (X[fs, 1] > 1e-9) &
+            np.isclose(X[fs, 1], M.wigley_hull_y(X[fs, 0], 0.0))]
+    assert wl.size > 0
+    for i in wl:
+        mates = idx[ptr[i]:ptr[i + 1]]
+        assert np.any(m.dof_material[mates] == M.HULL)
+    assert np.array_equal(m.surface_nodes, (m.dof_material == M.FREE_SURFACE).astype(float))
+    assert np.array_equal(m.other_nodes, 1 - m.surface_nodes)
+
+
+def test_wigley_boundary_data():
+    m = M.wigley_preset("2.5k")
+    bc, U = M.wigley_bc(m, froude=0.3, seed=0)
+    assert abs(U - 0.3 * math.sqrt(9.81)) < 1e-12
+    fs = m.dof_material == M.FREE_SURFACE
+    assert np.abs(bc[fs]).max() <= 1e-3 and np.abs(bc[fs]).max() > 9e-4
+    hull = m.dof_material == M.HULL
+    n = M.wigley_hull_normal_out_of_fluid(m.support_points[hull])
+    assert np.allclose(bc[hull], -U * n[:, 0])
+    assert np.all(n[:, 1] < 0)  # into the hull, away from the fluid at y > hull
+    other = ~(fs | hull)
+    assert np.all(bc[other] == 0)
+    # seeded: same seed, same data
+    assert np.array_equal(bc, M.wigley_bc(m, froude=0.3, seed=0)[0])
+
+
+@pytest.mark.parametrize("name", ["2.5k", "10k", "20k", "40k", "60k"])
+def test_presets_sizes(name):
+    target = {"2.5k": 2500, "10k": 10000, "20k": 20000, "40k": 40000, "60k": 60000}[name]
+    m = M.wigley_preset(name)
+    assert 0.9 * target < m.n_dofs < 1.3 * target

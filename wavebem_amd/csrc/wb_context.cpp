@@ -86,7 +86,8 @@ struct wb_ctx {
         sing_cell, sing_vtx, qpts, qw, srule, cellq, perm_d, iperm_d;
     // device: operator
     DevBuf G, H, Glo, Hlo, hsum_part, hsum_sing, alpha;
-    DevBuf surf_d, gmask, diag_op, diag_rhs;
+    DevBuf surf_d, gmask, diag_op, diag_rhs, segs;
+    int n_segs = 0;
     DevBuf con_rows, con_ptr, con_cols, con_w, con_inhom, constrained, free_rows;
     int n_lines = 0, n_free = 0;
     DevBuf binv, binv32;
@@ -260,8 +261,9 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
     const int *rows = free_rows_only ? c->free_rows.as<int>() : nullptr;
     const int nr = free_rows_only ? c->n_free : c->n;
     KernelScope ks(c, mode == 0 ? WB_K_GEMV64 : (mode == 1 ? WB_K_GEMV32 : WB_K_GEMVHL), nr);
-    launch_gemv_sel(mode, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld, c->n, c->gmask.as<uint8_t>(),
-                    which, s_sel, s_uns, x, diag, xd, rows, nr, y, c->stream);
+    launch_gemv_sel(mode, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld, c->segs.as<int>(),
+                    c->n_segs, c->gmask.as<uint8_t>(), which, s_sel, s_uns, x, diag, xd, rows, nr,
+                    y, c->stream);
 }
 
 // y = ConstrainedOperator(A) x in FP64 (mode 0 or 2) or FP32 (mode 1)
@@ -695,7 +697,7 @@ void wb_destroy(wb_ctx *c)
                       &c->sing_vtx,
                       &c->qpts, &c->qw, &c->srule, &c->cellq, &c->perm_d, &c->iperm_d, &c->G,
                       &c->H, &c->Glo, &c->Hlo, &c->hsum_part, &c->hsum_sing, &c->alpha,
-                      &c->surf_d, &c->gmask, &c->diag_op, &c->diag_rhs, &c->con_rows,
+                      &c->surf_d, &c->gmask, &c->segs, &c->diag_op, &c->diag_rhs, &c->con_rows,
                       &c->con_ptr, &c->con_cols, &c->con_w, &c->con_inhom, &c->constrained,
                       &c->free_rows, &c->binv, &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2,
                       &c->Pn, &c->hbuf, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
@@ -838,6 +840,27 @@ wb_status wb_set_masks(wb_ctx *c, const double *surface, const double *other)
         for (int j = 0; j < n; ++j)
             if (oi[j] != 0.0) gm[j / 4] |= (uint8_t)(1u << (j % 4));
         upload(c->gmask, gm.data(), gm.size(), c->stream);
+        // runs of 4-column groups reading only G, only H, or mixed
+        std::vector<int> sg;
+        const int ngroups = (n + 3) / 4;
+        for (int g = 0; g < ngroups;) {
+            const int nvalid = std::min(4, n - 4 * g);
+            const uint8_t full = (uint8_t)((1u << nvalid) - 1);
+            auto kind = [&](int gg) {
+                const int nv = std::min(4, n - 4 * gg);
+                const uint8_t f = (uint8_t)((1u << nv) - 1);
+                const uint8_t m = gm[gg] & f;
+                return m == 0 ? 0 : (m == f ? 1 : 2);
+            };
+            (void)full;
+            const int k = kind(g);
+            int e = g + 1;
+            while (e < ngroups && kind(e) == k) ++e;
+            sg.insert(sg.end(), {g, e, k, 0});
+            g = e;
+        }
+        c->n_segs = (int)sg.size() / 4;
+        upload(c->segs, sg.data(), sg.size(), c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
         c->have_masks = true;
         c->mask_ver++;
@@ -1160,6 +1183,7 @@ size_t wb_device_bytes(const wb_ctx *c)
                             &c->sing_cell, &c->sing_vtx, &c->qpts, &c->qw, &c->srule, &c->cellq,
                             &c->perm_d, &c->iperm_d, &c->G, &c->H, &c->Glo, &c->Hlo,
                             &c->hsum_part, &c->hsum_sing, &c->alpha, &c->surf_d, &c->gmask,
+                            &c->segs,
                             &c->diag_op, &c->diag_rhs, &c->con_rows, &c->con_ptr, &c->con_cols,
                             &c->con_w, &c->con_inhom, &c->constrained, &c->free_rows, &c->binv,
                             &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2, &c->Pn, &c->hbuf,

@@ -1,0 +1,265 @@
+// wb_gemv.hip -- column-selected GEMV of the BEM operator on gfx950 (HBM-bound).
+//
+// BEMProblem::vmult (source/bem_problem.cc:621-670) computes
+//   dst = -G (surface .* src) + H (other .* src) + alpha .* other .* src
+// and compute_rhs (:673-717) the complementary selection.  Every column j needs either
+// G(:,j) or H(:,j), never both, so one pass reads exactly one n x n matrix.  The host cuts
+// the columns (groups of 4) into segments that read only G, only H, or are mixed (the few
+// groups at a Dirichlet/Neumann boundary of the device ordering, resolved per column with a
+// 4-bit mask).  A segment is wave-uniform, so the matrix pointer needs no per-lane load and
+// all row streams stay in flight.  Each wavefront streams RPW rows with 16-byte lane loads
+// (the shapes measured fastest on MI355X: FP64 4 rows x 4 waves, FP32 8 rows x 2 waves,
+// both unrolled twice) and reduces with wavefront shuffles.  Constrained rows
+// (ConstrainedOperator, include/constrained_matrix.h:73-86) are skipped through the
+// caller's row list and rewritten by a fix-up kernel.
+#include "wb_internal.h"
+
+namespace wb {
+
+namespace {
+
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef unsigned short uvec4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wsumf(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double bf16d(unsigned short b)
+{
+    return (double)__uint_as_float((uint32_t)b << 16);
+}
+
+template <int RPW>
+__device__ __forceinline__ void load_rows(int slot0, const int *__restrict__ rows, int n_rows,
+                                          int (&r)[RPW], bool (&ok)[RPW])
+{
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int s = slot0 + k;
+        ok[k] = s < n_rows;
+        r[k] = ok[k] ? (rows ? rows[s] : s) : 0;
+    }
+}
+
+constexpr int F64_RPW = 4, F64_WAVES = 4;
+constexpr int F32_RPW = 8, F32_WAVES = 2;
+
+// segment = int4 {first group, end group, kind (0 G, 1 H, 2 mixed), unused}
+__global__ __launch_bounds__(64 * F64_WAVES) void k_gemv_f64(
+    const double *__restrict__ G, const double *__restrict__ H, size_t ld,
+    const int4 *__restrict__ segs, int n_segs, const uint8_t *__restrict__ gmask, int flip,
+    double s_sel, double s_uns, const double *__restrict__ x, const double *__restrict__ diag,
+    const double *__restrict__ xd, const int *__restrict__ rows, int n_rows,
+    double *__restrict__ y)
+{
+    constexpr int RPW = F64_RPW;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int r[RPW];
+    bool ok[RPW];
+    load_rows<RPW>((blockIdx.x * F64_WAVES + wave) * RPW, rows, n_rows, r, ok);
+    const double *Gr[RPW], *Hr[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        Gr[k] = G + r[k] * ld;
+        Hr[k] = H + r[k] * ld;
+    }
+    double acc[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) acc[k] = 0.0;
+    for (int si = 0; si < n_segs; ++si) {
+        const int4 sg = segs[si];
+        const int p0 = 2 * sg.x, p1 = 2 * sg.y;  // column pairs of the segment
+        if (sg.z < 2) {
+            const bool use_h = (sg.z ^ flip) != 0;
+            const double sc = use_h ? s_sel : s_uns;
+            auto step = [&](int p) {
+                const dvec2 xv = *((const dvec2 *)x + p);
+#pragma unroll
+                for (int k = 0; k < RPW; ++k) {
+                    const dvec2 a =
+                        __builtin_nontemporal_load((const dvec2 *)(use_h ? Hr[k] : Gr[k]) + p);
+                    acc[k] += (a.x * xv.x + a.y * xv.y) * sc;
+                }
+            };
+            int p = p0 + lane;
+            for (; p + 64 < p1; p += 128) {
+                step(p);
+                step(p + 64);
+            }
+            if (p < p1) step(p);
+        } else {
+            const unsigned fm = flip ? 3u : 0u;
+            for (int p = p0 + lane; p < p1; p += 64) {
+                const unsigned m2 = ((gmask[p >> 1] >> ((p & 1) * 2)) & 3u) ^ fm;
+                const dvec2 xv = *((const dvec2 *)x + p);
+                const double xe0 = xv.x * ((m2 & 1) ? s_sel : s_uns);
+                const double xe1 = xv.y * ((m2 & 2) ? s_sel : s_uns);
+#pragma unroll
+                for (int k = 0; k < RPW; ++k) {
+                    const dvec2 a = *((const dvec2 *)Gr[k] + p);
+                    const dvec2 b = *((const dvec2 *)Hr[k] + p);
+                    acc[k] += ((m2 & 1) ? b.x : a.x) * xe0 + ((m2 & 2) ? b.y : a.y) * xe1;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const double s = wsum(acc[k]);
+        if (lane == 0 && ok[k]) y[r[k]] = s + (diag ? diag[r[k]] * xd[r[k]] : 0.0);
+    }
+}
+
+// FP32 hi (MODE 1: float vectors, FP32 accumulation) or hi + bf16 lo (MODE 2: double
+// vectors, FP64 accumulation); 4 columns per lane per step (group g = columns 4g..4g+3)
+template <int MODE>
+__global__ __launch_bounds__(64 * F32_WAVES) void k_gemv_mixed(
+    const float *__restrict__ G, const float *__restrict__ H, const uint16_t *__restrict__ Glo,
+    const uint16_t *__restrict__ Hlo, size_t ld, const int4 *__restrict__ segs, int n_segs,
+    const uint8_t *__restrict__ gmask, int flip, double s_sel, double s_uns,
+    const void *__restrict__ xv_, const double *__restrict__ diag, const void *__restrict__ xd_,
+    const int *__restrict__ rows, int n_rows, void *__restrict__ y_)
+{
+    constexpr int RPW = F32_RPW;
+    typedef typename std::conditional<MODE == 1, float, double>::type T;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int r[RPW];
+    bool ok[RPW];
+    load_rows<RPW>((blockIdx.x * F32_WAVES + wave) * RPW, rows, n_rows, r, ok);
+    T acc[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) acc[k] = 0;
+    auto load_x = [&](int g, T (&xe)[4]) {
+        if (MODE == 1) {
+            const fvec4 xv = *((const fvec4 *)xv_ + g);
+            xe[0] = xv.x;
+            xe[1] = xv.y;
+            xe[2] = xv.z;
+            xe[3] = xv.w;
+        } else {
+            const dvec2 a = *((const dvec2 *)xv_ + 2 * g), b = *((const dvec2 *)xv_ + 2 * g + 1);
+            xe[0] = a.x;
+            xe[1] = a.y;
+            xe[2] = b.x;
+            xe[3] = b.y;
+        }
+    };
+    for (int si = 0; si < n_segs; ++si) {
+        const int4 sg = segs[si];
+        if (sg.z < 2) {
+            const bool use_h = (sg.z ^ flip) != 0;
+            const T sc = (T)(use_h ? s_sel : s_uns);
+            const float *M = use_h ? H : G;
+            const uint16_t *Ml = use_h ? Hlo : Glo;
+            auto step = [&](int g) {
+                T xe[4];
+                load_x(g, xe);
+#pragma unroll
+                for (int k = 0; k < RPW; ++k) {
+                    const fvec4 a = __builtin_nontemporal_load((const fvec4 *)(M + r[k] * ld) + g);
+                    if (MODE == 1) {
+                        acc[k] += (a.x * xe[0] + a.y * xe[1] + a.z * xe[2] + a.w * xe[3]) * sc;
+                    } else {
+                        const uvec4 l =
+                            __builtin_nontemporal_load((const uvec4 *)(Ml + r[k] * ld) + g);
+                        acc[k] += (((double)a.x + bf16d(l.x)) * xe[0] +
+                                   ((double)a.y + bf16d(l.y)) * xe[1] +
+                                   ((double)a.z + bf16d(l.z)) * xe[2] +
+                                   ((double)a.w + bf16d(l.w)) * xe[3]) *
+                                  sc;
+                    }
+                }
+            };
+            int g = sg.x + lane;
+            for (; g + 64 < sg.y; g += 128) {
+                step(g);
+                step(g + 64);
+            }
+            if (g < sg.y) step(g);
+        } else {
+            const unsigned fm = flip ? 0xfu : 0u;
+            const T ssel = (T)s_sel, suns = (T)s_uns;
+            for (int g = sg.x + lane; g < sg.y; g += 64) {
+                const unsigned m = gmask[g] ^ fm;
+                T xe[4];
+                load_x(g, xe);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) xe[e] *= ((m >> e) & 1) ? ssel : suns;
+#pragma unroll
+                for (int k = 0; k < RPW; ++k) {
+                    const fvec4 a = *((const fvec4 *)(G + r[k] * ld) + g);
+                    const fvec4 b = *((const fvec4 *)(H + r[k] * ld) + g);
+                    T av[4] = {(m & 1) ? b.x : a.x, (m & 2) ? b.y : a.y, (m & 4) ? b.z : a.z,
+                               (m & 8) ? b.w : a.w};
+                    if (MODE == 2) {
+                        const uvec4 la = *((const uvec4 *)(Glo + r[k] * ld) + g);
+                        const uvec4 lb = *((const uvec4 *)(Hlo + r[k] * ld) + g);
+                        av[0] += bf16d((m & 1) ? lb.x : la.x);
+                        av[1] += bf16d((m & 2) ? lb.y : la.y);
+                        av[2] += bf16d((m & 4) ? lb.z : la.z);
+                        av[3] += bf16d((m & 8) ? lb.w : la.w);
+                    }
+                    acc[k] += av[0] * xe[0] + av[1] * xe[1] + av[2] * xe[2] + av[3] * xe[3];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        if (MODE == 1) {
+            const float s = wsumf((float)acc[k]);
+            if (lane == 0 && ok[k]) {
+                const float *xd = (const float *)xd_;
+                ((float *)y_)[r[k]] = s + (diag ? (float)diag[r[k]] * xd[r[k]] : 0.0f);
+            }
+        } else {
+            const double s = wsum((double)acc[k]);
+            if (lane == 0 && ok[k]) {
+                const double *xd = (const double *)xd_;
+                ((double *)y_)[r[k]] = s + (diag ? diag[r[k]] * xd[r[k]] : 0.0);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, const void *Hlo,
+                     size_t ld, const int *segs, int n_segs, const uint8_t *gmask, int flip,
+                     double s_sel, double s_unsel, const void *x, const double *diag,
+                     const void *xd, const int *rows, int n_rows, void *y, hipStream_t s)
+{
+    if (n_rows <= 0) return;
+    if (mode == 0) {
+        const int per_block = F64_RPW * F64_WAVES;
+        hipLaunchKernelGGL(k_gemv_f64, dim3((n_rows + per_block - 1) / per_block),
+                           dim3(64 * F64_WAVES), 0, s, (const double *)G, (const double *)H, ld,
+                           (const int4 *)segs, n_segs, gmask, flip, s_sel, s_unsel,
+                           (const double *)x, diag, (const double *)xd, rows, n_rows, (double *)y);
+        return;
+    }
+    const int per_block = F32_RPW * F32_WAVES;
+    dim3 grid((n_rows + per_block - 1) / per_block), block(64 * F32_WAVES);
+    if (mode == 1)
+        hipLaunchKernelGGL(k_gemv_mixed<1>, grid, block, 0, s, (const float *)G, (const float *)H,
+                           (const uint16_t *)Glo, (const uint16_t *)Hlo, ld, (const int4 *)segs,
+                           n_segs, gmask, flip, s_sel, s_unsel, x, diag, xd, rows, n_rows, y);
+    else
+        hipLaunchKernelGGL(k_gemv_mixed<2>, grid, block, 0, s, (const float *)G, (const float *)H,
+                           (const uint16_t *)Glo, (const uint16_t *)Hlo, ld, (const int4 *)segs,
+                           n_segs, gmask, flip, s_sel, s_unsel, x, diag, xd, rows, n_rows, y);
+}
+
+}  // namespace wb

@@ -93,10 +93,11 @@ void launch_alpha_reduce(const double *hsum_part, int n_parts, const double *hsu
 // y[r] = sum_j M_sel(j)[r,j] * sign_sel(j) * x[j] + diag[r] * xd[r] for r in rows (or all)
 // sel bit per column pair in pairflag (0: G,G 1: H,H 2: G,H 3: H,G), flipped by `flip`.
 // mode: 0 = FP64 matrices, 1 = hi only (FP32 accumulate), 2 = hi+lo (FP64 accumulate)
+// segs: n_segs x int4 {first 4-column group, end group, kind 0=G 1=H 2=mixed, 0}
 void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, const void *Hlo,
-                     size_t ld, int n, const uint8_t *pairflag, int flip, double s_sel,
-                     double s_unsel, const void *x, const double *diag, const void *xd,
-                     const int *rows, int n_rows, void *y, hipStream_t s);
+                     size_t ld, const int *segs, int n_segs, const uint8_t *gmask, int flip,
+                     double s_sel, double s_unsel, const void *x, const double *diag,
+                     const void *xd, const int *rows, int n_rows, void *y, hipStream_t s);
 void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *Glo,
                            const void *Hlo, size_t ld, int n, const uint8_t *pairflag, int flip,
                            double s_sel, double s_unsel, int k, const void *X, size_t ldx,

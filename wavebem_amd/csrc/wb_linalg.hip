@@ -37,186 +37,6 @@ typedef double dvec4 __attribute__((ext_vector_type(4)));
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 typedef unsigned short uvec4 __attribute__((ext_vector_type(4)));
 
-constexpr int GEMV_RPW = 4;    // rows per wavefront
-constexpr int GEMV_WAVES = 4;  // wavefronts per workgroup
-
-// mode 0: FP64 matrices, FP64 vectors.  x, xd, y double.
-template <int RPW>
-__global__ __launch_bounds__(64 * GEMV_WAVES) void k_gemv_sel_f64(
-    const double *__restrict__ G, const double *__restrict__ H, size_t ld, int n,
-    const uint8_t *__restrict__ gmask, int flip, double s_sel, double s_uns,
-    const double *__restrict__ x, const double *__restrict__ diag, const double *__restrict__ xd,
-    const int *__restrict__ rows, int n_rows, double *__restrict__ y)
-{
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int slot0 = (blockIdx.x * GEMV_WAVES + wave) * RPW;
-    int r[RPW];
-    bool ok[RPW];
-#pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-        const int s = slot0 + k;
-        ok[k] = s < n_rows;
-        r[k] = ok[k] ? (rows ? rows[s] : s) : 0;
-    }
-    double acc[RPW];
-#pragma unroll
-    for (int k = 0; k < RPW; ++k) acc[k] = 0.0;
-    const int ngroups = (n + 3) >> 2;
-    const uint8_t fm = flip ? 0xf : 0x0;
-    for (int g = lane; g < ngroups; g += 64) {
-        const uint8_t m = gmask[g] ^ fm;
-        const dvec4 xv = *(const dvec4 *)(x + 4 * g);
-        double xe[4] = {xv.x, xv.y, xv.z, xv.w};
-#pragma unroll
-        for (int e = 0; e < 4; ++e) xe[e] *= ((m >> e) & 1) ? s_sel : s_uns;
-        if (m == 0 || m == 0xf) {
-            const double *M = m ? H : G;
-#pragma unroll
-            for (int k = 0; k < RPW; ++k) {
-                const dvec4 a = __builtin_nontemporal_load((const dvec4 *)(M + r[k] * ld + 4 * g));
-                acc[k] += a.x * xe[0] + a.y * xe[1] + a.z * xe[2] + a.w * xe[3];
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < RPW; ++k) {
-                const dvec4 a = *(const dvec4 *)(G + r[k] * ld + 4 * g);
-                const dvec4 b = *(const dvec4 *)(H + r[k] * ld + 4 * g);
-                acc[k] += ((m & 1) ? b.x : a.x) * xe[0] + ((m & 2) ? b.y : a.y) * xe[1] +
-                          ((m & 4) ? b.z : a.z) * xe[2] + ((m & 8) ? b.w : a.w) * xe[3];
-            }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-        const double s = wave_sum(acc[k]);
-        if (lane == 0 && ok[k]) y[r[k]] = s + (diag ? diag[r[k]] * xd[r[k]] : 0.0);
-    }
-}
-
-// mode 1: FP32 hi matrices, float vectors, FP32 accumulation (inner solver).
-// mode 2: hi + bf16 lo matrices, double vectors, FP64 accumulation (refinement residual).
-template <int RPW, int MODE>
-__global__ __launch_bounds__(64 * GEMV_WAVES) void k_gemv_sel_mixed(
-    const float *__restrict__ G, const float *__restrict__ H, const uint16_t *__restrict__ Glo,
-    const uint16_t *__restrict__ Hlo, size_t ld, int n, const uint8_t *__restrict__ gmask,
-    int flip, double s_sel, double s_uns, const void *__restrict__ xv_,
-    const double *__restrict__ diag, const void *__restrict__ xd_, const int *__restrict__ rows,
-    int n_rows, void *__restrict__ y_)
-{
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int slot0 = (blockIdx.x * GEMV_WAVES + wave) * RPW;
-    int r[RPW];
-    bool ok[RPW];
-#pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-        const int s = slot0 + k;
-        ok[k] = s < n_rows;
-        r[k] = ok[k] ? (rows ? rows[s] : s) : 0;
-    }
-    typedef typename std::conditional<MODE == 1, float, double>::type acc_t;
-    acc_t acc[RPW];
-#pragma unroll
-    for (int k = 0; k < RPW; ++k) acc[k] = 0;
-    const int ngroups = (n + 3) >> 2;
-    const uint8_t fm = flip ? 0xf : 0x0;
-    for (int g = lane; g < ngroups; g += 64) {
-        const uint8_t m = gmask[g] ^ fm;
-        acc_t xe[4];
-        if (MODE == 1) {
-            const fvec4 xv = *((const fvec4 *)xv_ + g);
-            xe[0] = xv.x;
-            xe[1] = xv.y;
-            xe[2] = xv.z;
-            xe[3] = xv.w;
-        } else {
-            const dvec4 xv = *((const dvec4 *)xv_ + g);
-            xe[0] = xv.x;
-            xe[1] = xv.y;
-            xe[2] = xv.z;
-            xe[3] = xv.w;
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) xe[e] *= (acc_t)(((m >> e) & 1) ? s_sel : s_uns);
-        if (m == 0 || m == 0xf) {
-            const float *M = m ? H : G;
-            const uint16_t *Ml = m ? Hlo : Glo;
-#pragma unroll
-            for (int k = 0; k < RPW; ++k) {
-                const fvec4 a = __builtin_nontemporal_load((const fvec4 *)(M + r[k] * ld + 4 * g));
-                if (MODE == 1) {
-                    acc[k] += a.x * xe[0] + a.y * xe[1] + a.z * xe[2] + a.w * xe[3];
-                } else {
-                    const uvec4 l =
-                        __builtin_nontemporal_load((const uvec4 *)(Ml + r[k] * ld + 4 * g));
-                    acc[k] += ((double)a.x + bf16_to_double(l.x)) * xe[0] +
-                              ((double)a.y + bf16_to_double(l.y)) * xe[1] +
-                              ((double)a.z + bf16_to_double(l.z)) * xe[2] +
-                              ((double)a.w + bf16_to_double(l.w)) * xe[3];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < RPW; ++k) {
-                const fvec4 a = *(const fvec4 *)(G + r[k] * ld + 4 * g);
-                const fvec4 b = *(const fvec4 *)(H + r[k] * ld + 4 * g);
-                acc_t av[4] = {((m & 1) ? b.x : a.x), ((m & 2) ? b.y : a.y), ((m & 4) ? b.z : a.z),
-                               ((m & 8) ? b.w : a.w)};
-                if (MODE == 2) {
-                    const uvec4 la = *(const uvec4 *)(Glo + r[k] * ld + 4 * g);
-                    const uvec4 lb = *(const uvec4 *)(Hlo + r[k] * ld + 4 * g);
-                    av[0] += bf16_to_double((m & 1) ? lb.x : la.x);
-                    av[1] += bf16_to_double((m & 2) ? lb.y : la.y);
-                    av[2] += bf16_to_double((m & 4) ? lb.z : la.z);
-                    av[3] += bf16_to_double((m & 8) ? lb.w : la.w);
-                }
-                acc[k] += av[0] * xe[0] + av[1] * xe[1] + av[2] * xe[2] + av[3] * xe[3];
-            }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < RPW; ++k) {
-        acc_t s;
-        if (MODE == 1)
-            s = wave_sumf((float)acc[k]);
-        else
-            s = wave_sum((double)acc[k]);
-        if (lane == 0 && ok[k]) {
-            if (MODE == 1) {
-                const float *xd = (const float *)xd_;
-                ((float *)y_)[r[k]] = (float)s + (diag ? (float)diag[r[k]] * xd[r[k]] : 0.0f);
-            } else {
-                const double *xd = (const double *)xd_;
-                ((double *)y_)[r[k]] = (double)s + (diag ? diag[r[k]] * xd[r[k]] : 0.0);
-            }
-        }
-    }
-}
-
-void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, const void *Hlo,
-                     size_t ld, int n, const uint8_t *gmask, int flip, double s_sel,
-                     double s_unsel, const void *x, const double *diag, const void *xd,
-                     const int *rows, int n_rows, void *y, hipStream_t s)
-{
-    if (n_rows <= 0) return;
-    const int per_block = GEMV_RPW * GEMV_WAVES;
-    dim3 grid((n_rows + per_block - 1) / per_block), block(64 * GEMV_WAVES);
-    if (mode == 0)
-        hipLaunchKernelGGL((k_gemv_sel_f64<GEMV_RPW>), grid, block, 0, s, (const double *)G,
-                           (const double *)H, ld, n, gmask, flip, s_sel, s_unsel,
-                           (const double *)x, diag, (const double *)xd, rows, n_rows,
-                           (double *)y);
-    else if (mode == 1)
-        hipLaunchKernelGGL((k_gemv_sel_mixed<GEMV_RPW, 1>), grid, block, 0, s, (const float *)G,
-                           (const float *)H, (const uint16_t *)Glo, (const uint16_t *)Hlo, ld, n,
-                           gmask, flip, s_sel, s_unsel, x, diag, xd, rows, n_rows, y);
-    else
-        hipLaunchKernelGGL((k_gemv_sel_mixed<GEMV_RPW, 2>), grid, block, 0, s, (const float *)G,
-                           (const float *)H, (const uint16_t *)Glo, (const uint16_t *)Hlo, ld, n,
-                           gmask, flip, s_sel, s_unsel, x, diag, xd, rows, n_rows, y);
-}
-
 // ------------------------------------------------------------------ constraint rows
 template <typename T>
 __global__ void k_constraint_rows(int n_lines, const int *__restrict__ rows,
