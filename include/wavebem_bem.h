@@ -160,10 +160,16 @@ wb_status wb_solve(wb_ctx *ctx, const double *tmp_rhs, double *phi, double *dphi
  * installed constraints. */
 wb_status wb_residual(wb_ctx *ctx, double *res, const double *phi, const double *dphi_dn);
 
-/* Multi right-hand side form of solve_system: k boundary-data vectors (column j at
- * tmp_rhs + j*n), solved together with one batched (MFMA) matrix product per GMRES
- * step.  Constraint inhomogeneities are taken per right-hand side from inhom_k
- * (n_lines x k, column-major; NULL = the installed ones for every column). */
+/* Multi right-hand side form of solve_system (no reference counterpart: the reference
+ * calls solve_system once per time step, source/bem_problem.cc:1013-1108): k
+ * boundary-data vectors (column j at tmp_rhs + j*n), solved in batches of up to
+ * WB_MULTI_MAX with one batched (MFMA) matrix product per GMRES step and per-column
+ * GMRES state (each column converges, restarts and fails on its own, as k separate
+ * solve_system calls would).  Constraint inhomogeneities: with automatic constraints
+ * they are rebuilt from each column of tmp_rhs; otherwise taken from inhom_k (n_lines x k,
+ * column-major; NULL = the installed ones for every column).  All-Neumann masks
+ * return WB_E_UNSUPPORTED. */
+#define WB_MULTI_MAX 16
 wb_status wb_solve_system_multi(wb_ctx *ctx, int k, const double *tmp_rhs, double *phi,
                                 double *dphi_dn, const double *inhom_k,
                                 wb_solve_stats *stats);

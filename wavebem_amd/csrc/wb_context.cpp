@@ -108,6 +108,19 @@ struct wb_ctx {
     };
     std::vector<Pending> pending;
     wb_kernel_stats kstats{};
+
+    // several right-hand sides at once
+    struct Multi {
+        int k = 0;
+        DevBuf V, V32, Z, W, Z32, W32, P1, P2, Pn, H, X, B, R, T, D32, R32, act, inh;
+        double *hpin = nullptr;  // pinned k x 132 coefficients
+        int *apin = nullptr;     // pinned k active flags
+        std::vector<DevBuf *> all()
+        {
+            return {&V, &V32, &Z, &W, &Z32, &W32, &P1, &P2, &Pn, &H,
+                    &X, &B, &R, &T, &D32, &R32, &act, &inh};
+        }
+    } mw;
 };
 
 // Brackets one launch with a pair of events when profiling is on.
@@ -630,6 +643,353 @@ static void solve_system_dev(wb_ctx *c, const double *tmp_dev_internal, wb_solve
     run_solver(c, st);
 }
 
+// ------------------------------------------------------------------ several right-hand sides
+// Y[:, v] = ConstrainedOperator(A) X[:, v] for the k columns (column strides ldx / ldy).
+static void apply_constrained_multi(wb_ctx *c, int mode, int k, const void *X, size_t ldx,
+                                    void *Y, size_t ldy)
+{
+    require(!c->pure_neumann, WB_E_UNSUPPORTED,
+            "several right-hand sides with all-Neumann masks are not supported");
+    const int is_float = mode == 1;
+    const size_t es = is_float ? 4 : 8;
+    {
+        KernelScope ks(c, mode == 0 ? WB_K_GEMV64 : (mode == 1 ? WB_K_GEMV32 : WB_K_GEMVHL),
+                       (long long)c->n_free);
+        launch_gemv_sel_multi(mode, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld, c->segs.as<int>(),
+                              c->n_segs, c->gmask.as<uint8_t>(), 0, 1.0, -1.0, k, X, ldx,
+                              c->diag_op.as<double>(), X, c->free_rows.as<int>(), c->n_free, Y,
+                              ldy, c->stream);
+    }
+    for (int v = 0; v < k; ++v)
+        launch_constraint_rows(is_float, c->n_lines, c->con_rows.as<int>(), c->con_ptr.as<int>(),
+                               c->con_cols.as<int>(), c->con_w.as<double>(),
+                               c->con_inhom.as<double>(), 0, (const char *)X + es * ldx * v,
+                               (char *)Y + es * ldy * v, c->stream);
+}
+
+static void precond_multi(wb_ctx *c, int is_float, int k, const void *X, void *Y)
+{
+    if (c->cfg.preconditioner == WB_PRECOND_BLOCK_JACOBI)
+        launch_block_jacobi_apply(is_float, c->n, c->pc_bs, is_float ? c->binv32.p : c->binv.p, X,
+                                  Y, k, c->vlen, c->stream);
+    else
+        HIPCHK(hipMemcpyAsync(Y, X, (is_float ? 4 : 8) * c->vlen * k, hipMemcpyDeviceToDevice,
+                              c->stream));
+}
+
+// k GMRES instances in lockstep (same restart cycle), deal.II semantics per instance.
+// B, X: k x vlen (column stride vlen).  state[v] in/out: 0 iterate, 1 converged, -1 failed;
+// instances entering with state != 0 are left alone.  Returns per-instance iterations.
+static std::vector<int> gmres_multi(wb_ctx *c, int is_float, int k, const void *B, void *X,
+                                    double abs_tol, double rel_tol, int max_steps,
+                                    std::vector<int> &state, std::vector<double> &resid)
+{
+    auto &M = c->mw;
+    const int n = c->n;
+    const size_t vl = c->vlen;
+    const int m = std::max(1, c->cfg.gmres_max_tmp_vectors - 2);
+    const size_t es = is_float ? 4 : 8;
+    const size_t sV = (size_t)(m + 1) * vl;
+    DevBuf &V = is_float ? M.V32 : M.V;
+    if (V.bytes < es * sV * k) {
+        V.ensure(es * sV * k);
+        HIPCHK(hipMemsetAsync(V.p, 0, V.bytes, c->stream));
+    }
+    void *Z = is_float ? M.Z32.p : M.Z.p;
+    void *W = is_float ? M.W32.p : M.W.p;
+    const int mode = is_float ? 1 : (c->cfg.precision == WB_PREC_FP64 ? 0 : 2);
+    const int parts = krylov_partials_blocks(n);
+    const size_t sH = 132;
+    auto basis = [&](int i) { return (void *)((char *)V.p + es * vl * i); };
+    std::vector<std::vector<double>> gamma(k, std::vector<double>(m + 2)),
+        ci(k, std::vector<double>(m + 1)), si(k, std::vector<double>(m + 1)),
+        Hh(k, std::vector<double>((size_t)(m + 1) * m));
+    std::vector<double> h(m + 2), y(m + 1);
+    std::vector<int> its(k, 0), dim(k, 0);
+    int *act = M.apin;
+    std::vector<double> tol(k, abs_tol);
+    bool first = true;
+    auto upload_act = [&]() {
+        HIPCHK(hipStreamSynchronize(c->stream));  // previous copy out of apin has finished
+        for (int v = 0; v < k; ++v) act[v] = state[v] == 0;
+        HIPCHK(hipMemcpyAsync(M.act.p, act, sizeof(int) * k, hipMemcpyHostToDevice, c->stream));
+    };
+    for (;;) {
+        upload_act();
+        if (first) {
+            precond_multi(c, is_float, k, B, Z);
+        } else {
+            apply_constrained_multi(c, mode, k, X, vl, W, vl);
+            launch_vec_ops(is_float ? 9 : 8, (int)(vl * k), W, B, W, 0, c->stream);
+            precond_multi(c, is_float, k, W, Z);
+        }
+        launch_norm2_partials(is_float, Z, n, M.Pn.as<double>(), c->stream, k, vl,
+                              M.act.as<int>());
+        launch_finalize(is_float, 0, M.P1.as<double>(), nullptr, M.Pn.as<double>(), parts, Z, n,
+                        basis(0), M.H.as<double>(), c->stream, k, vl, sV, sH, M.act.as<int>());
+        HIPCHK(hipMemcpyAsync(M.hpin, M.H.p, sizeof(double) * sH * k, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        bool any = false;
+        for (int v = 0; v < k; ++v) {
+            if (state[v] != 0) continue;
+            const double rho = M.hpin[v * sH];
+            if (first && rel_tol > 0) tol[v] = std::max(abs_tol, rel_tol * rho);
+            resid[v] = rho;
+            if (rho <= tol[v]) state[v] = 1;
+            else if (std::isnan(rho) || its[v] >= max_steps) state[v] = -1;
+            else {
+                gamma[v][0] = rho;
+                dim[v] = 0;
+                any = true;
+            }
+        }
+        first = false;
+        if (!any) break;
+        upload_act();
+        for (int inner = 0; inner < m; ++inner) {
+            bool still = false;
+            for (int v = 0; v < k; ++v) still |= (act[v] != 0);
+            if (!still) break;
+            apply_constrained_multi(c, mode, k, basis(inner), sV, W, vl);
+            precond_multi(c, is_float, k, W, Z);
+            const int kc = inner + 1;
+            launch_dot_partials(is_float, V.p, vl, kc, Z, n, M.P1.as<double>(), c->stream, k, sV,
+                                vl, M.act.as<int>());
+            launch_update_dot(is_float, V.p, vl, kc, M.P1.as<double>(), parts, Z, n,
+                              M.P2.as<double>(), nullptr, 0, c->stream, k, sV, vl,
+                              M.act.as<int>());
+            launch_update_dot(is_float, V.p, vl, kc, M.P2.as<double>(), parts, Z, n, nullptr,
+                              M.Pn.as<double>(), 1, c->stream, k, sV, vl, M.act.as<int>());
+            launch_finalize(is_float, kc, M.P1.as<double>(), M.P2.as<double>(),
+                            M.Pn.as<double>(), parts, Z, n, basis(kc), M.H.as<double>(),
+                            c->stream, k, vl, sV, sH, M.act.as<int>());
+            HIPCHK(hipMemcpyAsync(M.hpin, M.H.p, sizeof(double) * sH * k, hipMemcpyDeviceToHost,
+                                  c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            bool changed = false;
+            for (int v = 0; v < k; ++v) {
+                if (!act[v]) continue;
+                ++its[v];
+                dim[v] = kc;
+                for (int i = 0; i <= kc; ++i) h[i] = M.hpin[v * sH + i];
+                for (int i = 0; i < inner; ++i) {
+                    const double s = si[v][i], cc = ci[v][i], d0 = h[i];
+                    h[i] = cc * d0 + s * h[i + 1];
+                    h[i + 1] = -s * d0 + cc * h[i + 1];
+                }
+                const double r = 1. / std::sqrt(h[inner] * h[inner] + h[inner + 1] * h[inner + 1]);
+                si[v][inner] = h[inner + 1] * r;
+                ci[v][inner] = h[inner] * r;
+                h[inner] = ci[v][inner] * h[inner] + si[v][inner] * h[inner + 1];
+                gamma[v][inner + 1] = -si[v][inner] * gamma[v][inner];
+                gamma[v][inner] *= ci[v][inner];
+                for (int i = 0; i < kc; ++i) Hh[v][(size_t)i * m + inner] = h[i];
+                const double rho = std::fabs(gamma[v][kc]);
+                resid[v] = rho;
+                if (rho <= tol[v]) state[v] = 1;
+                else if (std::isnan(rho) || its[v] >= max_steps) state[v] = -1;
+                if (state[v] != 0) {
+                    act[v] = 0;
+                    changed = true;
+                }
+            }
+            if (changed)
+                HIPCHK(hipMemcpyAsync(M.act.p, act, sizeof(int) * k, hipMemcpyHostToDevice,
+                                      c->stream));
+        }
+        // x_v += V_v y_v for every instance that ran this cycle (y_v staged at H + v*sH)
+        for (int v = 0; v < k; ++v) {
+            const int d = dim[v];
+            if (d == 0) continue;
+            for (int i = d - 1; i >= 0; --i) {
+                double s = gamma[v][i];
+                for (int j = i + 1; j < d; ++j) s -= Hh[v][(size_t)i * m + j] * y[j];
+                y[i] = s / Hh[v][(size_t)i * m + i];
+            }
+            std::copy(y.begin(), y.begin() + d, M.hpin + v * sH);
+        }
+        HIPCHK(hipMemcpyAsync(M.H.p, M.hpin, sizeof(double) * sH * k, hipMemcpyHostToDevice,
+                              c->stream));
+        for (int v = 0; v < k; ++v) {
+            if (dim[v] == 0) continue;
+            launch_axpy_basis(is_float, (const char *)V.p + es * sV * v, vl, dim[v],
+                              M.H.as<double>() + v * sH, (char *)X + es * vl * v, is_float, n,
+                              c->stream);
+            dim[v] = 0;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return its;
+}
+
+static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, double *dphi,
+                        const double *inhom_k, wb_solve_stats *st)
+{
+    require(!c->pure_neumann, WB_E_UNSUPPORTED,
+            "several right-hand sides with all-Neumann masks are not supported");
+    auto &M = c->mw;
+    const int n = c->n;
+    const size_t vl = c->vlen;
+    ensure_work(c);
+    const int parts = krylov_partials_blocks(n);
+    const size_t d8 = sizeof(double) * vl * k, f4 = sizeof(float) * vl * k;
+    for (DevBuf *b : {&M.Z, &M.W, &M.X, &M.B, &M.R, &M.T})
+        if (b->bytes < d8) {
+            b->ensure(d8);
+            HIPCHK(hipMemsetAsync(b->p, 0, d8, c->stream));
+        }
+    for (DevBuf *b : {&M.Z32, &M.W32, &M.D32, &M.R32})
+        if (b->bytes < f4) {
+            b->ensure(f4);
+            HIPCHK(hipMemsetAsync(b->p, 0, f4, c->stream));
+        }
+    M.P1.ensure(sizeof(double) * parts * KMAX * k);
+    M.P2.ensure(sizeof(double) * parts * KMAX * k);
+    M.Pn.ensure(sizeof(double) * parts * k);
+    M.H.ensure(sizeof(double) * 132 * k);
+    M.act.ensure(sizeof(int) * k);
+    if (M.k < k) {
+        if (M.hpin) (void)hipHostFree(M.hpin);
+        if (M.apin) (void)hipHostFree(M.apin);
+        M.hpin = nullptr;
+        M.apin = nullptr;
+        M.k = 0;
+        HIPCHK(hipHostMalloc((void **)&M.hpin, sizeof(double) * 132 * k, 0));
+        HIPCHK(hipHostMalloc((void **)&M.apin, sizeof(int) * k, 0));
+        M.k = k;
+    }
+    // per right-hand side constraint inhomogeneities (the line structure is shared)
+    const int nl = c->n_lines;
+    std::vector<double> inh((size_t)std::max(nl, 1) * k, 0.0);
+    for (int v = 0; v < k; ++v) {
+        const double *t = tmp_rhs + (size_t)v * n;
+        if (c->cons_auto) {
+            ConstraintSet cs;
+            std::string e = build_constraints(
+                n, c->plan.dn_ptr.data(), c->plan.dn_idx.data(), c->surface.data(),
+                c->cons_normals.empty() ? nullptr : c->cons_normals.data(),
+                c->cons_grads.empty() ? nullptr : c->cons_grads.data(), t, cs);
+            require(e.empty(), WB_E_UNSUPPORTED, e.c_str());
+            if (v == 0) install_constraints(c, cs);
+            require(cs.rows == c->cons.rows && cs.ptr == c->cons.ptr &&
+                        cs.cols == c->cons.cols && cs.w == c->cons.w,
+                    WB_E_STATE, "constraint structure differs between right-hand sides");
+            std::copy(cs.inhom.begin(), cs.inhom.end(), inh.begin() + (size_t)v * nl);
+        } else if (inhom_k) {
+            std::copy(inhom_k + (size_t)v * nl, inhom_k + (size_t)(v + 1) * nl,
+                      inh.begin() + (size_t)v * nl);
+        } else {
+            std::copy(c->cons.inhom.begin(), c->cons.inhom.end(), inh.begin() + (size_t)v * nl);
+        }
+    }
+    upload(M.inh, inh.data(), inh.size(), c->stream);
+    // tmp_rhs -> T (internal order), B = distributed rhs per column
+    std::vector<double> tmp;
+    for (int v = 0; v < k; ++v) {
+        to_internal(c, tmp_rhs + (size_t)v * n, tmp);
+        HIPCHK(hipMemcpyAsync((double *)M.T.p + vl * v, tmp.data(), sizeof(double) * vl,
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    const int mode64 = c->cfg.precision == WB_PREC_FP64 ? 0 : 2;
+    double t0 = now_ms();
+    {
+        KernelScope ks(c, mode64 == 0 ? WB_K_GEMV64 : WB_K_GEMVHL, (long long)c->n_free);
+        launch_gemv_sel_multi(mode64, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld,
+                              c->segs.as<int>(), c->n_segs, c->gmask.as<uint8_t>(), 1, -1.0, 1.0,
+                              k, M.T.p, vl, c->diag_rhs.as<double>(), M.T.p,
+                              c->free_rows.as<int>(), c->n_free, M.B.p, vl, c->stream);
+    }
+    for (int v = 0; v < k; ++v)
+        launch_constraint_rows(0, nl, c->con_rows.as<int>(), c->con_ptr.as<int>(),
+                               c->con_cols.as<int>(), c->con_w.as<double>(),
+                               M.inh.as<double>() + (size_t)v * nl, 1, nullptr,
+                               (double *)M.B.p + vl * v, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double t1 = now_ms();
+    ensure_precond(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double t2 = now_ms();
+    launch_vec_ops(6, (int)(vl * k), nullptr, nullptr, M.X.p, 0, c->stream);
+    std::vector<int> state(k, 0);
+    std::vector<double> resid(k, 0.0);
+    wb_solve_stats s{};
+    if (c->cfg.precision == WB_PREC_FP64) {
+        std::vector<int> its = gmres_multi(c, 0, k, M.B.p, M.X.p, c->cfg.solver_tolerance, 0.0,
+                                           c->cfg.solver_max_steps, state, resid);
+        s.iterations = *std::max_element(its.begin(), its.end());
+    } else {
+        HIPCHK(hipMemcpyAsync(M.R.p, M.B.p, d8, hipMemcpyDeviceToDevice, c->stream));
+        int sweeps = 0, total = 0;
+        std::vector<int> done(k, 0);
+        for (;;) {
+            precond_multi(c, 0, k, M.R.p, M.T.p);
+            launch_norm2_partials(0, M.T.p, n, M.Pn.as<double>(), c->stream, k, vl);
+            std::vector<double> pn((size_t)parts * k);
+            HIPCHK(hipMemcpyAsync(pn.data(), M.Pn.p, sizeof(double) * parts * k,
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            bool all = true, bad = false;
+            for (int v = 0; v < k; ++v) {
+                double q = 0.0;
+                for (int p = 0; p < parts; ++p) q += pn[(size_t)v * parts + p];
+                resid[v] = std::sqrt(q);
+                done[v] = resid[v] <= c->cfg.solver_tolerance;
+                all &= done[v] != 0;
+                bad |= std::isnan(resid[v]);
+            }
+            if (all || bad || sweeps >= c->cfg.ir_max_sweeps ||
+                total >= c->cfg.solver_max_steps)
+                break;
+            launch_vec_ops(1, (int)(vl * k), M.R.p, nullptr, M.R32.p, 0, c->stream);
+            launch_vec_ops(7, (int)(vl * k), nullptr, nullptr, M.D32.p, 0, c->stream);
+            std::vector<int> st_in(k);
+            std::vector<double> r_in(k);
+            for (int v = 0; v < k; ++v) st_in[v] = done[v] ? 1 : 0;
+            std::vector<int> its =
+                gmres_multi(c, 1, k, M.R32.p, M.D32.p, 0.0, c->cfg.ir_inner_reduction,
+                            c->cfg.solver_max_steps - total, st_in, r_in);
+            total += *std::max_element(its.begin(), its.end());
+            launch_vec_ops(2, (int)(vl * k), M.D32.p, nullptr, M.X.p, 0, c->stream);
+            apply_constrained_multi(c, 2, k, M.X.p, vl, M.W.p, vl);
+            launch_vec_ops(8, (int)(vl * k), M.W.p, M.B.p, M.R.p, 0, c->stream);
+            sweeps++;
+        }
+        s.iterations = total;
+        s.ir_sweeps = sweeps;
+        for (int v = 0; v < k; ++v) state[v] = done[v] ? 1 : -1;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double t3 = now_ms();
+    for (int v = 0; v < k; ++v) s.residual = std::max(s.residual, resid[v]);
+    // true relative residuals with the FP64-accurate operator
+    apply_constrained_multi(c, mode64, k, M.X.p, vl, M.W.p, vl);
+    launch_vec_ops(8, (int)(vl * k), M.W.p, M.B.p, M.T.p, 0, c->stream);
+    for (int v = 0; v < k; ++v) {
+        const double rn = device_norm(c, (double *)M.T.p + vl * v, 0);
+        const double bn = device_norm(c, (double *)M.B.p + vl * v, 0);
+        s.true_rel_residual = std::max(s.true_rel_residual, bn > 0 ? rn / bn : rn);
+    }
+    s.t_rhs_ms = t1 - t0;
+    s.t_precond_ms = t2 - t1;
+    s.t_gmres_ms = t3 - t2;
+    if (st) *st = s;
+    for (int v = 0; v < k; ++v)
+        require(state[v] == 1, WB_E_NOCONV, "GMRES did not converge for every right-hand side");
+    std::vector<double> xi(vl * k);
+    HIPCHK(hipMemcpyAsync(xi.data(), M.X.p, d8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int v = 0; v < k; ++v)
+        for (int d = 0; d < n; ++d) {
+            const double val = xi[vl * v + c->plan.perm[d]];
+            if (c->surface[d] == 0)
+                phi[(size_t)v * n + d] = val;
+            else
+                dphi[(size_t)v * n + d] = val;
+        }
+}
+
 // ==================================================================== C-ABI
 extern "C" {
 
@@ -703,8 +1063,11 @@ void wb_destroy(wb_ctx *c)
                       &c->Pn, &c->hbuf, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
                       &c->z32, &c->w32, &c->r32, &c->d32};
     for (DevBuf *b : bufs) b->release();
+    for (DevBuf *b : c->mw.all()) b->release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->hpin) (void)hipHostFree(c->hpin);
+    if (c->mw.hpin) (void)hipHostFree(c->mw.hpin);
+    if (c->mw.apin) (void)hipHostFree(c->mw.apin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1138,31 +1501,17 @@ wb_status wb_solve_system_multi(wb_ctx *c, int k, const double *tmp_rhs, double 
     return guarded(c, [&] {
         check_ready(c, true);
         require(k >= 1 && tmp_rhs && phi && dphi_dn, WB_E_ARG, "bad multi-rhs arguments");
-        // one solve per right-hand side (the batched MFMA form is added separately)
+        // batches of up to 16 right-hand sides share every matrix sweep
         wb_solve_stats acc{};
-        for (int j = 0; j < k; ++j) {
-            const size_t off = (size_t)j * c->n;
-            if (inhom_k && !c->cons_auto) {
-                ConstraintSet cs = c->cons;
-                for (size_t l = 0; l < cs.rows.size(); ++l) cs.inhom[l] = inhom_k[j * cs.rows.size() + l];
-                install_constraints(c, cs);
-            }
-            refresh_auto_constraints(c, tmp_rhs + off);
-            ensure_work(c);
-            upload_vec(c, c->vs, tmp_rhs + off);
+        const int nl = c->n_lines;
+        for (int j0 = 0; j0 < k; j0 += WB_MULTI_MAX) {
+            const int kb = std::min(WB_MULTI_MAX, k - j0);
+            const size_t off = (size_t)j0 * c->n;
             wb_solve_stats s1{};
-            solve_system_dev(c, c->vs.as<double>(), &s1);
-            std::vector<double> sol(c->n);
-            download_vec(c, c->vx, sol.data());
-            for (int i = 0; i < c->n; ++i) {
-                if (c->surface[i] == 0)
-                    phi[off + i] = sol[i];
-                else
-                    dphi_dn[off + i] = sol[i];
-            }
+            solve_multi(c, kb, tmp_rhs + off, phi + off, dphi_dn + off,
+                        inhom_k ? inhom_k + (size_t)j0 * nl : nullptr, &s1);
             acc.iterations += s1.iterations;
-            acc.restarts += s1.restarts;
-            acc.ir_sweeps += s1.ir_sweeps;
+            acc.ir_sweeps = std::max(acc.ir_sweeps, s1.ir_sweeps);
             acc.residual = std::max(acc.residual, s1.residual);
             acc.true_rel_residual = std::max(acc.true_rel_residual, s1.true_rel_residual);
             acc.t_rhs_ms += s1.t_rhs_ms;
@@ -1190,6 +1539,7 @@ size_t wb_device_bytes(const wb_ctx *c)
                             &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32, &c->z32, &c->w32,
                             &c->r32, &c->d32};
     for (const DevBuf *b : bufs) t += b->bytes;
+    for (DevBuf *b : const_cast<wb_ctx *>(c)->mw.all()) t += b->bytes;
     return t;
 }
 

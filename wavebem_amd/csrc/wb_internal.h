@@ -98,11 +98,13 @@ void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, co
                      size_t ld, const int *segs, int n_segs, const uint8_t *gmask, int flip,
                      double s_sel, double s_unsel, const void *x, const double *diag,
                      const void *xd, const int *rows, int n_rows, void *y, hipStream_t s);
+// k <= 16 vectors at once (column v at X + v*ldx, Xd + v*ldx, Y + v*ldy); MFMA for modes
+// 0 and 1
 void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *Glo,
-                           const void *Hlo, size_t ld, int n, const uint8_t *pairflag, int flip,
-                           double s_sel, double s_unsel, int k, const void *X, size_t ldx,
-                           const double *diag, const void *Xd, const int *rows, int n_rows,
-                           void *Y, hipStream_t s);
+                           const void *Hlo, size_t ld, const int *segs, int n_segs,
+                           const uint8_t *gmask, int flip, double s_sel, double s_unsel, int k,
+                           const void *X, size_t ldx, const double *diag, const void *Xd,
+                           const int *rows, int n_rows, void *Y, size_t ldy, hipStream_t s);
 // constrained rows: y[row] = x[row] - sum w x[col]  (mode_inhom=0)  or y[row] = inhom (1)
 void launch_constraint_rows(int is_float, int n_lines, const int *rows, const int *ptr,
                             const int *cols, const double *w, const double *inhom, int mode_inhom,
@@ -115,19 +117,27 @@ void launch_block_jacobi_build(int n, int bs, const void *G, const void *H, cons
 void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, const void *x,
                                void *y, int k, size_t ldx, hipStream_t s);
 
-// Krylov helpers (is_float selects float vectors)
+// Krylov helpers (is_float selects float vectors).  batch instances on blockIdx.y with
+// element strides sV (basis), sZ / sX (vectors), sY / sH (coefficients); partial buffers
+// are per instance of nparts*KMAX (dots) or nparts (norms); active[b] == 0 skips instance b.
+constexpr int KMAX = 128;  // max vectors in one orthogonalization
 int krylov_partials_blocks(int n);
 void launch_dot_partials(int is_float, const void *V, size_t ldv, int kcount, const void *z,
-                         int n, double *P, hipStream_t s);
+                         int n, double *P, hipStream_t s, int batch = 1, size_t sV = 0,
+                         size_t sZ = 0, const int *active = nullptr);
 void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, const double *Pin,
-                       int nparts, void *z, int n, double *Pout, double *hacc, int want_norm,
-                       hipStream_t s);
+                       int nparts, void *z, int n, double *Pout, double *normp, int want_norm,
+                       hipStream_t s, int batch = 1, size_t sV = 0, size_t sZ = 0,
+                       const int *active = nullptr);
 void launch_finalize(int is_float, int kcount, const double *P1, const double *P2,
                      const double *Pn, int nparts, void *z, int n, void *vout, double *hout,
-                     hipStream_t s);
+                     hipStream_t s, int batch = 1, size_t sZ = 0, size_t sV = 0, size_t sH = 0,
+                     const int *active = nullptr);
 void launch_axpy_basis(int is_float, const void *V, size_t ldv, int kcount, const double *y,
-                       void *x_acc, int x_is_float, int n, hipStream_t s);
-void launch_norm2_partials(int is_float, const void *x, int n, double *P, hipStream_t s);
+                       void *x_acc, int x_is_float, int n, hipStream_t s, int batch = 1,
+                       size_t sV = 0, size_t sY = 0, size_t sX = 0, const int *active = nullptr);
+void launch_norm2_partials(int is_float, const void *x, int n, double *P, hipStream_t s,
+                           int batch = 1, size_t sX = 0, const int *active = nullptr);
 void launch_vec_ops(int op, int n, const void *a, const void *b, void *c, double alpha,
                     hipStream_t s);
 void launch_permute(int n, const int *idx, const double *src, double *dst, int gather,

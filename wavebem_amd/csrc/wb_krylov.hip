@@ -1,0 +1,290 @@
+// wb_krylov.hip -- Arnoldi building blocks for one or several GMRES instances.
+//
+// Vectors are split into chunks of KCHUNK entries, one workgroup per chunk; reductions
+// produce one partial per chunk and are finished, in chunk order, by the consumer, so every
+// result is deterministic.  blockIdx.y selects the instance (right-hand side) of a batch;
+// each instance has its own Krylov basis, work vector and partials at the given strides, and
+// an optional activity flag lets converged instances sit out.
+//
+// Orthogonalization is classical Gram-Schmidt with one re-orthogonalization pass (CGS2):
+//   h1 = V^T z, z -= V h1, h2 = V^T z, z -= V h2, h = h1 + h2, s = |z|, v_new = z / s
+// (dot_partials, update_dot x2, finalize); it spans the same Krylov space as the modified
+// Gram-Schmidt of deal.II's SolverGMRES with two global reductions per step.
+#include "wb_internal.h"
+
+namespace wb {
+
+namespace {
+
+constexpr int KTHREADS = 256;
+constexpr int KPER = 4;
+constexpr int KCHUNK = KTHREADS * KPER;
+
+__device__ __forceinline__ double ksum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename T>
+__device__ void chunk_dots(const T *__restrict__ V, size_t ldv, int kcount,
+                           const T *__restrict__ zs /*LDS chunk*/, int base, int m,
+                           double *__restrict__ P, int part)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = wave; k < kcount; k += KTHREADS / 64) {
+        const T *vk = V + (size_t)k * ldv + base;
+        double s = 0.0;
+        for (int e = lane; e < m; e += 64) s += (double)vk[e] * (double)zs[e];
+        s = ksum(s);
+        if (lane == 0) P[(size_t)part * KMAX + k] = s;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(KTHREADS) void k_dot_partials(const T *__restrict__ V, size_t ldv,
+                                                           size_t sV, int kcount,
+                                                           const T *__restrict__ z, size_t sZ,
+                                                           int n, double *__restrict__ P,
+                                                           const int *__restrict__ active)
+{
+    __shared__ T zs[KCHUNK];
+    const int b = blockIdx.y;
+    if (active && !active[b]) return;
+    V += b * sV;
+    z += b * sZ;
+    P += (size_t)b * gridDim.x * KMAX;
+    const int base = blockIdx.x * KCHUNK;
+    const int m = min(KCHUNK, n - base);
+    for (int e = threadIdx.x; e < KCHUNK; e += KTHREADS) zs[e] = e < m ? z[base + e] : (T)0;
+    __syncthreads();
+    chunk_dots(V, ldv, kcount, zs, base, m, P, blockIdx.x);
+}
+
+// h = sum_parts Pin;  z -= V h;  then optional partial dots (Pout) and partial |z|^2
+template <typename T>
+__global__ __launch_bounds__(KTHREADS) void k_update_dot(const T *__restrict__ V, size_t ldv,
+                                                         size_t sV, int kcount,
+                                                         const double *__restrict__ Pin,
+                                                         T *__restrict__ z, size_t sZ, int n,
+                                                         double *__restrict__ Pout,
+                                                         double *__restrict__ normp,
+                                                         int want_norm,
+                                                         const int *__restrict__ active)
+{
+    __shared__ double h[KMAX];
+    __shared__ T zs[KCHUNK];
+    __shared__ double red[KTHREADS / 64];
+    const int b = blockIdx.y;
+    if (active && !active[b]) return;
+    const int nparts = gridDim.x;
+    V += b * sV;
+    z += b * sZ;
+    Pin += (size_t)b * nparts * KMAX;
+    if (Pout) Pout += (size_t)b * nparts * KMAX;
+    if (normp) normp += (size_t)b * nparts;
+    const int base = blockIdx.x * KCHUNK;
+    const int m = min(KCHUNK, n - base);
+    for (int k = threadIdx.x; k < kcount; k += KTHREADS) {
+        double s = 0.0;
+        for (int p = 0; p < nparts; ++p) s += Pin[(size_t)p * KMAX + k];
+        h[k] = s;
+    }
+    __syncthreads();
+    double nrm = 0.0;
+    for (int e = threadIdx.x; e < KCHUNK; e += KTHREADS) {
+        T v = (T)0;
+        if (e < m) {
+            double a = (double)z[base + e];
+            for (int k = 0; k < kcount; ++k) a -= h[k] * (double)V[(size_t)k * ldv + base + e];
+            v = (T)a;
+            z[base + e] = v;
+            nrm += (double)v * (double)v;
+        }
+        zs[e] = v;
+    }
+    __syncthreads();
+    if (Pout) chunk_dots(V, ldv, kcount, zs, base, m, Pout, blockIdx.x);
+    if (want_norm) {
+        nrm = ksum(nrm);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int w = 0; w < KTHREADS / 64; ++w) s += red[w];
+            normp[blockIdx.x] = s;
+        }
+    }
+}
+
+// hout[k] = sum P1 + sum P2 (k < kcount), hout[kcount] = sqrt(sum Pn); vout = z / hout[kcount]
+template <typename T>
+__global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double *__restrict__ P1,
+                                                       const double *__restrict__ P2,
+                                                       const double *__restrict__ Pn,
+                                                       const T *__restrict__ z, size_t sZ,
+                                                       int n, T *__restrict__ vout, size_t sV,
+                                                       double *__restrict__ hout, size_t sH,
+                                                       const int *__restrict__ active)
+{
+    __shared__ double snorm;
+    const int b = blockIdx.y;
+    if (active && !active[b]) return;
+    const int nparts = gridDim.x;
+    P1 += (size_t)b * nparts * KMAX;
+    if (P2) P2 += (size_t)b * nparts * KMAX;
+    Pn += (size_t)b * nparts;
+    z += b * sZ;
+    if (vout) vout += b * sV;
+    hout += b * sH;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int p = 0; p < nparts; ++p) s += Pn[p];
+        snorm = sqrt(s);
+    }
+    __syncthreads();
+    const double nv = snorm;
+    if (blockIdx.x == 0) {
+        for (int k = threadIdx.x; k < kcount; k += KTHREADS) {
+            double s = 0.0;
+            for (int p = 0; p < nparts; ++p) s += P1[(size_t)p * KMAX + k];
+            if (P2)
+                for (int p = 0; p < nparts; ++p) s += P2[(size_t)p * KMAX + k];
+            hout[k] = s;
+        }
+        if (threadIdx.x == 0) hout[kcount] = nv;
+    }
+    if (vout) {
+        const double inv = nv > 0 ? 1.0 / nv : 0.0;
+        const int base = blockIdx.x * KCHUNK;
+        const int m = min(KCHUNK, n - base);
+        for (int e = threadIdx.x; e < m; e += KTHREADS) vout[base + e] = (T)((double)z[base + e] * inv);
+    }
+}
+
+// x += sum_i y_i v_i   (x may be double while V is float)
+template <typename T, typename TX>
+__global__ void k_axpy_basis(const T *__restrict__ V, size_t ldv, size_t sV, int kcount,
+                             const double *__restrict__ y, size_t sY, TX *__restrict__ x,
+                             size_t sX, int n, const int *__restrict__ active)
+{
+    __shared__ double ys[KMAX];
+    const int b = blockIdx.y;
+    if (active && !active[b]) return;
+    V += b * sV;
+    y += b * sY;
+    x += b * sX;
+    for (int k = threadIdx.x; k < kcount; k += blockDim.x) ys[k] = y[k];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = (double)x[i];
+    for (int k = 0; k < kcount; ++k) a += ys[k] * (double)V[(size_t)k * ldv + i];
+    x[i] = (TX)a;
+}
+
+template <typename T>
+__global__ __launch_bounds__(KTHREADS) void k_norm2_partials(const T *__restrict__ x, size_t sX,
+                                                             int n, double *__restrict__ P,
+                                                             const int *__restrict__ active)
+{
+    __shared__ double red[KTHREADS / 64];
+    const int b = blockIdx.y;
+    if (active && !active[b]) return;
+    x += b * sX;
+    P += (size_t)b * gridDim.x;
+    const int base = blockIdx.x * KCHUNK;
+    const int m = min(KCHUNK, n - base);
+    double s = 0.0;
+    for (int e = threadIdx.x; e < m; e += KTHREADS) {
+        const double v = (double)x[base + e];
+        s += v * v;
+    }
+    s = ksum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < KTHREADS / 64; ++w) t += red[w];
+        P[blockIdx.x] = t;
+    }
+}
+
+}  // namespace
+
+int krylov_partials_blocks(int n) { return (n + KCHUNK - 1) / KCHUNK; }
+
+void launch_dot_partials(int is_float, const void *V, size_t ldv, int kcount, const void *z,
+                         int n, double *P, hipStream_t s, int batch, size_t sV, size_t sZ,
+                         const int *active)
+{
+    dim3 grid(krylov_partials_blocks(n), batch);
+    if (is_float)
+        hipLaunchKernelGGL(k_dot_partials<float>, grid, dim3(KTHREADS), 0, s, (const float *)V,
+                           ldv, sV, kcount, (const float *)z, sZ, n, P, active);
+    else
+        hipLaunchKernelGGL(k_dot_partials<double>, grid, dim3(KTHREADS), 0, s, (const double *)V,
+                           ldv, sV, kcount, (const double *)z, sZ, n, P, active);
+}
+
+void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, const double *Pin,
+                       int nparts, void *z, int n, double *Pout, double *normp, int want_norm,
+                       hipStream_t s, int batch, size_t sV, size_t sZ, const int *active)
+{
+    (void)nparts;
+    dim3 grid(krylov_partials_blocks(n), batch);
+    if (is_float)
+        hipLaunchKernelGGL(k_update_dot<float>, grid, dim3(KTHREADS), 0, s, (const float *)V,
+                           ldv, sV, kcount, Pin, (float *)z, sZ, n, Pout, normp, want_norm,
+                           active);
+    else
+        hipLaunchKernelGGL(k_update_dot<double>, grid, dim3(KTHREADS), 0, s, (const double *)V,
+                           ldv, sV, kcount, Pin, (double *)z, sZ, n, Pout, normp, want_norm,
+                           active);
+}
+
+void launch_finalize(int is_float, int kcount, const double *P1, const double *P2,
+                     const double *Pn, int nparts, void *z, int n, void *vout, double *hout,
+                     hipStream_t s, int batch, size_t sZ, size_t sV, size_t sH,
+                     const int *active)
+{
+    (void)nparts;
+    dim3 grid(krylov_partials_blocks(n), batch);
+    if (is_float)
+        hipLaunchKernelGGL(k_finalize<float>, grid, dim3(KTHREADS), 0, s, kcount, P1, P2, Pn,
+                           (const float *)z, sZ, n, (float *)vout, sV, hout, sH, active);
+    else
+        hipLaunchKernelGGL(k_finalize<double>, grid, dim3(KTHREADS), 0, s, kcount, P1, P2, Pn,
+                           (const double *)z, sZ, n, (double *)vout, sV, hout, sH, active);
+}
+
+void launch_axpy_basis(int is_float, const void *V, size_t ldv, int kcount, const double *y,
+                       void *x, int x_is_float, int n, hipStream_t s, int batch, size_t sV,
+                       size_t sY, size_t sX, const int *active)
+{
+    dim3 grid((n + 255) / 256, batch);
+    if (is_float && x_is_float)
+        hipLaunchKernelGGL((k_axpy_basis<float, float>), grid, dim3(256), 0, s, (const float *)V,
+                           ldv, sV, kcount, y, sY, (float *)x, sX, n, active);
+    else if (is_float)
+        hipLaunchKernelGGL((k_axpy_basis<float, double>), grid, dim3(256), 0, s,
+                           (const float *)V, ldv, sV, kcount, y, sY, (double *)x, sX, n, active);
+    else
+        hipLaunchKernelGGL((k_axpy_basis<double, double>), grid, dim3(256), 0, s,
+                           (const double *)V, ldv, sV, kcount, y, sY, (double *)x, sX, n, active);
+}
+
+void launch_norm2_partials(int is_float, const void *x, int n, double *P, hipStream_t s,
+                           int batch, size_t sX, const int *active)
+{
+    dim3 grid(krylov_partials_blocks(n), batch);
+    if (is_float)
+        hipLaunchKernelGGL(k_norm2_partials<float>, grid, dim3(KTHREADS), 0, s, (const float *)x,
+                           sX, n, P, active);
+    else
+        hipLaunchKernelGGL(k_norm2_partials<double>, grid, dim3(KTHREADS), 0, s,
+                           (const double *)x, sX, n, P, active);
+}
+
+}  // namespace wb

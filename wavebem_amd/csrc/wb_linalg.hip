@@ -240,228 +240,6 @@ void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, co
                            (const double *)x, (double *)y, k, ldx);
 }
 
-// ------------------------------------------------------------------ Krylov helpers
-// Vectors are split into chunks of KCHUNK entries, one workgroup per chunk; reductions
-// produce one partial per chunk and are finished, in chunk order, by the consumer, so
-// every result is deterministic.
-constexpr int KTHREADS = 256;
-constexpr int KPER = 4;
-constexpr int KCHUNK = KTHREADS * KPER;
-constexpr int KMAX = 128;  // max vectors in one orthogonalization
-
-int krylov_partials_blocks(int n) { return (n + KCHUNK - 1) / KCHUNK; }
-
-template <typename T>
-__device__ void chunk_dots(const T *__restrict__ V, size_t ldv, int kcount,
-                           const T *__restrict__ zs /*LDS chunk*/, int base, int m,
-                           double *__restrict__ P, int part)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int k = wave; k < kcount; k += KTHREADS / 64) {
-        const T *vk = V + (size_t)k * ldv + base;
-        double s = 0.0;
-        for (int e = lane; e < m; e += 64) s += (double)vk[e] * (double)zs[e];
-        s = wave_sum(s);
-        if (lane == 0) P[(size_t)part * KMAX + k] = s;
-    }
-}
-
-template <typename T>
-__global__ __launch_bounds__(KTHREADS) void k_dot_partials(const T *__restrict__ V, size_t ldv,
-                                                           int kcount, const T *__restrict__ z,
-                                                           int n, double *__restrict__ P)
-{
-    __shared__ T zs[KCHUNK];
-    const int base = blockIdx.x * KCHUNK;
-    const int m = min(KCHUNK, n - base);
-    for (int e = threadIdx.x; e < KCHUNK; e += KTHREADS) zs[e] = e < m ? z[base + e] : (T)0;
-    __syncthreads();
-    chunk_dots(V, ldv, kcount, zs, base, m, P, blockIdx.x);
-}
-
-void launch_dot_partials(int is_float, const void *V, size_t ldv, int kcount, const void *z,
-                         int n, double *P, hipStream_t s)
-{
-    dim3 grid(krylov_partials_blocks(n));
-    if (is_float)
-        hipLaunchKernelGGL(k_dot_partials<float>, grid, dim3(KTHREADS), 0, s, (const float *)V,
-                           ldv, kcount, (const float *)z, n, P);
-    else
-        hipLaunchKernelGGL(k_dot_partials<double>, grid, dim3(KTHREADS), 0, s, (const double *)V,
-                           ldv, kcount, (const double *)z, n, P);
-}
-
-// h = sum_parts Pin;  z -= V h;  then optional partial dots (Pout) and partial |z|^2
-// (stored at Pout[part*KMAX + KMAX-1] when want_norm and Pout has no dots, else in
-// hacc[part]).
-template <typename T>
-__global__ __launch_bounds__(KTHREADS) void k_update_dot(const T *__restrict__ V, size_t ldv,
-                                                         int kcount, const double *__restrict__ Pin,
-                                                         int nparts, T *__restrict__ z, int n,
-                                                         double *__restrict__ Pout,
-                                                         double *__restrict__ normp, int want_norm)
-{
-    __shared__ double h[KMAX];
-    __shared__ T zs[KCHUNK];
-    __shared__ double red[KTHREADS / 64];
-    const int base = blockIdx.x * KCHUNK;
-    const int m = min(KCHUNK, n - base);
-    for (int k = threadIdx.x; k < kcount; k += KTHREADS) {
-        double s = 0.0;
-        for (int p = 0; p < nparts; ++p) s += Pin[(size_t)p * KMAX + k];
-        h[k] = s;
-    }
-    __syncthreads();
-    double nrm = 0.0;
-    for (int e = threadIdx.x; e < KCHUNK; e += KTHREADS) {
-        T v = (T)0;
-        if (e < m) {
-            double a = (double)z[base + e];
-            for (int k = 0; k < kcount; ++k) a -= h[k] * (double)V[(size_t)k * ldv + base + e];
-            v = (T)a;
-            z[base + e] = v;
-            nrm += (double)v * (double)v;
-        }
-        zs[e] = v;
-    }
-    __syncthreads();
-    if (Pout) chunk_dots(V, ldv, kcount, zs, base, m, Pout, blockIdx.x);
-    if (want_norm) {
-        nrm = wave_sum(nrm);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int w = 0; w < KTHREADS / 64; ++w) s += red[w];
-            normp[blockIdx.x] = s;
-        }
-    }
-}
-
-void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, const double *Pin,
-                       int nparts, void *z, int n, double *Pout, double *normp, int want_norm,
-                       hipStream_t s)
-{
-    dim3 grid(krylov_partials_blocks(n));
-    if (is_float)
-        hipLaunchKernelGGL(k_update_dot<float>, grid, dim3(KTHREADS), 0, s, (const float *)V,
-                           ldv, kcount, Pin, nparts, (float *)z, n, Pout, normp, want_norm);
-    else
-        hipLaunchKernelGGL(k_update_dot<double>, grid, dim3(KTHREADS), 0, s, (const double *)V,
-                           ldv, kcount, Pin, nparts, (double *)z, n, Pout, normp, want_norm);
-}
-
-// hout[k] = sum P1 + sum P2 (k < kcount), hout[kcount] = sqrt(sum Pn); vout = z / hout[kcount]
-template <typename T>
-__global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double *__restrict__ P1,
-                                                       const double *__restrict__ P2,
-                                                       const double *__restrict__ Pn, int nparts,
-                                                       const T *__restrict__ z, int n,
-                                                       T *__restrict__ vout,
-                                                       double *__restrict__ hout)
-{
-    __shared__ double snorm;
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int p = 0; p < nparts; ++p) s += Pn[p];
-        snorm = sqrt(s);
-    }
-    __syncthreads();
-    const double nv = snorm;
-    if (blockIdx.x == 0) {
-        for (int k = threadIdx.x; k < kcount; k += KTHREADS) {
-            double s = 0.0;
-            for (int p = 0; p < nparts; ++p) s += P1[(size_t)p * KMAX + k];
-            if (P2)
-                for (int p = 0; p < nparts; ++p) s += P2[(size_t)p * KMAX + k];
-            hout[k] = s;
-        }
-        if (threadIdx.x == 0) hout[kcount] = nv;
-    }
-    if (vout) {
-        const double inv = nv > 0 ? 1.0 / nv : 0.0;
-        const int base = blockIdx.x * KCHUNK;
-        const int m = min(KCHUNK, n - base);
-        for (int e = threadIdx.x; e < m; e += KTHREADS) vout[base + e] = (T)((double)z[base + e] * inv);
-    }
-}
-
-void launch_finalize(int is_float, int kcount, const double *P1, const double *P2,
-                     const double *Pn, int nparts, void *z, int n, void *vout, double *hout,
-                     hipStream_t s)
-{
-    dim3 grid(krylov_partials_blocks(n));
-    if (is_float)
-        hipLaunchKernelGGL(k_finalize<float>, grid, dim3(KTHREADS), 0, s, kcount, P1, P2, Pn,
-                           nparts, (const float *)z, n, (float *)vout, hout);
-    else
-        hipLaunchKernelGGL(k_finalize<double>, grid, dim3(KTHREADS), 0, s, kcount, P1, P2, Pn,
-                           nparts, (const double *)z, n, (double *)vout, hout);
-}
-
-// x += sum_i y_i v_i   (x may be double while V is float)
-template <typename T, typename TX>
-__global__ void k_axpy_basis(const T *__restrict__ V, size_t ldv, int kcount,
-                             const double *__restrict__ y, TX *__restrict__ x, int n)
-{
-    __shared__ double ys[KMAX];
-    for (int k = threadIdx.x; k < kcount; k += blockDim.x) ys[k] = y[k];
-    __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double a = (double)x[i];
-    for (int k = 0; k < kcount; ++k) a += ys[k] * (double)V[(size_t)k * ldv + i];
-    x[i] = (TX)a;
-}
-
-void launch_axpy_basis(int is_float, const void *V, size_t ldv, int kcount, const double *y,
-                       void *x, int x_is_float, int n, hipStream_t s)
-{
-    dim3 grid((n + 255) / 256);
-    if (is_float && x_is_float)
-        hipLaunchKernelGGL((k_axpy_basis<float, float>), grid, dim3(256), 0, s, (const float *)V,
-                           ldv, kcount, y, (float *)x, n);
-    else if (is_float)
-        hipLaunchKernelGGL((k_axpy_basis<float, double>), grid, dim3(256), 0, s, (const float *)V,
-                           ldv, kcount, y, (double *)x, n);
-    else
-        hipLaunchKernelGGL((k_axpy_basis<double, double>), grid, dim3(256), 0, s,
-                           (const double *)V, ldv, kcount, y, (double *)x, n);
-}
-
-template <typename T>
-__global__ __launch_bounds__(KTHREADS) void k_norm2_partials(const T *__restrict__ x, int n,
-                                                             double *__restrict__ P)
-{
-    __shared__ double red[KTHREADS / 64];
-    const int base = blockIdx.x * KCHUNK;
-    const int m = min(KCHUNK, n - base);
-    double s = 0.0;
-    for (int e = threadIdx.x; e < m; e += KTHREADS) {
-        const double v = (double)x[base + e];
-        s += v * v;
-    }
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < KTHREADS / 64; ++w) t += red[w];
-        P[blockIdx.x] = t;
-    }
-}
-
-void launch_norm2_partials(int is_float, const void *x, int n, double *P, hipStream_t s)
-{
-    dim3 grid(krylov_partials_blocks(n));
-    if (is_float)
-        hipLaunchKernelGGL(k_norm2_partials<float>, grid, dim3(KTHREADS), 0, s, (const float *)x,
-                           n, P);
-    else
-        hipLaunchKernelGGL(k_norm2_partials<double>, grid, dim3(KTHREADS), 0, s,
-                           (const double *)x, n, P);
-}
-
 // elementwise ops on n entries:
 //  0: c(d) = a(d) - b(d)          1: c(f) = (float) a(d)          2: c(d) += a(f)
 //  3: c(d) = a(d) * alpha         4: c(d) = a(d) * b(d)            5: c(f) = a(f) * alpha
