@@ -121,7 +121,7 @@ def main():
     ap.add_argument("--only", choices=["fp64", "fp32_ir"], default=None)
     ap.add_argument("--ir-inner", type=float, default=1e-6,
                     help="FP32-IR inner GMRES relative residual reduction per sweep")
-    ap.add_argument("--tile", type=int, default=16, choices=[16, 32])
+    ap.add_argument("--tile", type=int, default=16, help="max columns per assembly tile (1..16)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "

@@ -60,8 +60,9 @@ typedef struct {
     int preconditioner;            /* wb_preconditioner */
     int precond_block;             /* block size for block Jacobi (<= 128) */
     int ir_max_sweeps;             /* FP32_IR: outer refinement sweeps */
-    double ir_inner_reduction;     /* FP32_IR: inner relative residual reduction */
-    int assembly_tile;             /* columns per register tile of the assembly (16|32) */
+    double ir_inner_reduction;     /* FP32_IR: an inner FP32 solve stops at
+                                      max(this * rho, tol / 2), rho = outer residual */
+    int assembly_tile;             /* max columns per assembly column tile (1..16) */
 } wb_config;
 
 typedef struct {
@@ -109,6 +110,12 @@ wb_status wb_assemble(wb_ctx *ctx);
 /* Copies G and H back (row-major, n x n, caller dof order), for parity checks; either
  * pointer may be NULL. */
 wb_status wb_get_matrices(wb_ctx *ctx, double *G, double *H);
+
+/* The same for n_rows selected rows (caller numbering): G and H are n_rows x n_dofs,
+ * row r holding matrix row rows[r].  For spot checks of large assemblies, where the
+ * full matrices do not fit in host memory. */
+wb_status wb_get_matrix_rows(wb_ctx *ctx, int n_rows, const int32_t *rows, double *G,
+                             double *H);
 
 /* BEMProblem::compute_alpha (source/bem_problem.cc:594-618): alpha = H * (-1). */
 wb_status wb_compute_alpha(wb_ctx *ctx, double *alpha);

@@ -88,6 +88,7 @@ _SIGS = {
     "wb_set_masks": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_assemble": (ctypes.c_int, [_vp]),
     "wb_get_matrices": (ctypes.c_int, [_vp, _dp, _dp]),
+    "wb_get_matrix_rows": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _dp, _dp]),
     "wb_compute_alpha": (ctypes.c_int, [_vp, _dp]),
     "wb_vmult": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_compute_rhs": (ctypes.c_int, [_vp, _dp, _dp]),

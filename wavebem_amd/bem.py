@@ -161,6 +161,15 @@ class BEMProblem:
         self._check(L.lib().wb_get_matrices(self._ctx, _d(G), _d(H)), "wb_get_matrices")
         return G, H
 
+    def matrix_rows(self, rows):
+        """(G[rows], H[rows]) for a list of caller-numbered rows, float64."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        G = np.empty((r.size, self.n))
+        H = np.empty((r.size, self.n))
+        self._check(L.lib().wb_get_matrix_rows(self._ctx, int(r.size), _i(r), _d(G), _d(H)),
+                    "wb_get_matrix_rows")
+        return G, H
+
     @property
     def dirichlet_matrix(self):
         return self.matrices()[0]

@@ -26,6 +26,9 @@ FLAGS = [
     "-shared",
     # keep the compiler from forming packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # let both 16-double register tiles of the assembly kernel (G and H, 256 bytes) live
+    # in VGPRs; the default budget keeps only one and sends the other to scratch
+    "-mllvm", "-amdgpu-promote-alloca-to-vector-limit=1024",
     "-Wall",
     "-Wno-unused-result",
     "-Wno-unused-variable",

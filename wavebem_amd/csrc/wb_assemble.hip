@@ -8,10 +8,10 @@
 // (double-node membership test, source/bem_problem.cc:214-230).
 //
 // Regular cells: one wavefront owns 64 collocation rows (lane = row) and a column tile
-// of TILE_COLS consecutive device-ordered dofs; it sweeps the cells touching the tile.
-// Cell quadrature data is wave-uniform and arrives through scalar loads; each lane keeps
-// the 4+4 local G/H values of the current cell in registers and adds them into an LDS
-// accumulator [column][row]; the finished tile is written once, coalesced, row by row.
+// of up to TILE_COLS consecutive device-ordered dofs; it sweeps the cells touching the
+// tile.  Cell quadrature data is wave-uniform and arrives through scalar loads; each lane
+// adds the 4+4 local G/H values of the current cell into its row's 16-wide register tiles;
+// the finished tile goes out through LDS as whole row segments, written once.
 // Singular cells: a second kernel adds their 2 s^2-point contributions per row.
 #include "wb_internal.h"
 
@@ -48,7 +48,8 @@ __device__ __forceinline__ double join_hilo(float hi, uint16_t lo)
 
 // ------------------------------------------------------------------ cell geometry
 // deal.II FEValues<2,3> with a Q1 mapping: x = sum phi_k v_k, a = dx/dxi, b = dx/deta,
-// JxW = sqrt(det(J^T J)) w, n = orient (a x b)/|a x b|.  Record: y(3), n*JxW(3), JxW, 0.
+// JxW = sqrt(det(J^T J)) w, n = orient (a x b)/|a x b|.
+// Record: y(3), n*JxW(3), JxW, xi*JxW (xi = first reference coordinate of the point).
 __global__ void k_cell_geometry(const double *__restrict__ X, const int *__restrict__ cell_int,
                                 const int8_t *__restrict__ orient, int nc, int nq,
                                 const double *__restrict__ qpts, const double *__restrict__ qw,
@@ -86,7 +87,7 @@ __global__ void k_cell_geometry(const double *__restrict__ X, const int *__restr
     r[4] = nn[1] * s * jxw;
     r[5] = nn[2] * s * jxw;
     r[6] = jxw;
-    r[7] = 0.0;
+    r[7] = xi * jxw;
 }
 
 void launch_cell_geometry(const double *X, const int *cell_int, const int8_t *orient, int nc,
@@ -122,8 +123,7 @@ void k_assemble_tiles(
     uint16_t *__restrict__ Hlo, double *__restrict__ hsum_part)
 {
     constexpr int NQ = Q * Q;
-    static_assert(TC == 16 || TC == 32, "tile width");
-    // 16-wide register vectors (wider ones are lowered to scratch by the compiler)
+    static_assert(TC == 16, "tile width");
     typedef double accv __attribute__((ext_vector_type(16)));
     constexpr int STAGE_LD = TC + 1;
     __shared__ double stage[TILE_ROWS * STAGE_LD];
@@ -138,16 +138,11 @@ void k_assemble_tiles(
     const int t_begin = tg * TILES_PER_WG;
     const int t_end = min(n_tiles, t_begin + TILES_PER_WG);
     for (int tile = t_begin; tile < t_end; ++tile) {
-        accv aG0 = 0.0, aH0 = 0.0, aG1 = 0.0, aH1 = 0.0;
-        // c is wave-uniform: the branch is scalar and the vector index an s_set_gpr_idx
+        accv aG = 0.0, aH = 0.0;
+        // c is wave-uniform: the vector index becomes an s_set_gpr_idx register move
         auto upd = [&](int c, double g, double h) {
-            if (TC == 16 || c < 16) {
-                aG0[c & 15] += g;
-                aH0[c & 15] += h;
-            } else {
-                aG1[c & 15] += g;
-                aH1[c & 15] += h;
-            }
+            aG[c & 15] += g;
+            aH[c & 15] += h;
         };
         double hs = 0.0;
         const int p0 = tile_ptr[tile], p1 = tile_ptr[tile + 1];
@@ -166,12 +161,11 @@ void k_assemble_tiles(
                     const double R0 = r[0] - x0, R1 = r[1] - x1, R2 = r[2] - x2;
                     const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
                     const double ri = rsqrt_f64(r2);
-                    const double tgv = ri * r[6];
+                    sg = __builtin_fma(ri, r[6], sg);   // sum JxW / r
+                    sg1 = __builtin_fma(ri, r[7], sg1); // sum xi JxW / r
                     const double dn = __builtin_fma(R0, r[3], __builtin_fma(R1, r[4], R2 * r[5]));
-                    const double thv = dn * (ri * ri * ri);
-                    sg += tgv;
+                    const double thv = (dn * ri) * (ri * ri);
                     sh += thv;
-                    sg1 = __builtin_fma(nodes.phi[i], tgv, sg1);
                     sh1 = __builtin_fma(nodes.phi[i], thv, sh1);
                 }
                 const double sg0 = sg - sg1, sh0 = sh - sh1;  // sum (1 - xi_i) t
@@ -221,8 +215,7 @@ void k_assemble_tiles(
             __syncthreads();
 #pragma unroll
             for (int c = 0; c < TC; ++c) {
-                const double v = m == 0 ? (c < 16 ? aG0[c & 15] : aG1[c & 15])
-                                        : (c < 16 ? aH0[c & 15] : aH1[c & 15]);
+                const double v = m == 0 ? aG[c] : aH[c];
                 stage[lane * STAGE_LD + c] = v * scale;
             }
             __syncthreads();
@@ -279,12 +272,12 @@ static void launch_tiles_q(int precision, int tc, const double *xrow, const int 
                            size_t ld, const PhiTable &nodes, void *G, void *H, void *Glo,
                            void *Hlo, double *hsum_part, hipStream_t s)
 {
-    if (tc == 16)
-        launch_tiles_qt<Q, 16>(precision, xrow, row_cls, cellq, cell_cls, tile_ptr, tile_cell,
-                               tile_cols, tile_start, n, n_tiles, ld, nodes, G, H, Glo, Hlo, hsum_part, s);
-    else
-        launch_tiles_qt<Q, 32>(precision, xrow, row_cls, cellq, cell_cls, tile_ptr, tile_cell,
-                               tile_cols, tile_start, n, n_tiles, ld, nodes, G, H, Glo, Hlo, hsum_part, s);
+    // tiles of up to 16 columns share the 16-wide register tile (32 columns do not fit
+    // the register budget of three waves per SIMD)
+    (void)tc;
+    launch_tiles_qt<Q, TILE_COLS>(precision, xrow, row_cls, cellq, cell_cls, tile_ptr, tile_cell,
+                                  tile_cols, tile_start, n, n_tiles, ld, nodes, G, H, Glo, Hlo,
+                                  hsum_part, s);
 }
 
 // nodes.phi[0..q_order-1] = 1-D Gauss nodes on [0,1] of the regular rule

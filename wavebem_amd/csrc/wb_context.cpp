@@ -509,6 +509,13 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
 }
 
 // ------------------------------------------------------------------ solve_system
+// Absolute tolerance of one inner FP32 solve of the refinement loop, given the current
+// preconditioned outer residual rho.
+static double ir_inner_target(const wb_ctx *c, double rho)
+{
+    return std::max(c->cfg.ir_inner_reduction * rho, 0.5 * c->cfg.solver_tolerance);
+}
+
 // b (device, internal order, vlen) must already hold the distributed right-hand side;
 // the solution lands in c->vx.
 static void run_solver(wb_ctx *c, wb_solve_stats *st)
@@ -558,10 +565,12 @@ static void run_solver(wb_ctx *c, wb_solve_stats *st)
             if (std::isnan(rho) || sweeps >= c->cfg.ir_max_sweeps ||
                 total_inner >= c->cfg.solver_max_steps)
                 break;
-            // inner: A32 d = r, from d = 0, relative reduction ir_inner_reduction
+            // inner: A32 d = r from d = 0, down to the larger of ir_inner_reduction * rho and
+            // what the outer tolerance still asks for (half of it, leaving room for the
+            // FP32 operator's own error in the update)
             launch_vec_ops(1, n, c->vr.p, nullptr, c->r32.p, 0, c->stream);
             launch_vec_ops(7, n, nullptr, nullptr, c->d32.p, 0, c->stream);
-            GmresResult R = gmres(c, 1, c->r32.p, c->d32.p, 0.0, c->cfg.ir_inner_reduction,
+            GmresResult R = gmres(c, 1, c->r32.p, c->d32.p, ir_inner_target(c, rho), 0.0,
                                   c->cfg.solver_max_steps - total_inner, true);
             total_inner += R.iterations;
             s.restarts += R.restarts;
@@ -681,7 +690,7 @@ static void precond_multi(wb_ctx *c, int is_float, int k, const void *X, void *Y
 // B, X: k x vlen (column stride vlen).  state[v] in/out: 0 iterate, 1 converged, -1 failed;
 // instances entering with state != 0 are left alone.  Returns per-instance iterations.
 static std::vector<int> gmres_multi(wb_ctx *c, int is_float, int k, const void *B, void *X,
-                                    double abs_tol, double rel_tol, int max_steps,
+                                    const std::vector<double> &abs_tol, int max_steps,
                                     std::vector<int> &state, std::vector<double> &resid)
 {
     auto &M = c->mw;
@@ -707,7 +716,7 @@ static std::vector<int> gmres_multi(wb_ctx *c, int is_float, int k, const void *
     std::vector<double> h(m + 2), y(m + 1);
     std::vector<int> its(k, 0), dim(k, 0);
     int *act = M.apin;
-    std::vector<double> tol(k, abs_tol);
+    const std::vector<double> &tol = abs_tol;
     bool first = true;
     auto upload_act = [&]() {
         HIPCHK(hipStreamSynchronize(c->stream));  // previous copy out of apin has finished
@@ -734,7 +743,6 @@ static std::vector<int> gmres_multi(wb_ctx *c, int is_float, int k, const void *
         for (int v = 0; v < k; ++v) {
             if (state[v] != 0) continue;
             const double rho = M.hpin[v * sH];
-            if (first && rel_tol > 0) tol[v] = std::max(abs_tol, rel_tol * rho);
             resid[v] = rho;
             if (rho <= tol[v]) state[v] = 1;
             else if (std::isnan(rho) || its[v] >= max_steps) state[v] = -1;
@@ -916,8 +924,9 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
     std::vector<double> resid(k, 0.0);
     wb_solve_stats s{};
     if (c->cfg.precision == WB_PREC_FP64) {
-        std::vector<int> its = gmres_multi(c, 0, k, M.B.p, M.X.p, c->cfg.solver_tolerance, 0.0,
-                                           c->cfg.solver_max_steps, state, resid);
+        std::vector<int> its =
+            gmres_multi(c, 0, k, M.B.p, M.X.p, std::vector<double>(k, c->cfg.solver_tolerance),
+                        c->cfg.solver_max_steps, state, resid);
         s.iterations = *std::max_element(its.begin(), its.end());
     } else {
         HIPCHK(hipMemcpyAsync(M.R.p, M.B.p, d8, hipMemcpyDeviceToDevice, c->stream));
@@ -945,11 +954,13 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
             launch_vec_ops(1, (int)(vl * k), M.R.p, nullptr, M.R32.p, 0, c->stream);
             launch_vec_ops(7, (int)(vl * k), nullptr, nullptr, M.D32.p, 0, c->stream);
             std::vector<int> st_in(k);
-            std::vector<double> r_in(k);
-            for (int v = 0; v < k; ++v) st_in[v] = done[v] ? 1 : 0;
-            std::vector<int> its =
-                gmres_multi(c, 1, k, M.R32.p, M.D32.p, 0.0, c->cfg.ir_inner_reduction,
-                            c->cfg.solver_max_steps - total, st_in, r_in);
+            std::vector<double> r_in(k), t_in(k);
+            for (int v = 0; v < k; ++v) {
+                st_in[v] = done[v] ? 1 : 0;
+                t_in[v] = ir_inner_target(c, resid[v]);
+            }
+            std::vector<int> its = gmres_multi(c, 1, k, M.R32.p, M.D32.p, t_in,
+                                               c->cfg.solver_max_steps - total, st_in, r_in);
             total += *std::max_element(its.begin(), its.end());
             launch_vec_ops(2, (int)(vl * k), M.D32.p, nullptr, M.X.p, 0, c->stream);
             apply_constrained_multi(c, 2, k, M.X.p, vl, M.W.p, vl);
@@ -1028,7 +1039,7 @@ wb_status wb_create(wb_ctx **out, const wb_config *cfg)
         k.gmres_max_tmp_vectors < 3 || k.gmres_max_tmp_vectors > 129 ||
         (k.precision != WB_PREC_FP64 && k.precision != WB_PREC_FP32_IR) ||
         k.precond_block < 1 || k.precond_block > 128 || k.solver_max_steps < 0 ||
-        (k.assembly_tile != 16 && k.assembly_tile != 32) ||
+        k.assembly_tile < 1 || k.assembly_tile > 16 ||
         !(k.solver_tolerance >= 0)) {
         delete c;
         return WB_E_ARG;
@@ -1271,41 +1282,56 @@ wb_status wb_assemble(wb_ctx *c)
     });
 }
 
+// rows[r] (caller numbering) of G and H, caller column order, FP64 (hi + lo in FP32_IR)
+static void copy_rows(wb_ctx *c, int nr, const int *rows, double *G, double *H)
+{
+    require(c->assembled, WB_E_STATE, "matrix copy before wb_assemble");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int n = c->n;
+    const size_t ld = c->ld;
+    for (int which = 0; which < 2; ++which) {
+        double *out = which == 0 ? G : H;
+        if (!out) continue;
+        const DevBuf &hi = which == 0 ? c->G : c->H;
+        const DevBuf &lo = which == 0 ? c->Glo : c->Hlo;
+        std::vector<double> row(ld);
+        std::vector<float> rowf(ld);
+        std::vector<uint16_t> rowl(ld);
+        for (int r = 0; r < nr; ++r) {
+            const int d = rows ? rows[r] : r;
+            require(d >= 0 && d < n, WB_E_ARG, "row index out of range");
+            const int i = c->plan.perm[d];
+            if (c->cfg.precision == WB_PREC_FP64) {
+                HIPCHK(hipMemcpy(row.data(), (char *)hi.p + sizeof(double) * ld * i,
+                                 sizeof(double) * ld, hipMemcpyDeviceToHost));
+            } else {
+                HIPCHK(hipMemcpy(rowf.data(), (char *)hi.p + sizeof(float) * ld * i,
+                                 sizeof(float) * ld, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(rowl.data(), (char *)lo.p + sizeof(uint16_t) * ld * i,
+                                 sizeof(uint16_t) * ld, hipMemcpyDeviceToHost));
+                for (size_t e = 0; e < ld; ++e) {
+                    uint32_t b = (uint32_t)rowl[e] << 16;
+                    float lf;
+                    std::memcpy(&lf, &b, 4);
+                    row[e] = (double)rowf[e] + (double)lf;
+                }
+            }
+            double *o = out + (size_t)r * n;
+            for (int e = 0; e < n; ++e) o[e] = row[c->plan.perm[e]];
+        }
+    }
+}
+
 wb_status wb_get_matrices(wb_ctx *c, double *G, double *H)
 {
+    return guarded(c, [&] { copy_rows(c, c->n, nullptr, G, H); });
+}
+
+wb_status wb_get_matrix_rows(wb_ctx *c, int n_rows, const int32_t *rows, double *G, double *H)
+{
     return guarded(c, [&] {
-        require(c->assembled, WB_E_STATE, "wb_get_matrices before wb_assemble");
-        const int n = c->n;
-        const size_t ld = c->ld;
-        for (int which = 0; which < 2; ++which) {
-            double *out = which == 0 ? G : H;
-            if (!out) continue;
-            const DevBuf &hi = which == 0 ? c->G : c->H;
-            const DevBuf &lo = which == 0 ? c->Glo : c->Hlo;
-            std::vector<double> row(ld);
-            std::vector<float> rowf(ld);
-            std::vector<uint16_t> rowl(ld);
-            for (int d = 0; d < n; ++d) {
-                const int i = c->plan.perm[d];
-                if (c->cfg.precision == WB_PREC_FP64) {
-                    HIPCHK(hipMemcpy(row.data(), (char *)hi.p + sizeof(double) * ld * i,
-                                     sizeof(double) * ld, hipMemcpyDeviceToHost));
-                } else {
-                    HIPCHK(hipMemcpy(rowf.data(), (char *)hi.p + sizeof(float) * ld * i,
-                                     sizeof(float) * ld, hipMemcpyDeviceToHost));
-                    HIPCHK(hipMemcpy(rowl.data(), (char *)lo.p + sizeof(uint16_t) * ld * i,
-                                     sizeof(uint16_t) * ld, hipMemcpyDeviceToHost));
-                    for (size_t j = 0; j < ld; ++j) {
-                        uint32_t b = (uint32_t)rowl[j] << 16;
-                        float lf;
-                        std::memcpy(&lf, &b, 4);
-                        row[j] = (double)rowf[j] + (double)lf;
-                    }
-                }
-                double *o = out + (size_t)d * n;
-                for (int e = 0; e < n; ++e) o[e] = row[c->plan.perm[e]];
-            }
-        }
+        require(n_rows >= 0 && (rows || n_rows == 0), WB_E_ARG, "bad row list");
+        copy_rows(c, n_rows, rows, G, H);
     });
 }
 

@@ -24,7 +24,7 @@
 namespace wb {
 
 constexpr int TILE_ROWS = 64;   // rows per assembly workgroup (one wavefront, lane = row)
-constexpr int TILE_COLS = 16;   // default columns per assembly tile (16 or 32)
+constexpr int TILE_COLS = 16;   // columns per assembly register tile
 constexpr int TILES_PER_WG = 8; // column tiles swept by one workgroup
 constexpr int MAX_NQ1 = 6;      // supported regular QGauss orders 1..6
 constexpr int MAX_SING = 8;     // supported singular QGaussOneOverR orders 1..8

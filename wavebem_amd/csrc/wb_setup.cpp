@@ -182,7 +182,7 @@ std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const i
                       const int32_t *dnp, const int32_t *dni, int tc, MeshPlan &P)
 {
     P = MeshPlan();
-    if (tc != 16 && tc != 32) return "tile width must be 16 or 32";
+    if (tc < 1 || tc > TILE_COLS) return "tile width must be 1..16";
     P.tc = tc;
     P.n = n;
     P.nc = nc;
