@@ -116,12 +116,15 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-rows", type=int, default=2048)
+    ap.add_argument("--cpu-rows", type=int, default=24000)
     ap.add_argument("--cpu-gemv-n", type=int, default=8192)
     ap.add_argument("--only", choices=["fp64", "fp32_ir"], default=None)
     ap.add_argument("--ir-inner", type=float, default=1e-6,
                     help="FP32-IR inner GMRES relative residual reduction per sweep")
     ap.add_argument("--tile", type=int, default=16, help="max columns per assembly tile (1..16)")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (4 launches) and exit: the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -133,6 +136,16 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibrate:
+        nb = args.copy_calibrate << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": nb, "copy_bytes_written_per_launch": nb,
+                          "launches": 4}), flush=True)
+        return
     from wavebem_amd import mesh as M
 
     t = time.perf_counter()

@@ -6,8 +6,9 @@
 // vectors in one pass over the matrix.  Each wavefront owns 16 rows; the product is a dense
 // contraction, so it runs on the matrix cores: v_mfma_f32_16x16x4f32 for the FP32 (hi)
 // matrix, v_mfma_f64_16x16x4f64 for FP64.  Operand layout (checked on the device):
-// A[i][kk] in lane i + 16 kk, B[kk][j] in lane j + 16 kk, D[i][j] in lane j + 16 (i / 4),
-// accumulator register i % 4.  Lane l streams 8 (FP32) or 4 (FP64) consecutive columns of
+// A[i][kk] in lane i + 16 kk and B[kk][j] in lane j + 16 kk for both; the result D[i][j]
+// is in lane j + 16 (i / 4), register i % 4 for FP32 but in lane j + 16 (i % 4), register
+// i / 4 for FP64.  Lane l streams 8 (FP32) or 4 (FP64) consecutive columns of
 // row l % 16, so every load is 32 contiguous bytes and a row segment is 128 bytes.
 #include "wb_internal.h"
 
@@ -163,10 +164,11 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_f64(
             for (int s = 0; s < 4; ++s) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], c, 0, 0, 0);
         }
     }
+    // FP64 result layout differs from FP32: D[i][j] in lane j + 16 (i % 4), register i / 4
     if (vok) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int s2 = tile0 + 4 * q + r;
+            const int s2 = tile0 + q + 4 * r;
             if (s2 < n_rows) {
                 const int rr = rows ? rows[s2] : s2;
                 double val = c[r];
