@@ -32,14 +32,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_precision(mesh, bc, precision, steps, warmup, tol, ir_inner=1e-6, tile=16):
+def run_precision(mesh, bc, precision, steps, warmup, tol, ir_inner=1e-6, tile=16, pblock=64):
     import torch
 
     from wavebem_amd.bem import BEMProblem, Config
 
     n = mesh.n_dofs
     cfg = Config(precision=precision, solver_tolerance=tol, solver_max_steps=5000,
-                 ir_inner_reduction=ir_inner, assembly_tile=tile)
+                 ir_inner_reduction=ir_inner, assembly_tile=tile, precond_block=pblock)
     p = BEMProblem(mesh, cfg)
     p.compute_constraints(bc)
     bc_d = torch.tensor(bc, dtype=torch.float64, device="cuda")
@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--ir-inner", type=float, default=1e-6,
                     help="FP32-IR inner GMRES relative residual reduction per sweep")
     ap.add_argument("--tile", type=int, default=16, help="max columns per assembly tile (1..16)")
+    ap.add_argument("--precond-block", type=int, default=64,
+                    help="block-Jacobi block size (rows of the device ordering, <= 128)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (4 launches) and exit: the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -159,7 +161,7 @@ def main():
         if args.only and prec != args.only:
             continue
         res[prec] = run_precision(mesh, bc, prec, args.steps, args.warmup, args.tol,
-                                  args.ir_inner, args.tile)
+                                  args.ir_inner, args.tile, args.precond_block)
 
     n = mesh.n_dofs
     N = mesh.other_nodes > 0
@@ -181,7 +183,7 @@ def main():
         "config": {"workload": f"wigley_closed_halfbox_{args.workload}", "n_dofs": n,
                    "n_cells": mesh.n_cells, "quadrature": "QGauss 4x4 + QGaussOneOverR(5)",
                    "gmres_tol_abs_preconditioned": args.tol, "gmres_restart": 98,
-                   "preconditioner": "block_jacobi_64", "froude": 0.3},
+                   "preconditioner": f"block_jacobi_{args.precond_block}", "froude": 0.3},
     }
     if "fp64" in res:
         line["fp64_ms_per_step"] = round(res["fp64"]["ms_per_step"], 3)
@@ -191,6 +193,9 @@ def main():
         st = res["fp32_ir"]["stats"]
         line["fp32_ir_inner_iterations"] = st["iterations"]
         line["fp32_ir_sweeps"] = st["ir_sweeps"]
+        k = min(st["ir_sweeps"], 8)
+        line["fp32_ir_sweep_iterations"] = st["sweep_iterations"][:k]
+        line["fp32_ir_sweep_residual"] = st["sweep_residual"][:k]
         line["fp32_ir_true_rel_residual"] = st["true_rel_residual"]
     if "fp64" in res and "fp32_ir" in res:
         a, b = res["fp32_ir"], res["fp64"]

@@ -72,6 +72,8 @@ typedef struct {
     double residual;        /* last preconditioned residual (convergence test value) */
     double true_rel_residual; /* ||b - A x|| / ||b|| of the constrained system, FP64 */
     double t_alpha_ms, t_rhs_ms, t_precond_ms, t_gmres_ms;
+    int sweep_iterations[8];  /* FP32_IR: inner iterations of the first 8 sweeps */
+    double sweep_residual[8]; /* FP32_IR: outer preconditioned residual before each sweep */
 } wb_solve_stats;
 
 typedef struct wb_ctx wb_ctx;

@@ -48,10 +48,16 @@ class WbSolveStats(ctypes.Structure):
         ("t_rhs_ms", ctypes.c_double),
         ("t_precond_ms", ctypes.c_double),
         ("t_gmres_ms", ctypes.c_double),
+        ("sweep_iterations", ctypes.c_int * 8),
+        ("sweep_residual", ctypes.c_double * 8),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            d[k] = list(v) if not isinstance(v, (int, float)) else v
+        return d
 
 
 WB_K_NAMES = ["assemble", "singular", "gemv64", "gemv32", "gemvhl"]

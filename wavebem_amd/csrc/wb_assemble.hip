@@ -10,8 +10,9 @@
 // Regular cells: one wavefront owns 64 collocation rows (lane = row) and a column tile
 // of up to TILE_COLS consecutive device-ordered dofs; it sweeps the cells touching the
 // tile.  Cell quadrature data is wave-uniform and arrives through scalar loads; each lane
-// adds the 4+4 local G/H values of the current cell into its row's 16-wide register tiles;
-// the finished tile goes out through LDS as whole row segments, written once.
+// adds the 4+4 local G/H values of the current cell into its row of the tile (G in a
+// 16-wide register vector, H in lane-private LDS words); the finished tile goes out through
+// LDS as whole row segments, written once.
 // Singular cells: a second kernel adds their 2 s^2-point contributions per row.
 #include "wb_internal.h"
 
@@ -119,12 +120,28 @@ void launch_cell_geometry(const double *X, const int *cell_int, const int8_t *or
 // are formed per rule row j from S0 = sum_i (1-xi_i) t_ij and S1 = sum_i xi_i t_ij:
 // only the Q Gauss nodes are needed (kept in SGPRs).
 //
-// The TC tile columns of each lane's row live in registers (a TC-wide vector per matrix);
-// a cell's four tile-local columns are wave-uniform, so the compiler updates them with
-// s_set_gpr_idx-indexed register moves, no LDS and no atomics.
+// The TC tile columns of each lane's row: G in a TC-wide register vector, H in LDS.  A
+// cell's four tile-local columns are wave-uniform, so G is updated with s_set_gpr_idx
+// register moves and H with one no-return LDS add per column; both are lane-private, so
+// there is no contention and the summation order is fixed.  Keeping H out of the registers
+// is what lets four waves share a SIMD.
+// Measured at 41k dofs (FP64 assembly): 3 waves per SIMD with both accumulator tiles in
+// registers 67 ms; 4 waves with the H tile in LDS 51 ms (the few spilled registers of the
+// 4-wave budget cost less than the lost occupancy).
 #ifndef WB_ASM_WAVES
-#define WB_ASM_WAVES 3  // requested waves per SIMD for the regular-cell kernel
+#define WB_ASM_WAVES 4  // requested waves per SIMD for the regular-cell kernel
 #endif
+#ifndef WB_ASM_JUNROLL
+#define WB_ASM_JUNROLL 8  // unroll of the rule-row loop (>= Q: fully unrolled)
+#endif
+#ifndef WB_ASM_IUNROLL
+#define WB_ASM_IUNROLL 8  // unroll of the points of one rule row
+#endif
+#ifndef WB_ASM_HLDS
+#define WB_ASM_HLDS 1  // H accumulators in LDS (1) or in registers like G (0)
+#endif
+#define WB_STR(x) #x
+#define WB_UNROLL(n) _Pragma(WB_STR(unroll n))
 template <int Q, int TC, int OUT>
 __global__ __launch_bounds__(TILE_ROWS) __attribute__((amdgpu_waves_per_eu(WB_ASM_WAVES)))
 void k_assemble_tiles(
@@ -139,8 +156,10 @@ void k_assemble_tiles(
     constexpr int NQ = Q * Q;
     static_assert(TC == 16, "tile width");
     typedef double accv __attribute__((ext_vector_type(16)));
-    constexpr int STAGE_LD = TC + 1;
-    __shared__ double stage[TILE_ROWS * STAGE_LD];
+    // [column][row] tile in LDS, row stride ACC_LD (68: the remapped row-segment reads of
+    // the write-out hit distinct banks); holds the H accumulators, then stages G
+    constexpr int ACC_LD = TILE_ROWS + 4;
+    __shared__ double lacc[TC * ACC_LD];
     const int lane = threadIdx.x;
     const int rb = blockIdx.x;
     const int tg = blockIdx.y;
@@ -152,11 +171,23 @@ void k_assemble_tiles(
     const int t_begin = tg * TILES_PER_WG;
     const int t_end = min(n_tiles, t_begin + TILES_PER_WG);
     for (int tile = t_begin; tile < t_end; ++tile) {
-        accv aG = 0.0, aH = 0.0;
-        // c is wave-uniform: the vector index becomes an s_set_gpr_idx register move
+        accv aG = 0.0;
+#if WB_ASM_HLDS
+#pragma unroll
+        for (int c = 0; c < TC; ++c) lacc[c * ACC_LD + lane] = 0.0;
+#else
+        accv aH = 0.0;
+#endif
+        // c is wave-uniform: the G vector index becomes an s_set_gpr_idx register move; the
+        // H entry is a lane-private LDS word, updated by a no-return LDS add (ds_add_f64)
         auto upd = [&](int c, double g, double h) {
             aG[c & 15] += g;
+#if WB_ASM_HLDS
+            __hip_atomic_fetch_add(&lacc[(c & 15) * ACC_LD + lane], h, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
+#else
             aH[c & 15] += h;
+#endif
         };
         double hs = 0.0;
         const int p0 = tile_ptr[tile], p1 = tile_ptr[tile + 1];
@@ -166,10 +197,10 @@ void k_assemble_tiles(
             const int4 cc = cell_cls[cell];
             const double *rec = cellq + (size_t)cell * (NQ * 8);
             double g0 = 0, g1 = 0, g2 = 0, g3 = 0, h0 = 0, h1 = 0, h2 = 0, h3 = 0;
-#pragma unroll
+            WB_UNROLL(WB_ASM_JUNROLL)
             for (int j = 0; j < Q; ++j) {
                 double sg = 0, sg1 = 0, sh = 0, sh1 = 0;
-#pragma unroll
+                WB_UNROLL(WB_ASM_IUNROLL)
                 for (int i = 0; i < Q; ++i) {
                     const double *r = rec + (i + Q * j) * 8;
                     const double R0 = r[0] - x0, R1 = r[1] - x1, R2 = r[2] - x2;
@@ -219,19 +250,23 @@ void k_assemble_tiles(
                 hs += h3;
             }
         }
-        // Write-out through LDS: each lane parks its row, then lanes are remapped to
-        // (row, column) so every store instruction writes whole contiguous row segments.
+        // Write-out through LDS: H (already there), then G parked by each lane; lanes are
+        // remapped to (row, column) so every store instruction writes whole row segments.
         const int c_begin = tile_start[tile];
         const int ncols = min(TC, tile_start[tile + 1] - c_begin);
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
+        for (int m = 1; m >= 0; --m) {
             const double scale = m == 0 ? INV4PI : -INV4PI;
             __syncthreads();
+#if WB_ASM_HLDS
+            if (m == 0) {
 #pragma unroll
-            for (int c = 0; c < TC; ++c) {
-                const double v = m == 0 ? aG[c] : aH[c];
-                stage[lane * STAGE_LD + c] = v * scale;
+                for (int c = 0; c < TC; ++c) lacc[c * ACC_LD + lane] = aG[c];
             }
+#else
+#pragma unroll
+            for (int c = 0; c < TC; ++c) lacc[c * ACC_LD + lane] = m == 0 ? aG[c] : aH[c];
+#endif
             __syncthreads();
             constexpr int RPI = TILE_ROWS / TC;  // rows per store instruction
             const int cl = lane % TC, rsub = lane / TC;
@@ -240,7 +275,7 @@ void k_assemble_tiles(
                 const int rl = r0 + rsub;
                 const int grow = rb * TILE_ROWS + rl;
                 if (grow < n && cl < ncols) {
-                    const double v = stage[rl * STAGE_LD + cl];
+                    const double v = lacc[cl * ACC_LD + rl] * scale;
                     const size_t off = (size_t)grow * ld + c_begin + cl;
                     if (OUT == 0) {
                         ((double *)(m == 0 ? Gv : Hv))[off] = v;
@@ -418,20 +453,29 @@ void launch_assemble_singular(int precision, const double *X, const int *cell_in
 }
 
 // alpha_i = -(sum over tile groups of the regular H row sums + singular H row sum)
-__global__ void k_alpha_reduce(const double *__restrict__ part, int n_parts,
-                               const double *__restrict__ sing, int n, double *__restrict__ alpha)
+// alpha_i = -(sum over tile groups of the partial row sums + singular part).  A block
+// owns 64 rows; its 4 wavefronts sum interleaved quarters of the partials and the quarters
+// are added in a fixed order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_alpha_reduce(const double *__restrict__ part,
+                                                      int n_parts,
+                                                      const double *__restrict__ sing, int n,
+                                                      double *__restrict__ alpha)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    __shared__ double q[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
     double s = 0.0;
-    for (int p = 0; p < n_parts; ++p) s += part[(size_t)p * n + i];
-    alpha[i] = -(s + sing[i]);
+    if (i < n)
+        for (int p = wave; p < n_parts; p += 4) s += part[(size_t)p * n + i];
+    q[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && i < n) alpha[i] = -(((q[0][lane] + q[1][lane]) + (q[2][lane] + q[3][lane])) + sing[i]);
 }
 
 void launch_alpha_reduce(const double *hsum_part, int n_parts, const double *hsum_sing, int n,
                          double *alpha, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_alpha_reduce, dim3((n + 255) / 256), dim3(256), 0, s, hsum_part, n_parts,
+    hipLaunchKernelGGL(k_alpha_reduce, dim3((n + 63) / 64), dim3(256), 0, s, hsum_part, n_parts,
                        hsum_sing, n, alpha);
 }
 

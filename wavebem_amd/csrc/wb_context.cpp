@@ -572,6 +572,10 @@ static void run_solver(wb_ctx *c, wb_solve_stats *st)
             launch_vec_ops(7, n, nullptr, nullptr, c->d32.p, 0, c->stream);
             GmresResult R = gmres(c, 1, c->r32.p, c->d32.p, ir_inner_target(c, rho), 0.0,
                                   c->cfg.solver_max_steps - total_inner, true);
+            if (sweeps < 8) {
+                s.sweep_iterations[sweeps] = R.iterations;
+                s.sweep_residual[sweeps] = rho;
+            }
             total_inner += R.iterations;
             s.restarts += R.restarts;
             launch_vec_ops(2, n, c->d32.p, nullptr, c->vx.p, 0, c->stream);

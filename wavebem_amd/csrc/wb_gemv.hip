@@ -41,20 +41,21 @@ __device__ __forceinline__ double bf16d(unsigned short b)
     return (double)__uint_as_float((uint32_t)b << 16);
 }
 
+// The rows of a wavefront are wave-uniform: keep them (and the row base addresses built
+// from them) in SGPRs, so each matrix load is an SGPR base plus the lane's column offset.
 template <int RPW>
 __device__ __forceinline__ void load_rows(int slot0, const int *__restrict__ rows, int n_rows,
                                           int (&r)[RPW], bool (&ok)[RPW])
 {
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        const int s = slot0 + k;
+        const int s = __builtin_amdgcn_readfirstlane(slot0 + k);
         ok[k] = s < n_rows;
-        r[k] = ok[k] ? (rows ? rows[s] : s) : 0;
+        r[k] = __builtin_amdgcn_readfirstlane(ok[k] ? (rows ? rows[s] : s) : 0);
     }
 }
 
 constexpr int F64_RPW = 4, F64_WAVES = 4;
-constexpr int F32_RPW = 8, F32_WAVES = 2;
 
 // segment = int4 {first group, end group, kind (0 G, 1 H, 2 mixed), unused}
 __global__ __launch_bounds__(64 * F64_WAVES) void k_gemv_f64(
@@ -123,24 +124,34 @@ __global__ __launch_bounds__(64 * F64_WAVES) void k_gemv_f64(
 }
 
 // FP32 hi (MODE 1: float vectors, FP32 accumulation) or hi + bf16 lo (MODE 2: double
-// vectors, FP64 accumulation); 4 columns per lane per step (group g = columns 4g..4g+3)
+// vectors, FP64 accumulation); 4 columns per lane per step (group g = columns 4g..4g+3).
+// Pure segments stream one matrix with vector loads; the few mixed groups load each column
+// from its own matrix with scalar loads, so they add no registers to the streaming loop.
+template <int MODE> struct MixedShape;
+template <> struct MixedShape<1> { static constexpr int RPW = 8, WAVES = 2; };
+template <> struct MixedShape<2> { static constexpr int RPW = 4, WAVES = 4; };
+
 template <int MODE>
-__global__ __launch_bounds__(64 * F32_WAVES) void k_gemv_mixed(
+__global__ __launch_bounds__(64 * MixedShape<MODE>::WAVES) void k_gemv_mixed(
     const float *__restrict__ G, const float *__restrict__ H, const uint16_t *__restrict__ Glo,
     const uint16_t *__restrict__ Hlo, size_t ld, const int4 *__restrict__ segs, int n_segs,
     const uint8_t *__restrict__ gmask, int flip, double s_sel, double s_uns,
     const void *__restrict__ xv_, const double *__restrict__ diag, const void *__restrict__ xd_,
     const int *__restrict__ rows, int n_rows, void *__restrict__ y_)
 {
-    constexpr int RPW = F32_RPW;
+    constexpr int RPW = MixedShape<MODE>::RPW, WAVES = MixedShape<MODE>::WAVES;
     typedef typename std::conditional<MODE == 1, float, double>::type T;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int r[RPW];
     bool ok[RPW];
-    load_rows<RPW>((blockIdx.x * F32_WAVES + wave) * RPW, rows, n_rows, r, ok);
+    load_rows<RPW>((blockIdx.x * WAVES + wave) * RPW, rows, n_rows, r, ok);
+    size_t off[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) off[k] = (size_t)r[k] * ld;
     T acc[RPW];
 #pragma unroll
     for (int k = 0; k < RPW; ++k) acc[k] = 0;
+    // x of group g, each column scaled by its selection sign
     auto load_x = [&](int g, T (&xe)[4]) {
         if (MODE == 1) {
             const fvec4 xv = *((const fvec4 *)xv_ + g);
@@ -167,18 +178,19 @@ __global__ __launch_bounds__(64 * F32_WAVES) void k_gemv_mixed(
                 T xe[4];
                 load_x(g, xe);
 #pragma unroll
+                for (int e = 0; e < 4; ++e) xe[e] *= sc;
+#pragma unroll
                 for (int k = 0; k < RPW; ++k) {
-                    const fvec4 a = __builtin_nontemporal_load((const fvec4 *)(M + r[k] * ld) + g);
+                    const fvec4 a = __builtin_nontemporal_load((const fvec4 *)(M + off[k]) + g);
                     if (MODE == 1) {
-                        acc[k] += (a.x * xe[0] + a.y * xe[1] + a.z * xe[2] + a.w * xe[3]) * sc;
+                        acc[k] += a.x * xe[0] + a.y * xe[1] + a.z * xe[2] + a.w * xe[3];
                     } else {
                         const uvec4 l =
-                            __builtin_nontemporal_load((const uvec4 *)(Ml + r[k] * ld) + g);
-                        acc[k] += (((double)a.x + bf16d(l.x)) * xe[0] +
-                                   ((double)a.y + bf16d(l.y)) * xe[1] +
-                                   ((double)a.z + bf16d(l.z)) * xe[2] +
-                                   ((double)a.w + bf16d(l.w)) * xe[3]) *
-                                  sc;
+                            __builtin_nontemporal_load((const uvec4 *)(Ml + off[k]) + g);
+                        acc[k] += ((double)a.x + bf16d(l.x)) * xe[0] +
+                                  ((double)a.y + bf16d(l.y)) * xe[1] +
+                                  ((double)a.z + bf16d(l.z)) * xe[2] +
+                                  ((double)a.w + bf16d(l.w)) * xe[3];
                     }
                 }
             };
@@ -190,28 +202,23 @@ __global__ __launch_bounds__(64 * F32_WAVES) void k_gemv_mixed(
             if (g < sg.y) step(g);
         } else {
             const unsigned fm = flip ? 0xfu : 0u;
-            const T ssel = (T)s_sel, suns = (T)s_uns;
             for (int g = sg.x + lane; g < sg.y; g += 64) {
                 const unsigned m = gmask[g] ^ fm;
                 T xe[4];
                 load_x(g, xe);
 #pragma unroll
-                for (int e = 0; e < 4; ++e) xe[e] *= ((m >> e) & 1) ? ssel : suns;
+                for (int e = 0; e < 4; ++e) {
+                    const bool h = (m >> e) & 1u;
+                    const float *M = h ? H : G;
+                    const uint16_t *Ml = h ? Hlo : Glo;
+                    const T xs = xe[e] * (T)(h ? s_sel : s_uns);
+                    const size_t c = (size_t)4 * g + e;
 #pragma unroll
-                for (int k = 0; k < RPW; ++k) {
-                    const fvec4 a = *((const fvec4 *)(G + r[k] * ld) + g);
-                    const fvec4 b = *((const fvec4 *)(H + r[k] * ld) + g);
-                    T av[4] = {(m & 1) ? b.x : a.x, (m & 2) ? b.y : a.y, (m & 4) ? b.z : a.z,
-                               (m & 8) ? b.w : a.w};
-                    if (MODE == 2) {
-                        const uvec4 la = *((const uvec4 *)(Glo + r[k] * ld) + g);
-                        const uvec4 lb = *((const uvec4 *)(Hlo + r[k] * ld) + g);
-                        av[0] += bf16d((m & 1) ? lb.x : la.x);
-                        av[1] += bf16d((m & 2) ? lb.y : la.y);
-                        av[2] += bf16d((m & 4) ? lb.z : la.z);
-                        av[3] += bf16d((m & 8) ? lb.w : la.w);
+                    for (int k = 0; k < RPW; ++k) {
+                        T a = (T)M[off[k] + c];
+                        if (MODE == 2) a += (T)bf16d(Ml[off[k] + c]);
+                        acc[k] += a * xs;
                     }
-                    acc[k] += av[0] * xe[0] + av[1] * xe[1] + av[2] * xe[2] + av[3] * xe[3];
                 }
             }
         }
@@ -250,16 +257,21 @@ void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, co
                            (const double *)x, diag, (const double *)xd, rows, n_rows, (double *)y);
         return;
     }
-    const int per_block = F32_RPW * F32_WAVES;
-    dim3 grid((n_rows + per_block - 1) / per_block), block(64 * F32_WAVES);
-    if (mode == 1)
-        hipLaunchKernelGGL(k_gemv_mixed<1>, grid, block, 0, s, (const float *)G, (const float *)H,
-                           (const uint16_t *)Glo, (const uint16_t *)Hlo, ld, (const int4 *)segs,
-                           n_segs, gmask, flip, s_sel, s_unsel, x, diag, xd, rows, n_rows, y);
-    else
-        hipLaunchKernelGGL(k_gemv_mixed<2>, grid, block, 0, s, (const float *)G, (const float *)H,
-                           (const uint16_t *)Glo, (const uint16_t *)Hlo, ld, (const int4 *)segs,
-                           n_segs, gmask, flip, s_sel, s_unsel, x, diag, xd, rows, n_rows, y);
+    if (mode == 1) {
+        constexpr int pb = MixedShape<1>::RPW * MixedShape<1>::WAVES;
+        hipLaunchKernelGGL(k_gemv_mixed<1>, dim3((n_rows + pb - 1) / pb),
+                           dim3(64 * MixedShape<1>::WAVES), 0, s, (const float *)G,
+                           (const float *)H, (const uint16_t *)Glo, (const uint16_t *)Hlo, ld,
+                           (const int4 *)segs, n_segs, gmask, flip, s_sel, s_unsel, x, diag, xd,
+                           rows, n_rows, y);
+    } else {
+        constexpr int pb = MixedShape<2>::RPW * MixedShape<2>::WAVES;
+        hipLaunchKernelGGL(k_gemv_mixed<2>, dim3((n_rows + pb - 1) / pb),
+                           dim3(64 * MixedShape<2>::WAVES), 0, s, (const float *)G,
+                           (const float *)H, (const uint16_t *)Glo, (const uint16_t *)Hlo, ld,
+                           (const int4 *)segs, n_segs, gmask, flip, s_sel, s_unsel, x, diag, xd,
+                           rows, n_rows, y);
+    }
 }
 
 }  // namespace wb

@@ -17,7 +17,7 @@ namespace wb {
 namespace {
 
 constexpr int KTHREADS = 256;
-constexpr int KPER = 4;
+constexpr int KPER = 1;  // 256-entry chunks: about n/256 workgroups per reduction
 constexpr int KCHUNK = KTHREADS * KPER;
 
 __device__ __forceinline__ double ksum(double v)
