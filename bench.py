@@ -81,6 +81,20 @@ def kernel_avg_ms(ks, name):
     return k["total_ms"] / k["launches"] if k["launches"] else float("nan")
 
 
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this
+    workload (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", f"bem_step_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_per_launch {kernel} = "
+    with open(path) as f:
+        for ln in f:
+            if ln.startswith(key):
+                return int(float(ln[len(key):]))
+    return None
+
+
 def cpu_baseline(mesh, iters64, threads, rows_sample, gemv_n):
     """Oracle (CPU restatement of the reference dense path) on a bounded sample."""
     import oracle as O
@@ -119,7 +133,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=24000)
     ap.add_argument("--cpu-gemv-n", type=int, default=8192)
     ap.add_argument("--only", choices=["fp64", "fp32_ir"], default=None)
-    ap.add_argument("--ir-inner", type=float, default=1e-6,
+    ap.add_argument("--ir-inner", type=float, default=1e-7,
                     help="FP32-IR inner GMRES relative residual reduction per sweep")
     ap.add_argument("--tile", type=int, default=16, help="max columns per assembly tile (1..16)")
     ap.add_argument("--precond-block", type=int, default=64,
@@ -217,9 +231,11 @@ def main():
     rows_per_launch = g["rows"] / max(g["launches"], 1)
     bytes_per_launch = rows_per_launch * n * esz + 2 * n * esz
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    line["roofline"] = {"kernel": f"k_gemv_sel ({gname})", "bound": "hbm",
+    kname = "k_gemv_mixed<1>" if gname == "gemv32" else "k_gemv_f64"
+    line["roofline"] = {"kernel": kname, "bound": "hbm",
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": profiled_traffic(args.workload, kname),
                         "avg_launch_ms": round(avg_ms, 4), "launches": g["launches"],
                         "algorithmic_bytes_per_launch": int(bytes_per_launch)}
     kern = {}

@@ -58,7 +58,7 @@ class Config:
     preconditioner: str = "block_jacobi"  # "block_jacobi" | "none"
     precond_block: int = 64
     ir_max_sweeps: int = 10
-    ir_inner_reduction: float = 1e-6
+    ir_inner_reduction: float = 1e-7
     assembly_tile: int = 16
 
     def to_c(self) -> L.WbConfig:

@@ -92,7 +92,7 @@ struct wb_ctx {
     int n_lines = 0, n_free = 0;
     DevBuf binv, binv32;
     // device: solver work
-    DevBuf V, z, w, P1, P2, Pn, hbuf, vx, vb, vr, vt, vs;   // double / raw
+    DevBuf V, z, w, P1, P2, Pn, hbuf, hred, vx, vb, vr, vt, vs;   // double / raw
     DevBuf V32, z32, w32, r32, d32;
     double *hpin = nullptr;  // pinned host copy of hbuf
     size_t vlen = 0;         // padded vector length
@@ -112,12 +112,12 @@ struct wb_ctx {
     // several right-hand sides at once
     struct Multi {
         int k = 0;
-        DevBuf V, V32, Z, W, Z32, W32, P1, P2, Pn, H, X, B, R, T, D32, R32, act, inh;
+        DevBuf V, V32, Z, W, Z32, W32, P1, P2, Pn, H, hred, X, B, R, T, D32, R32, act, inh;
         double *hpin = nullptr;  // pinned k x 132 coefficients
         int *apin = nullptr;     // pinned k active flags
         std::vector<DevBuf *> all()
         {
-            return {&V, &V32, &Z, &W, &Z32, &W32, &P1, &P2, &Pn, &H,
+            return {&V, &V32, &Z, &W, &Z32, &W32, &P1, &P2, &Pn, &H, &hred,
                     &X, &B, &R, &T, &D32, &R32, &act, &inh};
         }
     } mw;
@@ -248,6 +248,7 @@ static void ensure_work(wb_ctx *c)
     c->P2.ensure(sizeof(double) * parts * 128);
     c->Pn.ensure(sizeof(double) * parts);
     c->hbuf.ensure(sizeof(double) * 132);
+    c->hred.ensure(sizeof(double) * 2 * KMAX);
     if (!c->hpin) HIPCHK(hipHostMalloc((void **)&c->hpin, sizeof(double) * 132, 0));
 }
 
@@ -437,8 +438,8 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
             apply_precond(c, is_float, w, z);
         }
         launch_norm2_partials(is_float, z, n, c->Pn.as<double>(), c->stream);
-        launch_finalize(is_float, 0, c->P1.as<double>(), nullptr, c->Pn.as<double>(), parts, z, n,
-                        vec(0), c->hbuf.as<double>(), c->stream);
+        launch_finalize(is_float, 0, nullptr, nullptr, c->Pn.as<double>(), z, n, vec(0),
+                        c->hbuf.as<double>(), c->stream);
         HIPCHK(hipMemcpyAsync(c->hpin, c->hbuf.p, sizeof(double), hipMemcpyDeviceToHost,
                               c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -460,14 +461,16 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
             apply_constrained(c, mode, vec(inner), w);
             apply_precond(c, is_float, w, z);
             dim = inner + 1;
+            double *h1 = c->hred.as<double>(), *h2 = h1 + KMAX;
             launch_dot_partials(is_float, V.p, ldv, dim, z, n, c->P1.as<double>(), c->stream);
-            launch_update_dot(is_float, V.p, ldv, dim, c->P1.as<double>(), parts, z, n,
-                              c->P2.as<double>(), nullptr, 0, c->stream);
-            launch_update_dot(is_float, V.p, ldv, dim, c->P2.as<double>(), parts, z, n, nullptr,
-                              c->Pn.as<double>(), 1, c->stream);
-            launch_finalize(is_float, dim, c->P1.as<double>(), c->P2.as<double>(),
-                            c->Pn.as<double>(), parts, z, n, vec(dim), c->hbuf.as<double>(),
-                            c->stream);
+            launch_reduce_parts(dim, c->P1.as<double>(), n, h1, c->stream);
+            launch_update_dot(is_float, V.p, ldv, dim, h1, z, n, c->P2.as<double>(), nullptr, 0,
+                              c->stream);
+            launch_reduce_parts(dim, c->P2.as<double>(), n, h2, c->stream);
+            launch_update_dot(is_float, V.p, ldv, dim, h2, z, n, nullptr, c->Pn.as<double>(), 1,
+                              c->stream);
+            launch_finalize(is_float, dim, h1, h2, c->Pn.as<double>(), z, n, vec(dim),
+                            c->hbuf.as<double>(), c->stream);
             HIPCHK(hipMemcpyAsync(c->hpin, c->hbuf.p, sizeof(double) * (dim + 1),
                                   hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
@@ -738,8 +741,8 @@ static std::vector<int> gmres_multi(wb_ctx *c, int is_float, int k, const void *
         }
         launch_norm2_partials(is_float, Z, n, M.Pn.as<double>(), c->stream, k, vl,
                               M.act.as<int>());
-        launch_finalize(is_float, 0, M.P1.as<double>(), nullptr, M.Pn.as<double>(), parts, Z, n,
-                        basis(0), M.H.as<double>(), c->stream, k, vl, sV, sH, M.act.as<int>());
+        launch_finalize(is_float, 0, nullptr, nullptr, M.Pn.as<double>(), Z, n, basis(0),
+                        M.H.as<double>(), c->stream, k, vl, sV, 0, sH, M.act.as<int>());
         HIPCHK(hipMemcpyAsync(M.hpin, M.H.p, sizeof(double) * sH * k, hipMemcpyDeviceToHost,
                               c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -766,16 +769,18 @@ static std::vector<int> gmres_multi(wb_ctx *c, int is_float, int k, const void *
             apply_constrained_multi(c, mode, k, basis(inner), sV, W, vl);
             precond_multi(c, is_float, k, W, Z);
             const int kc = inner + 1;
+            const size_t sR = 2 * KMAX;
+            double *h1 = M.hred.as<double>(), *h2 = h1 + KMAX;
             launch_dot_partials(is_float, V.p, vl, kc, Z, n, M.P1.as<double>(), c->stream, k, sV,
                                 vl, M.act.as<int>());
-            launch_update_dot(is_float, V.p, vl, kc, M.P1.as<double>(), parts, Z, n,
-                              M.P2.as<double>(), nullptr, 0, c->stream, k, sV, vl,
-                              M.act.as<int>());
-            launch_update_dot(is_float, V.p, vl, kc, M.P2.as<double>(), parts, Z, n, nullptr,
-                              M.Pn.as<double>(), 1, c->stream, k, sV, vl, M.act.as<int>());
-            launch_finalize(is_float, kc, M.P1.as<double>(), M.P2.as<double>(),
-                            M.Pn.as<double>(), parts, Z, n, basis(kc), M.H.as<double>(),
-                            c->stream, k, vl, sV, sH, M.act.as<int>());
+            launch_reduce_parts(kc, M.P1.as<double>(), n, h1, c->stream, k, sR, M.act.as<int>());
+            launch_update_dot(is_float, V.p, vl, kc, h1, Z, n, M.P2.as<double>(), nullptr, 0,
+                              c->stream, k, sV, vl, sR, M.act.as<int>());
+            launch_reduce_parts(kc, M.P2.as<double>(), n, h2, c->stream, k, sR, M.act.as<int>());
+            launch_update_dot(is_float, V.p, vl, kc, h2, Z, n, nullptr, M.Pn.as<double>(), 1,
+                              c->stream, k, sV, vl, sR, M.act.as<int>());
+            launch_finalize(is_float, kc, h1, h2, M.Pn.as<double>(), Z, n, basis(kc),
+                            M.H.as<double>(), c->stream, k, vl, sV, sR, sH, M.act.as<int>());
             HIPCHK(hipMemcpyAsync(M.hpin, M.H.p, sizeof(double) * sH * k, hipMemcpyDeviceToHost,
                                   c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
@@ -860,6 +865,7 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
     M.P2.ensure(sizeof(double) * parts * KMAX * k);
     M.Pn.ensure(sizeof(double) * parts * k);
     M.H.ensure(sizeof(double) * 132 * k);
+    M.hred.ensure(sizeof(double) * 2 * KMAX * k);
     M.act.ensure(sizeof(int) * k);
     if (M.k < k) {
         if (M.hpin) (void)hipHostFree(M.hpin);
@@ -1021,7 +1027,7 @@ void wb_config_default(wb_config *cfg)
     cfg->preconditioner = WB_PRECOND_BLOCK_JACOBI;
     cfg->precond_block = 64;
     cfg->ir_max_sweeps = 10;
-    cfg->ir_inner_reduction = 1e-6;
+    cfg->ir_inner_reduction = 1e-7;
     cfg->assembly_tile = 16;
 }
 
@@ -1075,7 +1081,7 @@ void wb_destroy(wb_ctx *c)
                       &c->surf_d, &c->gmask, &c->segs, &c->diag_op, &c->diag_rhs, &c->con_rows,
                       &c->con_ptr, &c->con_cols, &c->con_w, &c->con_inhom, &c->constrained,
                       &c->free_rows, &c->binv, &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2,
-                      &c->Pn, &c->hbuf, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
+                      &c->Pn, &c->hbuf, &c->hred, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
                       &c->z32, &c->w32, &c->r32, &c->d32};
     for (DevBuf *b : bufs) b->release();
     for (DevBuf *b : c->mw.all()) b->release();
@@ -1565,7 +1571,7 @@ size_t wb_device_bytes(const wb_ctx *c)
                             &c->segs,
                             &c->diag_op, &c->diag_rhs, &c->con_rows, &c->con_ptr, &c->con_cols,
                             &c->con_w, &c->con_inhom, &c->constrained, &c->free_rows, &c->binv,
-                            &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2, &c->Pn, &c->hbuf,
+                            &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2, &c->Pn, &c->hbuf, &c->hred,
                             &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32, &c->z32, &c->w32,
                             &c->r32, &c->d32};
     for (const DevBuf *b : bufs) t += b->bytes;

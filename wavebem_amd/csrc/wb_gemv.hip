@@ -8,8 +8,8 @@
 // groups at a Dirichlet/Neumann boundary of the device ordering, resolved per column with a
 // 4-bit mask).  A segment is wave-uniform, so the matrix pointer needs no per-lane load and
 // all row streams stay in flight.  Each wavefront streams RPW rows with 16-byte lane loads
-// (the shapes measured fastest on MI355X: FP64 4 rows x 4 waves, FP32 8 rows x 2 waves,
-// both unrolled twice) and reduces with wavefront shuffles.  Constrained rows
+// (the shapes measured fastest on MI355X: 4 rows x 4 waves for FP64 and FP32, unrolled
+// twice) and reduces with wavefront shuffles.  Constrained rows
 // (ConstrainedOperator, include/constrained_matrix.h:73-86) are skipped through the
 // caller's row list and rewritten by a fix-up kernel.
 #include "wb_internal.h"
@@ -23,13 +23,6 @@ typedef float fvec4 __attribute__((ext_vector_type(4)));
 typedef unsigned short uvec4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ double wsum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ __forceinline__ float wsumf(float v)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -123,12 +116,20 @@ __global__ __launch_bounds__(64 * F64_WAVES) void k_gemv_f64(
     }
 }
 
-// FP32 hi (MODE 1: float vectors, FP32 accumulation) or hi + bf16 lo (MODE 2: double
-// vectors, FP64 accumulation); 4 columns per lane per step (group g = columns 4g..4g+3).
+// FP32 hi with float vectors (MODE 1) or hi + bf16 lo with double vectors (MODE 2); 4
+// columns per lane per step (group g = columns 4g..4g+3).  Both accumulate in FP64: the
+// products of FP32 values are exact in FP64 and the kernel is bound by HBM, not by the
+// FP64 issue rate, so the FP32 operator's only error is the rounding of its entries.
 // Pure segments stream one matrix with vector loads; the few mixed groups load each column
 // from its own matrix with scalar loads, so they add no registers to the streaming loop.
 template <int MODE> struct MixedShape;
-template <> struct MixedShape<1> { static constexpr int RPW = 8, WAVES = 2; };
+// FP32 shape: 4 rows x 4 waves measured 6.25 TB/s at 41k, 8 x 2 6.22 TB/s
+#ifndef WB_GEMV32_RPW
+#define WB_GEMV32_RPW 4
+#endif
+template <> struct MixedShape<1> {
+    static constexpr int RPW = WB_GEMV32_RPW, WAVES = 16 / WB_GEMV32_RPW;
+};
 template <> struct MixedShape<2> { static constexpr int RPW = 4, WAVES = 4; };
 
 template <int MODE>
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(64 * MixedShape<MODE>::WAVES) void k_gemv_mixed(
     const int *__restrict__ rows, int n_rows, void *__restrict__ y_)
 {
     constexpr int RPW = MixedShape<MODE>::RPW, WAVES = MixedShape<MODE>::WAVES;
-    typedef typename std::conditional<MODE == 1, float, double>::type T;
+    typedef double T;  // products and sums in FP64 in both modes
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int r[RPW];
     bool ok[RPW];
@@ -155,10 +156,10 @@ __global__ __launch_bounds__(64 * MixedShape<MODE>::WAVES) void k_gemv_mixed(
     auto load_x = [&](int g, T (&xe)[4]) {
         if (MODE == 1) {
             const fvec4 xv = *((const fvec4 *)xv_ + g);
-            xe[0] = xv.x;
-            xe[1] = xv.y;
-            xe[2] = xv.z;
-            xe[3] = xv.w;
+            xe[0] = (double)xv.x;
+            xe[1] = (double)xv.y;
+            xe[2] = (double)xv.z;
+            xe[3] = (double)xv.w;
         } else {
             const dvec2 a = *((const dvec2 *)xv_ + 2 * g), b = *((const dvec2 *)xv_ + 2 * g + 1);
             xe[0] = a.x;
@@ -183,7 +184,8 @@ __global__ __launch_bounds__(64 * MixedShape<MODE>::WAVES) void k_gemv_mixed(
                 for (int k = 0; k < RPW; ++k) {
                     const fvec4 a = __builtin_nontemporal_load((const fvec4 *)(M + off[k]) + g);
                     if (MODE == 1) {
-                        acc[k] += a.x * xe[0] + a.y * xe[1] + a.z * xe[2] + a.w * xe[3];
+                        acc[k] += (double)a.x * xe[0] + (double)a.y * xe[1] +
+                                  (double)a.z * xe[2] + (double)a.w * xe[3];
                     } else {
                         const uvec4 l =
                             __builtin_nontemporal_load((const uvec4 *)(Ml + off[k]) + g);
@@ -226,10 +228,10 @@ __global__ __launch_bounds__(64 * MixedShape<MODE>::WAVES) void k_gemv_mixed(
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
         if (MODE == 1) {
-            const float s = wsumf((float)acc[k]);
+            const double s = wsum(acc[k]);
             if (lane == 0 && ok[k]) {
                 const float *xd = (const float *)xd_;
-                ((float *)y_)[r[k]] = s + (diag ? (float)diag[r[k]] * xd[r[k]] : 0.0f);
+                ((float *)y_)[r[k]] = (float)(s + (diag ? diag[r[k]] * (double)xd[r[k]] : 0.0));
             }
         } else {
             const double s = wsum((double)acc[k]);

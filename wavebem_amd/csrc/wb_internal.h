@@ -125,14 +125,17 @@ int krylov_partials_blocks(int n);
 void launch_dot_partials(int is_float, const void *V, size_t ldv, int kcount, const void *z,
                          int n, double *P, hipStream_t s, int batch = 1, size_t sV = 0,
                          size_t sZ = 0, const int *active = nullptr);
-void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, const double *Pin,
-                       int nparts, void *z, int n, double *Pout, double *normp, int want_norm,
-                       hipStream_t s, int batch = 1, size_t sV = 0, size_t sZ = 0,
+// h[b*sH + k] = sum over chunks of the partials P (k < kcount)
+void launch_reduce_parts(int kcount, const double *P, int n, double *h, hipStream_t s,
+                         int batch = 1, size_t sH = 0, const int *active = nullptr);
+void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, const double *h,
+                       void *z, int n, double *Pout, double *normp, int want_norm, hipStream_t s,
+                       int batch = 1, size_t sV = 0, size_t sZ = 0, size_t sH = 0,
                        const int *active = nullptr);
-void launch_finalize(int is_float, int kcount, const double *P1, const double *P2,
-                     const double *Pn, int nparts, void *z, int n, void *vout, double *hout,
-                     hipStream_t s, int batch = 1, size_t sZ = 0, size_t sV = 0, size_t sH = 0,
-                     const int *active = nullptr);
+void launch_finalize(int is_float, int kcount, const double *h1, const double *h2,
+                     const double *Pn, void *z, int n, void *vout, double *hout, hipStream_t s,
+                     int batch = 1, size_t sZ = 0, size_t sV = 0, size_t sHin = 0,
+                     size_t sH = 0, const int *active = nullptr);
 void launch_axpy_basis(int is_float, const void *V, size_t ldv, int kcount, const double *y,
                        void *x_acc, int x_is_float, int n, hipStream_t s, int batch = 1,
                        size_t sV = 0, size_t sY = 0, size_t sX = 0, const int *active = nullptr);

@@ -8,8 +8,10 @@
 //
 // Orthogonalization is classical Gram-Schmidt with one re-orthogonalization pass (CGS2):
 //   h1 = V^T z, z -= V h1, h2 = V^T z, z -= V h2, h = h1 + h2, s = |z|, v_new = z / s
-// (dot_partials, update_dot x2, finalize); it spans the same Krylov space as the modified
-// Gram-Schmidt of deal.II's SolverGMRES with two global reductions per step.
+// (dot_partials, reduce_parts, update_dot, reduce_parts, update_dot, finalize); it spans
+// the same Krylov space as the modified Gram-Schmidt of deal.II's SolverGMRES with two
+// global reductions per step.  Chunk partials are summed once, by one small workgroup
+// (reduce_parts), not redundantly by every chunk of the next pass.
 #include "wb_internal.h"
 
 namespace wb {
@@ -62,12 +64,41 @@ __global__ __launch_bounds__(KTHREADS) void k_dot_partials(const T *__restrict__
     chunk_dots(V, ldv, kcount, zs, base, m, P, blockIdx.x);
 }
 
-// h = sum_parts Pin;  z -= V h;  then optional partial dots (Pout) and partial |z|^2
+// out[k] = sum over the chunks of P[p][k] (k < kcount), one workgroup per instance: thread
+// (k, q) sums every KQ-th chunk and the KQ partial sums are added in a fixed order.
+constexpr int KQ = KTHREADS / KMAX;
+__global__ __launch_bounds__(KTHREADS) void k_reduce_parts(int kcount, const double *__restrict__ P,
+                                                           int nparts, double *__restrict__ out,
+                                                           size_t sOut,
+                                                           const int *__restrict__ active)
+{
+    __shared__ double red[KQ][KMAX];
+    const int b = blockIdx.x;
+    if (active && !active[b]) return;
+    P += (size_t)b * nparts * KMAX;
+    out += b * sOut;
+    const int k = threadIdx.x % KMAX, q = threadIdx.x / KMAX;
+    double s = 0.0;
+    if (k < kcount) {
+#pragma unroll 8
+        for (int p = q; p < nparts; p += KQ) s += P[(size_t)p * KMAX + k];
+    }
+    red[q][k] = s;
+    __syncthreads();
+    if (q == 0 && k < kcount) {
+        double t = red[0][k];
+        for (int j = 1; j < KQ; ++j) t += red[j][k];
+        out[k] = t;
+    }
+}
+
+// z -= V h (h reduced);  then optional partial dots (Pout) and partial |z|^2
 template <typename T>
 __global__ __launch_bounds__(KTHREADS) void k_update_dot(const T *__restrict__ V, size_t ldv,
                                                          size_t sV, int kcount,
-                                                         const double *__restrict__ Pin,
-                                                         T *__restrict__ z, size_t sZ, int n,
+                                                         const double *__restrict__ hin,
+                                                         size_t sH, T *__restrict__ z,
+                                                         size_t sZ, int n,
                                                          double *__restrict__ Pout,
                                                          double *__restrict__ normp,
                                                          int want_norm,
@@ -81,16 +112,12 @@ __global__ __launch_bounds__(KTHREADS) void k_update_dot(const T *__restrict__ V
     const int nparts = gridDim.x;
     V += b * sV;
     z += b * sZ;
-    Pin += (size_t)b * nparts * KMAX;
+    hin += b * sH;
     if (Pout) Pout += (size_t)b * nparts * KMAX;
     if (normp) normp += (size_t)b * nparts;
     const int base = blockIdx.x * KCHUNK;
     const int m = min(KCHUNK, n - base);
-    for (int k = threadIdx.x; k < kcount; k += KTHREADS) {
-        double s = 0.0;
-        for (int p = 0; p < nparts; ++p) s += Pin[(size_t)p * KMAX + k];
-        h[k] = s;
-    }
+    for (int k = threadIdx.x; k < kcount; k += KTHREADS) h[k] = hin[k];
     __syncthreads();
     double nrm = 0.0;
     for (int e = threadIdx.x; e < KCHUNK; e += KTHREADS) {
@@ -118,10 +145,12 @@ __global__ __launch_bounds__(KTHREADS) void k_update_dot(const T *__restrict__ V
     }
 }
 
-// hout[k] = sum P1 + sum P2 (k < kcount), hout[kcount] = sqrt(sum Pn); vout = z / hout[kcount]
+// hout[k] = h1[k] + h2[k] (k < kcount), hout[kcount] = sqrt(sum Pn); vout = z / hout[kcount].
+// Every workgroup forms the norm itself, wavefront-parallel and in the same order.
 template <typename T>
-__global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double *__restrict__ P1,
-                                                       const double *__restrict__ P2,
+__global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double *__restrict__ h1,
+                                                       const double *__restrict__ h2,
+                                                       size_t sHin,
                                                        const double *__restrict__ Pn,
                                                        const T *__restrict__ z, size_t sZ,
                                                        int n, T *__restrict__ vout, size_t sV,
@@ -132,27 +161,23 @@ __global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double 
     const int b = blockIdx.y;
     if (active && !active[b]) return;
     const int nparts = gridDim.x;
-    P1 += (size_t)b * nparts * KMAX;
-    if (P2) P2 += (size_t)b * nparts * KMAX;
+    if (h1) h1 += b * sHin;
+    if (h2) h2 += b * sHin;
     Pn += (size_t)b * nparts;
     z += b * sZ;
     if (vout) vout += b * sV;
     hout += b * sH;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
         double s = 0.0;
-        for (int p = 0; p < nparts; ++p) s += Pn[p];
-        snorm = sqrt(s);
+        for (int p = threadIdx.x; p < nparts; p += 64) s += Pn[p];
+        s = ksum(s);
+        if (threadIdx.x == 0) snorm = sqrt(s);
     }
     __syncthreads();
     const double nv = snorm;
     if (blockIdx.x == 0) {
-        for (int k = threadIdx.x; k < kcount; k += KTHREADS) {
-            double s = 0.0;
-            for (int p = 0; p < nparts; ++p) s += P1[(size_t)p * KMAX + k];
-            if (P2)
-                for (int p = 0; p < nparts; ++p) s += P2[(size_t)p * KMAX + k];
-            hout[k] = s;
-        }
+        for (int k = threadIdx.x; k < kcount; k += KTHREADS)
+            hout[k] = (h1 ? h1[k] : 0.0) + (h2 ? h2[k] : 0.0);
         if (threadIdx.x == 0) hout[kcount] = nv;
     }
     if (vout) {
@@ -228,35 +253,41 @@ void launch_dot_partials(int is_float, const void *V, size_t ldv, int kcount, co
                            ldv, sV, kcount, (const double *)z, sZ, n, P, active);
 }
 
-void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, const double *Pin,
-                       int nparts, void *z, int n, double *Pout, double *normp, int want_norm,
-                       hipStream_t s, int batch, size_t sV, size_t sZ, const int *active)
+void launch_reduce_parts(int kcount, const double *P, int n, double *h, hipStream_t s, int batch,
+                         size_t sH, const int *active)
 {
-    (void)nparts;
+    if (kcount <= 0) return;
+    hipLaunchKernelGGL(k_reduce_parts, dim3(batch), dim3(KTHREADS), 0, s, kcount, P,
+                       krylov_partials_blocks(n), h, sH, active);
+}
+
+void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, const double *h,
+                       void *z, int n, double *Pout, double *normp, int want_norm, hipStream_t s,
+                       int batch, size_t sV, size_t sZ, size_t sH, const int *active)
+{
     dim3 grid(krylov_partials_blocks(n), batch);
     if (is_float)
         hipLaunchKernelGGL(k_update_dot<float>, grid, dim3(KTHREADS), 0, s, (const float *)V,
-                           ldv, sV, kcount, Pin, (float *)z, sZ, n, Pout, normp, want_norm,
+                           ldv, sV, kcount, h, sH, (float *)z, sZ, n, Pout, normp, want_norm,
                            active);
     else
         hipLaunchKernelGGL(k_update_dot<double>, grid, dim3(KTHREADS), 0, s, (const double *)V,
-                           ldv, sV, kcount, Pin, (double *)z, sZ, n, Pout, normp, want_norm,
+                           ldv, sV, kcount, h, sH, (double *)z, sZ, n, Pout, normp, want_norm,
                            active);
 }
 
-void launch_finalize(int is_float, int kcount, const double *P1, const double *P2,
-                     const double *Pn, int nparts, void *z, int n, void *vout, double *hout,
-                     hipStream_t s, int batch, size_t sZ, size_t sV, size_t sH,
+void launch_finalize(int is_float, int kcount, const double *h1, const double *h2,
+                     const double *Pn, void *z, int n, void *vout, double *hout, hipStream_t s,
+                     int batch, size_t sZ, size_t sV, size_t sHin, size_t sH,
                      const int *active)
 {
-    (void)nparts;
     dim3 grid(krylov_partials_blocks(n), batch);
     if (is_float)
-        hipLaunchKernelGGL(k_finalize<float>, grid, dim3(KTHREADS), 0, s, kcount, P1, P2, Pn,
-                           (const float *)z, sZ, n, (float *)vout, sV, hout, sH, active);
+        hipLaunchKernelGGL(k_finalize<float>, grid, dim3(KTHREADS), 0, s, kcount, h1, h2, sHin,
+                           Pn, (const float *)z, sZ, n, (float *)vout, sV, hout, sH, active);
     else
-        hipLaunchKernelGGL(k_finalize<double>, grid, dim3(KTHREADS), 0, s, kcount, P1, P2, Pn,
-                           (const double *)z, sZ, n, (double *)vout, sV, hout, sH, active);
+        hipLaunchKernelGGL(k_finalize<double>, grid, dim3(KTHREADS), 0, s, kcount, h1, h2, sHin,
+                           Pn, (const double *)z, sZ, n, (double *)vout, sV, hout, sH, active);
 }
 
 void launch_axpy_basis(int is_float, const void *V, size_t ldv, int kcount, const double *y,
