@@ -101,6 +101,12 @@ wb_status wb_set_mesh(wb_ctx *ctx, int n_dofs, int n_cells, const double *suppor
                       const int32_t *cell_material, const int32_t *dn_ptr,
                       const int32_t *dn_idx);
 
+/* Moves the support points of the current mesh (the reference's MappingQEulerian update
+ * before each assemble_system, source/free_surface.cc:8744-8883): same cells, dofs and
+ * double-node sets, so coincident dofs must stay coincident.  Keeps masks and constraints;
+ * the next wb_assemble uses the new geometry. */
+wb_status wb_set_support_points(wb_ctx *ctx, const double *support_points);
+
 /* comp_dom.surface_nodes / other_nodes (source/numerical_towing_tank.cc:1799-1807,
  * source/free_surface.cc:8756-8772): 1 = Dirichlet (phi known) / Neumann. */
 wb_status wb_set_masks(wb_ctx *ctx, const double *surface_nodes, const double *other_nodes);
@@ -168,6 +174,29 @@ wb_status wb_solve(wb_ctx *ctx, const double *tmp_rhs, double *phi, double *dphi
 /* BEMProblem::residual (source/bem_problem.cc:903-961), Direct branch, with the
  * installed constraints. */
 wb_status wb_residual(wb_ctx *ctx, double *res, const double *phi, const double *dphi_dn);
+
+/* ---- adjoint (no reference counterpart: the reference differentiates its residual with
+ * Sacado, source/free_surface.cc:4918-5053, 6099-6109) ------------------------------ */
+
+/* Solves A_c^T lambda = rhs with the transposed ConstrainedOperator of the installed
+ * constraints and masks (the operator of solve_system), preconditioned by the transposed
+ * block Jacobi, with the same GMRES settings.  rhs and lambda in the caller's dof
+ * numbering.  FP64 contexts only (WB_E_UNSUPPORTED otherwise and for all-Neumann
+ * masks); WB_E_NOCONV leaves lambda unchanged. */
+wb_status wb_solve_adjoint(wb_ctx *ctx, const double *rhs, double *lambda,
+                           wb_solve_stats *stats);
+
+/* Backward of assemble_system with respect to the support points: grad (n x 3, caller
+ * numbering) = sum_i w_i dR_i/dx_d for every support point x_d, where
+ *   R_i = (alpha_i phi_i + sum_j H_ij phi_j) - sum_j G_ij dphi_dn_j
+ * is the BEM row of vmult/compute_rhs on the full vectors (alpha = -H 1), with the
+ * quadrature of the current assembly (regular and singular cells).  A point moves as the
+ * collocation node of its row and as the vertex of every cell that uses its dof; callers
+ * moving coincident dofs together add their rows.  With free_rows_only the rows of the
+ * installed constraint lines get weight 0 (their residual does not depend on geometry).
+ * Needs wb_assemble at the current geometry. */
+wb_status wb_residual_vjp(wb_ctx *ctx, const double *w, const double *phi,
+                          const double *dphi_dn, int free_rows_only, double *grad);
 
 /* Multi right-hand side form of solve_system (no reference counterpart: the reference
  * calls solve_system once per time step, source/bem_problem.cc:1013-1108): k

@@ -130,6 +130,22 @@ def assemble_rows(support, cell_dofs, cell_orient, dn_ptr, dn_idx, row0, row1, q
     return H, G
 
 
+def residual_vjp(support, cell_dofs, cell_orient, dn_ptr, dn_idx, w, phi, psi, q_order=4,
+                 sing_order=5):
+    """(n, 3): sum_i w_i dR_i/dx_d, R_i = sum_j H_ij (phi_j - phi_i) - sum_j G_ij psi_j."""
+    support = _f64(support)
+    cd = _i32(cell_dofs)
+    orient = np.ascontiguousarray(cell_orient, dtype=np.int8)
+    n = support.shape[0]
+    out = np.zeros((n, 3))
+    rc = lib().orc_residual_vjp(n, cd.shape[0], _i(cd), orient.ctypes.data_as(_bp),
+                                _d(support), _i(_i32(dn_ptr)), _i(_i32(dn_idx)), q_order,
+                                sing_order, _d(_f64(w)), _d(_f64(phi)), _d(_f64(psi)), _d(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_residual_vjp failed: {rc}")
+    return out
+
+
 def gemv_time(n, reps=3, n_threads=0):
     """Seconds per dense FP64 row-major mat-vec of size n (FullMatrix::vmult loop)."""
     import time

@@ -965,3 +965,113 @@ void orc_residual(int n, const double *G, const double *H, const double *surface
     free(rr);
     free(sol);
 }
+
+/* ------------------------------------------------------------------------------------
+ * Backward of the assembly: out(d, :) = sum_i w_i dR_i/dx_d for the support point x_d of
+ * every dof d, where R_i = (alpha_i phi_i + sum_j H_ij phi_j) - sum_j G_ij psi_j is the
+ * BEM row of source/bem_problem.cc:621-717 (vmult + compute_rhs on the full vectors), with
+ * alpha_i = -sum_j H_ij (compute_alpha, :594-618), so that
+ *   R_i = sum_j H_ij (phi_j - phi_i) - sum_j G_ij psi_j.
+ * Each (row i, cell c, point q) term of assemble_system (:106-590) is
+ *   s = w_i/(4 pi) [ -(Phi_q - phi_i) (R.N) / r^3 - Psi_q J / r ],
+ * R = y_q - x_i, N = n_q JxW_q = o w_q (a x b), J = JxW_q = w_q |a x b|, Phi_q / Psi_q the Q1
+ * interpolants of phi / psi at q; its derivatives with respect to R, N and J are chained
+ * through y = sum phi_k v_k, a = sum dxi_k v_k, b = sum deta_k v_k to the cell vertices, and
+ * -dS/dR goes to x_i.  Regular and singular (QGaussOneOverR) points alike, with the same
+ * rule selection as assemble_system.  Plain loops, no parallelism: small meshes only.
+ * ---------------------------------------------------------------------------------- */
+int orc_residual_vjp(int n, int nc, const int *cell_dofs, const signed char *orient,
+                     const double *support, const int *dn_ptr, const int *dn_idx, int q_order,
+                     int sing_order, const double *w, const double *phi, const double *psi,
+                     double *out)
+{
+    if (n <= 0 || nc <= 0 || q_order <= 0 || q_order > 16 || sing_order <= 0 ||
+        sing_order > 16)
+        return -1;
+    const int nq = q_order * q_order;
+    const int ns = 2 * sing_order * sing_order;
+    double *qpts = (double *)malloc(sizeof(double) * 2 * nq);
+    double *qw = (double *)malloc(sizeof(double) * nq);
+    double *spts = (double *)malloc(sizeof(double) * 4 * 2 * ns);
+    double *sw = (double *)malloc(sizeof(double) * 4 * ns);
+    if (!qpts || !qw || !spts || !sw) return -2;
+    orc_qgauss_2d(q_order, qpts, qw);
+    for (int k = 0; k < 4; ++k)
+        orc_qgauss_one_over_r(sing_order, k, 1, spts + 2 * ns * k, sw + ns * k);
+    memset(out, 0, sizeof(double) * 3 * (size_t)n);
+    const double inv4pi = 1.0 / (4.0 * ORC_PI);
+    for (int i = 0; i < n; ++i) {
+        if (w[i] == 0.0) continue;
+        const double *xi = support + 3 * i;
+        const double wi = w[i] * inv4pi;
+        for (int c = 0; c < nc; ++c) {
+            const int *dofs = cell_dofs + 4 * c;
+            int sing = -1;
+            for (int j = 0; j < 4; ++j)
+                if (in_set(dn_ptr, dn_idx, i, dofs[j])) {
+                    sing = j;
+                    break;
+                }
+            const double *pts = sing < 0 ? qpts : spts + 2 * ns * sing;
+            const double *pw = sing < 0 ? qw : sw + ns * sing;
+            const int np = sing < 0 ? nq : ns;
+            double v[12];
+            for (int k = 0; k < 4; ++k)
+                for (int d = 0; d < 3; ++d) v[3 * k + d] = support[3 * dofs[k] + d];
+            const double so = orient[c] < 0 ? -1.0 : 1.0;
+            for (int q = 0; q < np; ++q) {
+                double sh[4], dx[4], de[4];
+                q1_shape(pts[2 * q], pts[2 * q + 1], sh, dx, de);
+                double y[3] = {0, 0, 0}, a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+                double Phi = 0.0, Psi = 0.0;
+                for (int k = 0; k < 4; ++k) {
+                    for (int d = 0; d < 3; ++d) {
+                        y[d] += sh[k] * v[3 * k + d];
+                        a[d] += dx[k] * v[3 * k + d];
+                        b[d] += de[k] * v[3 * k + d];
+                    }
+                    Phi += sh[k] * phi[dofs[k]];
+                    Psi += sh[k] * psi[dofs[k]];
+                }
+                const double cx[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                                      a[0] * b[1] - a[1] * b[0]};
+                const double cl = sqrt(cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
+                const double N[3] = {so * pw[q] * cx[0], so * pw[q] * cx[1], so * pw[q] * cx[2]};
+                const double J = pw[q] * cl;
+                const double R[3] = {y[0] - xi[0], y[1] - xi[1], y[2] - xi[2]};
+                const double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+                const double ri = 1.0 / r, ri3 = ri * ri * ri, ri5 = ri3 * ri * ri;
+                const double RN = R[0] * N[0] + R[1] * N[1] + R[2] * N[2];
+                const double D = Phi - phi[i], E = Psi;
+                double gR[3], gN[3];
+                for (int d = 0; d < 3; ++d) {
+                    gR[d] = wi * (-D * (N[d] * ri3 - 3.0 * RN * ri5 * R[d]) + E * J * ri3 * R[d]);
+                    gN[d] = wi * (-D * ri3 * R[d]);
+                }
+                const double gJ = wi * (-E * ri);
+                /* x_i */
+                for (int d = 0; d < 3; ++d) out[3 * i + d] -= gR[d];
+                /* N = so w (a x b): d/da = so w (b x gN), d/db = so w (gN x a);
+                 * J = w |a x b|: d/da = w gJ (b x m), d/db = w gJ (m x a), m = (a x b)/|a x b| */
+                const double m[3] = {cx[0] / cl, cx[1] / cl, cx[2] / cl};
+                const double sN = so * pw[q], sJ = pw[q] * gJ;
+                const double ga[3] = {
+                    sN * (b[1] * gN[2] - b[2] * gN[1]) + sJ * (b[1] * m[2] - b[2] * m[1]),
+                    sN * (b[2] * gN[0] - b[0] * gN[2]) + sJ * (b[2] * m[0] - b[0] * m[2]),
+                    sN * (b[0] * gN[1] - b[1] * gN[0]) + sJ * (b[0] * m[1] - b[1] * m[0])};
+                const double gb[3] = {
+                    sN * (gN[1] * a[2] - gN[2] * a[1]) + sJ * (m[1] * a[2] - m[2] * a[1]),
+                    sN * (gN[2] * a[0] - gN[0] * a[2]) + sJ * (m[2] * a[0] - m[0] * a[2]),
+                    sN * (gN[0] * a[1] - gN[1] * a[0]) + sJ * (m[0] * a[1] - m[1] * a[0])};
+                for (int k = 0; k < 4; ++k)
+                    for (int d = 0; d < 3; ++d)
+                        out[3 * dofs[k] + d] += sh[k] * gR[d] + dx[k] * ga[d] + de[k] * gb[d];
+            }
+        }
+    }
+    free(qpts);
+    free(qw);
+    free(spts);
+    free(sw);
+    return 0;
+}

@@ -85,6 +85,13 @@ void orc_residual(int n, const double *G, const double *H, const double *surface
                   const int *ent_col, const double *ent_w, const double *inhom,
                   const double *phi, const double *dphi_dn, double *res);
 
+/* sum_i w_i dR_i/dx_d for every support point x_d (n x 3), R_i the BEM row
+ * sum_j H_ij (phi_j - phi_i) - sum_j G_ij psi_j of the assembled operator. */
+int orc_residual_vjp(int n, int nc, const int *cell_dofs, const signed char *orient,
+                     const double *support, const int *dn_ptr, const int *dn_idx, int q_order,
+                     int sing_order, const double *w, const double *phi, const double *psi,
+                     double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,9 +92,12 @@ _SIGS = {
     "wb_set_mesh": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _i32p, _i8p, _i32p,
                                    _i32p, _i32p]),
     "wb_set_masks": (ctypes.c_int, [_vp, _dp, _dp]),
+    "wb_set_support_points": (ctypes.c_int, [_vp, _dp]),
     "wb_assemble": (ctypes.c_int, [_vp]),
     "wb_get_matrices": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_get_matrix_rows": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _dp, _dp]),
+    "wb_solve_adjoint": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_void_p]),
+    "wb_residual_vjp": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, _dp]),
     "wb_compute_alpha": (ctypes.c_int, [_vp, _dp]),
     "wb_vmult": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_compute_rhs": (ctypes.c_int, [_vp, _dp, _dp]),

@@ -151,6 +151,11 @@ class BEMProblem:
         self._check(L.lib().wb_set_masks(self._ctx, _d(s), _d(o)), "wb_set_masks")
         self.surface_nodes, self.other_nodes = s.copy(), o.copy()
 
+    def set_support_points(self, support_points):
+        """Moves the mesh nodes (same cells, dofs and double nodes); re-assemble afterwards."""
+        sp = _f64(support_points, 3 * self.n)
+        self._check(L.lib().wb_set_support_points(self._ctx, _d(sp)), "wb_set_support_points")
+
     def assemble_system(self):
         self._check(L.lib().wb_assemble(self._ctx), "wb_assemble")
 
@@ -258,6 +263,25 @@ class BEMProblem:
         self.last_stats = st.as_dict()
         self._check(status, "wb_solve_system_multi")
         return self.last_stats
+
+    def solve_adjoint(self, rhs):
+        """lambda with A_c^T lambda = rhs (transposed constrained operator), float64 (n,)."""
+        r = _f64(rhs, self.n)
+        lam = np.zeros(self.n)
+        st = L.WbSolveStats()
+        status = L.lib().wb_solve_adjoint(self._ctx, _d(r), _d(lam), ctypes.byref(st))
+        self.last_stats = st.as_dict()
+        self._check(status, "wb_solve_adjoint")
+        return lam
+
+    def residual_vjp(self, w, phi, dphi_dn, free_rows_only=True):
+        """(n, 3): sum_i w_i dR_i/dx_d, R_i = (alpha_i phi_i + sum_j H_ij phi_j)
+        - sum_j G_ij dphi_dn_j, at the current assembly."""
+        g = np.zeros((self.n, 3))
+        self._check(L.lib().wb_residual_vjp(self._ctx, _d(_f64(w, self.n)),
+                                            _d(_f64(phi, self.n)), _d(_f64(dphi_dn, self.n)),
+                                            int(free_rows_only), _d(g)), "wb_residual_vjp")
+        return g
 
     def solve_system_dev(self, phi_ptr, dphi_dn_ptr, tmp_rhs_ptr):
         """Device-pointer form (caller dof order, n float64 each)."""

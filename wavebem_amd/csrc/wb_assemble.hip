@@ -15,37 +15,9 @@
 // LDS as whole row segments, written once.
 // Singular cells: a second kernel adds their 2 s^2-point contributions per row.
 #include "wb_internal.h"
+#include "wb_device.h"
 
 namespace wb {
-
-// 1/sqrt(r2) in FP64 from the hardware estimate y0 (relative error eps).
-// WB_RSQ_THIRD_ORDER: one step y = y0 (1 + e/2 + 3 e^2/8), e = 1 - r2 y0^2, the series
-// of (1 - e)^(-1/2) to second order, leaves about (5/16) eps^3 (5 operations).
-// Otherwise two Newton steps y <- y + y (1/2 - (r2/2) y^2), each squaring the error
-// (7 operations).
-// v_rsq_f64 measured on gfx950: max relative error 5.2e-8 (2^-24.2) over 4M inputs in
-// [1e-8, 1e4]; after the third-order step 1.66e-16 (1.5 ulp).
-#ifndef WB_RSQ_THIRD_ORDER
-#define WB_RSQ_THIRD_ORDER 1
-#endif
-__device__ __forceinline__ double rsqrt_f64(double r2)
-{
-    double y = __builtin_amdgcn_rsq(r2);
-#if WB_RSQ_THIRD_ORDER
-    const double e = __builtin_fma(-r2, y * y, 1.0);
-    const double p = __builtin_fma(0.375, e, 0.5);
-    return __builtin_fma(y * e, p, y);
-#else
-    const double h = 0.5 * r2;
-    double t = y * y;
-    double e = __builtin_fma(-h, t, 0.5);
-    y = __builtin_fma(y, e, y);
-    t = y * y;
-    e = __builtin_fma(-h, t, 0.5);
-    y = __builtin_fma(y, e, y);
-    return y;
-#endif
-}
 
 __device__ __forceinline__ void split_hilo(double v, float &hi, uint16_t &lo)
 {

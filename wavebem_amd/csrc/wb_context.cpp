@@ -89,6 +89,14 @@ struct wb_ctx {
     DevBuf surf_d, gmask, diag_op, diag_rhs, segs;
     int n_segs = 0;
     DevBuf con_rows, con_ptr, con_cols, con_w, con_inhom, constrained, free_rows;
+    // transposed constraint lines, gathered by target column: y[tgt] += coef * x[src]
+    DevBuf ct_tgt, ct_ptr, ct_src, ct_coef;
+    int n_ct = 0;
+    bool op_transpose = false;  // apply_constrained / apply_precond act with A^T
+    DevBuf gt_part;             // chunk partials of the transposed product
+    // backward of the assembly
+    DevBuf vjp_w, vjp_phi, vjp_psi, vjp_rows, vjp_cstate, vjp_p1, vjp_p2, vjp_xg, vjp_vbar,
+        vjp_xs, vjp_spart;
     int n_lines = 0, n_free = 0;
     DevBuf binv, binv32;
     // device: solver work
@@ -284,6 +292,22 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
 static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
 {
     const int is_float = mode == 1;
+    if (c->op_transpose) {
+        // A_c^T x: transposed product over the unconstrained rows (the diagonal alpha term
+        // of those rows included), then the transposed constraint lines
+        require(mode == 0, WB_E_UNSUPPORTED, "transposed operator needs FP64 matrices");
+        const int chunks = gemvT_chunks(c->n_free);
+        c->gt_part.ensure(sizeof(double) * c->ld * chunks);
+        KernelScope ks(c, WB_K_GEMV64, (long long)c->n_free);
+        launch_gemvT(c->G.as<double>(), c->H.as<double>(), c->ld, c->n, c->gmask.as<uint8_t>(), 0,
+                     1.0, -1.0, (const double *)x, c->free_rows.as<int>(), c->n_free,
+                     c->diag_op.as<double>(), c->constrained.as<uint8_t>(),
+                     c->gt_part.as<double>(), (double *)y, c->stream);
+        launch_constraint_T(c->n_ct, c->ct_tgt.as<int>(), c->ct_ptr.as<int>(),
+                            c->ct_src.as<int>(), c->ct_coef.as<double>(), (const double *)x,
+                            (double *)y, c->stream);
+        return;
+    }
     if (c->pure_neumann) {
         gemv(c, mode, 0, x, x, y, false);
         const double nrm = device_norm(c, y, is_float);
@@ -312,7 +336,7 @@ static void apply_precond(wb_ctx *c, int is_float, const void *x, void *y)
 {
     if (c->cfg.preconditioner == WB_PRECOND_BLOCK_JACOBI) {
         launch_block_jacobi_apply(is_float, c->n, c->pc_bs, is_float ? c->binv32.p : c->binv.p, x,
-                                  y, 1, c->vlen, c->stream);
+                                  y, 1, c->vlen, c->stream, c->op_transpose ? 1 : 0);
     } else {
         HIPCHK(hipMemcpyAsync(y, x, (is_float ? 4 : 8) * (size_t)c->n, hipMemcpyDeviceToDevice,
                               c->stream));
@@ -369,6 +393,30 @@ static void install_constraints(wb_ctx *c, const ConstraintSet &cs)
         if (!flag[i]) fr.push_back(i);
     c->n_free = (int)fr.size();
     upload(c->free_rows, fr.data(), fr.size(), c->stream);
+    // A_c^T adds, for line l with row r: +x[r] to y[r] and -w_p x[r] to y[col_p]
+    {
+        std::vector<std::vector<std::pair<int, double>>> by(n);
+        for (int l = 0; l < nl; ++l) {
+            by[rows[l]].push_back({rows[l], 1.0});
+            for (int p = ptr[l]; p < ptr[l + 1]; ++p) by[cols[p]].push_back({rows[l], -cs.w[p]});
+        }
+        std::vector<int> tgt, tptr{0}, src;
+        std::vector<double> coef;
+        for (int j = 0; j < n; ++j) {
+            if (by[j].empty()) continue;
+            tgt.push_back(j);
+            for (auto &e : by[j]) {
+                src.push_back(e.first);
+                coef.push_back(e.second);
+            }
+            tptr.push_back((int)src.size());
+        }
+        c->n_ct = (int)tgt.size();
+        upload(c->ct_tgt, tgt.data(), tgt.size(), c->stream);
+        upload(c->ct_ptr, tptr.data(), tptr.size(), c->stream);
+        upload(c->ct_src, src.data(), src.size(), c->stream);
+        upload(c->ct_coef, coef.data(), coef.size(), c->stream);
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     if (!same_struct) c->cons_struct_ver++;
 }
@@ -1082,7 +1130,10 @@ void wb_destroy(wb_ctx *c)
                       &c->con_ptr, &c->con_cols, &c->con_w, &c->con_inhom, &c->constrained,
                       &c->free_rows, &c->binv, &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2,
                       &c->Pn, &c->hbuf, &c->hred, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
-                      &c->z32, &c->w32, &c->r32, &c->d32};
+                      &c->z32, &c->w32, &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
+                      &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,
+                      &c->vjp_rows, &c->vjp_cstate, &c->vjp_p1, &c->vjp_p2, &c->vjp_xg,
+                      &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart};
     for (DevBuf *b : bufs) b->release();
     for (DevBuf *b : c->mw.all()) b->release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -1195,6 +1246,26 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         c->cons_struct_ver++;
         HIPCHK(hipStreamSynchronize(s));
         c->have_mesh = true;
+    });
+}
+
+wb_status wb_set_support_points(wb_ctx *c, const double *support)
+{
+    return guarded(c, [&] {
+        require(c->have_mesh, WB_E_STATE, "wb_set_support_points before wb_set_mesh");
+        require(support != nullptr, WB_E_ARG, "null support points");
+        const int n = c->n, nc = c->plan.nc;
+        std::vector<double> Xi(3 * (size_t)n);
+        for (int d = 0; d < n; ++d)
+            for (int k = 0; k < 3; ++k) Xi[3 * c->plan.perm[d] + k] = support[3 * d + k];
+        upload(c->X, Xi.data(), Xi.size(), c->stream);
+        const int nq = c->quad.q_order * c->quad.q_order;
+        launch_cell_geometry(c->X.as<double>(), c->cell_int.as<int>(), c->orient.as<int8_t>(), nc,
+                             nq, c->qpts.as<double>(), c->qw.as<double>(), c->cellq.as<double>(),
+                             c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->assembled = false;
     });
 }
 
@@ -1531,6 +1602,136 @@ wb_status wb_residual(wb_ctx *c, double *res, const double *phi, const double *d
     });
 }
 
+// ------------------------------------------------------------------ adjoint
+wb_status wb_solve_adjoint(wb_ctx *c, const double *rhs, double *lambda, wb_solve_stats *st)
+{
+    return guarded(c, [&] {
+        check_ready(c, true);
+        require(rhs && lambda, WB_E_ARG, "null adjoint vector");
+        require(c->cfg.precision == WB_PREC_FP64, WB_E_UNSUPPORTED,
+                "the adjoint solve needs an FP64 context");
+        require(!c->pure_neumann, WB_E_UNSUPPORTED,
+                "the adjoint solve does not support all-Neumann masks");
+        ensure_work(c);
+        upload_vec(c, c->vb, rhs);
+        double t0 = now_ms();
+        ensure_precond(c);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double t1 = now_ms();
+        launch_vec_ops(6, c->n, nullptr, nullptr, c->vx.p, 0, c->stream);
+        struct Flip {
+            wb_ctx *c;
+            Flip(wb_ctx *c_) : c(c_) { c->op_transpose = true; }
+            ~Flip() { c->op_transpose = false; }
+        } flip(c);
+        GmresResult R = gmres(c, 0, c->vb.p, c->vx.p, c->cfg.solver_tolerance, 0.0,
+                              c->cfg.solver_max_steps, true);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double t2 = now_ms();
+        wb_solve_stats s{};
+        s.iterations = R.iterations;
+        s.restarts = R.restarts;
+        s.residual = R.residual;
+        s.t_precond_ms = t1 - t0;
+        s.t_gmres_ms = t2 - t1;
+        // true relative residual of the transposed system
+        apply_constrained(c, 0, c->vx.p, c->w.p);
+        launch_vec_ops(8, c->n, c->w.p, c->vb.p, c->vt.p, 0, c->stream);
+        const double rn = device_norm(c, c->vt.p, 0), bn = device_norm(c, c->vb.p, 0);
+        s.true_rel_residual = bn > 0 ? rn / bn : rn;
+        if (st) *st = s;
+        require(R.status == 1, WB_E_NOCONV, "adjoint GMRES did not converge");
+        download_vec(c, c->vx, lambda);
+    });
+}
+
+wb_status wb_residual_vjp(wb_ctx *c, const double *w, const double *phi, const double *dphi_dn,
+                          int free_rows_only, double *grad)
+{
+    return guarded(c, [&] {
+        require(c->assembled, WB_E_STATE, "wb_residual_vjp before wb_assemble");
+        require(w && phi && dphi_dn && grad, WB_E_ARG, "null vjp argument");
+        const int n = c->n, nc = c->plan.nc;
+        const int nq = c->quad.q_order * c->quad.q_order;
+        const int ns = 2 * c->quad.s_order * c->quad.s_order;
+        require(nq <= 64, WB_E_UNSUPPORTED, "regular rule too large for the backward pass");
+        std::vector<double> wi, pi, si;
+        to_internal(c, w, wi);
+        to_internal(c, phi, pi);
+        to_internal(c, dphi_dn, si);
+        if (free_rows_only)
+            for (int l = 0; l < (int)c->cons.rows.size(); ++l) wi[c->plan.perm[c->cons.rows[l]]] = 0.0;
+        // rowpack[i] = {x, y, z, w_i, phi_i, class}
+        std::vector<double> xs_host(3 * (size_t)n);
+        HIPCHK(hipMemcpyAsync(xs_host.data(), c->X.p, sizeof(double) * 3 * n,
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<double> rp(8 * (size_t)n, 0.0);
+        for (int i = 0; i < n; ++i) {
+            for (int d = 0; d < 3; ++d) rp[8 * i + d] = xs_host[3 * i + d];
+            rp[8 * i + 3] = wi[i];
+            rp[8 * i + 4] = pi[i];
+            rp[8 * i + 5] = (double)c->plan.cls[i];
+        }
+        upload(c->vjp_w, wi.data(), n, c->stream);
+        upload(c->vjp_phi, pi.data(), n, c->stream);
+        upload(c->vjp_psi, si.data(), n, c->stream);
+        upload(c->vjp_rows, rp.data(), rp.size(), c->stream);
+        const int npairs = (int)c->plan.sing_cell.size();
+        c->vjp_cstate.ensure(sizeof(double) * 2 * (size_t)nc * nq);
+        c->vjp_p1.ensure(sizeof(double) * 3 * (size_t)vjp_row_chunks(nc) * n);
+        c->vjp_p2.ensure(sizeof(double) * 7 * (size_t)vjp_point_chunks(n) * nc * nq);
+        c->vjp_xg.ensure(sizeof(double) * 3 * (size_t)n);
+        c->vjp_vbar.ensure(sizeof(double) * 12 * (size_t)nc);
+        c->vjp_xs.ensure(sizeof(double) * 3 * (size_t)n);
+        c->vjp_spart.ensure(sizeof(double) * 12 * (size_t)std::max(npairs, 1));
+        launch_residual_vjp(c->quad.q_order, n, nc, c->X.as<double>(), c->row_cls.as<int>(),
+                            c->vjp_w.as<double>(), c->vjp_phi.as<double>(),
+                            c->vjp_psi.as<double>(), c->vjp_rows.as<double>(),
+                            c->cellq.as<double>(), c->cell_int.as<int>(), c->cell_cls.as<int>(),
+                            c->orient.as<int8_t>(), c->qpts.as<double>(), c->qw.as<double>(),
+                            c->sing_ptr.as<int>(), c->sing_cell.as<int>(), c->sing_vtx.as<int>(),
+                            ns, c->srule.as<double>(), c->vjp_cstate.as<double>(),
+                            c->vjp_p1.as<double>(), c->vjp_p2.as<double>(),
+                            c->vjp_xg.as<double>(), c->vjp_vbar.as<double>(),
+                            c->vjp_xs.as<double>(), c->vjp_spart.as<double>(), c->stream);
+        HIPCHK(hipGetLastError());
+        std::vector<double> xg(3 * (size_t)n), xsg(3 * (size_t)n), vbar(12 * (size_t)nc),
+            sp(12 * (size_t)npairs);
+        HIPCHK(hipMemcpyAsync(xg.data(), c->vjp_xg.p, sizeof(double) * xg.size(),
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(xsg.data(), c->vjp_xs.p, sizeof(double) * xsg.size(),
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(vbar.data(), c->vjp_vbar.p, sizeof(double) * vbar.size(),
+                              hipMemcpyDeviceToHost, c->stream));
+        if (npairs)
+            HIPCHK(hipMemcpyAsync(sp.data(), c->vjp_spart.p, sizeof(double) * sp.size(),
+                                  hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        // per support point (internal index), in a fixed order: row part, regular cell
+        // vertices, singular cell vertices; then to the caller's numbering
+        std::vector<double> g(3 * (size_t)n, 0.0);
+        for (int i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) g[3 * i + d] = xg[3 * i + d] + xsg[3 * i + d];
+        for (int cc = 0; cc < nc; ++cc)
+            for (int k = 0; k < 4; ++k) {
+                const int v = c->plan.cell_int[4 * cc + k];
+                for (int d = 0; d < 3; ++d) g[3 * v + d] += vbar[12 * (size_t)cc + 3 * k + d];
+            }
+        for (int p = 0; p < npairs; ++p) {
+            const int cc = c->plan.sing_cell[p];
+            for (int k = 0; k < 4; ++k) {
+                const int v = c->plan.cell_int[4 * cc + k];
+                for (int d = 0; d < 3; ++d) g[3 * v + d] += sp[12 * (size_t)p + 3 * k + d];
+            }
+        }
+        for (int dof = 0; dof < n; ++dof) {
+            const int i = c->plan.perm[dof];
+            for (int d = 0; d < 3; ++d) grad[3 * (size_t)dof + d] = g[3 * i + d];
+        }
+    });
+}
+
 wb_status wb_solve_system_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi,
                                 double *dphi_dn, const double *inhom_k, wb_solve_stats *st)
 {
@@ -1573,7 +1774,10 @@ size_t wb_device_bytes(const wb_ctx *c)
                             &c->con_w, &c->con_inhom, &c->constrained, &c->free_rows, &c->binv,
                             &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2, &c->Pn, &c->hbuf, &c->hred,
                             &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32, &c->z32, &c->w32,
-                            &c->r32, &c->d32};
+                            &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
+                            &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,
+                            &c->vjp_rows, &c->vjp_cstate, &c->vjp_p1, &c->vjp_p2, &c->vjp_xg,
+                            &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart};
     for (const DevBuf *b : bufs) t += b->bytes;
     for (DevBuf *b : const_cast<wb_ctx *>(c)->mw.all()) t += b->bytes;
     return t;

@@ -115,7 +115,30 @@ void launch_block_jacobi_build(int n, int bs, const void *G, const void *H, cons
                                const uint8_t *constrained, double *binv, float *binv32,
                                hipStream_t s);
 void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, const void *x,
-                               void *y, int k, size_t ldx, hipStream_t s);
+                               void *y, int k, size_t ldx, hipStream_t s, int trans = 0);
+
+// ---------------------------------------------------------------- adjoint (wb_adjoint.hip)
+// y = A_sel^T x over the listed rows (FP64 matrices) + diag .* x on unconstrained rows;
+// part: gemvT_chunks(n_rows) x ld doubles
+int gemvT_chunks(int n_rows);
+void launch_gemvT(const double *G, const double *H, size_t ld, int n, const uint8_t *gmask,
+                  int flip, double s_sel, double s_unsel, const double *x, const int *rows,
+                  int n_rows, const double *diag, const uint8_t *constrained, double *part,
+                  double *y, hipStream_t s);
+void launch_constraint_T(int n_targets, const int *tgt, const int *ptr, const int *src,
+                         const double *coef, const double *x, double *y, hipStream_t s);
+// backward of the assembly; buffer sizes: cstate nc*nq*2, part1 vjp_row_chunks(nc)*n*3,
+// part2 vjp_point_chunks(n)*nc*nq*7, xgrad n*3, vbar nc*12, xs n*3, spart n_sing*12
+int vjp_row_chunks(int nc);
+int vjp_point_chunks(int n);
+void launch_residual_vjp(int q_order, int n, int nc, const double *X, const int *row_cls,
+                         const double *wrow, const double *phi, const double *psi,
+                         const double *rowpack, const double *cellq, const int *cell_int,
+                         const int *cell_cls, const int8_t *orient, const double *qpts,
+                         const double *qw, const int *sing_ptr, const int *sing_cell,
+                         const int *sing_vtx, int ns, const double *srule, double *cstate,
+                         double *part1, double *part2, double *xgrad, double *vbar,
+                         double *xs, double *spart, hipStream_t s);
 
 // Krylov helpers (is_float selects float vectors).  batch instances on blockIdx.y with
 // element strides sV (basis), sZ / sX (vectors), sY / sH (coefficients); partial buffers

@@ -206,7 +206,8 @@ void launch_block_jacobi_build(int n, int bs, const void *G, const void *H, cons
 // y_blk = Binv_blk x_blk for k vectors (column stride ldx); one thread per block row.
 template <typename T>
 __global__ void k_block_jacobi_apply(int n, int bs, const T *__restrict__ binv,
-                                     const T *__restrict__ x, T *__restrict__ y, int k, size_t ldx)
+                                     const T *__restrict__ x, T *__restrict__ y, int k, size_t ldx,
+                                     int trans)
 {
     extern __shared__ char smraw[];
     T *xs = (T *)smraw;
@@ -218,7 +219,10 @@ __global__ void k_block_jacobi_apply(int n, int bs, const T *__restrict__ binv,
         __syncthreads();
         if (r < m) {
             T acc = 0;
-            for (int c = 0; c < m; ++c) acc += B[c * bs + r] * xs[c];
+            if (trans)  // the transposed block inverse, for the adjoint system
+                for (int c = 0; c < m; ++c) acc += B[r * bs + c] * xs[c];
+            else
+                for (int c = 0; c < m; ++c) acc += B[c * bs + r] * xs[c];
             y[(size_t)v * ldx + base + r] = acc;
         }
         __syncthreads();
@@ -226,18 +230,18 @@ __global__ void k_block_jacobi_apply(int n, int bs, const T *__restrict__ binv,
 }
 
 void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, const void *x,
-                               void *y, int k, size_t ldx, hipStream_t s)
+                               void *y, int k, size_t ldx, hipStream_t s, int trans)
 {
     const int nb = (n + bs - 1) / bs;
     const int threads = bs <= 64 ? 64 : 128;
     if (is_float)
         hipLaunchKernelGGL(k_block_jacobi_apply<float>, dim3(nb), dim3(threads),
                            sizeof(float) * bs, s, n, bs, (const float *)binv, (const float *)x,
-                           (float *)y, k, ldx);
+                           (float *)y, k, ldx, trans);
     else
         hipLaunchKernelGGL(k_block_jacobi_apply<double>, dim3(nb), dim3(threads),
                            sizeof(double) * bs, s, n, bs, (const double *)binv,
-                           (const double *)x, (double *)y, k, ldx);
+                           (const double *)x, (double *)y, k, ldx, trans);
 }
 
 // elementwise ops on n entries:
