@@ -81,6 +81,61 @@ def kernel_avg_ms(ks, name):
     return k["total_ms"] / k["launches"] if k["launches"] else float("nan")
 
 
+def run_adjoint(workload, steps):
+    """Hull-drag gradient with respect to 500 free-surface elevations and 200 hull node
+    positions (the adjoint configuration): forward solve, transposed solve, backward of
+    the assembly; FP64.  Returns a dict of timings (ms, mean over `steps`)."""
+    import torch
+
+    from wavebem_amd import mesh as M
+    from wavebem_amd.adjoint import bem_solve, hull_drag
+    from wavebem_amd.bem import BEMProblem, Config
+
+    m = M.wigley_preset(workload)
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    bc, U = M.wigley_bc(m, seed=0)
+    rng = np.random.default_rng(7)
+    rep = np.array([idx[ptr[d]:ptr[d + 1]].min() for d in range(m.n_dofs)])
+    fs = np.unique(rep[np.unique(m.cell_dofs[m.cell_material == M.FREE_SURFACE])])
+    hull_cells = m.cell_material == M.HULL
+    hn = np.unique(rep[np.unique(m.cell_dofs[hull_cells])])
+    fs_sel = rng.choice(fs, 500, replace=False)
+    hull_sel = rng.choice(hn, 200, replace=False)
+    X0 = m.support_points.copy()
+    for d in fs_sel:
+        X0[idx[ptr[d]:ptr[d + 1]], 2] += rng.uniform(-5e-3, 5e-3)
+    phi_t = torch.zeros(m.n_dofs, dtype=torch.float64)
+    times = {"forward": [], "adjoint_solve": [], "backward_assembly": [], "total": []}
+    with BEMProblem(m, Config(solver_tolerance=1e-10), double_nodes=(ptr, idx)) as p:
+        p.compute_constraints(bc)
+        cells, orient = m.cell_dofs[hull_cells], m.cell_orient[hull_cells]
+        for s in range(steps + 1):
+            X = torch.tensor(X0, dtype=torch.float64, requires_grad=True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            phi, dphi = bem_solve(X, p, bc)
+            t1 = time.perf_counter()
+            J = hull_drag(X, cells, orient, phi, dphi, phi_t, U)
+            # the backward pieces, timed one by one
+            g_phi, g_dphi = torch.autograd.grad(J, (phi, dphi), retain_graph=True)
+            g_u = g_phi.numpy() * p.other_nodes + g_dphi.numpy() * p.surface_nodes
+            t2 = time.perf_counter()
+            lam = p.solve_adjoint(g_u)
+            t3 = time.perf_counter()
+            p.residual_vjp(lam, phi.detach().numpy(), dphi.detach().numpy())
+            t4 = time.perf_counter()
+            if s == 0:
+                continue  # warmup
+            times["forward"].append((t1 - t0) * 1e3)
+            times["adjoint_solve"].append((t3 - t2) * 1e3)
+            times["backward_assembly"].append((t4 - t3) * 1e3)
+            times["total"].append((t4 - t0) * 1e3)
+    out = {k: round(float(np.mean(v)), 3) for k, v in times.items()}
+    out["n_dofs"] = m.n_dofs
+    out["gradient_entries"] = 500 + 3 * 200
+    return out
+
+
 def profiled_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this
     workload (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy kernel), or None."""
@@ -138,6 +193,8 @@ def main():
     ap.add_argument("--tile", type=int, default=16, help="max columns per assembly tile (1..16)")
     ap.add_argument("--precond-block", type=int, default=64,
                     help="block-Jacobi block size (rows of the device ordering, <= 128)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only time the drag-gradient adjoint (default workload 10k) and exit")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (4 launches) and exit: the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -152,6 +209,16 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.adjoint:
+        wl = args.workload if args.workload != "40k" else "10k"
+        r = run_adjoint(wl, args.steps)
+        print(json.dumps({"metric": "hull-drag gradient (adjoint) time", "unit": "ms",
+                          "value": r["total"], "higher_is_better": False, "dtype": "f64",
+                          "data": "synthetic", "config": {"workload": f"wigley_closed_halfbox_{wl}",
+                                                          "n_dofs": r["n_dofs"],
+                                                          "gradient_entries": r["gradient_entries"]},
+                          "phases_ms": r}), flush=True)
+        return
     if args.copy_calibrate:
         nb = args.copy_calibrate << 20
         src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")

@@ -6,9 +6,9 @@
 // system of source/bem_problem.cc:821-895), dJ/dtheta = dJ/dtheta|_u - lambda^T dR/dtheta
 // with A_c^T lambda = dJ/du.  This file holds:
 //   * the transposed column-selected product A_sel^T x (kernel k_gemvT_f64): every
-//     workgroup owns a strip of 128 columns and a chunk of rows; lanes read whole row
-//     segments (coalesced, like the forward product) and keep their two columns' sums in
-//     registers; the chunk partials are added in a fixed order;
+//     workgroup owns a strip of 128 columns and a chunk of 512 rows; lanes read whole row
+//     segments (coalesced, like the forward product), four rows per step, and keep their
+//     two columns' sums in registers; the chunk partials are added in a fixed order;
 //   * the transposed constraint lines (k_constraint_T, a column-wise gather built on the
 //     host, so there are no atomics);
 //   * the backward of assemble_system with respect to the support points: for row i with
@@ -35,11 +35,14 @@ typedef double dvec2 __attribute__((ext_vector_type(2)));
 constexpr int GT_WAVES = 4;
 constexpr int GT_COLS = 128;  // columns per workgroup: 64 lanes x 2
 
-// part[chunk][col] = sum over the chunk's rows i of sel(col) * A_sel(col)(i, col) * x[i]
+// part[chunk][col] = sum over the chunk's unconstrained rows i of
+// sel(col) * A_sel(col)(i, col) * x[i].  Rows are taken in order (constrained rows read
+// with weight 0), four per wavefront step, so the row addresses are affine and the loads
+// of a step are independent.
 __global__ __launch_bounds__(64 * GT_WAVES) void k_gemvT_f64(
     const double *__restrict__ G, const double *__restrict__ H, size_t ld, int n,
     const uint8_t *__restrict__ gmask, int flip, double s_sel, double s_uns,
-    const double *__restrict__ x, const int *__restrict__ rows, int n_rows, int rows_per_chunk,
+    const double *__restrict__ x, const uint8_t *__restrict__ constrained, int rows_per_chunk,
     double *__restrict__ part)
 {
     __shared__ double red[GT_WAVES][64][2];
@@ -55,22 +58,44 @@ __global__ __launch_bounds__(64 * GT_WAVES) void k_gemvT_f64(
     const double *M0 = b0 ? H : G, *M1 = b1 ? H : G;
     const bool same = b0 == b1;
     double a0 = 0.0, a1 = 0.0;
-    const int s0 = blockIdx.y * rows_per_chunk;
-    const int s1 = min(n_rows, s0 + rows_per_chunk);
+    const int i0 = blockIdx.y * rows_per_chunk;
+    const int i1 = min(n, i0 + rows_per_chunk);
+    constexpr int U = 4;  // rows per wavefront step
+    auto xw = [&](int i) { return constrained[i] ? 0.0 : x[i]; };
     if (live) {
-        for (int s = s0 + wave; s < s1; s += GT_WAVES) {
-            const int i = __builtin_amdgcn_readfirstlane(rows ? rows[s] : s);
-            const double xi = x[i];
-            const size_t off = (size_t)i * ld + col;
+        const int nfull = (i1 - i0) / U;  // whole groups of U rows; group g to wave g % 4
+        for (int g = wave; g < nfull; g += GT_WAVES) {
+            const int i = i0 + g * U;
+            double xi[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) xi[u] = xw(i + u);
             if (same) {
-                const dvec2 v = __builtin_nontemporal_load((const dvec2 *)(M0 + off));
-                a0 += v.x * xi;
-                a1 += v.y * xi;
+                dvec2 v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    v[u] = __builtin_nontemporal_load(
+                        (const dvec2 *)(M0 + (size_t)(i + u) * ld + col));
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    a0 += v[u].x * xi[u];
+                    a1 += v[u].y * xi[u];
+                }
             } else {
-                a0 += M0[off] * xi;
-                a1 += M1[off + 1] * xi;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const size_t off = (size_t)(i + u) * ld + col;
+                    a0 += M0[off] * xi[u];
+                    a1 += M1[off + 1] * xi[u];
+                }
             }
         }
+        if (wave == 0)  // the last (i1 - i0) % U rows
+            for (int i = i0 + nfull * U; i < i1; ++i) {
+                const double xv = xw(i);
+                const size_t off = (size_t)i * ld + col;
+                a0 += M0[off] * xv;
+                a1 += M1[off + 1] * xv;
+            }
     }
     red[wave][lane][0] = a0 * (b0 ? s_sel : s_uns);
     red[wave][lane][1] = a1 * (b1 ? s_sel : s_uns);
@@ -391,18 +416,19 @@ __global__ void k_sum_chunks(const double *__restrict__ part, int n_chunks, size
 
 }  // namespace
 
-int gemvT_chunks(int n_rows) { return max(1, min(32, (n_rows + 2047) / 2048)); }
+// 512-row chunks: about 4 x n / 128 wavefronts, enough to keep HBM busy at 10k dofs
+int gemvT_chunks(int n) { return max(1, min(128, (n + 511) / 512)); }
 
 void launch_gemvT(const double *G, const double *H, size_t ld, int n, const uint8_t *gmask,
-                  int flip, double s_sel, double s_unsel, const double *x, const int *rows,
-                  int n_rows, const double *diag, const uint8_t *constrained, double *part,
-                  double *y, hipStream_t s)
+                  int flip, double s_sel, double s_unsel, const double *x,
+                  const double *diag, const uint8_t *constrained, double *part, double *y,
+                  hipStream_t s)
 {
-    const int chunks = gemvT_chunks(n_rows);
-    const int rpc = (n_rows + chunks - 1) / chunks;
+    const int chunks = gemvT_chunks(n);
+    const int rpc = (n + chunks - 1) / chunks;
     dim3 grid((n + GT_COLS - 1) / GT_COLS, chunks);
     hipLaunchKernelGGL(k_gemvT_f64, grid, dim3(64 * GT_WAVES), 0, s, G, H, ld, n, gmask, flip,
-                       s_sel, s_unsel, x, rows, n_rows, rpc, part);
+                       s_sel, s_unsel, x, constrained, rpc, part);
     hipLaunchKernelGGL(k_gemvT_reduce, dim3((n + 255) / 256), dim3(256), 0, s, part, chunks, ld,
                        n, diag, constrained, x, y);
 }

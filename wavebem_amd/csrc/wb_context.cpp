@@ -296,13 +296,13 @@ static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
         // A_c^T x: transposed product over the unconstrained rows (the diagonal alpha term
         // of those rows included), then the transposed constraint lines
         require(mode == 0, WB_E_UNSUPPORTED, "transposed operator needs FP64 matrices");
-        const int chunks = gemvT_chunks(c->n_free);
+        const int chunks = gemvT_chunks(c->n);
         c->gt_part.ensure(sizeof(double) * c->ld * chunks);
         KernelScope ks(c, WB_K_GEMV64, (long long)c->n_free);
         launch_gemvT(c->G.as<double>(), c->H.as<double>(), c->ld, c->n, c->gmask.as<uint8_t>(), 0,
-                     1.0, -1.0, (const double *)x, c->free_rows.as<int>(), c->n_free,
-                     c->diag_op.as<double>(), c->constrained.as<uint8_t>(),
-                     c->gt_part.as<double>(), (double *)y, c->stream);
+                     1.0, -1.0, (const double *)x, c->diag_op.as<double>(),
+                     c->constrained.as<uint8_t>(), c->gt_part.as<double>(), (double *)y,
+                     c->stream);
         launch_constraint_T(c->n_ct, c->ct_tgt.as<int>(), c->ct_ptr.as<int>(),
                             c->ct_src.as<int>(), c->ct_coef.as<double>(), (const double *)x,
                             (double *)y, c->stream);

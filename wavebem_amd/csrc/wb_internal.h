@@ -118,13 +118,13 @@ void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, co
                                void *y, int k, size_t ldx, hipStream_t s, int trans = 0);
 
 // ---------------------------------------------------------------- adjoint (wb_adjoint.hip)
-// y = A_sel^T x over the listed rows (FP64 matrices) + diag .* x on unconstrained rows;
-// part: gemvT_chunks(n_rows) x ld doubles
-int gemvT_chunks(int n_rows);
+// y = A_sel^T x over the unconstrained rows (FP64 matrices) + diag .* x on those rows;
+// part: gemvT_chunks(n) x ld doubles
+int gemvT_chunks(int n);
 void launch_gemvT(const double *G, const double *H, size_t ld, int n, const uint8_t *gmask,
-                  int flip, double s_sel, double s_unsel, const double *x, const int *rows,
-                  int n_rows, const double *diag, const uint8_t *constrained, double *part,
-                  double *y, hipStream_t s);
+                  int flip, double s_sel, double s_unsel, const double *x,
+                  const double *diag, const uint8_t *constrained, double *part, double *y,
+                  hipStream_t s);
 void launch_constraint_T(int n_targets, const int *tgt, const int *ptr, const int *src,
                          const double *coef, const double *x, double *y, hipStream_t s);
 // backward of the assembly; buffer sizes: cstate nc*nq*2, part1 vjp_row_chunks(nc)*n*3,
