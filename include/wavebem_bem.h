@@ -198,6 +198,19 @@ wb_status wb_solve_adjoint(wb_ctx *ctx, const double *rhs, double *lambda,
 wb_status wb_residual_vjp(wb_ctx *ctx, const double *w, const double *phi,
                           const double *dphi_dn, int free_rows_only, double *grad);
 
+/* Tangent of the same residual: dres (n, caller numbering) with
+ * dres_i = sum_d dR_i/dx_d . dx_d for a direction dx (n x 3) of the support points; rows of
+ * the installed constraint lines are 0 with free_rows_only.  Needs wb_assemble at the
+ * current geometry. */
+wb_status wb_residual_jvp(wb_ctx *ctx, const double *dx, const double *phi,
+                          const double *dphi_dn, int free_rows_only, double *dres);
+
+/* Solves A_c sol = rhs with the operator, preconditioner and solver of solve_system for a
+ * given right-hand side (no boundary-data mapping, no constraint inhomogeneities): the
+ * linear solve of a tangent, e.g. du = -A_c^-1 (dR/dx dx) for a Newton JVP.  sol holds the
+ * unknowns (phi on Neumann dofs, dphi_dn on Dirichlet dofs), caller numbering. */
+wb_status wb_solve_rhs(wb_ctx *ctx, const double *rhs, double *sol, wb_solve_stats *stats);
+
 /* Multi right-hand side form of solve_system (no reference counterpart: the reference
  * calls solve_system once per time step, source/bem_problem.cc:1013-1108): k
  * boundary-data vectors (column j at tmp_rhs + j*n), solved in batches of up to

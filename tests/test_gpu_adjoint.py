@@ -155,3 +155,62 @@ def test_drag_gradient_matches_central_differences(gpu_available, preset, n_fs, 
           f"entries; checked {fd.size}: |fd| max {np.abs(fd).max():.3e}, "
           f"adjoint vs FD max rel err {err:.2e}")
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("kind", ["wigley_small", "sphere4"])
+def test_residual_jvp_is_transpose_of_vjp(gpu_available, kind):
+    """<w, JVP(dx)> = <VJP(w), dx>: the tangent and the backward are the same derivative."""
+    m, ptr, idx, H, G = T.oracle_assembly(kind)
+    n = m.n_dofs
+    rng = np.random.default_rng(13)
+    w, phi, psi = (rng.standard_normal(n) for _ in range(3))
+    dx = rng.standard_normal((n, 3))
+    with BEMProblem(m, Config(), double_nodes=(ptr, idx)) as p:
+        p.assemble_system()
+        jv = p.residual_jvp(dx, phi, psi, free_rows_only=False)
+        vj = p.residual_vjp(w, phi, psi, free_rows_only=False)
+    a, b = float(w @ jv), float((vj * dx).sum())
+    print(f"  [{kind}] <w, Jdx> = {a:.12e}  <J^T w, dx> = {b:.12e}")
+    assert abs(a - b) <= 1e-12 * max(abs(a), abs(b))
+
+
+def test_solve_rhs_matches_dense(gpu_available):
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    bc, _ = M.wigley_bc(m, seed=4)
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, bc)
+    A, _ = T.explicit_system(H, G, m.surface_nodes, m.other_nodes, cons, bc)
+    r = np.random.default_rng(14).standard_normal(m.n_dofs)
+    ref = np.linalg.solve(A, r)
+    with BEMProblem(m, Config(solver_tolerance=1e-13), double_nodes=(ptr, idx)) as p:
+        p.compute_constraints(bc)
+        p.assemble_system()
+        x = p.solve_rhs(r)
+    assert np.abs(x - ref).max() / np.abs(ref).max() < 1e-8
+
+
+def test_solution_jvp_matches_central_differences(gpu_available):
+    """Tangent of the solution along a seeded motion of free-surface and hull nodes (whole
+    double-node classes) against central differences of the GPU forward solve."""
+    from wavebem_amd.adjoint import bem_solve_jvp
+
+    m = M.wigley_preset("2.5k")
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    bc, U = M.wigley_bc(m, seed=0)
+    rng = np.random.default_rng(15)
+    X0 = m.support_points.copy()
+    dx = np.zeros_like(X0)
+    nodes = rng.choice(m.n_dofs, 60, replace=False)
+    for d in nodes:
+        dx[idx[ptr[d]:ptr[d + 1]]] = rng.uniform(-1.0, 1.0, 3)
+    h = 1e-6
+    with BEMProblem(m, Config(solver_tolerance=1e-13, solver_max_steps=3000),
+                    double_nodes=(ptr, idx)) as p:
+        p.compute_constraints(bc)
+        phi, dphi, tphi, tdphi = bem_solve_jvp(p, X0, bc, dx)
+        pp, dp, _, _ = bem_solve_jvp(p, X0 + h * dx, bc, np.zeros_like(dx))
+        pm, dm, _, _ = bem_solve_jvp(p, X0 - h * dx, bc, np.zeros_like(dx))
+    fd = np.concatenate([(pp - pm) / (2 * h), (dp - dm) / (2 * h)])
+    jv = np.concatenate([tphi, tdphi])
+    err = np.abs(jv - fd).max() / np.abs(fd).max()
+    print(f"  tangent vs FD: max |fd| {np.abs(fd).max():.3e}, rel err {err:.2e}")
+    assert err < 1e-6

@@ -98,6 +98,8 @@ _SIGS = {
     "wb_get_matrix_rows": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _dp, _dp]),
     "wb_solve_adjoint": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_void_p]),
     "wb_residual_vjp": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, _dp]),
+    "wb_residual_jvp": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, _dp]),
+    "wb_solve_rhs": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_void_p]),
     "wb_compute_alpha": (ctypes.c_int, [_vp, _dp]),
     "wb_vmult": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_compute_rhs": (ctypes.c_int, [_vp, _dp, _dp]),

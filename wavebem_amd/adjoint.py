@@ -100,6 +100,23 @@ def bem_solve(support, problem, tmp_rhs):
     return _BEMSolve.apply(support, problem, np.ascontiguousarray(tmp_rhs, dtype=np.float64))
 
 
+def bem_solve_jvp(problem, support, tmp_rhs, dx):
+    """Tangent of the BEM solve: (phi, dphi_dn) at ``support`` and their directional
+    derivatives along a support-point direction ``dx`` (n, 3), boundary data fixed:
+    du = -A_c^-1 (dR/dx dx), scattered like solve_system (phi on Neumann dofs, dphi_dn on
+    Dirichlet dofs; the known entries do not move)."""
+    x = np.ascontiguousarray(support, dtype=np.float64)
+    tmp = np.ascontiguousarray(tmp_rhs, dtype=np.float64)
+    problem.set_support_points(x)
+    problem.assemble_system()
+    phi = tmp * problem.surface_nodes
+    dphi = tmp * problem.other_nodes
+    problem.solve_system(phi, dphi, tmp)
+    dr = problem.residual_jvp(dx, phi, dphi, free_rows_only=True)
+    du = problem.solve_rhs(-dr)
+    return phi, dphi, du * problem.other_nodes, du * problem.surface_nodes
+
+
 def class_gradient(grad, dn_ptr, dn_idx):
     """Per double-node class: moving a geometric node moves all its coincident dofs, so
     the gradient of a node is the sum over its dofs.  Returns (n, 3) with every dof holding

@@ -283,6 +283,24 @@ class BEMProblem:
                                             int(free_rows_only), _d(g)), "wb_residual_vjp")
         return g
 
+    def residual_jvp(self, dx, phi, dphi_dn, free_rows_only=True):
+        """(n,): sum_d dR_i/dx_d . dx_d for a support-point direction dx (n, 3)."""
+        out = np.zeros(self.n)
+        self._check(L.lib().wb_residual_jvp(self._ctx, _d(_f64(dx, 3 * self.n)),
+                                            _d(_f64(phi, self.n)), _d(_f64(dphi_dn, self.n)),
+                                            int(free_rows_only), _d(out)), "wb_residual_jvp")
+        return out
+
+    def solve_rhs(self, rhs):
+        """sol with A_c sol = rhs (the solve_system operator, a given right-hand side)."""
+        sol = np.zeros(self.n)
+        st = L.WbSolveStats()
+        status = L.lib().wb_solve_rhs(self._ctx, _d(_f64(rhs, self.n)), _d(sol),
+                                      ctypes.byref(st))
+        self.last_stats = st.as_dict()
+        self._check(status, "wb_solve_rhs")
+        return sol
+
     def solve_system_dev(self, phi_ptr, dphi_dn_ptr, tmp_rhs_ptr):
         """Device-pointer form (caller dof order, n float64 each)."""
         st = L.WbSolveStats()

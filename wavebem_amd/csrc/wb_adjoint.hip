@@ -403,6 +403,185 @@ __global__ __launch_bounds__(64) void k_vjp_singular(
     for (int d = 0; d < 3; ++d) xs[3 * i + d] = gx[d];
 }
 
+// ------------------------------------------------------------------ tangent (JVP)
+// Directional derivatives of (y, N, J) at every regular point for vertex velocities dX:
+// dy = sum sh_k dv_k, dc = da x b + a x db, dN = o w dc, dJ = w (c . dc) / |c|.
+// trec[c][q][8] = {dy (3), dN (3), dJ, 0}.
+__global__ void k_tangent_points(int nc, int nq, const double *__restrict__ X,
+                                 const double *__restrict__ dX, const int *__restrict__ cell_int,
+                                 const int8_t *__restrict__ orient,
+                                 const double *__restrict__ qpts, const double *__restrict__ qw,
+                                 double *__restrict__ trec)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nc * nq) return;
+    const int c = gid / nq, q = gid % nq;
+    double sh[4], dx[4], de[4];
+    q1_shape(qpts[2 * q], qpts[2 * q + 1], sh, dx, de);
+    double a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, da[3] = {0, 0, 0}, db[3] = {0, 0, 0},
+           dy[3] = {0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        const int v = cell_int[4 * c + k];
+        for (int d = 0; d < 3; ++d) {
+            const double p = X[3 * v + d], dp = dX[3 * v + d];
+            a[d] += dx[k] * p;
+            b[d] += de[k] * p;
+            da[d] += dx[k] * dp;
+            db[d] += de[k] * dp;
+            dy[d] += sh[k] * dp;
+        }
+    }
+    const double cx[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                          a[0] * b[1] - a[1] * b[0]};
+    const double dc[3] = {da[1] * b[2] - da[2] * b[1] + a[1] * db[2] - a[2] * db[1],
+                          da[2] * b[0] - da[0] * b[2] + a[2] * db[0] - a[0] * db[2],
+                          da[0] * b[1] - da[1] * b[0] + a[0] * db[1] - a[1] * db[0]};
+    const double cl = sqrt(cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
+    const double so = orient[c] < 0 ? -1.0 : 1.0, w = qw[q];
+    double *t = trec + (size_t)gid * 8;
+    for (int d = 0; d < 3; ++d) {
+        t[d] = dy[d];
+        t[3 + d] = so * w * dc[d];
+    }
+    t[6] = w * (cx[0] * dc[0] + cx[1] * dc[1] + cx[2] * dc[2]) / cl;
+    t[7] = 0.0;
+}
+
+// part[chunk][row] = sum over the chunk's regular (row, cell) pairs of
+// ds/dR . (dy - dx_i) + ds/dN . dN + ds/dJ dJ  (without the 1/(4 pi)); lane = row.
+template <int Q>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_jvp_rows(
+    const double *__restrict__ xrow, const double *__restrict__ dxrow,
+    const int *__restrict__ row_cls, const double *__restrict__ phi,
+    const double *__restrict__ cellq, const double *__restrict__ cstate,
+    const double *__restrict__ trec, const int4 *__restrict__ cell_cls, int nc,
+    int cells_per_chunk, int n, double *__restrict__ part)
+{
+    constexpr int NQ = Q * Q;
+    const int lane = threadIdx.x;
+    const int row = blockIdx.x * 64 + lane;
+    const int rr = row < n ? row : n - 1;
+    const double x0 = xrow[3 * rr], x1 = xrow[3 * rr + 1], x2 = xrow[3 * rr + 2];
+    const double e0 = dxrow[3 * rr], e1 = dxrow[3 * rr + 1], e2 = dxrow[3 * rr + 2];
+    const int rc = row_cls[rr];
+    const double phi_i = phi[rr];
+    double acc = 0.0;
+    const int c0 = blockIdx.y * cells_per_chunk;
+    const int c1 = min(nc, c0 + cells_per_chunk);
+    for (int c = c0; c < c1; ++c) {
+        const int4 cc = cell_cls[c];
+        const double *rec = cellq + (size_t)c * (NQ * 8);
+        const double *st = cstate + (size_t)c * (NQ * 2);
+        const double *tr = trec + (size_t)c * (NQ * 8);
+        double g = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const double *r = rec + 8 * q, *t = tr + 8 * q;
+            const double R0 = r[0] - x0, R1 = r[1] - x1, R2 = r[2] - x2;
+            const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
+            const double ri = rsqrt_f64(r2);
+            const double ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
+            const double RN = __builtin_fma(R0, r[3], __builtin_fma(R1, r[4], R2 * r[5]));
+            const double D = st[2 * q] - phi_i, E = st[2 * q + 1];
+            const double t1 = -D * ri3;
+            const double t2 = __builtin_fma(3.0 * D * RN, ri5, E * r[6] * ri3);
+            const double d0 = t[0] - e0, d1 = t[1] - e1, d2 = t[2] - e2;  // dR
+            const double Nd = __builtin_fma(r[3], d0, __builtin_fma(r[4], d1, r[5] * d2));
+            const double Rd = __builtin_fma(R0, d0, __builtin_fma(R1, d1, R2 * d2));
+            const double RdN = __builtin_fma(R0, t[3], __builtin_fma(R1, t[4], R2 * t[5]));
+            g += t1 * (Nd + RdN) + t2 * Rd - E * ri * t[6];
+        }
+        const bool sing = (rc == cc.x) | (rc == cc.y) | (rc == cc.z) | (rc == cc.w);
+        if (!sing) acc += g;
+    }
+    if (row < n) part[(size_t)blockIdx.y * n + row] = acc;
+}
+
+// Singular pairs of the tangent: ds[i] (without 1/(4 pi)), one thread per row.
+__global__ __launch_bounds__(64) void k_jvp_singular(
+    const double *__restrict__ X, const double *__restrict__ dX, const int *__restrict__ cell_int,
+    const int8_t *__restrict__ orient, const int *__restrict__ sing_ptr,
+    const int *__restrict__ sing_cell, const int *__restrict__ sing_vtx, int n, int ns,
+    const double *__restrict__ rule, const double *__restrict__ phi,
+    const double *__restrict__ psi, double *__restrict__ ds)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double xi0[3] = {X[3 * i], X[3 * i + 1], X[3 * i + 2]};
+    const double dxi[3] = {dX[3 * i], dX[3 * i + 1], dX[3 * i + 2]};
+    const double phi_i = phi[i];
+    double acc = 0.0;
+    for (int p = sing_ptr[i]; p < sing_ptr[i + 1]; ++p) {
+        const int c = sing_cell[p], vtx = sing_vtx[p];
+        double v[4][3], dv[4][3], pv[4], sv[4];
+        for (int k = 0; k < 4; ++k) {
+            const int id = cell_int[4 * c + k];
+            for (int d = 0; d < 3; ++d) {
+                v[k][d] = X[3 * id + d];
+                dv[k][d] = dX[3 * id + d];
+            }
+            pv[k] = phi[id];
+            sv[k] = psi[id];
+        }
+        const double so = orient[c] < 0 ? -1.0 : 1.0;
+        const double *ru = rule + (size_t)vtx * ns * 3;
+        for (int q = 0; q < ns; ++q) {
+            const double xi = ru[3 * q], eta = ru[3 * q + 1], w = ru[3 * q + 2];
+            double sh[4], dx[4], de[4];
+            q1_shape(xi, eta, sh, dx, de);
+            double y[3] = {0, 0, 0}, a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, dy[3] = {0, 0, 0},
+                   da[3] = {0, 0, 0}, db[3] = {0, 0, 0};
+            double P = 0.0, S = 0.0;
+            for (int k = 0; k < 4; ++k) {
+                for (int d = 0; d < 3; ++d) {
+                    y[d] += sh[k] * v[k][d];
+                    a[d] += dx[k] * v[k][d];
+                    b[d] += de[k] * v[k][d];
+                    dy[d] += sh[k] * dv[k][d];
+                    da[d] += dx[k] * dv[k][d];
+                    db[d] += de[k] * dv[k][d];
+                }
+                P += sh[k] * pv[k];
+                S += sh[k] * sv[k];
+            }
+            const double cx[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                                  a[0] * b[1] - a[1] * b[0]};
+            const double dc[3] = {da[1] * b[2] - da[2] * b[1] + a[1] * db[2] - a[2] * db[1],
+                                  da[2] * b[0] - da[0] * b[2] + a[2] * db[0] - a[0] * db[2],
+                                  da[0] * b[1] - da[1] * b[0] + a[0] * db[1] - a[1] * db[0]};
+            const double cl = sqrt(cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
+            const double N[3] = {so * w * cx[0], so * w * cx[1], so * w * cx[2]};
+            const double dN[3] = {so * w * dc[0], so * w * dc[1], so * w * dc[2]};
+            const double J = w * cl;
+            const double dJ = w * (cx[0] * dc[0] + cx[1] * dc[1] + cx[2] * dc[2]) / cl;
+            const double R[3] = {y[0] - xi0[0], y[1] - xi0[1], y[2] - xi0[2]};
+            const double dR[3] = {dy[0] - dxi[0], dy[1] - dxi[1], dy[2] - dxi[2]};
+            const double ri = rsqrt_f64(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+            const double ri3 = ri * ri * ri, ri5 = ri3 * ri * ri;
+            const double RN = R[0] * N[0] + R[1] * N[1] + R[2] * N[2];
+            const double D = P - phi_i;
+            const double t1 = -D * ri3, t2 = 3.0 * D * RN * ri5 + S * J * ri3;
+            const double Nd = N[0] * dR[0] + N[1] * dR[1] + N[2] * dR[2];
+            const double Rd = R[0] * dR[0] + R[1] * dR[1] + R[2] * dR[2];
+            const double RdN = R[0] * dN[0] + R[1] * dN[1] + R[2] * dN[2];
+            acc += t1 * (Nd + RdN) + t2 * Rd - S * ri * dJ;
+        }
+    }
+    ds[i] = acc;
+}
+
+// out[i] = (sum_c part[c][i] + ds[i]) / (4 pi), zero on the rows flagged in skip
+__global__ void k_jvp_finish(const double *__restrict__ part, int n_chunks, int n,
+                             const double *__restrict__ ds, const uint8_t *__restrict__ skip,
+                             double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < n_chunks; ++c) s += part[(size_t)c * n + i];
+    out[i] = (skip && skip[i]) ? 0.0 : (s + ds[i]) * INV4PI;
+}
+
 // out[e] = sum_c part[c * stride + e]
 __global__ void k_sum_chunks(const double *__restrict__ part, int n_chunks, size_t stride,
                              size_t count, double *__restrict__ out)
@@ -484,6 +663,42 @@ void launch_residual_vjp(int q_order, int n, int nc, const double *X, const int 
                        X, cell_int, orient, qpts, qw, vbar);
     hipLaunchKernelGGL(k_vjp_singular, dim3((n + 63) / 64), dim3(64), 0, s, X, cell_int, orient,
                        sing_ptr, sing_cell, sing_vtx, n, ns, srule, wrow, phi, psi, xs, spart);
+}
+
+void launch_residual_jvp(int q_order, int n, int nc, const double *X, const double *dX,
+                         const int *row_cls, const double *phi, const double *psi,
+                         const double *cellq, const int *cell_int, const int *cell_cls,
+                         const int8_t *orient, const double *qpts, const double *qw,
+                         const int *sing_ptr, const int *sing_cell, const int *sing_vtx, int ns,
+                         const double *srule, const uint8_t *skip, double *cstate,
+                         double *trec, double *part, double *ds, double *out, hipStream_t s)
+{
+    const int nq = q_order * q_order;
+    hipLaunchKernelGGL(k_cell_state, dim3((nc * nq + 255) / 256), dim3(256), 0, s, nc, nq,
+                       cell_int, qpts, phi, psi, cstate);
+    hipLaunchKernelGGL(k_tangent_points, dim3((nc * nq + 255) / 256), dim3(256), 0, s, nc, nq,
+                       X, dX, cell_int, orient, qpts, qw, trec);
+    const int rch = vjp_row_chunks(nc), cpc = (nc + rch - 1) / rch;
+    dim3 g1((n + 63) / 64, rch);
+#define WB_JVP(QV)                                                                          \
+    case QV:                                                                                \
+        hipLaunchKernelGGL(k_jvp_rows<QV>, g1, dim3(64), 0, s, X, dX, row_cls, phi, cellq,    \
+                           cstate, trec, (const int4 *)cell_cls, nc, cpc, n, part);          \
+        break;
+    switch (q_order) {
+        WB_JVP(1)
+        WB_JVP(2)
+        WB_JVP(3)
+        WB_JVP(4)
+        WB_JVP(5)
+        WB_JVP(6)
+    default: break;
+    }
+#undef WB_JVP
+    hipLaunchKernelGGL(k_jvp_singular, dim3((n + 63) / 64), dim3(64), 0, s, X, dX, cell_int,
+                       orient, sing_ptr, sing_cell, sing_vtx, n, ns, srule, phi, psi, ds);
+    hipLaunchKernelGGL(k_jvp_finish, dim3((n + 255) / 256), dim3(256), 0, s, part, rch, n, ds,
+                       skip, out);
 }
 
 }  // namespace wb

@@ -1732,6 +1732,62 @@ wb_status wb_residual_vjp(wb_ctx *c, const double *w, const double *phi, const d
     });
 }
 
+wb_status wb_residual_jvp(wb_ctx *c, const double *dx, const double *phi, const double *dphi_dn,
+                          int free_rows_only, double *dres)
+{
+    return guarded(c, [&] {
+        require(c->assembled, WB_E_STATE, "wb_residual_jvp before wb_assemble");
+        require(dx && phi && dphi_dn && dres, WB_E_ARG, "null jvp argument");
+        const int n = c->n, nc = c->plan.nc;
+        const int nq = c->quad.q_order * c->quad.q_order;
+        const int ns = 2 * c->quad.s_order * c->quad.s_order;
+        std::vector<double> pi, si, dxi(3 * (size_t)n);
+        to_internal(c, phi, pi);
+        to_internal(c, dphi_dn, si);
+        for (int d = 0; d < n; ++d)
+            for (int k = 0; k < 3; ++k) dxi[3 * c->plan.perm[d] + k] = dx[3 * (size_t)d + k];
+        upload(c->vjp_phi, pi.data(), n, c->stream);
+        upload(c->vjp_psi, si.data(), n, c->stream);
+        upload(c->vjp_w, dxi.data(), dxi.size(), c->stream);  // the direction, 3 per point
+        c->vjp_cstate.ensure(sizeof(double) * 2 * (size_t)nc * nq);
+        c->vjp_p2.ensure(sizeof(double) * 8 * (size_t)nc * nq);              // tangent records
+        c->vjp_p1.ensure(sizeof(double) * (size_t)vjp_row_chunks(nc) * n);   // row partials
+        c->vjp_xs.ensure(sizeof(double) * n);                                // singular part
+        c->vjp_xg.ensure(sizeof(double) * n);                                // result
+        launch_residual_jvp(c->quad.q_order, n, nc, c->X.as<double>(), c->vjp_w.as<double>(),
+                            c->row_cls.as<int>(), c->vjp_phi.as<double>(),
+                            c->vjp_psi.as<double>(), c->cellq.as<double>(), c->cell_int.as<int>(),
+                            c->cell_cls.as<int>(), c->orient.as<int8_t>(), c->qpts.as<double>(),
+                            c->qw.as<double>(), c->sing_ptr.as<int>(), c->sing_cell.as<int>(),
+                            c->sing_vtx.as<int>(), ns, c->srule.as<double>(),
+                            free_rows_only && c->have_cons ? c->constrained.as<uint8_t>()
+                                                           : nullptr,
+                            c->vjp_cstate.as<double>(), c->vjp_p2.as<double>(),
+                            c->vjp_p1.as<double>(), c->vjp_xs.as<double>(),
+                            c->vjp_xg.as<double>(), c->stream);
+        HIPCHK(hipGetLastError());
+        std::vector<double> out(n);
+        HIPCHK(hipMemcpyAsync(out.data(), c->vjp_xg.p, sizeof(double) * n, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int d = 0; d < n; ++d) dres[d] = out[c->plan.perm[d]];
+    });
+}
+
+wb_status wb_solve_rhs(wb_ctx *c, const double *rhs, double *sol, wb_solve_stats *st)
+{
+    return guarded(c, [&] {
+        check_ready(c, true);
+        require(rhs && sol, WB_E_ARG, "null vector");
+        ensure_work(c);
+        upload_vec(c, c->vb, rhs);
+        wb_solve_stats s{};
+        run_solver(c, &s);
+        if (st) *st = s;
+        download_vec(c, c->vx, sol);
+    });
+}
+
 wb_status wb_solve_system_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi,
                                 double *dphi_dn, const double *inhom_k, wb_solve_stats *st)
 {

@@ -169,4 +169,14 @@ void launch_vec_ops(int op, int n, const void *a, const void *b, void *c, double
 void launch_permute(int n, const int *idx, const double *src, double *dst, int gather,
                     hipStream_t s);
 
+// tangent of the residual: out[i] = sum_d dR_i/dx_d . dX_d (0 on rows flagged in skip);
+// buffers: cstate nc*nq*2, trec nc*nq*8, part vjp_row_chunks(nc)*n, ds n, out n
+void launch_residual_jvp(int q_order, int n, int nc, const double *X, const double *dX,
+                         const int *row_cls, const double *phi, const double *psi,
+                         const double *cellq, const int *cell_int, const int *cell_cls,
+                         const int8_t *orient, const double *qpts, const double *qw,
+                         const int *sing_ptr, const int *sing_cell, const int *sing_vtx, int ns,
+                         const double *srule, const uint8_t *skip, double *cstate,
+                         double *trec, double *part, double *ds, double *out, hipStream_t s);
+
 }  // namespace wb
