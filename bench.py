@@ -136,6 +136,61 @@ def run_adjoint(workload, steps):
     return out
 
 
+def run_multi(workload, k, steps):
+    """k seeded Dirichlet data sets solved together in FP32-IR (batched matrix-core
+    products), against k single FP32-IR solves and k single FP64 solves of the same
+    systems; ms per solve phase (assembly excluded) and the worst error against FP64."""
+    from wavebem_amd import mesh as M
+    from wavebem_amd.bem import BEMProblem, Config
+
+    m = M.wigley_preset(workload)
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    tmp = np.stack([M.wigley_bc(m, seed=s)[0] for s in range(k)])
+    N = m.other_nodes > 0
+    S = ~N
+    out = {"n_dofs": m.n_dofs, "k": k}
+    sols = {}
+    for prec in ("fp32_ir", "fp64"):
+        with BEMProblem(m, Config(precision=prec), double_nodes=(ptr, idx)) as p:
+            p.compute_constraints(tmp[0])
+            t = time.perf_counter()
+            p.assemble_system()
+            p.solve_system(tmp[0] * m.surface_nodes, tmp[0] * m.other_nodes, tmp[0])
+            out[f"{prec}_assembly_and_first_solve_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+            ts = []
+            for _ in range(steps):
+                phi_s, dphi_s = [], []
+                t = time.perf_counter()
+                for v in range(k):
+                    phi = tmp[v] * m.surface_nodes
+                    dphi = tmp[v] * m.other_nodes
+                    p.solve_system(phi, dphi, tmp[v])
+                    phi_s.append(phi)
+                    dphi_s.append(dphi)
+                ts.append(time.perf_counter() - t)
+            out[f"{prec}_single_solves_ms"] = round(float(np.mean(ts)) * 1e3, 2)
+            sols[prec] = (np.array(phi_s), np.array(dphi_s))
+            if prec == "fp32_ir":
+                tm = []
+                for _ in range(steps):
+                    phi = tmp * m.surface_nodes
+                    dphi = tmp * m.other_nodes
+                    t = time.perf_counter()
+                    st = p.solve_system_multi(phi, dphi, tmp)
+                    tm.append(time.perf_counter() - t)
+                out["fp32_ir_multi_ms"] = round(float(np.mean(tm)) * 1e3, 2)
+                out["fp32_ir_multi_iterations"] = st["iterations"]
+                out["fp32_ir_multi_sweeps"] = st["ir_sweeps"]
+                sols["multi"] = (phi, dphi)
+    p64, d64 = sols["fp64"]
+    for key in ("multi", "fp32_ir"):
+        ph, dp = sols[key]
+        out[f"{key}_max_err_vs_fp64"] = float(max(
+            np.abs(ph[:, N] - p64[:, N]).max() / np.abs(p64[:, N]).max(),
+            np.abs(dp[:, S] - d64[:, S]).max() / np.abs(d64[:, S]).max()))
+    return out
+
+
 def profiled_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this
     workload (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy kernel), or None."""
@@ -193,6 +248,9 @@ def main():
     ap.add_argument("--tile", type=int, default=16, help="max columns per assembly tile (1..16)")
     ap.add_argument("--precond-block", type=int, default=64,
                     help="block-Jacobi block size (rows of the device ordering, <= 128)")
+    ap.add_argument("--multi", type=int, default=0, metavar="K",
+                    help="only time K right-hand sides solved together (default workload "
+                         "60k) and exit")
     ap.add_argument("--adjoint", action="store_true",
                     help="only time the drag-gradient adjoint (default workload 10k) and exit")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
@@ -209,6 +267,16 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.multi:
+        wl = args.workload if args.workload != "40k" else "60k"
+        r = run_multi(wl, args.multi, args.steps)
+        print(json.dumps({"metric": "several right-hand sides: solve time", "unit": "ms",
+                          "value": r["fp32_ir_multi_ms"], "higher_is_better": False,
+                          "dtype": "fp32+fp64-ir", "data": "synthetic",
+                          "config": {"workload": f"wigley_closed_halfbox_{wl}",
+                                     "n_dofs": r["n_dofs"], "rhs": args.multi},
+                          "detail": r}), flush=True)
+        return
     if args.adjoint:
         wl = args.workload if args.workload != "40k" else "10k"
         r = run_adjoint(wl, args.steps)

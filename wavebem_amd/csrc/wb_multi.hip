@@ -41,12 +41,22 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_f32(
     const int v = lane & 15;  // right-hand side fed by this lane's B operand
     const bool vok = v < k;
     const float *Xv = X + (size_t)(vok ? v : 0) * ldx;
+    // The matrix cores accumulate in FP32; every 128 columns the FP32 sums move into FP64
+    // accumulators, so rounding grows with the 128-column blocks, not with n.
     fvec4 c = {0.f, 0.f, 0.f, 0.f};
+    double cd[4] = {0.0, 0.0, 0.0, 0.0};
+    int since = 0;
     for (int si = 0; si < n_segs; ++si) {
         const int4 sg = segs[si];
         const int c_end = 4 * sg.y;
         const unsigned fm = flip ? 0xfu : 0u;
         for (int cb = 4 * sg.x; cb < c_end; cb += 32) {
+            if (++since == 4) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cd[r] += (double)c[r];
+                c = fvec4{0.f, 0.f, 0.f, 0.f};
+                since = 0;
+            }
             const int col = cb + 8 * q;  // this lane's 8 columns: col .. col+7
             float a[8], b[8];
 #pragma unroll
@@ -96,9 +106,9 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_f32(
             const int s2 = tile0 + 4 * q + r;
             if (s2 < n_rows) {
                 const int rr = rows ? rows[s2] : s2;
-                float val = c[r];
-                if (diag) val += (float)diag[rr] * Xd[(size_t)v * ldx + rr];
-                Y[(size_t)v * ldy + rr] = val;
+                double val = cd[r] + (double)c[r];
+                if (diag) val += diag[rr] * (double)Xd[(size_t)v * ldx + rr];
+                Y[(size_t)v * ldy + rr] = (float)val;
             }
         }
     }
