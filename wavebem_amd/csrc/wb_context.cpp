@@ -120,13 +120,13 @@ struct wb_ctx {
     // several right-hand sides at once
     struct Multi {
         int k = 0;
-        DevBuf V, V32, Z, W, Z32, W32, P1, P2, Pn, H, hred, X, B, R, T, D32, R32, act, inh;
+        DevBuf V, V32, Z, W, Z32, W32, P1, P2, Pn, H, hred, X, B, R, T, D32, R32, act, inh, part;
         double *hpin = nullptr;  // pinned k x 132 coefficients
         int *apin = nullptr;     // pinned k active flags
         std::vector<DevBuf *> all()
         {
             return {&V, &V32, &Z, &W, &Z32, &W32, &P1, &P2, &Pn, &H, &hred,
-                    &X, &B, &R, &T, &D32, &R32, &act, &inh};
+                    &X, &B, &R, &T, &D32, &R32, &act, &inh, &part};
         }
     } mw;
 };
@@ -709,6 +709,15 @@ static void solve_system_dev(wb_ctx *c, const double *tmp_dev_internal, wb_solve
 
 // ------------------------------------------------------------------ several right-hand sides
 // Y[:, v] = ConstrainedOperator(A) X[:, v] for the k columns (column strides ldx / ldy).
+// chunk partials of the batched product for k vectors over the unconstrained rows
+static double *multi_part(wb_ctx *c, int k)
+{
+    const size_t need =
+        sizeof(double) * (size_t)multi_chunks(c->n_free, (c->n + 3) / 4) * k * c->n_free;
+    c->mw.part.ensure(std::max<size_t>(need, 8));
+    return c->mw.part.as<double>();
+}
+
 static void apply_constrained_multi(wb_ctx *c, int mode, int k, const void *X, size_t ldx,
                                     void *Y, size_t ldy)
 {
@@ -719,10 +728,11 @@ static void apply_constrained_multi(wb_ctx *c, int mode, int k, const void *X, s
     {
         KernelScope ks(c, mode == 0 ? WB_K_GEMV64 : (mode == 1 ? WB_K_GEMV32 : WB_K_GEMVHL),
                        (long long)c->n_free);
+        double *part = multi_part(c, k);
         launch_gemv_sel_multi(mode, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld, c->segs.as<int>(),
                               c->n_segs, c->gmask.as<uint8_t>(), 0, 1.0, -1.0, k, X, ldx,
                               c->diag_op.as<double>(), X, c->free_rows.as<int>(), c->n_free, Y,
-                              ldy, c->stream);
+                              ldy, part, (c->n + 3) / 4, c->stream);
     }
     for (int v = 0; v < k; ++v)
         launch_constraint_rows(is_float, c->n_lines, c->con_rows.as<int>(), c->con_ptr.as<int>(),
@@ -962,10 +972,12 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
     double t0 = now_ms();
     {
         KernelScope ks(c, mode64 == 0 ? WB_K_GEMV64 : WB_K_GEMVHL, (long long)c->n_free);
+        double *part = multi_part(c, k);
         launch_gemv_sel_multi(mode64, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld,
                               c->segs.as<int>(), c->n_segs, c->gmask.as<uint8_t>(), 1, -1.0, 1.0,
                               k, M.T.p, vl, c->diag_rhs.as<double>(), M.T.p,
-                              c->free_rows.as<int>(), c->n_free, M.B.p, vl, c->stream);
+                              c->free_rows.as<int>(), c->n_free, M.B.p, vl, part,
+                              (c->n + 3) / 4, c->stream);
     }
     for (int v = 0; v < k; ++v)
         launch_constraint_rows(0, nl, c->con_rows.as<int>(), c->con_ptr.as<int>(),

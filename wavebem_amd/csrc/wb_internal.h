@@ -98,13 +98,16 @@ void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, co
                      size_t ld, const int *segs, int n_segs, const uint8_t *gmask, int flip,
                      double s_sel, double s_unsel, const void *x, const double *diag,
                      const void *xd, const int *rows, int n_rows, void *y, hipStream_t s);
-// k <= 16 vectors at once (column v at X + v*ldx, Xd + v*ldx, Y + v*ldy); MFMA for modes
-// 0 and 1
+// k <= 16 vectors at once (column v at X + v*ldx, Xd + v*ldx, Y + v*ldy) on the matrix
+// cores; the columns are split into multi_chunks(n_rows, n_groups) chunks whose partials
+// (part: chunks * k * n_rows doubles) are added in order
+int multi_chunks(int n_rows, int n_groups);
 void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *Glo,
                            const void *Hlo, size_t ld, const int *segs, int n_segs,
                            const uint8_t *gmask, int flip, double s_sel, double s_unsel, int k,
                            const void *X, size_t ldx, const double *diag, const void *Xd,
-                           const int *rows, int n_rows, void *Y, size_t ldy, hipStream_t s);
+                           const int *rows, int n_rows, void *Y, size_t ldy, double *part,
+                           int n_groups, hipStream_t s);
 // constrained rows: y[row] = x[row] - sum w x[col]  (mode_inhom=0)  or y[row] = inhom (1)
 void launch_constraint_rows(int is_float, int n_lines, const int *rows, const int *ptr,
                             const int *cols, const double *w, const double *inhom, int mode_inhom,
