@@ -191,6 +191,68 @@ def run_multi(workload, k, steps):
     return out
 
 
+def run_moving(workload, steps):
+    """Prescribed free-surface motion (no DAE integrator): every step moves the free-surface
+    nodes to a travelling wave of 2e-3 L amplitude, re-assembles G and H and solves, in
+    FP32-IR and in FP64 side by side, and evaluates the hull drag.  Returns per-step times
+    and the drag drift of FP32-IR against FP64."""
+    import torch
+
+    from wavebem_amd import mesh as M
+    from wavebem_amd.adjoint import hull_drag
+    from wavebem_amd.bem import BEMProblem, Config
+
+    m = M.wigley_preset(workload)
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    bc, U = M.wigley_bc(m, seed=0)
+    fs = np.unique(m.cell_dofs[m.cell_material == M.FREE_SURFACE])
+    moving = np.unique(np.concatenate([idx[ptr[d]:ptr[d + 1]] for d in fs]))
+    X0 = m.support_points.copy()
+    hull = m.cell_material == M.HULL
+    cells, orient = m.cell_dofs[hull], m.cell_orient[hull]
+    phi_t = torch.zeros(m.n_dofs, dtype=torch.float64)
+    dt, lam = 0.01, 0.5
+    out = {}
+    drags = {}
+    probs = {prec: BEMProblem(m, Config(precision=prec), double_nodes=(ptr, idx))
+             for prec in ("fp32_ir", "fp64")}
+    try:
+        for p in probs.values():
+            p.compute_constraints(bc)
+        times = {k: [] for k in probs}
+        for s in range(steps + 1):
+            t = s * dt
+            X = X0.copy()
+            xm = X0[moving]
+            X[moving, 2] += 2e-3 * np.sin(2 * np.pi * (xm[:, 0] - U * t) / lam)
+            tmp = bc * (1.0 + 0.1 * np.sin(2 * np.pi * t))
+            for prec, p in probs.items():
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                p.set_support_points(X)
+                p.assemble_system()
+                phi = tmp * m.surface_nodes
+                dphi = tmp * m.other_nodes
+                p.solve_system(phi, dphi, tmp)
+                t1 = time.perf_counter()
+                if s:
+                    times[prec].append((t1 - t0) * 1e3)
+                d = float(hull_drag(torch.from_numpy(X), cells, orient, torch.from_numpy(phi),
+                                    torch.from_numpy(dphi), phi_t, U))
+                drags.setdefault(prec, []).append(d)
+        for prec in probs:
+            out[f"{prec}_ms_per_step"] = round(float(np.mean(times[prec])), 2)
+    finally:
+        for p in probs.values():
+            p.close()
+    d32, d64 = np.array(drags["fp32_ir"]), np.array(drags["fp64"])
+    out["drag_rel_drift_max"] = float(np.abs(d32 - d64).max() / np.abs(d64).max())
+    out["drag_fp64_first_last"] = [float(d64[0]), float(d64[-1])]
+    out["n_dofs"] = m.n_dofs
+    out["steps"] = steps
+    return out
+
+
 def profiled_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary of this
     workload (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy kernel), or None."""
@@ -251,6 +313,9 @@ def main():
     ap.add_argument("--multi", type=int, default=0, metavar="K",
                     help="only time K right-hand sides solved together (default workload "
                          "60k) and exit")
+    ap.add_argument("--moving", type=int, default=0, metavar="STEPS",
+                    help="only run STEPS re-assembled steps of a prescribed free-surface "
+                         "motion in FP32-IR and FP64 (drag drift, ms per step) and exit")
     ap.add_argument("--adjoint", action="store_true",
                     help="only time the drag-gradient adjoint (default workload 10k) and exit")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
@@ -267,6 +332,15 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.moving:
+        r = run_moving(args.workload, args.moving)
+        print(json.dumps({"metric": "moving-mesh step time (re-assembly + solve)", "unit": "ms",
+                          "value": r["fp32_ir_ms_per_step"], "higher_is_better": False,
+                          "dtype": "fp32+fp64-ir", "data": "synthetic",
+                          "config": {"workload": f"wigley_closed_halfbox_{args.workload}",
+                                     "n_dofs": r["n_dofs"], "steps": r["steps"]},
+                          "detail": r}), flush=True)
+        return
     if args.multi:
         wl = args.workload if args.workload != "40k" else "60k"
         r = run_multi(wl, args.multi, args.steps)
