@@ -111,6 +111,10 @@ wb_status wb_set_support_points(wb_ctx *ctx, const double *support_points);
  * source/free_surface.cc:8756-8772): 1 = Dirichlet (phi known) / Neumann. */
 wb_status wb_set_masks(wb_ctx *ctx, const double *surface_nodes, const double *other_nodes);
 
+/* The installed masks (as set by wb_set_masks or wb_prepare_bem_vectors); either pointer
+ * may be NULL. */
+wb_status wb_get_masks(wb_ctx *ctx, double *surface_nodes, double *other_nodes);
+
 /* BEMProblem::assemble_system (source/bem_problem.cc:106-590): G (dirichlet_matrix) and
  * H (neumann_matrix) on the device, plus the solid-angle row sums. */
 wb_status wb_assemble(wb_ctx *ctx);
@@ -141,12 +145,51 @@ wb_status wb_set_constraints(wb_ctx *ctx, int n_lines, const int32_t *line_rows,
                              const int32_t *line_ptr, const int32_t *ent_col,
                              const double *ent_w, const double *inhom);
 
+/* ComputationalDomain::compute_normals (source/computational_domain.cc:1525-1620) with
+ * normalize = 1, NumericalTowingTank::compute_normals_at_nodes
+ * (source/numerical_towing_tank.cc:3358-3433; the comp_dom.node_normals that
+ * FreeSurface::prepare_bem_vectors reads) with normalize = 0: the L2 projection of the
+ * cell normals onto the Q1 space at the current support points, solved on the device by
+ * Jacobi-preconditioned CG on the mass matrix to a relative residual of 1e-14 (the
+ * reference uses a direct solve).  normals: n x 3, caller order.  *cg_iterations (may be
+ * NULL) receives the CG iteration count. */
+wb_status wb_compute_normals(wb_ctx *ctx, int normalize, double *normals, int *cg_iterations);
+
+/* FreeSurface::prepare_bem_vectors (source/free_surface.cc:8744-8883) for the current
+ * mesh (its cell materials are required):
+ *   - masks: surface_nodes = 1 on every dof of a cell whose material is in fs_ids, installed
+ *     as by wb_set_masks;
+ *   - on the dofs of the other cells, in cell order: wall cells (wall_ids)
+ *     bem_bc = dphi_dn = -node_normals . wind, with node_normals the unnormalised
+ *     projection of wb_compute_normals(normalize = 0) at the current support points and
+ *     wind (n x 3) the wind function sampled at the support points; inflow cells
+ *     (inflow_ids) inflow_dphi_dn (n, NULL reads as 0); other cells 0.  Free-surface
+ *     entries of bem_bc / dphi_dn are left unchanged;
+ *   - dofs with transom[i] != 0 (comp_dom.flags & transom_on_water; NULL = none) become
+ *     Neumann with bem_bc = dphi_dn = the mean of prev_dphi_dn (the previous solution's
+ *     bem_dphi_dn) over their coincident dofs. */
+wb_status wb_prepare_bem_vectors(wb_ctx *ctx, int n_fs_ids, const int32_t *fs_ids,
+                                 int n_wall_ids, const int32_t *wall_ids, int n_inflow_ids,
+                                 const int32_t *inflow_ids, const double *wind,
+                                 const double *inflow_dphi_dn, const uint8_t *transom,
+                                 const double *prev_dphi_dn, double *bem_bc, double *dphi_dn);
+
+/* BEMProblem::compute_surface_gradients (source/bem_problem.cc:1154-1290): the L2
+ * projection of the surface gradient of phi = tmp_rhs .* surface_nodes (needs the masks),
+ * solved as in wb_compute_normals.  grads: n x 3, caller order. */
+wb_status wb_compute_surface_gradients(wb_ctx *ctx, const double *tmp_rhs, double *grads,
+                                       int *cg_iterations);
+
 /* BEMProblem::compute_constraints (source/bem_problem.cc:990-1105) for the current masks
- * and double nodes, from tmp_rhs.  node_normals (n x 3, from
- * ComputationalDomain::compute_normals) is needed only when two Dirichlet dofs
- * coincide, node_surface_gradients (n x 3, compute_surface_gradients) only on sharp
- * Dirichlet-Dirichlet edges; both may be NULL otherwise.  Installs the result as with
- * wb_set_constraints and returns the number of lines in *n_lines (may be NULL). */
+ * and double nodes, from tmp_rhs.  The rules read node normals (n x 3,
+ * ComputationalDomain::compute_normals) only where two Dirichlet dofs coincide and
+ * surface gradients (n x 3, compute_surface_gradients(tmp_rhs)) only on sharp
+ * Dirichlet-Dirichlet edges.  Pass them to use fixed values; pass NULL to have them
+ * computed on the device as by wb_compute_normals / wb_compute_surface_gradients at the
+ * current support points (and, when the constraints are rebuilt by wb_solve_system for a
+ * new tmp_rhs, again from that tmp_rhs), as the reference does at the start of
+ * compute_constraints.  Installs the result as with wb_set_constraints and returns the
+ * number of lines in *n_lines (may be NULL). */
 wb_status wb_compute_constraints(wb_ctx *ctx, const double *tmp_rhs, const double *node_normals,
                                  const double *node_surface_gradients, int *n_lines);
 

@@ -146,6 +146,85 @@ def residual_vjp(support, cell_dofs, cell_orient, dn_ptr, dn_idx, w, phi, psi, q
     return out
 
 
+def node_normals(support, cell_dofs, cell_orient, q_order=4, normalize=True):
+    """(n, 3) nodal normals: L2 projection of the cell normals, normalised per node unless
+    ``normalize`` is False."""
+    support = _f64(support)
+    cd = _i32(cell_dofs)
+    orient = np.ascontiguousarray(cell_orient, dtype=np.int8)
+    out = np.zeros((support.shape[0], 3))
+    rc = lib().orc_node_normals(support.shape[0], cd.shape[0], _i(cd),
+                                orient.ctypes.data_as(_bp), _d(support), q_order,
+                                int(bool(normalize)), _d(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_node_normals failed: {rc}")
+    return out
+
+
+def surface_gradients(support, cell_dofs, cell_orient, phi, q_order=4):
+    """(n, 3) L2 projection of the surface gradient of the Q1 interpolant of phi."""
+    support = _f64(support)
+    cd = _i32(cell_dofs)
+    orient = np.ascontiguousarray(cell_orient, dtype=np.int8)
+    out = np.zeros((support.shape[0], 3))
+    rc = lib().orc_surface_gradients(support.shape[0], cd.shape[0], _i(cd),
+                                     orient.ctypes.data_as(_bp), _d(support), q_order,
+                                     _d(_f64(phi)), _d(out))
+    if rc != 0:
+        raise RuntimeError(f"orc_surface_gradients failed: {rc}")
+    return out
+
+
+def prepare_bem_vectors(support, cell_dofs, cell_orient, cell_material, dn_ptr, dn_idx,
+                        fs_ids, wall_ids, inflow_ids, wind, bem_bc, dphi_dn,
+                        inflow_dphi_dn=None, transom=None, prev_dphi_dn=None, q_order=4):
+    """FreeSurface::prepare_bem_vectors (source/free_surface.cc:8744-8883).  Returns
+    (surface_nodes, other_nodes, bem_bc, dphi_dn); the inputs are not modified.
+      * surface_nodes = 1 on the dofs of free-surface cells (:8753-8781);
+      * on the dofs of the other cells, in cell order (:8787-8847): wall cells
+        -node_normals . wind with the unnormalised projected normals
+        (NumericalTowingTank::compute_normals_at_nodes, numerical_towing_tank.cc:3358-3433),
+        inflow cells inflow_dphi_dn, others 0, written to both vectors;
+      * transom dofs (:8856-8880): Neumann, mean of prev_dphi_dn over the other dofs of
+        their double-node set."""
+    n = support.shape[0]
+    surf = np.zeros(n)
+    oth = np.ones(n)
+    bc = np.array(bem_bc, dtype=np.float64)
+    dn = np.array(dphi_dn, dtype=np.float64)
+    for k in range(cell_dofs.shape[0]):
+        if cell_material[k] in fs_ids:
+            surf[cell_dofs[k]] = 1.0
+            oth[cell_dofs[k]] = 0.0
+    nn = node_normals(support, cell_dofs, cell_orient, q_order, normalize=False)
+    for k in range(cell_dofs.shape[0]):
+        m = cell_material[k]
+        if m in fs_ids:
+            continue
+        for j in cell_dofs[k]:
+            if m in wall_ids:
+                v = -(nn[j, 0] * wind[j, 0] + nn[j, 1] * wind[j, 1] + nn[j, 2] * wind[j, 2])
+            elif m in inflow_ids:
+                v = 0.0 if inflow_dphi_dn is None else inflow_dphi_dn[j]
+            else:
+                v = 0.0
+            bc[j] = v
+            dn[j] = v
+    if transom is not None:
+        for i in range(n):
+            if not transom[i]:
+                continue
+            surf[i] = 0.0
+            oth[i] = 1.0
+            dup = [int(d) for d in dn_idx[dn_ptr[i]:dn_ptr[i + 1]] if d != i]
+            v = 0.0
+            for d in dup:
+                v += prev_dphi_dn[d] / len(dup)
+            bc[i] = v
+            dn[i] = v
+    return surf, oth, bc, dn
+
+
 def gemv_time(n, reps=3, n_threads=0):
     """Seconds per dense FP64 row-major mat-vec of size n (FullMatrix::vmult loop)."""
     import time

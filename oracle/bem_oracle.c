@@ -1075,3 +1075,120 @@ int orc_residual_vjp(int n, int nc, const int *cell_dofs, const signed char *ori
     free(sw);
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------
+ * L2 projections onto the continuous Q1 space of the dofs (one mass matrix, three
+ * components), solved directly (dense Cholesky; the reference uses SparseDirectUMFPACK):
+ *   kind 0: ComputationalDomain::compute_normals (source/computational_domain.cc:1525-1620),
+ *           rhs_i = sum_q phi_i n_q JxW_q, then every nodal vector normalised; without
+ *           the normalisation NumericalTowingTank::compute_normals_at_nodes
+ *           (source/numerical_towing_tank.cc:3358-3433);
+ *   kind 1: BEMProblem::compute_surface_gradients (source/bem_problem.cc:1154-1290),
+ *           rhs_i = sum_q phi_i grad_s(u)_q JxW_q with u = phi (the caller passes
+ *           tmp_rhs .* surface_nodes), grad_s the FEValues gradient of the Q1 interpolant
+ *           (J (J^T J)^-1 of the reference gradient).
+ * ---------------------------------------------------------------------------------- */
+static int l2_project(int n, int nc, const int *cell_dofs, const signed char *orient,
+                      const double *support, int q_order, int kind, int normalize,
+                      const double *u, double *out)
+{
+    const int nq = q_order * q_order;
+    double *pts = (double *)malloc(sizeof(double) * 2 * nq);
+    double *w = (double *)malloc(sizeof(double) * nq);
+    double *M = (double *)calloc((size_t)n * n, sizeof(double));
+    double *B = (double *)calloc((size_t)n * 3, sizeof(double));
+    if (!pts || !w || !M || !B) return -2;
+    orc_qgauss_2d(q_order, pts, w);
+    for (int c = 0; c < nc; ++c) {
+        const int *dofs = cell_dofs + 4 * c;
+        double v[12];
+        for (int k = 0; k < 4; ++k)
+            for (int d = 0; d < 3; ++d) v[3 * k + d] = support[3 * dofs[k] + d];
+        for (int q = 0; q < nq; ++q) {
+            double sh[4], dx[4], de[4];
+            q1_shape(pts[2 * q], pts[2 * q + 1], sh, dx, de);
+            double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+            for (int k = 0; k < 4; ++k)
+                for (int d = 0; d < 3; ++d) {
+                    a[d] += dx[k] * v[3 * k + d];
+                    b[d] += de[k] * v[3 * k + d];
+                }
+            const double g00 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+            const double g01 = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+            const double g11 = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+            const double det = g00 * g11 - g01 * g01;
+            const double jxw = sqrt(det) * w[q];
+            double f[3];
+            if (kind == 0) {
+                double nn[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                                a[0] * b[1] - a[1] * b[0]};
+                const double nl = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+                for (int d = 0; d < 3; ++d) f[d] = nn[d] / nl * (orient[c] < 0 ? -1.0 : 1.0);
+            } else {
+                double ux = 0.0, ue = 0.0;
+                for (int k = 0; k < 4; ++k) {
+                    ux += dx[k] * u[dofs[k]];
+                    ue += de[k] * u[dofs[k]];
+                }
+                const double ca = (g11 * ux - g01 * ue) / det, cb = (-g01 * ux + g00 * ue) / det;
+                for (int d = 0; d < 3; ++d) f[d] = ca * a[d] + cb * b[d];
+            }
+            for (int i = 0; i < 4; ++i) {
+                for (int j = 0; j < 4; ++j) M[(size_t)dofs[i] * n + dofs[j]] += sh[i] * sh[j] * jxw;
+                for (int d = 0; d < 3; ++d) B[3 * (size_t)dofs[i] + d] += sh[i] * f[d] * jxw;
+            }
+        }
+    }
+    /* Cholesky M = L L^T in place (lower triangle) */
+    for (int j = 0; j < n; ++j) {
+        double s = M[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) s -= M[(size_t)j * n + k] * M[(size_t)j * n + k];
+        if (s <= 0.0) {
+            free(pts); free(w); free(M); free(B);
+            return -3;
+        }
+        const double ljj = sqrt(s);
+        M[(size_t)j * n + j] = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            double t = M[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) t -= M[(size_t)i * n + k] * M[(size_t)j * n + k];
+            M[(size_t)i * n + j] = t / ljj;
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        for (int i = 0; i < n; ++i) {  /* L y = b */
+            double t = B[3 * (size_t)i + d];
+            for (int k = 0; k < i; ++k) t -= M[(size_t)i * n + k] * out[3 * (size_t)k + d];
+            out[3 * (size_t)i + d] = t / M[(size_t)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; --i) {  /* L^T x = y */
+            double t = out[3 * (size_t)i + d];
+            for (int k = i + 1; k < n; ++k) t -= M[(size_t)k * n + i] * out[3 * (size_t)k + d];
+            out[3 * (size_t)i + d] = t / M[(size_t)i * n + i];
+        }
+    }
+    if (kind == 0 && normalize)
+        for (int i = 0; i < n; ++i) {
+            double *o = out + 3 * (size_t)i;
+            const double l = sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+            for (int d = 0; d < 3; ++d) o[d] /= l;
+        }
+    free(pts);
+    free(w);
+    free(M);
+    free(B);
+    return 0;
+}
+
+int orc_node_normals(int n, int nc, const int *cell_dofs, const signed char *orient,
+                     const double *support, int q_order, int normalize, double *normals)
+{
+    return l2_project(n, nc, cell_dofs, orient, support, q_order, 0, normalize, NULL, normals);
+}
+
+int orc_surface_gradients(int n, int nc, const int *cell_dofs, const signed char *orient,
+                          const double *support, int q_order, const double *phi,
+                          double *grads)
+{
+    return l2_project(n, nc, cell_dofs, orient, support, q_order, 1, 0, phi, grads);
+}

@@ -92,6 +92,15 @@ int orc_residual_vjp(int n, int nc, const int *cell_dofs, const signed char *ori
                      int sing_order, const double *w, const double *phi, const double *psi,
                      double *out);
 
+/* Nodal normals (computational_domain.cc:1525-1620; normalize = 0:
+ * numerical_towing_tank.cc:3358-3433) and surface gradients of phi (bem_problem.cc:1154-1290):
+ * L2 projections onto the Q1 space, n x 3, dense direct (Cholesky) solve. */
+int orc_node_normals(int n, int nc, const int *cell_dofs, const signed char *orient,
+                     const double *support, int q_order, int normalize, double *normals);
+int orc_surface_gradients(int n, int nc, const int *cell_dofs, const signed char *orient,
+                          const double *support, int q_order, const double *phi,
+                          double *grads);
+
 #ifdef __cplusplus
 }
 #endif

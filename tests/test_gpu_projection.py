@@ -1,0 +1,164 @@
+"""GPU L2 projections (nodal normals, surface gradients) and the Dirichlet-Dirichlet
+constraints that read them.
+
+wb_compute_normals / wb_compute_surface_gradients solve the mass-matrix projections of
+ComputationalDomain::compute_normals (source/computational_domain.cc:1525-1620),
+NumericalTowingTank::compute_normals_at_nodes (source/numerical_towing_tank.cc:3358-3433)
+and BEMProblem::compute_surface_gradients (source/bem_problem.cc:1154-1290) by CG on the
+device; the oracle solves them directly.  Bar: 1e-12 relative to the largest entry.
+With no normals passed, compute_constraints and solve_system project them on the device
+(source/bem_problem.cc:996-997); the solves are compared with a dense solve of the system
+whose constraints the oracle built from its own projections (bar 1e-9), and on the box
+the sharp-edge dofs must carry dphi/dn = a.n of phi = a.x to 1e-11.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests import helpers as T
+from wavebem_amd import mesh as M
+from wavebem_amd.bem import BEMProblem, Config
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh(kind):
+    if kind == "box_split":
+        return M.box_mesh(n=6, lo=(-0.3, 0.1, 0.0), hi=(0.9, 0.7, 1.3),
+                          dirichlet_faces=[(2, 1), (0, 1)], split_top=True)
+    if kind == "sphere4":
+        return M.cube_sphere_mesh(n=4)
+    return T.wigley_small()
+
+
+@pytest.mark.parametrize("kind", ["box_split", "sphere4", "wigley_small"])
+def test_projections_match_oracle(gpu_available, kind):
+    m = _mesh(kind)
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    rng = np.random.default_rng(21)
+    t = rng.standard_normal(m.n_dofs)
+    X = m.support_points
+    with BEMProblem(m, Config(), double_nodes=(ptr, idx)) as p:
+        nr = p.compute_normals(True)
+        it = p.last_projection_iterations
+        raw = p.compute_normals(False)
+        gr = p.compute_surface_gradients(t)
+    o_nr = O.node_normals(X, m.cell_dofs, m.cell_orient)
+    o_raw = O.node_normals(X, m.cell_dofs, m.cell_orient, normalize=False)
+    o_gr = O.surface_gradients(X, m.cell_dofs, m.cell_orient, t * m.surface_nodes)
+    errs = [np.abs(a - b).max() / np.abs(b).max() for a, b in
+            ((nr, o_nr), (raw, o_raw), (gr, o_gr))]
+    print(f"  [{kind}] n={m.n_dofs} CG iterations {it}; normals {errs[0]:.1e}, "
+          f"unnormalised {errs[1]:.1e}, gradients {errs[2]:.1e}")
+    assert max(errs) < 1e-12
+
+
+def _box_problem():
+    m = M.box_mesh(n=6, dirichlet_faces=[(2, 1), (0, 1)], split_top=True)
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    H, G = O.assemble(m.support_points, m.cell_dofs, m.cell_orient, ptr, idx)
+    nn = np.zeros((m.n_dofs, 3))
+    for ax in range(3):
+        for sd in (0, 1):
+            nn[m.dof_material == M.BOX + 2 * ax + sd, ax] = 1.0 if sd else -1.0
+    return m, ptr, idx, H, G, nn
+
+
+def _bc(m, nn, a):
+    return np.where(m.surface_nodes == 1, m.support_points @ a, nn @ a)
+
+
+def _oracle_solution(m, ptr, idx, H, G, bc):
+    nr = O.node_normals(m.support_points, m.cell_dofs, m.cell_orient)
+    gr = O.surface_gradients(m.support_points, m.cell_dofs, m.cell_orient,
+                             bc * m.surface_nodes)
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, bc, nr, gr)
+    return T.dense_solution(H, G, m.surface_nodes, m.other_nodes, cons, bc), cons
+
+
+def test_dirichlet_dirichlet_solve_projects_on_device(gpu_available):
+    """Box with two adjacent Dirichlet faces (a sharp edge) and a top face built of two
+    patches (a flat seam): compute_constraints without normals, then two solves whose
+    tmp_rhs differ, so the second rebuilds the constraints from new gradients."""
+    m, ptr, idx, H, G, nn = _box_problem()
+    sharp = None
+    with BEMProblem(m, Config(solver_tolerance=1e-13), double_nodes=(ptr, idx)) as p:
+        for seed, a in enumerate((np.array([0.3, -0.7, 1.1]), np.array([-1.2, 0.4, 0.25]))):
+            bc = _bc(m, nn, a)
+            ref, cons = _oracle_solution(m, ptr, idx, H, G, bc)
+            if seed == 0:
+                nl = p.compute_constraints(bc)
+                assert nl == cons.n_lines
+                p.assemble_system()
+                sharp = [r for l, r in enumerate(cons.rows)
+                         if m.surface_nodes[r] == 1 and cons.ptr[l + 1] == cons.ptr[l]]
+                assert len(sharp) == 14       # 7 edge nodes x 2 Dirichlet dofs
+            phi = bc * m.surface_nodes
+            dphi = bc * m.other_nodes
+            st = p.solve_system(phi, dphi, bc)
+            sol = np.where(m.surface_nodes == 1, dphi, phi)
+            err = np.abs(sol - ref).max() / np.abs(ref).max()
+            edge = max(abs(dphi[r] - nn[r] @ a) for r in sharp)
+            print(f"  a={a}: {st['iterations']} it, vs dense {err:.2e}, sharp-edge dphi/dn "
+                  f"vs a.n {edge:.2e}")
+            assert err < 1e-9
+            assert edge < 1e-11
+
+
+def test_dirichlet_dirichlet_multi_rhs(gpu_available):
+    m, ptr, idx, H, G, nn = _box_problem()
+    rng = np.random.default_rng(22)
+    A = rng.standard_normal((3, 3))
+    bcs = np.stack([_bc(m, nn, a) for a in A])
+    with BEMProblem(m, Config(solver_tolerance=1e-13), double_nodes=(ptr, idx)) as p:
+        p.compute_constraints(bcs[0])
+        p.assemble_system()
+        phi = bcs * m.surface_nodes[None, :]
+        dphi = bcs * m.other_nodes[None, :]
+        p.solve_system_multi(phi, dphi, bcs)
+    for v in range(3):
+        ref, _ = _oracle_solution(m, ptr, idx, H, G, bcs[v])
+        sol = np.where(m.surface_nodes == 1, dphi[v], phi[v])
+        err = np.abs(sol - ref).max() / np.abs(ref).max()
+        print(f"  rhs {v}: vs dense {err:.2e}")
+        assert err < 1e-9
+
+
+def test_prepare_bem_vectors_matches_oracle(gpu_available):
+    """wb_prepare_bem_vectors against the oracle's restatement: masks and untouched
+    entries bit-exact, wall values (device-projected normals) within 1e-12, and the solve
+    that follows against a dense solve of the oracle's system."""
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    rng = np.random.default_rng(23)
+    wind = np.tile(np.array([0.9, -0.2, 0.35]), (m.n_dofs, 1))
+    bc0 = rng.uniform(-1e-3, 1e-3, m.n_dofs)
+    prev = rng.standard_normal(m.n_dofs)
+    inflow = rng.standard_normal(m.n_dofs)
+    fs = m.dof_material == M.FREE_SURFACE
+    cand = np.flatnonzero(fs & (np.diff(ptr) > 1))
+    tr = np.zeros(m.n_dofs, dtype=np.uint8)
+    tr[cand[:3]] = 1
+    ids = ([M.FREE_SURFACE], [M.HULL, M.BOTTOM], [M.INFLOW])
+    surf, oth, bc_o, dn_o = O.prepare_bem_vectors(
+        m.support_points, m.cell_dofs, m.cell_orient, m.cell_material, ptr, idx, *ids, wind,
+        bc0, bc0.copy(), inflow_dphi_dn=inflow, transom=tr, prev_dphi_dn=prev)
+    bc, dn = bc0.copy(), bc0.copy()
+    with BEMProblem(m, Config(solver_tolerance=1e-13), double_nodes=(ptr, idx)) as p:
+        p.prepare_bem_vectors(*ids, wind, bc, dn, inflow_dphi_dn=inflow, transom=tr,
+                              prev_dphi_dn=prev)
+        assert np.array_equal(p.surface_nodes, surf) and np.array_equal(p.other_nodes, oth)
+        err = max(np.abs(bc - bc_o).max(), np.abs(dn - dn_o).max()) / np.abs(bc_o).max()
+        keep = fs & (tr == 0)
+        assert np.array_equal(bc[keep], bc0[keep])
+        p.compute_constraints(bc)
+        p.assemble_system()
+        phi = bc * p.surface_nodes
+        dphi = bc * p.other_nodes
+        p.solve_system(phi, dphi, bc)
+    cons = O.compute_constraints(ptr, idx, surf, bc_o)
+    ref = T.dense_solution(H, G, surf, oth, cons, bc_o)
+    sol = np.where(surf == 1, dphi, phi)
+    serr = np.abs(sol - ref).max() / np.abs(ref).max()
+    print(f"  bc vs oracle {err:.2e}; solve vs dense {serr:.2e}")
+    assert err < 1e-12
+    assert serr < 1e-9

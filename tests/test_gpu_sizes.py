@@ -149,3 +149,45 @@ def test_multi_rhs_60k(gpu_available):
         assert e_ms < 1e-8
         assert e_phi < 1e-8 and e_dn < 1e-8
     assert n > 60000
+
+
+def test_projections_40k(gpu_available):
+    """Device L2 projections at 41k dofs.  Every tank patch except the hull is flat, so its
+    projected normal is its own outward unit normal exactly (both the normalised and the
+    raw projection), and on the flat free surface the projected surface gradient of
+    phi = a.x (tmp_rhs * surface_nodes) is (a_x, a_y, 0) exactly; hull normals lie within
+    a few degrees of the analytic Wigley normal."""
+    import time
+
+    m, ptr, idx = _mesh("40k")
+    a = np.array([0.3, -0.7, 1.1])
+    t = m.support_points @ a
+    with BEMProblem(m, Config(), double_nodes=(ptr, idx)) as p:
+        p.compute_normals(True)                       # warm-up
+        t0 = time.perf_counter()
+        nr = p.compute_normals(True)
+        t_n = time.perf_counter() - t0
+        it_n = p.last_projection_iterations
+        raw = p.compute_normals(False)
+        t0 = time.perf_counter()
+        gr = p.compute_surface_gradients(t)
+        t_g = time.perf_counter() - t0
+        it_g = p.last_projection_iterations
+    hull = m.dof_material == M.HULL
+    worst = 0.0
+    for mat in np.unique(m.dof_material[~hull]):
+        s = m.dof_material == mat
+        ax = int(np.argmax(np.abs(nr[s][0])))
+        ex = np.zeros(3)
+        ex[ax] = np.sign(nr[s][0][ax])
+        worst = max(worst, np.abs(nr[s] - ex).max(), np.abs(raw[s] - ex).max())
+    fs = m.dof_material == M.FREE_SURFACE
+    g_err = np.abs(gr[fs] - np.array([a[0], a[1], 0.0])).max()
+    exh = M.wigley_hull_normal_out_of_fluid(m.support_points[hull])
+    cos_min = (nr[hull] * exh).sum(1).min()
+    print(f"  n={m.n_dofs}: normals {t_n * 1e3:.2f} ms ({it_n} CG it), gradients "
+          f"{t_g * 1e3:.2f} ms ({it_g} CG it); flat-patch normal error {worst:.1e}, "
+          f"free-surface gradient error {g_err:.1e}, hull cos min {cos_min:.4f}")
+    assert worst < 1e-12
+    assert g_err < 1e-12
+    assert cos_min > 0.98

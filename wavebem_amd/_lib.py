@@ -104,6 +104,12 @@ _SIGS = {
     "wb_vmult": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_compute_rhs": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_set_constraints": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, _dp, _dp]),
+    "wb_compute_normals": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int)]),
+    "wb_compute_surface_gradients": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]),
+    "wb_get_masks": (ctypes.c_int, [_vp, _dp, _dp]),
+    "wb_prepare_bem_vectors": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, ctypes.c_int, _i32p,
+                                              ctypes.c_int, _i32p, _dp, _dp,
+                                              ctypes.POINTER(ctypes.c_uint8), _dp, _dp, _dp]),
     "wb_compute_constraints": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]),
     "wb_constrained_vmult": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_solve_system": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.POINTER(WbSolveStats)]),

@@ -201,6 +201,53 @@ class BEMProblem:
         self._check(L.lib().wb_compute_rhs(self._ctx, _d(dst), _d(src)), "wb_compute_rhs")
         return dst
 
+    def compute_normals(self, normalize=True):
+        """ComputationalDomain::compute_normals (normalize=True) or
+        NumericalTowingTank::compute_normals_at_nodes (normalize=False): (n, 3) L2
+        projection of the cell normals at the current support points, on the device."""
+        out = np.empty((self.n, 3))
+        it = ctypes.c_int(0)
+        self._check(L.lib().wb_compute_normals(self._ctx, int(bool(normalize)), _d(out),
+                                               ctypes.byref(it)), "wb_compute_normals")
+        self.last_projection_iterations = it.value
+        return out
+
+    def compute_surface_gradients(self, tmp_rhs):
+        """BEMProblem::compute_surface_gradients: (n, 3) L2 projection of the surface
+        gradient of tmp_rhs * surface_nodes, on the device."""
+        t = _f64(tmp_rhs, self.n)
+        out = np.empty((self.n, 3))
+        it = ctypes.c_int(0)
+        self._check(L.lib().wb_compute_surface_gradients(self._ctx, _d(t), _d(out),
+                                                         ctypes.byref(it)),
+                    "wb_compute_surface_gradients")
+        self.last_projection_iterations = it.value
+        return out
+
+    def prepare_bem_vectors(self, free_surface_ids, wall_ids, inflow_ids, wind, bem_bc,
+                            dphi_dn, inflow_dphi_dn=None, transom=None, prev_dphi_dn=None):
+        """FreeSurface::prepare_bem_vectors: installs the masks of the free-surface cells
+        and fills bem_bc / dphi_dn (updated in place) on the other dofs; ``wind`` is the
+        (n, 3) wind velocity at the support points."""
+        fs = np.ascontiguousarray(free_surface_ids, dtype=np.int32)
+        wl = np.ascontiguousarray(wall_ids, dtype=np.int32)
+        inf = np.ascontiguousarray(inflow_ids, dtype=np.int32)
+        w = _f64(wind, 3 * self.n)
+        self._inplace_ok(bem_bc)
+        self._inplace_ok(dphi_dn)
+        idp = _f64(inflow_dphi_dn, self.n) if inflow_dphi_dn is not None else None
+        tr = (np.ascontiguousarray(transom, dtype=np.uint8) if transom is not None else None)
+        pv = _f64(prev_dphi_dn, self.n) if prev_dphi_dn is not None else None
+        self._check(L.lib().wb_prepare_bem_vectors(
+            self._ctx, fs.size, _i(fs), wl.size, _i(wl), inf.size, _i(inf), _d(w),
+            _d(idp) if idp is not None else None,
+            tr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)) if tr is not None else None,
+            _d(pv) if pv is not None else None, _d(bem_bc), _d(dphi_dn)),
+            "wb_prepare_bem_vectors")
+        s, o = np.empty(self.n), np.empty(self.n)
+        self._check(L.lib().wb_get_masks(self._ctx, _d(s), _d(o)), "wb_get_masks")
+        self.surface_nodes, self.other_nodes = s, o
+
     def compute_constraints(self, tmp_rhs, node_normals=None, node_surface_gradients=None):
         t = _f64(tmp_rhs, self.n)
         nn = _f64(node_normals).reshape(-1) if node_normals is not None else None

@@ -69,11 +69,15 @@ struct wb_ctx {
     int n = 0;
     size_t ld = 0;
     std::vector<double> surface, other;  // caller order
+    std::vector<int> cell_dofs_h, cell_mat_h;  // caller numbering; materials may be empty
     bool pure_neumann = false;
     // constraints (caller numbering) and whether they are rebuilt from tmp_rhs
     ConstraintSet cons;
     bool cons_auto = false;
     std::vector<double> cons_normals, cons_grads;
+    // rebuilt constraints take normals / surface gradients from the device projections
+    // when the caller passed none
+    bool cons_project = false;
     bool have_cons = false;
     // versions for the preconditioner cache
     long assembly_ver = 0, mask_ver = 0, cons_struct_ver = 0;
@@ -97,6 +101,10 @@ struct wb_ctx {
     // backward of the assembly
     DevBuf vjp_w, vjp_phi, vjp_psi, vjp_rows, vjp_cstate, vjp_p1, vjp_p2, vjp_xg, vjp_vbar,
         vjp_xs, vjp_spart;
+    // L2 projections: node -> (cell, local vertex) incidence, element blocks, CG vectors
+    DevBuf inc_ptr, inc_item, pj_emass, pj_erhs, pj_u, pj_B, pj_X, pj_R, pj_Z, pj_P, pj_Q,
+        pj_dinv, pj_part, pj_part2, pj_state;
+    int proj_iterations = 0;
     int n_lines = 0, n_free = 0;
     DevBuf binv, binv32;
     // device: solver work
@@ -676,15 +684,119 @@ static void build_rhs(wb_ctx *c, const void *tmp_dev)
                            1, nullptr, c->vb.p, c->stream);
 }
 
+// L2 projections at the current support points (caller order, n x 3 each, either output
+// may be NULL): the cell normals, normalised per node when `normalize`, and the surface
+// gradient of u (caller order; NULL reads as 0).  Jacobi-preconditioned CG on the Q1 mass
+// matrix to a relative residual of 1e-14 per column, in place of the reference's direct
+// solve (the mass matrix is SPD and, diagonally scaled, well conditioned).
+static void project_fields(wb_ctx *c, const double *u, int normalize, double *normals,
+                           double *grads)
+{
+    const int n = c->n, nc = c->plan.nc, nq = c->quad.q_order * c->quad.q_order;
+    hipStream_t s = c->stream;
+    const size_t v6 = sizeof(double) * 6 * (size_t)n;
+    c->pj_emass.ensure(sizeof(double) * 10 * (size_t)nc);
+    c->pj_erhs.ensure(sizeof(double) * 24 * (size_t)nc);
+    for (DevBuf *b : {&c->pj_B, &c->pj_X, &c->pj_R, &c->pj_Z, &c->pj_P, &c->pj_Q}) b->ensure(v6);
+    c->pj_dinv.ensure(sizeof(double) * n);
+    const int nb = proj_blocks(n);
+    c->pj_part.ensure(sizeof(double) * 6 * nb);
+    c->pj_part2.ensure(sizeof(double) * 12 * nb);
+    c->pj_state.ensure(sizeof(double) * 24);
+    const double *ud = nullptr;
+    if (u) {
+        std::vector<double> ui;
+        to_internal(c, u, ui);
+        upload(c->pj_u, ui.data(), ui.size(), s);
+        ud = c->pj_u.as<double>();
+    }
+    double *X = c->pj_X.as<double>(), *R = c->pj_R.as<double>(), *Z = c->pj_Z.as<double>(),
+           *P = c->pj_P.as<double>(), *Q = c->pj_Q.as<double>(), *st = c->pj_state.as<double>();
+    launch_proj_setup(c->X.as<double>(), c->cell_int.as<int>(), c->orient.as<int8_t>(), ud, nc,
+                      nq, c->qpts.as<double>(), c->qw.as<double>(), c->inc_ptr.as<int>(),
+                      c->inc_item.as<int>(), n, c->pj_emass.as<double>(),
+                      c->pj_erhs.as<double>(), c->pj_B.as<double>(), c->pj_dinv.as<double>(), s);
+    launch_proj_cg_start(c->pj_B.as<double>(), c->pj_dinv.as<double>(), n, X, R, Z, P,
+                         c->pj_part2.as<double>(), st, s);
+    HIPCHK(hipGetLastError());
+    const double tol = 1e-14;
+    const int max_it = 1000, batch = 8;
+    double hs[24];
+    int it = 0;
+    for (;;) {
+        HIPCHK(hipMemcpyAsync(hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        bool done = true;
+        for (int e = 0; e < 6; ++e) done = done && hs[18 + e] <= tol * tol * hs[12 + e];
+        if (done) break;
+        require(it < max_it, WB_E_NOCONV, "L2 projection: CG on the mass matrix did not converge");
+        for (int k = 0; k < batch; ++k, ++it)
+            launch_proj_cg_iter(c->inc_ptr.as<int>(), c->inc_item.as<int>(),
+                                c->cell_int.as<int>(), c->pj_emass.as<double>(),
+                                c->pj_dinv.as<double>(), n, it, X, R, Z, P, Q,
+                                c->pj_part.as<double>(), c->pj_part2.as<double>(), st, s);
+        HIPCHK(hipGetLastError());
+    }
+    c->proj_iterations = it;
+    std::vector<double> xi(6 * (size_t)n);
+    HIPCHK(hipMemcpyAsync(xi.data(), X, v6, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int d = 0; d < n; ++d) {
+        const double *x = xi.data() + 6 * (size_t)c->plan.perm[d];
+        if (normals) {
+            double sc = 1.0;
+            if (normalize) {
+                const double l = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+                require(l > 0.0, WB_E_ARG, "zero projected normal");
+                sc = 1.0 / l;
+            }
+            for (int e = 0; e < 3; ++e) normals[3 * (size_t)d + e] = x[e] * sc;
+        }
+        if (grads)
+            for (int e = 0; e < 3; ++e) grads[3 * (size_t)d + e] = x[3 + e];
+    }
+}
+
+// Whether two Dirichlet dofs share a double-node set: the only case in which the
+// constraint rules read nodal normals and surface gradients.
+static bool has_dirichlet_pairs(const wb_ctx *c)
+{
+    const MeshPlan &p = c->plan;
+    for (int i = 0; i < c->n; ++i) {
+        if (c->surface[i] != 1.0) continue;
+        for (int q = p.dn_ptr[i]; q < p.dn_ptr[i + 1]; ++q)
+            if (p.dn_idx[q] != i && c->surface[p.dn_idx[q]] == 1.0) return true;
+    }
+    return false;
+}
+
+// BEMProblem::compute_constraints (source/bem_problem.cc:990-1105) for tmp_rhs.  The
+// reference starts with comp_dom.compute_normals() and compute_surface_gradients(tmp_rhs)
+// (:996-997); here they run on the device, and only when the rules will read them.
+static void auto_constraints(wb_ctx *c, const double *tmp_rhs, ConstraintSet &cs)
+{
+    const double *nn = c->cons_normals.empty() ? nullptr : c->cons_normals.data();
+    const double *gg = c->cons_grads.empty() ? nullptr : c->cons_grads.data();
+    std::vector<double> dn, dg;
+    if (c->cons_project && (!nn || !gg) && has_dirichlet_pairs(c)) {
+        std::vector<double> phi(c->n);
+        for (int i = 0; i < c->n; ++i) phi[i] = tmp_rhs[i] * c->surface[i];
+        dn.resize(3 * (size_t)c->n);
+        dg.resize(3 * (size_t)c->n);
+        project_fields(c, phi.data(), 1, nn ? nullptr : dn.data(), gg ? nullptr : dg.data());
+        if (!nn) nn = dn.data();
+        if (!gg) gg = dg.data();
+    }
+    std::string e = build_constraints(c->n, c->plan.dn_ptr.data(), c->plan.dn_idx.data(),
+                                      c->surface.data(), nn, gg, tmp_rhs, cs);
+    require(e.empty(), WB_E_UNSUPPORTED, e.c_str());
+}
+
 static void refresh_auto_constraints(wb_ctx *c, const double *tmp_rhs_caller)
 {
     if (!c->cons_auto) return;
     ConstraintSet cs;
-    std::string e = build_constraints(
-        c->n, c->plan.dn_ptr.data(), c->plan.dn_idx.data(), c->surface.data(),
-        c->cons_normals.empty() ? nullptr : c->cons_normals.data(),
-        c->cons_grads.empty() ? nullptr : c->cons_grads.data(), tmp_rhs_caller, cs);
-    require(e.empty(), WB_E_UNSUPPORTED, e.c_str());
+    auto_constraints(c, tmp_rhs_caller, cs);
     install_constraints(c, cs);
 }
 
@@ -942,11 +1054,7 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
         const double *t = tmp_rhs + (size_t)v * n;
         if (c->cons_auto) {
             ConstraintSet cs;
-            std::string e = build_constraints(
-                n, c->plan.dn_ptr.data(), c->plan.dn_idx.data(), c->surface.data(),
-                c->cons_normals.empty() ? nullptr : c->cons_normals.data(),
-                c->cons_grads.empty() ? nullptr : c->cons_grads.data(), t, cs);
-            require(e.empty(), WB_E_UNSUPPORTED, e.c_str());
+            auto_constraints(c, t, cs);
             if (v == 0) install_constraints(c, cs);
             require(cs.rows == c->cons.rows && cs.ptr == c->cons.ptr &&
                         cs.cols == c->cons.cols && cs.w == c->cons.w,
@@ -1145,7 +1253,10 @@ void wb_destroy(wb_ctx *c)
                       &c->z32, &c->w32, &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
                       &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,
                       &c->vjp_rows, &c->vjp_cstate, &c->vjp_p1, &c->vjp_p2, &c->vjp_xg,
-                      &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart};
+                      &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart, &c->inc_ptr, &c->inc_item,
+                      &c->pj_emass, &c->pj_erhs, &c->pj_u, &c->pj_B, &c->pj_X, &c->pj_R,
+                      &c->pj_Z, &c->pj_P, &c->pj_Q, &c->pj_dinv, &c->pj_part, &c->pj_part2,
+                      &c->pj_state};
     for (DevBuf *b : bufs) b->release();
     for (DevBuf *b : c->mw.all()) b->release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -1177,6 +1288,11 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         require(e.empty(), WB_E_ARG, e.c_str());
         c->have_mesh = c->have_masks = c->assembled = c->have_cons = false;
         c->plan = std::move(P);
+        c->cell_dofs_h.assign(cell_dofs, cell_dofs + 4 * (size_t)nc);
+        if (cell_material)
+            c->cell_mat_h.assign(cell_material, cell_material + nc);
+        else
+            c->cell_mat_h.clear();
         c->n = n;
         c->ld = c->plan.ld;
         c->vlen = c->ld;
@@ -1199,6 +1315,17 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         upload(c->sing_vtx, p.sing_vtx.data(), p.sing_vtx.size(), s);
         upload(c->perm_d, p.perm.data(), p.perm.size(), s);
         upload(c->iperm_d, p.iperm.data(), p.iperm.size(), s);
+        {
+            // node -> (cell, local vertex) incidence in cell order, for the L2 projections
+            std::vector<int> ip(n + 1, 0), it(4 * (size_t)nc);
+            for (size_t k = 0; k < 4 * (size_t)nc; ++k) ip[p.cell_int[k] + 1]++;
+            for (int i = 0; i < n; ++i) ip[i + 1] += ip[i];
+            std::vector<int> fill(ip.begin(), ip.end() - 1);
+            for (int cc = 0; cc < nc; ++cc)
+                for (int k = 0; k < 4; ++k) it[fill[p.cell_int[4 * cc + k]]++] = 4 * cc + k;
+            upload(c->inc_ptr, ip.data(), ip.size(), s);
+            upload(c->inc_item, it.data(), it.size(), s);
+        }
         const Quadrature &q = c->quad;
         upload(c->qpts, q.reg_pts.data(), q.reg_pts.size(), s);
         upload(c->qw, q.reg_w.data(), q.reg_w.size(), s);
@@ -1281,57 +1408,166 @@ wb_status wb_set_support_points(wb_ctx *c, const double *support)
     });
 }
 
+wb_status wb_compute_normals(wb_ctx *c, int normalize, double *normals, int *cg_iterations)
+{
+    return guarded(c, [&] {
+        require(c->have_mesh, WB_E_STATE, "wb_compute_normals before wb_set_mesh");
+        require(normals != nullptr, WB_E_ARG, "null normals");
+        project_fields(c, nullptr, normalize, normals, nullptr);
+        if (cg_iterations) *cg_iterations = c->proj_iterations;
+    });
+}
+
+wb_status wb_compute_surface_gradients(wb_ctx *c, const double *tmp_rhs, double *grads,
+                                       int *cg_iterations)
+{
+    return guarded(c, [&] {
+        require(c->have_mesh && c->have_masks, WB_E_STATE,
+                "wb_compute_surface_gradients needs a mesh and masks");
+        require(tmp_rhs && grads, WB_E_ARG, "null vector");
+        std::vector<double> phi(c->n);
+        for (int i = 0; i < c->n; ++i) phi[i] = tmp_rhs[i] * c->surface[i];
+        project_fields(c, phi.data(), 0, nullptr, grads);
+        if (cg_iterations) *cg_iterations = c->proj_iterations;
+    });
+}
+
+// comp_dom.surface_nodes / other_nodes: host copies, device mask, column groups
+static void install_masks(wb_ctx *c, const double *surface, const double *other)
+{
+    const int n = c->n;
+    // vmult/compute_rhs use the masks as multipliers; they must be complementary 0/1
+    for (int i = 0; i < n; ++i)
+        require((surface[i] == 0.0 || surface[i] == 1.0) && other[i] == 1.0 - surface[i],
+                WB_E_UNSUPPORTED, "masks must be complementary 0/1 vectors");
+    c->surface.assign(surface, surface + n);
+    c->other.assign(other, other + n);
+    double linf = 0;
+    for (int i = 0; i < n; ++i) linf = std::max(linf, std::fabs(surface[i]));
+    c->pure_neumann = linf < 1e-10;
+    std::vector<double> si;
+    to_internal(c, surface, si);
+    upload(c->surf_d, si.data(), c->vlen, c->stream);
+    // column selection: bit set -> column comes from H in vmult (other_nodes != 0)
+    std::vector<double> oi;
+    to_internal(c, other, oi);
+    const size_t ng = (c->vlen + 3) / 4;
+    std::vector<uint8_t> gm(ng, 0);
+    for (int j = 0; j < n; ++j)
+        if (oi[j] != 0.0) gm[j / 4] |= (uint8_t)(1u << (j % 4));
+    upload(c->gmask, gm.data(), gm.size(), c->stream);
+    // runs of 4-column groups reading only G, only H, or mixed
+    std::vector<int> sg;
+    const int ngroups = (n + 3) / 4;
+    auto kind = [&](int gg) {
+        const int nv = std::min(4, n - 4 * gg);
+        const uint8_t f = (uint8_t)((1u << nv) - 1);
+        const uint8_t m = gm[gg] & f;
+        return m == 0 ? 0 : (m == f ? 1 : 2);
+    };
+    for (int g = 0; g < ngroups;) {
+        const int k = kind(g);
+        int e = g + 1;
+        while (e < ngroups && kind(e) == k) ++e;
+        sg.insert(sg.end(), {g, e, k, 0});
+        g = e;
+    }
+    c->n_segs = (int)sg.size() / 4;
+    upload(c->segs, sg.data(), sg.size(), c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_masks = true;
+    c->mask_ver++;
+    if (c->assembled) refresh_diag(c);
+}
+
+wb_status wb_prepare_bem_vectors(wb_ctx *c, int n_fs_ids, const int32_t *fs_ids,
+                                 int n_wall_ids, const int32_t *wall_ids, int n_inflow_ids,
+                                 const int32_t *inflow_ids, const double *wind,
+                                 const double *inflow_dphi_dn, const uint8_t *transom,
+                                 const double *prev_dphi_dn, double *bem_bc, double *dphi_dn)
+{
+    return guarded(c, [&] {
+        require(c->have_mesh, WB_E_STATE, "wb_prepare_bem_vectors before wb_set_mesh");
+        require(!c->cell_mat_h.empty(), WB_E_STATE,
+                "wb_prepare_bem_vectors needs the cell materials of wb_set_mesh");
+        require(bem_bc && dphi_dn && wind, WB_E_ARG, "null vector");
+        require(n_fs_ids >= 0 && n_wall_ids >= 0 && n_inflow_ids >= 0, WB_E_ARG,
+                "negative id count");
+        require((n_fs_ids == 0 || fs_ids) && (n_wall_ids == 0 || wall_ids) &&
+                    (n_inflow_ids == 0 || inflow_ids),
+                WB_E_ARG, "null id list");
+        require(!transom || prev_dphi_dn, WB_E_ARG, "transom flags need prev_dphi_dn");
+        auto in = [](int m, int k, const int32_t *ids) {
+            for (int q = 0; q < k; ++q)
+                if (ids[q] == m) return true;
+            return false;
+        };
+        const int n = c->n, nc = c->plan.nc;
+        const int *cd = c->cell_dofs_h.data();
+        const int *mat = c->cell_mat_h.data();
+        // masks: 1 on the dofs of free-surface cells (free_surface.cc:8753-8781)
+        std::vector<double> surf(n, 0.0), oth(n, 1.0);
+        for (int k = 0; k < nc; ++k)
+            if (in(mat[k], n_fs_ids, fs_ids))
+                for (int v = 0; v < 4; ++v) {
+                    surf[cd[4 * k + v]] = 1.0;
+                    oth[cd[4 * k + v]] = 0.0;
+                }
+        // comp_dom.node_normals: the unnormalised projection
+        // (NumericalTowingTank::compute_normals_at_nodes) at the current support points
+        std::vector<double> nn(3 * (size_t)n);
+        project_fields(c, nullptr, 0, nn.data(), nullptr);
+        // boundary values on the dofs of the other cells, in cell order (:8787-8847)
+        for (int k = 0; k < nc; ++k) {
+            if (in(mat[k], n_fs_ids, fs_ids)) continue;
+            for (int v = 0; v < 4; ++v) {
+                const int j = cd[4 * k + v];
+                double b = 0.0;
+                if (in(mat[k], n_wall_ids, wall_ids)) {
+                    const double *w = wind + 3 * (size_t)j, *q = nn.data() + 3 * (size_t)j;
+                    b = -(q[0] * w[0] + q[1] * w[1] + q[2] * w[2]);
+                } else if (in(mat[k], n_inflow_ids, inflow_ids)) {
+                    b = inflow_dphi_dn ? inflow_dphi_dn[j] : 0.0;
+                }
+                bem_bc[j] = b;
+                dphi_dn[j] = b;
+            }
+        }
+        // transom stern nodes on the water: Neumann, with the mean of the previous dphi_dn
+        // of their coincident dofs (:8856-8880)
+        if (transom)
+            for (int i = 0; i < n; ++i) {
+                if (!transom[i]) continue;
+                surf[i] = 0.0;
+                oth[i] = 1.0;
+                const int p0 = c->plan.dn_ptr[i], p1 = c->plan.dn_ptr[i + 1];
+                int cnt = 0;
+                for (int p = p0; p < p1; ++p) cnt += c->plan.dn_idx[p] != i;
+                double b = 0.0;
+                for (int p = p0; p < p1; ++p)
+                    if (c->plan.dn_idx[p] != i) b += prev_dphi_dn[c->plan.dn_idx[p]] / cnt;
+                bem_bc[i] = b;
+                dphi_dn[i] = b;
+            }
+        install_masks(c, surf.data(), oth.data());
+    });
+}
+
+wb_status wb_get_masks(wb_ctx *c, double *surface, double *other)
+{
+    return guarded(c, [&] {
+        require(c->have_masks, WB_E_STATE, "no masks installed");
+        if (surface) std::copy(c->surface.begin(), c->surface.end(), surface);
+        if (other) std::copy(c->other.begin(), c->other.end(), other);
+    });
+}
+
 wb_status wb_set_masks(wb_ctx *c, const double *surface, const double *other)
 {
     return guarded(c, [&] {
         require(c->have_mesh, WB_E_STATE, "wb_set_masks before wb_set_mesh");
         require(surface && other, WB_E_ARG, "null mask");
-        const int n = c->n;
-        // vmult/compute_rhs use the masks as multipliers; they must be complementary 0/1
-        for (int i = 0; i < n; ++i)
-            require((surface[i] == 0.0 || surface[i] == 1.0) && other[i] == 1.0 - surface[i],
-                    WB_E_UNSUPPORTED, "masks must be complementary 0/1 vectors");
-        c->surface.assign(surface, surface + n);
-        c->other.assign(other, other + n);
-        double linf = 0;
-        for (int i = 0; i < n; ++i) linf = std::max(linf, std::fabs(surface[i]));
-        c->pure_neumann = linf < 1e-10;
-        std::vector<double> si;
-        to_internal(c, surface, si);
-        upload(c->surf_d, si.data(), c->vlen, c->stream);
-        // column selection: bit set -> column comes from H in vmult (other_nodes != 0)
-        std::vector<double> oi;
-        to_internal(c, other, oi);
-        const size_t ng = (c->vlen + 3) / 4;
-        std::vector<uint8_t> gm(ng, 0);
-        for (int j = 0; j < n; ++j)
-            if (oi[j] != 0.0) gm[j / 4] |= (uint8_t)(1u << (j % 4));
-        upload(c->gmask, gm.data(), gm.size(), c->stream);
-        // runs of 4-column groups reading only G, only H, or mixed
-        std::vector<int> sg;
-        const int ngroups = (n + 3) / 4;
-        for (int g = 0; g < ngroups;) {
-            const int nvalid = std::min(4, n - 4 * g);
-            const uint8_t full = (uint8_t)((1u << nvalid) - 1);
-            auto kind = [&](int gg) {
-                const int nv = std::min(4, n - 4 * gg);
-                const uint8_t f = (uint8_t)((1u << nv) - 1);
-                const uint8_t m = gm[gg] & f;
-                return m == 0 ? 0 : (m == f ? 1 : 2);
-            };
-            (void)full;
-            const int k = kind(g);
-            int e = g + 1;
-            while (e < ngroups && kind(e) == k) ++e;
-            sg.insert(sg.end(), {g, e, k, 0});
-            g = e;
-        }
-        c->n_segs = (int)sg.size() / 4;
-        upload(c->segs, sg.data(), sg.size(), c->stream);
-        HIPCHK(hipStreamSynchronize(c->stream));
-        c->have_masks = true;
-        c->mask_ver++;
-        if (c->assembled) refresh_diag(c);
+        install_masks(c, surface, other);
     });
 }
 
@@ -1502,6 +1738,7 @@ wb_status wb_compute_constraints(wb_ctx *c, const double *tmp_rhs, const double 
         c->cons_grads.clear();
         if (normals) c->cons_normals.assign(normals, normals + 3 * (size_t)c->n);
         if (grads) c->cons_grads.assign(grads, grads + 3 * (size_t)c->n);
+        c->cons_project = !normals || !grads;
         c->cons_auto = true;
         refresh_auto_constraints(c, tmp_rhs);
         if (n_lines) *n_lines = c->n_lines;
@@ -1845,7 +2082,10 @@ size_t wb_device_bytes(const wb_ctx *c)
                             &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
                             &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,
                             &c->vjp_rows, &c->vjp_cstate, &c->vjp_p1, &c->vjp_p2, &c->vjp_xg,
-                            &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart};
+                            &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart, &c->inc_ptr,
+                            &c->inc_item, &c->pj_emass, &c->pj_erhs, &c->pj_u, &c->pj_B,
+                            &c->pj_X, &c->pj_R, &c->pj_Z, &c->pj_P, &c->pj_Q, &c->pj_dinv,
+                            &c->pj_part, &c->pj_part2, &c->pj_state};
     for (const DevBuf *b : bufs) t += b->bytes;
     for (DevBuf *b : const_cast<wb_ctx *>(c)->mw.all()) t += b->bytes;
     return t;

@@ -87,6 +87,19 @@ void launch_assemble_singular(int precision, const double *support_int, const in
                               const int *sing_vtx, int n, int ns, const double *sing_rule,
                               size_t ld, void *G, void *H, void *Glo, void *Hlo,
                               double *hsum_sing, hipStream_t s);
+// L2 projections onto the Q1 space (wb_project.hip); vectors [n][6]: normal x,y,z and
+// surface gradient x,y,z.  state holds rho[2][6], |b|^2[6], |r|^2[6].
+int proj_blocks(int n);
+void launch_proj_setup(const double *X, const int *cell_int, const int8_t *orient,
+                       const double *u, int nc, int nq, const double *qpts, const double *qw,
+                       const int *inc_ptr, const int *inc_item, int n, double *emass,
+                       double *erhs, double *B, double *dinv, hipStream_t s);
+void launch_proj_cg_start(const double *B, const double *dinv, int n, double *Xs, double *R,
+                          double *Z, double *P, double *part2, double *state, hipStream_t s);
+void launch_proj_cg_iter(const int *inc_ptr, const int *inc_item, const int *cell_int,
+                         const double *emass, const double *dinv, int n, int iter, double *Xs,
+                         double *R, double *Z, double *P, double *Q, double *part,
+                         double *part2, double *state, hipStream_t s);
 void launch_alpha_reduce(const double *hsum_part, int n_parts, const double *hsum_sing, int n,
                          double *alpha, hipStream_t s);
 

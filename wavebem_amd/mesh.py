@@ -279,11 +279,16 @@ def wigley_preset(name: str) -> SurfaceMesh:
     return wigley_tank(**WIGLEY_PRESETS[name])
 
 
-def box_mesh(n=4, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_top=False) -> SurfaceMesh:
+def box_mesh(n=4, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_top=False,
+             dirichlet_faces=(), split_top=False) -> SurfaceMesh:
     """Closed axis-aligned box, six separately numbered faces, normals outward.
 
     Used for the solid-angle checks: alpha = 1/2 on faces, 1/4 on edges, 1/8 at
-    corners for the interior problem.
+    corners for the interior problem.  Face (axis, side) has material
+    BOX + 2 axis + side; ``dirichlet_faces`` lists the (axis, side) faces whose dofs are
+    Dirichlet (``dirichlet_top`` adds (2, 1)).  Two adjacent Dirichlet faces give sharp
+    Dirichlet-Dirichlet double nodes; ``split_top`` builds the top face as two patches
+    with their own vertices, whose shared line (x = midpoint) carries flat ones.
     """
     lo = np.asarray(lo, dtype=np.float64)
     hi = np.asarray(hi, dtype=np.float64)
@@ -302,12 +307,19 @@ def box_mesh(n=4, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dirichlet_top=False) -
                     p[v] = lo[v] + t[c] * (hi[v] - lo[v])
                     P[a, c] = p
             mat = BOX + 2 * axis + side
-            b.add_grid(P, mat, lambda c, cc=centre: c - cc)
+            if split_top and (axis, side) == (2, 1):
+                h = n // 2
+                b.add_grid(P[:h + 1], mat, lambda c, cc=centre: c - cc)
+                b.add_grid(P[h:], mat, lambda c, cc=centre: c - cc)
+            else:
+                b.add_grid(P, mat, lambda c, cc=centre: c - cc)
     pts, cells, orient, mat, dmat = b.finish()
+    faces = set(tuple(f) for f in dirichlet_faces)
     if dirichlet_top:
-        surf = (dmat == BOX + 2 * 2 + 1).astype(np.float64)
-    else:
-        surf = np.zeros(pts.shape[0])
+        faces.add((2, 1))
+    surf = np.zeros(pts.shape[0])
+    for axis, side in faces:
+        surf[dmat == BOX + 2 * axis + side] = 1.0
     return SurfaceMesh(pts, cells, orient, mat, surf, 1.0 - surf, dmat,
                        dict(kind="box", n=n))
 
