@@ -267,7 +267,7 @@ def profiled_traffic(workload, kernel):
     return None
 
 
-def cpu_baseline(mesh, iters64, threads, rows_sample, gemv_n):
+def cpu_baseline(mesh, iters64, threads, rows_sample, gemv_n, fma=True):
     """Oracle (CPU restatement of the reference dense path) on a bounded sample."""
     import oracle as O
     from wavebem_amd.mesh import double_nodes_csr
@@ -285,12 +285,42 @@ def cpu_baseline(mesh, iters64, threads, rows_sample, gemv_n):
     t_gemv = O.gemv_time(gn, reps=3, n_threads=threads) * (n / gn) ** 2
     # the reference's GMRES operator reads both G and H (two FullMatrix::vmult per step)
     t_step = t_asm + iters64 * 2 * t_gemv
-    return dict(value=t_step * 1e3, unit="ms", cores=threads, kind="port",
-                sample=(f"oracle assemble_system rows [{r0},{r0 + rows}) of {n} ({t_rows:.1f} s, "
-                        f"scaled by n/rows) + FullMatrix::vmult timed at n={gn} "
-                        f"(scaled by (n/{gn})^2) x 2 per GMRES step x {iters64} steps "
-                        f"(the FP64 GPU iteration count); preconditioner not counted"),
-                assembly_ms=t_asm * 1e3, gemv_ms=t_gemv * 1e3)
+    out = dict(value=t_step * 1e3, unit="ms", cores=threads, kind="port",
+               sample=(f"oracle assemble_system rows [{r0},{r0 + rows}) of {n} ({t_rows:.1f} s, "
+                       f"scaled by n/rows) + FullMatrix::vmult timed at n={gn} "
+                       f"(scaled by (n/{gn})^2) x 2 per GMRES step x {iters64} steps "
+                       f"(the FP64 GPU iteration count); preconditioner not counted"),
+               assembly_ms=t_asm * 1e3, gemv_ms=t_gemv * 1e3)
+    if fma:
+        out["bemfma"] = bemfma_baseline(mesh, ptr, idx, iters64)
+    return out
+
+
+def bemfma_baseline(mesh, ptr, idx, iters64, trunc_order=4, levels=10):
+    """The reference's other solution method, BEMFMA, restated on the host (oracle/bem_fma.c,
+    serial like the reference) at the reference's default parameters (FMA Truncation Order
+    4, Number of Octree Levels 10): octree + near-field + element-expansion set-up once per
+    geometry, then one FMM product per GMRES step (plus alpha and the rhs)."""
+    import oracle as O
+
+    rng = np.random.default_rng(0)
+    phi, psi = rng.standard_normal(mesh.n_dofs), rng.standard_normal(mesh.n_dofs)
+    t0 = time.perf_counter()
+    f = O.FMA(mesh.support_points, mesh.cell_dofs, mesh.cell_orient, ptr, idx,
+              trunc_order=trunc_order, num_levels=levels)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    f.products(phi, psi)
+    t_prod = time.perf_counter() - t0
+    info = f.info()
+    f.close()
+    step = t_setup + (iters64 + 2) * t_prod
+    return dict(value=step * 1e3, unit="ms", cores=1, kind="port",
+                sample=(f"oracle BEMFMA at n={mesh.n_dofs}: set-up {t_setup:.1f} s once, one "
+                        f"product {t_prod:.2f} s, x ({iters64} GMRES steps + alpha + rhs)"),
+                setup_s=round(t_setup, 2), product_ms=round(t_prod * 1e3, 1),
+                trunc_order=trunc_order, octree_levels=levels, blocks=info["blocks"],
+                near_field_entries=info["near_entries"])
 
 
 def main():
@@ -308,6 +338,8 @@ def main():
     ap.add_argument("--ir-inner", type=float, default=1e-7,
                     help="FP32-IR inner GMRES relative residual reduction per sweep")
     ap.add_argument("--tile", type=int, default=16, help="max columns per assembly tile (1..16)")
+    ap.add_argument("--no-fma-baseline", action="store_true",
+                    help="skip the host BEMFMA timing inside the CPU baseline")
     ap.add_argument("--precond-block", type=int, default=64,
                     help="block-Jacobi block size (rows of the device ordering, <= 128)")
     ap.add_argument("--multi", type=int, default=0, metavar="K",
@@ -472,7 +504,8 @@ def main():
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         iters = res["fp64"]["stats"]["iterations"] if "fp64" in res else 100
         t = time.perf_counter()
-        line["cpu_baseline"] = cpu_baseline(mesh, iters, threads, args.cpu_rows, args.cpu_gemv_n)
+        line["cpu_baseline"] = cpu_baseline(mesh, iters, threads, args.cpu_rows, args.cpu_gemv_n,
+                                            fma=not args.no_fma_baseline)
         log(f"cpu baseline {time.perf_counter() - t:.1f} s")
     print(json.dumps(line), flush=True)
 

@@ -225,6 +225,79 @@ def prepare_bem_vectors(support, cell_dofs, cell_orient, cell_material, dn_ptr, 
     return surf, oth, bc, dn
 
 
+class FMA:
+    """BEMFMA restated (bem_oracle.h orc_fma_*): octree + near field + multipole far field.
+    ``products(phi, dphi_dn)`` returns (prodN ~ H phi, prodD ~ G dphi_dn)."""
+
+    def __init__(self, support, cell_dofs, cell_orient, dn_ptr, dn_idx, trunc_order=4,
+                 num_levels=10, q_order=4, sing_order=5):
+        self._keep = [np.ascontiguousarray(support, dtype=np.float64),
+                      np.ascontiguousarray(cell_dofs, dtype=np.int32),
+                      np.ascontiguousarray(cell_orient, dtype=np.int8),
+                      np.ascontiguousarray(dn_ptr, dtype=np.int32),
+                      np.ascontiguousarray(dn_idx, dtype=np.int32)]
+        X, cd, orient, dp, di = self._keep
+        self.n = X.shape[0]
+        L = lib()
+        L.orc_fma_create.restype = ctypes.c_void_p
+        L.orc_fma_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_fma_products.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_fma_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long)]
+        self._h = L.orc_fma_create(self.n, cd.shape[0], _i(cd), orient.ctypes.data_as(_bp),
+                                   _d(X), _i(dp), _i(di), q_order, sing_order, trunc_order,
+                                   num_levels)
+        if not self._h:
+            raise RuntimeError("orc_fma_create failed")
+
+    def info(self):
+        a = (ctypes.c_long * 5)()
+        lib().orc_fma_info(self._h, a)
+        return dict(blocks=a[0], childless=a[1], near_entries=a[2], elem_expansions=a[3],
+                    levels=a[4])
+
+    def coverage(self, n_cells, nq=16):
+        """(n, n_cells * nq) uint8: how often each (node, quadrature point) pair is used."""
+        c = np.zeros((self.n, n_cells * nq), dtype=np.uint8)
+        lib().orc_fma_coverage(ctypes.c_void_p(self._h), c.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+        return c
+
+    def products(self, phi, dphi_dn):
+        pn = np.zeros(self.n)
+        pd = np.zeros(self.n)
+        lib().orc_fma_products(self._h, _d(_f64(phi)), _d(_f64(dphi_dn)), _d(pn), _d(pd))
+        return pn, pd
+
+    def close(self):
+        if self._h:
+            lib().orc_fma_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fma_expansion_check(p, kind, src, centres, x, normal=(0.0, 0.0, 1.0)):
+    c = np.zeros(9)
+    cc = np.asarray(centres, dtype=np.float64).ravel()
+    c[:cc.size] = cc
+    out = ctypes.c_double(0.0)
+    rc = lib().orc_fma_expansion_check(p, kind, _d(_f64(src)), _d(_f64(normal)), _d(c),
+                                       _d(_f64(x)), ctypes.byref(out))
+    if rc != 0:
+        raise ValueError("bad expansion check arguments")
+    return out.value
+
+
+def fma_legendre(n, m, x):
+    lib().orc_fma_legendre.restype = ctypes.c_double
+    ds = ctypes.c_double(0.0)
+    v = lib().orc_fma_legendre(n, m, ctypes.c_double(x), ctypes.byref(ds))
+    return v, ds.value
+
+
 def gemv_time(n, reps=3, n_threads=0):
     """Seconds per dense FP64 row-major mat-vec of size n (FullMatrix::vmult loop)."""
     import time

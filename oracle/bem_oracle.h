@@ -101,6 +101,26 @@ int orc_surface_gradients(int n, int nc, const int *cell_dofs, const signed char
                           const double *support, int q_order, const double *phi,
                           double *grads);
 
+/* BEMFMA (bem_fma.cc) with the octree of ComputationalDomain::generate_octree_blocking
+ * (computational_domain.cc:312-1453): build once per mesh (octree, near-field matrices,
+ * element multipole expansions), then prodN ~ H phi, prodD ~ G dphi_dn per call. */
+typedef struct orc_fma orc_fma;
+orc_fma *orc_fma_create(int n, int nc, const int *cell_dofs, const signed char *orient,
+                        const double *support, const int *dn_ptr, const int *dn_idx,
+                        int q_order, int sing_order, int trunc_order, int num_levels);
+void orc_fma_destroy(orc_fma *f);
+int orc_fma_products(orc_fma *f, const double *phi, const double *dphi_dn, double *prodN,
+                     double *prodD);
+/* info[0] blocks, [1] childless blocks, [2] near-field entries, [3] element expansions,
+ * [4] deepest level */
+void orc_fma_info(const orc_fma *f, long *info);
+/* times each (node, regular quadrature point) pair enters the product; count is
+ * n x (n_cells * q_order^2), zeroed by the caller (every entry must end at 1) */
+int orc_fma_coverage(const orc_fma *f, unsigned char *count);
+int orc_fma_expansion_check(int p, int kind, const double *src, const double *normal,
+                            const double *c, const double *x, double *out);
+double orc_fma_legendre(int n, int m, double x, double *dsin);
+
 #ifdef __cplusplus
 }
 #endif
