@@ -32,14 +32,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_precision(mesh, bc, precision, steps, warmup, tol, ir_inner=1e-6, tile=16, pblock=64):
+def run_precision(mesh, bc, precision, steps, warmup, tol, ir_inner=1e-6, tile=16, pblock=64,
+                  inner_format=1, inner16=1e-3):
     import torch
 
     from wavebem_amd.bem import BEMProblem, Config
 
     n = mesh.n_dofs
     cfg = Config(precision=precision, solver_tolerance=tol, solver_max_steps=5000,
-                 ir_inner_reduction=ir_inner, assembly_tile=tile, precond_block=pblock)
+                 ir_inner_reduction=ir_inner, assembly_tile=tile, precond_block=pblock,
+                 ir_inner_format=inner_format, ir_inner16_reduction=inner16)
     p = BEMProblem(mesh, cfg)
     p.compute_constraints(bc)
     bc_d = torch.tensor(bc, dtype=torch.float64, device="cuda")
@@ -335,6 +337,10 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=24000)
     ap.add_argument("--cpu-gemv-n", type=int, default=8192)
     ap.add_argument("--only", choices=["fp64", "fp32_ir"], default=None)
+    ap.add_argument("--ir-inner16", type=float, default=1e-3,
+                    help="FP32-IR with the FP16 inner operator: inner reduction per sweep")
+    ap.add_argument("--ir-inner-format", type=int, default=1, choices=(0, 1),
+                    help="FP32-IR inner Krylov operator: 0 FP32 (hi), 1 row-scaled FP16 copy")
     ap.add_argument("--ir-inner", type=float, default=1e-7,
                     help="FP32-IR inner GMRES relative residual reduction per sweep")
     ap.add_argument("--tile", type=int, default=16, help="max columns per assembly tile (1..16)")
@@ -416,7 +422,8 @@ def main():
         if args.only and prec != args.only:
             continue
         res[prec] = run_precision(mesh, bc, prec, args.steps, args.warmup, args.tol,
-                                  args.ir_inner, args.tile, args.precond_block)
+                                  args.ir_inner, args.tile, args.precond_block,
+                                  args.ir_inner_format, args.ir_inner16)
 
     n = mesh.n_dofs
     N = mesh.other_nodes > 0
@@ -433,7 +440,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64-ir" if "fp32_ir" in res else "f64",
+        "dtype": (("fp16+fp32/bf16+fp64-ir" if args.ir_inner_format == 1 else "fp32+fp64-ir")
+                  if "fp32_ir" in res else "f64"),
         "data": "synthetic",
         "config": {"workload": f"wigley_closed_halfbox_{args.workload}", "n_dofs": n,
                    "n_cells": mesh.n_cells, "quadrature": "QGauss 4x4 + QGaussOneOverR(5)",
@@ -465,14 +473,20 @@ def main():
 
     # roofline of the dominant kernel of the headline path (column-selected GEMV)
     ks = head["kernels"]
-    gname = "gemv32" if "fp32_ir" in res else "gemv64"
-    esz = 4 if gname == "gemv32" else 8
+    if "fp32_ir" not in res:
+        gname, esz, xsz, kname = "gemv64", 8, 8, "k_gemv_f64"
+    elif args.ir_inner_format == 1:
+        gname, esz, xsz, kname = "gemv16", 2, 4, "k_gemv_h16"
+    else:
+        gname, esz, xsz, kname = "gemv32", 4, 4, "k_gemv_mixed<1>"
     g = ks[gname]
     avg_ms = kernel_avg_ms(ks, gname)
     rows_per_launch = g["rows"] / max(g["launches"], 1)
-    bytes_per_launch = rows_per_launch * n * esz + 2 * n * esz
+    # matrix rows + x read + y written (the FP16 rows also read one 4-byte row scale)
+    bytes_per_launch = rows_per_launch * n * esz + 2 * n * xsz
+    if esz == 2:
+        bytes_per_launch += rows_per_launch * 4
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    kname = "k_gemv_mixed<1>" if gname == "gemv32" else "k_gemv_f64"
     line["roofline"] = {"kernel": kname, "bound": "hbm",
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -492,7 +506,7 @@ def main():
             kern[f"{prec}:assemble"]["tflops_algorithmic"] = round(flops / t_a / 1e12, 2)
             kern[f"{prec}:assemble"]["valu_frac_fp64_peak"] = round(
                 flops / t_a / 1e12 / FP64_VALU_PEAK_TFLOPS, 4)
-        for gk, es in (("gemv64", 8), ("gemvhl", 6), ("gemv32", 4)):
+        for gk, es in (("gemv64", 8), ("gemvhl", 6), ("gemv32", 4), ("gemv16", 2)):
             k = r["kernels"][gk]
             if k["launches"]:
                 rpl = k["rows"] / k["launches"]

@@ -63,6 +63,14 @@ typedef struct {
     double ir_inner_reduction;     /* FP32_IR: an inner FP32 solve stops at
                                       max(this * rho, tol / 2), rho = outer residual */
     int assembly_tile;             /* max columns per assembly column tile (1..16) */
+    int ir_inner_format;           /* FP32_IR: the inner Krylov solves run on 0 = the FP32
+                                      (hi) operator, 1 (default) = a row-scaled FP16 copy
+                                      of it (2 bytes per entry; the refinement residual
+                                      keeps the FP32 hi + bf16 lo operator; a sweep that
+                                      fails to halve the residual switches the remaining
+                                      sweeps to the FP32 operator) */
+    double ir_inner16_reduction;   /* FP32_IR with ir_inner_format = 1: the inner solve
+                                      stops at max(this * rho, tol / 2) */
 } wb_config;
 
 typedef struct {
@@ -275,7 +283,8 @@ typedef enum {
     WB_K_GEMV64 = 2,    /* column-selected GEMV, FP64 matrices */
     WB_K_GEMV32 = 3,    /* column-selected GEMV, FP32 (hi) matrices, inner solver */
     WB_K_GEMVHL = 4,    /* column-selected GEMV, FP32 hi + bf16 lo, FP64 accumulation */
-    WB_K_COUNT = 5
+    WB_K_GEMV16 = 5,    /* GEMV of the row-scaled FP16 operator, inner solver */
+    WB_K_COUNT = 6
 } wb_kernel_kind;
 
 typedef struct {

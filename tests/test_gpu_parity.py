@@ -76,7 +76,7 @@ def test_constrained_vmult_matches_oracle(gpu_available):
     assert np.abs(d - o).max() / np.abs(o).max() < 1e-12
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir", "fp32_ir_inner32"])
 def test_solve_system_matches_dense_oracle(gpu_available, precision):
     m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
     bc, _ = M.wigley_bc(m, seed=0)
@@ -84,7 +84,10 @@ def test_solve_system_matches_dense_oracle(gpu_available, precision):
     x = T.dense_solution(H, G, m.surface_nodes, m.other_nodes, cons, bc)
     phi = bc * m.surface_nodes
     dphi = bc * m.other_nodes
-    with _problem(m, ptr, idx, precision=precision, solver_tolerance=1e-13) as p:
+    # fp32_ir: inner solves on the FP16 operator (the default); fp32_ir_inner32: on FP32
+    kw = dict(precision="fp32_ir", ir_inner_format=0) if precision == "fp32_ir_inner32" \
+        else dict(precision=precision)
+    with _problem(m, ptr, idx, solver_tolerance=1e-13, **kw) as p:
         p.compute_constraints(bc)
         st = p.solve_system(phi, dphi, bc)
     print(f"  [{precision}] stats {st}")
@@ -98,6 +101,25 @@ def test_solve_system_matches_dense_oracle(gpu_available, precision):
     # known entries untouched
     assert np.array_equal(phi[S], bc[S])
     assert np.array_equal(dphi[N], bc[N])
+
+
+def test_fp16_inner_hands_over_to_fp32(gpu_available):
+    """An FP16 sweep that does not halve the outer residual (forced here by an inner
+    reduction of 0.9 per sweep) switches the remaining sweeps to the FP32 operator; the
+    solve still reaches the dense solution."""
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    bc, _ = M.wigley_bc(m, seed=2)
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, bc)
+    x = T.dense_solution(H, G, m.surface_nodes, m.other_nodes, cons, bc)
+    phi = bc * m.surface_nodes
+    dphi = bc * m.other_nodes
+    with _problem(m, ptr, idx, precision="fp32_ir", ir_inner_format=1,
+                  ir_inner16_reduction=0.9, solver_tolerance=1e-13) as p:
+        p.compute_constraints(bc)
+        st = p.solve_system(phi, dphi, bc)
+    print(f"  sweeps {st['ir_sweeps']} {st['sweep_iterations'][:st['ir_sweeps']]}")
+    sol = np.where(m.surface_nodes == 1, dphi, phi)
+    assert np.abs(sol - x).max() / np.abs(x).max() < 1e-8
 
 
 def test_solve_matches_oracle_gmres(gpu_available):

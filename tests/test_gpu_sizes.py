@@ -78,29 +78,34 @@ def test_solid_angles_40k(gpu_available):
 
 
 def test_fp32_ir_matches_fp64_40k(gpu_available):
-    """phi and dphi/dn of the FP32-IR solve within 1e-8 (relative to the largest value) of
-    the FP64 solve of the same constrained system."""
+    """phi and dphi/dn of the FP32-IR solves (inner Krylov on the row-scaled FP16 copy of the
+    operator, the default, and on the FP32 operator) within 1e-8 (relative to the largest value) of the FP64 solve of
+    the same constrained system."""
     m, ptr, idx = _mesh("40k")
     bc, _ = M.wigley_bc(m, seed=0)
     out = {}
-    for prec in ("fp64", "fp32_ir"):
+    for name, kw in (("fp64", dict(precision="fp64")),
+                     ("fp32_ir_inner32", dict(precision="fp32_ir", ir_inner_format=0)),
+                     ("fp32_ir", dict(precision="fp32_ir"))):
         phi = bc * m.surface_nodes
         dphi = bc * m.other_nodes
-        with BEMProblem(m, Config(precision=prec, solver_tolerance=1e-10),
-                        double_nodes=(ptr, idx)) as p:
+        with BEMProblem(m, Config(solver_tolerance=1e-10, **kw), double_nodes=(ptr, idx)) as p:
             p.compute_constraints(bc)
             p.assemble_system()
             st = p.solve_system(phi, dphi, bc)
-        print(f"  [{prec}] {st['iterations']} it, sweeps {st['ir_sweeps']}, "
+        print(f"  [{name}] {st['iterations']} it, sweeps {st['ir_sweeps']} "
+              f"{st['sweep_iterations'][:st['ir_sweeps']]}, "
               f"true rel res {st['true_rel_residual']:.2e}")
-        out[prec] = (phi, dphi)
+        out[name] = (phi, dphi)
     N = m.other_nodes > 0
     S = ~N
-    (p64, d64), (p32, d32) = out["fp64"], out["fp32_ir"]
-    e_phi = np.abs(p32[N] - p64[N]).max() / np.abs(p64[N]).max()
-    e_dn = np.abs(d32[S] - d64[S]).max() / np.abs(d64[S]).max()
-    print(f"  FP32-IR vs FP64: phi {e_phi:.2e}, dphi/dn {e_dn:.2e}")
-    assert e_phi < 1e-8 and e_dn < 1e-8
+    p64, d64 = out["fp64"]
+    for name in ("fp32_ir_inner32", "fp32_ir"):
+        p32, d32 = out[name]
+        e_phi = np.abs(p32[N] - p64[N]).max() / np.abs(p64[N]).max()
+        e_dn = np.abs(d32[S] - d64[S]).max() / np.abs(d64[S]).max()
+        print(f"  {name} vs FP64: phi {e_phi:.2e}, dphi/dn {e_dn:.2e}")
+        assert e_phi < 1e-8 and e_dn < 1e-8
 
 
 def test_multi_rhs_60k(gpu_available):

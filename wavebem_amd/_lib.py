@@ -34,6 +34,8 @@ class WbConfig(ctypes.Structure):
         ("ir_max_sweeps", ctypes.c_int),
         ("ir_inner_reduction", ctypes.c_double),
         ("assembly_tile", ctypes.c_int),
+        ("ir_inner_format", ctypes.c_int),
+        ("ir_inner16_reduction", ctypes.c_double),
     ]
 
 
@@ -60,14 +62,14 @@ class WbSolveStats(ctypes.Structure):
         return d
 
 
-WB_K_NAMES = ["assemble", "singular", "gemv64", "gemv32", "gemvhl"]
+WB_K_NAMES = ["assemble", "singular", "gemv64", "gemv32", "gemvhl", "gemv16"]
 
 
 class WbKernelStats(ctypes.Structure):
     _fields_ = [
-        ("launches", ctypes.c_int * 5),
-        ("total_ms", ctypes.c_double * 5),
-        ("rows", ctypes.c_longlong * 5),
+        ("launches", ctypes.c_int * 6),
+        ("total_ms", ctypes.c_double * 6),
+        ("rows", ctypes.c_longlong * 6),
     ]
 
     def as_dict(self):

@@ -60,6 +60,8 @@ class Config:
     ir_max_sweeps: int = 10
     ir_inner_reduction: float = 1e-7
     assembly_tile: int = 16
+    ir_inner_format: int = 1
+    ir_inner16_reduction: float = 1e-3
 
     def to_c(self) -> L.WbConfig:
         c = L.WbConfig()
@@ -77,6 +79,8 @@ class Config:
         c.ir_max_sweeps = self.ir_max_sweeps
         c.ir_inner_reduction = self.ir_inner_reduction
         c.assembly_tile = self.assembly_tile
+        c.ir_inner_format = self.ir_inner_format
+        c.ir_inner16_reduction = self.ir_inner16_reduction
         return c
 
 

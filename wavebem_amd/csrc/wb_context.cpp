@@ -107,6 +107,10 @@ struct wb_ctx {
     int proj_iterations = 0;
     int n_lines = 0, n_free = 0;
     DevBuf binv, binv32;
+    // row-scaled FP16 copy of the FP32 vmult operator for the inner solves
+    DevBuf A16, rs16;
+    long h16_assembly_ver = -1, h16_mask_ver = -1;
+    int float_mode = 1;  // operator mode of the FP32-vector solves: 1 = FP32 hi, 3 = FP16
     // device: solver work
     DevBuf V, z, w, P1, P2, Pn, hbuf, hred, vx, vb, vr, vt, vs;   // double / raw
     DevBuf V32, z32, w32, r32, d32;
@@ -290,16 +294,24 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
     const double s_sel = which == 0 ? 1.0 : -1.0, s_uns = which == 0 ? -1.0 : 1.0;
     const int *rows = free_rows_only ? c->free_rows.as<int>() : nullptr;
     const int nr = free_rows_only ? c->n_free : c->n;
+    if (mode == 3) {
+        require(which == 0, WB_E_STATE, "the FP16 operator holds the vmult selection only");
+        KernelScope ks(c, WB_K_GEMV16, nr);
+        launch_gemv_h16(c->A16.p, c->rs16.as<float>(), c->ld, c->n, (const float *)x, diag,
+                        (const float *)xd, rows, nr, (float *)y, c->stream);
+        return;
+    }
     KernelScope ks(c, mode == 0 ? WB_K_GEMV64 : (mode == 1 ? WB_K_GEMV32 : WB_K_GEMVHL), nr);
     launch_gemv_sel(mode, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld, c->segs.as<int>(),
                     c->n_segs, c->gmask.as<uint8_t>(), which, s_sel, s_uns, x, diag, xd, rows, nr,
                     y, c->stream);
 }
 
-// y = ConstrainedOperator(A) x in FP64 (mode 0 or 2) or FP32 (mode 1)
+// y = ConstrainedOperator(A) x in FP64 (mode 0 or 2) or FP32 vectors (mode 1: FP32
+// operator, mode 3: FP16 operator)
 static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
 {
-    const int is_float = mode == 1;
+    const int is_float = mode == 1 || mode == 3;
     if (c->op_transpose) {
         // A_c^T x: transposed product over the unconstrained rows (the diagonal alpha term
         // of those rows included), then the transposed constraint lines
@@ -474,7 +486,7 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
     }
     void *z = is_float ? c->z32.p : c->z.p;
     void *w = is_float ? c->w32.p : c->w.p;
-    const int mode = is_float ? 1 : (c->cfg.precision == WB_PREC_FP64 ? 0 : 2);
+    const int mode = is_float ? c->float_mode : (c->cfg.precision == WB_PREC_FP64 ? 0 : 2);
     const int parts = krylov_partials_blocks(n);
     auto vec = [&](int k) { return (void *)((char *)V.p + es * ldv * k); };
     std::vector<double> h(m + 2), gamma(m + 2), ci(m + 1), si(m + 1), Hh((size_t)(m + 1) * m),
@@ -572,8 +584,32 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
 // preconditioned outer residual rho.
 static double ir_inner_target(const wb_ctx *c, double rho)
 {
-    return std::max(c->cfg.ir_inner_reduction * rho, 0.5 * c->cfg.solver_tolerance);
+    const double red =
+        c->float_mode == 3 ? c->cfg.ir_inner16_reduction : c->cfg.ir_inner_reduction;
+    return std::max(red * rho, 0.5 * c->cfg.solver_tolerance);
 }
+
+// The FP16 inner operator for the current assembly and masks.
+static void ensure_h16(wb_ctx *c)
+{
+    if (c->h16_assembly_ver == c->assembly_ver && c->h16_mask_ver == c->mask_ver) return;
+    const size_t entries = (size_t)c->n * c->ld;
+    c->A16.ensure(entries * 2);
+    c->rs16.ensure(sizeof(float) * c->vlen);
+    launch_build_h16(c->G.as<float>(), c->H.as<float>(), c->ld, c->n, c->gmask.as<uint8_t>(),
+                     c->A16.p, c->rs16.as<float>(), c->stream);
+    HIPCHK(hipGetLastError());
+    c->h16_assembly_ver = c->assembly_ver;
+    c->h16_mask_ver = c->mask_ver;
+}
+
+// selects the operator of the FP32-vector solves for the lifetime of the guard
+struct FloatMode {
+    wb_ctx *c;
+    int old;
+    FloatMode(wb_ctx *c_, int m) : c(c_), old(c_->float_mode) { c->float_mode = m; }
+    ~FloatMode() { c->float_mode = old; }
+};
 
 // b (device, internal order, vlen) must already hold the distributed right-hand side;
 // the solution lands in c->vx.
@@ -599,7 +635,12 @@ static void run_solver(wb_ctx *c, wb_solve_stats *st)
             throw WbError(WB_E_NOCONV, "GMRES did not converge (SolverControl::NoConvergence)");
         }
     } else {
-        // FP64 iterative refinement around an FP32 GMRES on the FP32 (hi) operator.
+        // FP64 iterative refinement around an FP32-vector GMRES on the FP32 (hi) operator
+        // or on its row-scaled FP16 copy.
+        const int fmode = c->cfg.ir_inner_format == 1 ? 3 : 1;
+        if (fmode == 3) ensure_h16(c);
+        FloatMode fm(c, fmode);
+        double rho_prev = 0.0;
         const size_t vb4 = sizeof(float) * c->vlen;
         for (DevBuf *b : {&c->z32, &c->w32, &c->r32, &c->d32}) {
             if (b->bytes < vb4) {
@@ -624,6 +665,10 @@ static void run_solver(wb_ctx *c, wb_solve_stats *st)
             if (std::isnan(rho) || sweeps >= c->cfg.ir_max_sweeps ||
                 total_inner >= c->cfg.solver_max_steps)
                 break;
+            // an FP16 sweep that no longer halves the residual hands over to the FP32
+            // operator for the remaining sweeps
+            if (c->float_mode == 3 && sweeps > 0 && !(rho < 0.5 * rho_prev)) c->float_mode = 1;
+            rho_prev = rho;
             // inner: A32 d = r from d = 0, down to the larger of ir_inner_reduction * rho and
             // what the outer tolerance still asks for (half of it, leaving room for the
             // FP32 operator's own error in the update)
@@ -1197,6 +1242,8 @@ void wb_config_default(wb_config *cfg)
     cfg->ir_max_sweeps = 10;
     cfg->ir_inner_reduction = 1e-7;
     cfg->assembly_tile = 16;
+    cfg->ir_inner_format = 1;
+    cfg->ir_inner16_reduction = 1e-3;
 }
 
 const char *wb_version(void) { return "wavebem_amd 0.1 (gfx950)"; }
@@ -1218,6 +1265,8 @@ wb_status wb_create(wb_ctx **out, const wb_config *cfg)
         (k.precision != WB_PREC_FP64 && k.precision != WB_PREC_FP32_IR) ||
         k.precond_block < 1 || k.precond_block > 128 || k.solver_max_steps < 0 ||
         k.assembly_tile < 1 || k.assembly_tile > 16 ||
+        (k.ir_inner_format != 0 && k.ir_inner_format != 1) ||
+        !(k.ir_inner16_reduction > 0 && k.ir_inner16_reduction < 1) ||
         !(k.solver_tolerance >= 0)) {
         delete c;
         return WB_E_ARG;
@@ -1256,7 +1305,7 @@ void wb_destroy(wb_ctx *c)
                       &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart, &c->inc_ptr, &c->inc_item,
                       &c->pj_emass, &c->pj_erhs, &c->pj_u, &c->pj_B, &c->pj_X, &c->pj_R,
                       &c->pj_Z, &c->pj_P, &c->pj_Q, &c->pj_dinv, &c->pj_part, &c->pj_part2,
-                      &c->pj_state};
+                      &c->pj_state, &c->A16, &c->rs16};
     for (DevBuf *b : bufs) b->release();
     for (DevBuf *b : c->mw.all()) b->release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -2085,7 +2134,7 @@ size_t wb_device_bytes(const wb_ctx *c)
                             &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart, &c->inc_ptr,
                             &c->inc_item, &c->pj_emass, &c->pj_erhs, &c->pj_u, &c->pj_B,
                             &c->pj_X, &c->pj_R, &c->pj_Z, &c->pj_P, &c->pj_Q, &c->pj_dinv,
-                            &c->pj_part, &c->pj_part2, &c->pj_state};
+                            &c->pj_part, &c->pj_part2, &c->pj_state, &c->A16, &c->rs16};
     for (const DevBuf *b : bufs) t += b->bytes;
     for (DevBuf *b : const_cast<wb_ctx *>(c)->mw.all()) t += b->bytes;
     return t;

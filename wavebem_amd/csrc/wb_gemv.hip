@@ -243,7 +243,117 @@ __global__ __launch_bounds__(64 * MixedShape<MODE>::WAVES) void k_gemv_mixed(
     }
 }
 
+// ------------------------------------------------------------------ FP16 inner operator
+// A16 = row-scaled FP16 copy of the column-selected vmult operator (H columns of Neumann
+// dofs, -G columns of Dirichlet dofs; the alpha diagonal stays separate, as in the FP32
+// path), built once per assembly and masks from the FP32 (hi) matrices:
+//   A16[i][j] = fp16(A_ij / s_i),  s_i = the power of two at or above max_j |A_ij|,
+// so the scaling is exact and every entry lies in [-1, 1].  An inner Krylov solve on A16
+// reads 2 bytes per entry; the refinement residuals keep the FP32 hi + bf16 lo operator.
+typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
+typedef float fvec8 __attribute__((ext_vector_type(8)));
+
+constexpr int H16_WAVES = 4;
+
+__device__ __forceinline__ float sel_entry(const float *__restrict__ G,
+                                           const float *__restrict__ H, size_t off, int j,
+                                           const uint8_t *__restrict__ gmask)
+{
+    const bool h = (gmask[j >> 2] >> (j & 3)) & 1u;
+    return h ? H[off + j] : -G[off + j];
+}
+
+// one wavefront per row
+__global__ __launch_bounds__(64 * H16_WAVES) void k_build_h16(
+    const float *__restrict__ G, const float *__restrict__ H, size_t ld, int n,
+    const uint8_t *__restrict__ gmask, _Float16 *__restrict__ A16, float *__restrict__ rs)
+{
+    const int lane = threadIdx.x & 63;
+    const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * H16_WAVES + (threadIdx.x >> 6));
+    if (row >= n) return;
+    const size_t off = (size_t)row * ld;
+    float mx = 0.0f;
+    for (int j = lane; j < n; j += 64) mx = fmaxf(mx, fabsf(sel_entry(G, H, off, j, gmask)));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    int e = 0;
+    if (mx > 0.0f) (void)frexpf(mx, &e);  // mx = f 2^e, f in [0.5, 1): 2^e >= mx
+    const float sc = mx > 0.0f ? ldexpf(1.0f, e) : 1.0f, inv = 1.0f / sc;
+    for (int j = lane; j < (int)ld; j += 64)
+        A16[off + j] = (_Float16)(j < n ? sel_entry(G, H, off, j, gmask) * inv : 0.0f);
+    if (lane == 0) rs[row] = sc;
+}
+
+// y_i = s_i sum_j A16_ij x_j + diag_i xd_i over the listed rows (FP32 vectors, FP32 sums
+// per lane, FP64 wavefront reduction); 8 columns per lane per step.
+constexpr int H16_RPW = 4;
+__global__ __launch_bounds__(64 * H16_WAVES) void k_gemv_h16(
+    const _Float16 *__restrict__ A16, const float *__restrict__ rs, size_t ld, int n8,
+    const float *__restrict__ x, const double *__restrict__ diag, const float *__restrict__ xd,
+    const int *__restrict__ rows, int n_rows, float *__restrict__ y)
+{
+    constexpr int RPW = H16_RPW;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int r[RPW];
+    bool ok[RPW];
+    load_rows<RPW>((blockIdx.x * H16_WAVES + wave) * RPW, rows, n_rows, r, ok);
+    size_t off[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) off[k] = (size_t)r[k] * ld;
+    float acc[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) acc[k] = 0.0f;
+    auto step = [&](int g) {
+        const fvec4 x0 = *((const fvec4 *)x + 2 * g), x1 = *((const fvec4 *)x + 2 * g + 1);
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+            const hvec8 a = __builtin_nontemporal_load((const hvec8 *)(A16 + off[k]) + g);
+            const fvec8 f = __builtin_convertvector(a, fvec8);
+            float t = acc[k];
+            t = fmaf(f.s0, x0.x, t);
+            t = fmaf(f.s1, x0.y, t);
+            t = fmaf(f.s2, x0.z, t);
+            t = fmaf(f.s3, x0.w, t);
+            t = fmaf(f.s4, x1.x, t);
+            t = fmaf(f.s5, x1.y, t);
+            t = fmaf(f.s6, x1.z, t);
+            t = fmaf(f.s7, x1.w, t);
+            acc[k] = t;
+        }
+    };
+    int g = lane;
+    for (; g + 64 < n8; g += 128) {
+        step(g);
+        step(g + 64);
+    }
+    if (g < n8) step(g);
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const double s = wsum((double)acc[k]);
+        if (lane == 0 && ok[k])
+            y[r[k]] = (float)(s * (double)rs[r[k]] +
+                              (diag ? diag[r[k]] * (double)xd[r[k]] : 0.0));
+    }
+}
+
 }  // namespace
+
+void launch_build_h16(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
+                      void *A16, float *rs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_build_h16, dim3((n + H16_WAVES - 1) / H16_WAVES), dim3(64 * H16_WAVES),
+                       0, s, G, H, ld, n, gmask, (_Float16 *)A16, rs);
+}
+
+void launch_gemv_h16(const void *A16, const float *rs, size_t ld, int n, const float *x,
+                     const double *diag, const float *xd, const int *rows, int n_rows, float *y,
+                     hipStream_t s)
+{
+    if (n_rows <= 0) return;
+    constexpr int pb = H16_RPW * H16_WAVES;
+    hipLaunchKernelGGL(k_gemv_h16, dim3((n_rows + pb - 1) / pb), dim3(64 * H16_WAVES), 0, s,
+                       (const _Float16 *)A16, rs, ld, (n + 7) / 8, x, diag, xd, rows, n_rows, y);
+}
 
 void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, const void *Hlo,
                      size_t ld, const int *segs, int n_segs, const uint8_t *gmask, int flip,
