@@ -33,7 +33,7 @@ def log(*a):
 
 
 def run_precision(mesh, bc, precision, steps, warmup, tol, ir_inner=1e-6, tile=16, pblock=64,
-                  inner_format=1, inner16=1e-3):
+                  inner_format=1, inner16=3e-4):
     import torch
 
     from wavebem_amd.bem import BEMProblem, Config
@@ -337,7 +337,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=24000)
     ap.add_argument("--cpu-gemv-n", type=int, default=8192)
     ap.add_argument("--only", choices=["fp64", "fp32_ir"], default=None)
-    ap.add_argument("--ir-inner16", type=float, default=1e-3,
+    ap.add_argument("--ir-inner16", type=float, default=3e-4,
                     help="FP32-IR with the FP16 inner operator: inner reduction per sweep")
     ap.add_argument("--ir-inner-format", type=int, default=1, choices=(0, 1),
                     help="FP32-IR inner Krylov operator: 0 FP32 (hi), 1 row-scaled FP16 copy")

@@ -61,7 +61,7 @@ class Config:
     ir_inner_reduction: float = 1e-7
     assembly_tile: int = 16
     ir_inner_format: int = 1
-    ir_inner16_reduction: float = 1e-3
+    ir_inner16_reduction: float = 3e-4
 
     def to_c(self) -> L.WbConfig:
         c = L.WbConfig()

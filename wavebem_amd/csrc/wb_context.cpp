@@ -596,8 +596,9 @@ static void ensure_h16(wb_ctx *c)
     const size_t entries = (size_t)c->n * c->ld;
     c->A16.ensure(entries * 2);
     c->rs16.ensure(sizeof(float) * c->vlen);
-    launch_build_h16(c->G.as<float>(), c->H.as<float>(), c->ld, c->n, c->gmask.as<uint8_t>(),
-                     c->A16.p, c->rs16.as<float>(), c->stream);
+    launch_build_h16(c->G.as<float>(), c->H.as<float>(), c->ld, c->n, c->segs.as<int>(),
+                     c->n_segs, c->gmask.as<uint8_t>(), c->A16.p, c->rs16.as<float>(),
+                     c->stream);
     HIPCHK(hipGetLastError());
     c->h16_assembly_ver = c->assembly_ver;
     c->h16_mask_ver = c->mask_ver;
@@ -1243,7 +1244,7 @@ void wb_config_default(wb_config *cfg)
     cfg->ir_inner_reduction = 1e-7;
     cfg->assembly_tile = 16;
     cfg->ir_inner_format = 1;
-    cfg->ir_inner16_reduction = 1e-3;
+    cfg->ir_inner16_reduction = 3e-4;
 }
 
 const char *wb_version(void) { return "wavebem_amd 0.1 (gfx950)"; }

@@ -254,34 +254,72 @@ typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
 typedef float fvec8 __attribute__((ext_vector_type(8)));
 
 constexpr int H16_WAVES = 4;
+constexpr int B16_THREADS = 256;  // one workgroup per row of the build
 
-__device__ __forceinline__ float sel_entry(const float *__restrict__ G,
-                                           const float *__restrict__ H, size_t off, int j,
-                                           const uint8_t *__restrict__ gmask)
+typedef _Float16 hvec4 __attribute__((ext_vector_type(4)));
+
+// the 4 selected entries of column group g of a row (segment kind: 0 G, 1 H, 2 mixed)
+__device__ __forceinline__ fvec4 sel_group(const float *__restrict__ G,
+                                           const float *__restrict__ H, size_t off, int g,
+                                           int kind, const uint8_t *__restrict__ gmask)
 {
-    const bool h = (gmask[j >> 2] >> (j & 3)) & 1u;
-    return h ? H[off + j] : -G[off + j];
+    if (kind == 0) return -__builtin_nontemporal_load((const fvec4 *)(G + off) + g);
+    if (kind == 1) return __builtin_nontemporal_load((const fvec4 *)(H + off) + g);
+    const fvec4 a = *((const fvec4 *)(G + off) + g), b = *((const fvec4 *)(H + off) + g);
+    const unsigned m = gmask[g];
+    fvec4 r;
+    r.x = (m & 1u) ? b.x : -a.x;
+    r.y = (m & 2u) ? b.y : -a.y;
+    r.z = (m & 4u) ? b.z : -a.z;
+    r.w = (m & 8u) ? b.w : -a.w;
+    return r;
 }
 
-// one wavefront per row
-__global__ __launch_bounds__(64 * H16_WAVES) void k_build_h16(
+// One workgroup per row: the row maximum over the segments (4 columns per lane per step),
+// then the scaled FP16 row.  Columns past n (the last group's tail and the padding up to
+// ld) are written as 0.
+__global__ __launch_bounds__(B16_THREADS) void k_build_h16(
     const float *__restrict__ G, const float *__restrict__ H, size_t ld, int n,
-    const uint8_t *__restrict__ gmask, _Float16 *__restrict__ A16, float *__restrict__ rs)
+    const int4 *__restrict__ segs, int n_segs, const uint8_t *__restrict__ gmask,
+    _Float16 *__restrict__ A16, float *__restrict__ rs)
 {
-    const int lane = threadIdx.x & 63;
-    const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * H16_WAVES + (threadIdx.x >> 6));
-    if (row >= n) return;
+    __shared__ float red[B16_THREADS / 64];
+    const int row = blockIdx.x;
     const size_t off = (size_t)row * ld;
+    const int ng = (n + 3) / 4;
     float mx = 0.0f;
-    for (int j = lane; j < n; j += 64) mx = fmaxf(mx, fabsf(sel_entry(G, H, off, j, gmask)));
+    for (int si = 0; si < n_segs; ++si) {
+        const int4 sg = segs[si];
+        for (int g = sg.x + (int)threadIdx.x; g < sg.y; g += B16_THREADS) {
+            const fvec4 v = sel_group(G, H, off, g, sg.z, gmask);
+            // entries past n in the last group are zero padding of G and H
+            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+        }
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
     int e = 0;
     if (mx > 0.0f) (void)frexpf(mx, &e);  // mx = f 2^e, f in [0.5, 1): 2^e >= mx
     const float sc = mx > 0.0f ? ldexpf(1.0f, e) : 1.0f, inv = 1.0f / sc;
-    for (int j = lane; j < (int)ld; j += 64)
-        A16[off + j] = (_Float16)(j < n ? sel_entry(G, H, off, j, gmask) * inv : 0.0f);
-    if (lane == 0) rs[row] = sc;
+    for (int si = 0; si < n_segs; ++si) {
+        const int4 sg = segs[si];
+        for (int g = sg.x + (int)threadIdx.x; g < sg.y; g += B16_THREADS) {
+            const fvec4 v = sel_group(G, H, off, g, sg.z, gmask) * inv;
+            hvec4 h = __builtin_convertvector(v, hvec4);
+            if (4 * g + 3 >= n) {
+                if (4 * g + 1 >= n) h.y = (_Float16)0.0f;
+                if (4 * g + 2 >= n) h.z = (_Float16)0.0f;
+                if (4 * g + 3 >= n) h.w = (_Float16)0.0f;
+            }
+            *((hvec4 *)(A16 + off) + g) = h;
+        }
+    }
+    for (int g = ng + (int)threadIdx.x; g < (int)(ld / 4); g += B16_THREADS)
+        *((hvec4 *)(A16 + off) + g) = hvec4{0, 0, 0, 0};
+    if (threadIdx.x == 0) rs[row] = sc;
 }
 
 // y_i = s_i sum_j A16_ij x_j + diag_i xd_i over the listed rows (FP32 vectors, FP32 sums
@@ -338,11 +376,11 @@ __global__ __launch_bounds__(64 * H16_WAVES) void k_gemv_h16(
 
 }  // namespace
 
-void launch_build_h16(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
-                      void *A16, float *rs, hipStream_t s)
+void launch_build_h16(const float *G, const float *H, size_t ld, int n, const int *segs,
+                      int n_segs, const uint8_t *gmask, void *A16, float *rs, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_build_h16, dim3((n + H16_WAVES - 1) / H16_WAVES), dim3(64 * H16_WAVES),
-                       0, s, G, H, ld, n, gmask, (_Float16 *)A16, rs);
+    hipLaunchKernelGGL(k_build_h16, dim3(n), dim3(B16_THREADS), 0, s, G, H, ld, n,
+                       (const int4 *)segs, n_segs, gmask, (_Float16 *)A16, rs);
 }
 
 void launch_gemv_h16(const void *A16, const float *rs, size_t ld, int n, const float *x,

@@ -117,8 +117,8 @@ void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, co
 int multi_chunks(int n_rows, int n_groups);
 // FP16 inner operator (wb_gemv.hip): A16 (n x ld halfs, row scales rs) from the FP32
 // column-selected vmult operator, and y = diag(rs) A16 x + diag .* xd over the listed rows
-void launch_build_h16(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
-                      void *A16, float *rs, hipStream_t s);
+void launch_build_h16(const float *G, const float *H, size_t ld, int n, const int *segs,
+                      int n_segs, const uint8_t *gmask, void *A16, float *rs, hipStream_t s);
 void launch_gemv_h16(const void *A16, const float *rs, size_t ld, int n, const float *x,
                      const double *diag, const float *xd, const int *rows, int n_rows, float *y,
                      hipStream_t s);
