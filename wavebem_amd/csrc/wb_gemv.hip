@@ -253,7 +253,14 @@ __global__ __launch_bounds__(64 * MixedShape<MODE>::WAVES) void k_gemv_mixed(
 typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
 typedef float fvec8 __attribute__((ext_vector_type(8)));
 
-constexpr int H16_WAVES = 4;
+// FP16 GEMV shape: rows per wavefront x wavefronts per workgroup
+#ifndef WB_GEMV16_RPW
+#define WB_GEMV16_RPW 4
+#endif
+#ifndef WB_GEMV16_WAVES
+#define WB_GEMV16_WAVES 4
+#endif
+constexpr int H16_WAVES = WB_GEMV16_WAVES;
 constexpr int B16_THREADS = 256;  // one workgroup per row of the build
 
 typedef _Float16 hvec4 __attribute__((ext_vector_type(4)));
@@ -324,7 +331,7 @@ __global__ __launch_bounds__(B16_THREADS) void k_build_h16(
 
 // y_i = s_i sum_j A16_ij x_j + diag_i xd_i over the listed rows (FP32 vectors, FP32 sums
 // per lane, FP64 wavefront reduction); 8 columns per lane per step.
-constexpr int H16_RPW = 4;
+constexpr int H16_RPW = WB_GEMV16_RPW;
 __global__ __launch_bounds__(64 * H16_WAVES) void k_gemv_h16(
     const _Float16 *__restrict__ A16, const float *__restrict__ rs, size_t ld, int n8,
     const float *__restrict__ x, const double *__restrict__ diag, const float *__restrict__ xd,
