@@ -109,6 +109,9 @@ void launch_cell_geometry(const double *X, const int *cell_int, const int8_t *or
 #ifndef WB_ASM_IUNROLL
 #define WB_ASM_IUNROLL 8  // unroll of the points of one rule row
 #endif
+#ifndef WB_ASM_BATCH
+#define WB_ASM_BATCH 4  // quadrature-point records per batch of scalar loads
+#endif
 #ifndef WB_ASM_HLDS
 #define WB_ASM_HLDS 1  // H accumulators in LDS (1) or in registers like G (0)
 #endif
@@ -172,18 +175,31 @@ void k_assemble_tiles(
             WB_UNROLL(WB_ASM_JUNROLL)
             for (int j = 0; j < Q; ++j) {
                 double sg = 0, sg1 = 0, sh = 0, sh1 = 0;
+                // The records of WB_ASM_BATCH points arrive in one batch of scalar loads:
+                // scalar loads return out of order, so every use of one waits for all
+                // outstanding ones (lgkmcnt(0)), and batching divides those waits.
                 WB_UNROLL(WB_ASM_IUNROLL)
-                for (int i = 0; i < Q; ++i) {
-                    const double *r = rec + (i + Q * j) * 8;
-                    const double R0 = r[0] - x0, R1 = r[1] - x1, R2 = r[2] - x2;
-                    const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
-                    const double ri = rsqrt_f64(r2);
-                    sg = __builtin_fma(ri, r[6], sg);   // sum JxW / r
-                    sg1 = __builtin_fma(ri, r[7], sg1); // sum xi JxW / r
-                    const double dn = __builtin_fma(R0, r[3], __builtin_fma(R1, r[4], R2 * r[5]));
-                    const double thv = (dn * ri) * (ri * ri);
-                    sh += thv;
-                    sh1 = __builtin_fma(nodes.phi[i], thv, sh1);
+                for (int i = 0; i < Q; i += WB_ASM_BATCH) {
+                    constexpr int B = WB_ASM_BATCH;
+                    double rr[8 * B];
+#pragma unroll
+                    for (int k = 0; k < 8 * B; ++k)
+                        rr[k] = (i + k / 8 < Q) ? rec[(i + Q * j) * 8 + k] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < B; ++u) {
+                        if (i + u >= Q) break;
+                        const double *r = rr + 8 * u;
+                        const double R0 = r[0] - x0, R1 = r[1] - x1, R2 = r[2] - x2;
+                        const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
+                        const double ri = rsqrt_f64(r2);
+                        sg = __builtin_fma(ri, r[6], sg);   // sum JxW / r
+                        sg1 = __builtin_fma(ri, r[7], sg1); // sum xi JxW / r
+                        const double dn =
+                            __builtin_fma(R0, r[3], __builtin_fma(R1, r[4], R2 * r[5]));
+                        const double thv = (dn * ri) * (ri * ri);
+                        sh += thv;
+                        sh1 = __builtin_fma(nodes.phi[i + u], thv, sh1);
+                    }
                 }
                 const double sg0 = sg - sg1, sh0 = sh - sh1;  // sum (1 - xi_i) t
                 const double eta = nodes.phi[j], ome = 1.0 - eta;
