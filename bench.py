@@ -374,7 +374,7 @@ def main():
         r = run_moving(args.workload, args.moving)
         print(json.dumps({"metric": "moving-mesh step time (re-assembly + solve)", "unit": "ms",
                           "value": r["fp32_ir_ms_per_step"], "higher_is_better": False,
-                          "dtype": "fp32+fp64-ir", "data": "synthetic",
+                          "dtype": "fp16+fp32/bf16+fp64-ir", "data": "synthetic",
                           "config": {"workload": f"wigley_closed_halfbox_{args.workload}",
                                      "n_dofs": r["n_dofs"], "steps": r["steps"]},
                           "detail": r}), flush=True)
