@@ -126,12 +126,18 @@ def run_adjoint(workload, steps):
             t3 = time.perf_counter()
             p.residual_vjp(lam, phi.detach().numpy(), dphi.detach().numpy())
             t4 = time.perf_counter()
+            # one tangent (Newton JVP) of the residual along a seeded node motion
+            dxv = rng.standard_normal((m.n_dofs, 3)) * 1e-3
+            t5 = time.perf_counter()
+            p.residual_jvp(dxv, phi.detach().numpy(), dphi.detach().numpy())
+            t6 = time.perf_counter()
             if s == 0:
                 continue  # warmup
             times["forward"].append((t1 - t0) * 1e3)
             times["adjoint_solve"].append((t3 - t2) * 1e3)
             times["backward_assembly"].append((t4 - t3) * 1e3)
             times["total"].append((t4 - t0) * 1e3)
+            times.setdefault("tangent_residual_jvp", []).append((t6 - t5) * 1e3)
     out = {k: round(float(np.mean(v)), 3) for k, v in times.items()}
     out["n_dofs"] = m.n_dofs
     out["gradient_entries"] = 500 + 3 * 200

@@ -26,6 +26,16 @@
 #include "wb_device.h"
 #include "wb_internal.h"
 
+// Wave-uniform records (point records of a cell, row records) are read by scalar loads,
+// which return out of order: each use waits for every outstanding scalar load.  Records
+// are therefore fetched in batches, one wait per batch.
+#ifndef WB_ADJ_BATCH
+#define WB_ADJ_BATCH 4  // points (rows) per batch in k_vjp_rows and k_vjp_points
+#endif
+#ifndef WB_JVP_BATCH
+#define WB_JVP_BATCH 2  // points per batch in k_jvp_rows (18 doubles per point)
+#endif
+
 namespace wb {
 
 namespace {
@@ -186,20 +196,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const double *rec = cellq + (size_t)c * (NQ * 8);
             const double *st = cstate + (size_t)c * (NQ * 2);
             double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+            constexpr int B = WB_ADJ_BATCH;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const double *r = rec + 8 * q;
-                const double R0 = r[0] - x0, R1 = r[1] - x1, R2 = r[2] - x2;
-                const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
-                const double ri = rsqrt_f64(r2);
-                const double ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
-                const double RN = __builtin_fma(R0, r[3], __builtin_fma(R1, r[4], R2 * r[5]));
-                const double D = st[2 * q] - phi_i;
-                const double t1 = -D * ri3;
-                const double t2 = __builtin_fma(3.0 * D * RN, ri5, st[2 * q + 1] * r[6] * ri3);
-                g0 = __builtin_fma(t1, r[3], __builtin_fma(t2, R0, g0));
-                g1 = __builtin_fma(t1, r[4], __builtin_fma(t2, R1, g1));
-                g2 = __builtin_fma(t1, r[5], __builtin_fma(t2, R2, g2));
+            for (int q0 = 0; q0 < NQ; q0 += B) {
+                double rb[8 * B], sb[2 * B];
+#pragma unroll
+                for (int k = 0; k < 8 * B; ++k) rb[k] = (q0 + k / 8 < NQ) ? rec[8 * q0 + k] : 0.0;
+#pragma unroll
+                for (int k = 0; k < 2 * B; ++k) sb[k] = (q0 + k / 2 < NQ) ? st[2 * q0 + k] : 0.0;
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    if (q0 + u >= NQ) break;
+                    const double *r = rb + 8 * u;
+                    const double R0 = r[0] - x0, R1 = r[1] - x1, R2 = r[2] - x2;
+                    const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
+                    const double ri = rsqrt_f64(r2);
+                    const double ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
+                    const double RN = __builtin_fma(R0, r[3], __builtin_fma(R1, r[4], R2 * r[5]));
+                    const double D = sb[2 * u] - phi_i;
+                    const double t1 = -D * ri3;
+                    const double t2 = __builtin_fma(3.0 * D * RN, ri5, sb[2 * u + 1] * r[6] * ri3);
+                    g0 = __builtin_fma(t1, r[3], __builtin_fma(t2, R0, g0));
+                    g1 = __builtin_fma(t1, r[4], __builtin_fma(t2, R1, g1));
+                    g2 = __builtin_fma(t1, r[5], __builtin_fma(t2, R2, g2));
+                }
             }
             const bool sing = (rc == cc.x) | (rc == cc.y) | (rc == cc.z) | (rc == cc.w);
             if (!sing) {
@@ -241,29 +261,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     double ay0 = 0, ay1 = 0, ay2 = 0, aN0 = 0, aN1 = 0, aN2 = 0, aJ = 0;
     const int i0 = blockIdx.y * rows_per_chunk;
     const int i1 = min(n, i0 + rows_per_chunk);
-    for (int i = i0; i < i1; ++i) {
-        const double *rp = rowpack + 8 * (size_t)i;
-        const double wi = rp[3];
-        if (wi == 0.0) continue;
-        const int ci = (int)rp[5];
-        const double R0 = y0 - rp[0], R1 = y1 - rp[1], R2 = y2 - rp[2];
-        const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
-        const double ri = rsqrt_f64(r2);
-        const double ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
-        const double RN = __builtin_fma(R0, N0, __builtin_fma(R1, N1, R2 * N2));
-        const double D = P - rp[4];
-        const double t1 = -D * ri3;
-        const double t2 = __builtin_fma(3.0 * D * RN, ri5, SJ * ri3);
-        const bool sing = (ci == cc.x) | (ci == cc.y) | (ci == cc.z) | (ci == cc.w);
-        const double ws = sing ? 0.0 : wi;
-        const double u1 = ws * t1, u2 = ws * t2;
-        ay0 = __builtin_fma(u1, N0, __builtin_fma(u2, R0, ay0));
-        ay1 = __builtin_fma(u1, N1, __builtin_fma(u2, R1, ay1));
-        ay2 = __builtin_fma(u1, N2, __builtin_fma(u2, R2, ay2));
-        aN0 = __builtin_fma(u1, R0, aN0);
-        aN1 = __builtin_fma(u1, R1, aN1);
-        aN2 = __builtin_fma(u1, R2, aN2);
-        aJ = __builtin_fma(-ws * S, ri, aJ);
+    constexpr int B = WB_ADJ_BATCH;
+    for (int ib = i0; ib < i1; ib += B) {
+        double pb[8 * B];
+#pragma unroll
+        for (int k = 0; k < 8 * B; ++k)
+            pb[k] = (ib + k / 8 < i1) ? rowpack[8 * (size_t)ib + k] : 0.0;
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const double *rp = pb + 8 * u;
+            const double wi = rp[3];  // 0 for rows past the chunk (padding above)
+            if (wi == 0.0) continue;
+            const int ci = (int)rp[5];
+            const double R0 = y0 - rp[0], R1 = y1 - rp[1], R2 = y2 - rp[2];
+            const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
+            const double ri = rsqrt_f64(r2);
+            const double ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
+            const double RN = __builtin_fma(R0, N0, __builtin_fma(R1, N1, R2 * N2));
+            const double D = P - rp[4];
+            const double t1 = -D * ri3;
+            const double t2 = __builtin_fma(3.0 * D * RN, ri5, SJ * ri3);
+            const bool sing = (ci == cc.x) | (ci == cc.y) | (ci == cc.z) | (ci == cc.w);
+            const double ws = sing ? 0.0 : wi;
+            const double u1 = ws * t1, u2 = ws * t2;
+            ay0 = __builtin_fma(u1, N0, __builtin_fma(u2, R0, ay0));
+            ay1 = __builtin_fma(u1, N1, __builtin_fma(u2, R1, ay1));
+            ay2 = __builtin_fma(u1, N2, __builtin_fma(u2, R2, ay2));
+            aN0 = __builtin_fma(u1, R0, aN0);
+            aN1 = __builtin_fma(u1, R1, aN1);
+            aN2 = __builtin_fma(u1, R2, aN2);
+            aJ = __builtin_fma(-ws * S, ri, aJ);
+        }
     }
     if (live) {
         double *o = part + (((size_t)blockIdx.y * nc + c) * NQ + q) * 7;
@@ -474,15 +502,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const double *st = cstate + (size_t)c * (NQ * 2);
         const double *tr = trec + (size_t)c * (NQ * 8);
         double g = 0.0;
+        constexpr int B = WB_JVP_BATCH;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const double *r = rec + 8 * q, *t = tr + 8 * q;
+        for (int q0 = 0; q0 < NQ; q0 += B) {
+          double rb[8 * B], tb[8 * B], sb[2 * B];
+#pragma unroll
+          for (int k = 0; k < 8 * B; ++k) {
+              rb[k] = (q0 + k / 8 < NQ) ? rec[8 * q0 + k] : 0.0;
+              tb[k] = (q0 + k / 8 < NQ) ? tr[8 * q0 + k] : 0.0;
+          }
+#pragma unroll
+          for (int k = 0; k < 2 * B; ++k) sb[k] = (q0 + k / 2 < NQ) ? st[2 * q0 + k] : 0.0;
+#pragma unroll
+          for (int u = 0; u < B; ++u) {
+            if (q0 + u >= NQ) break;
+            const int q = u;  // index into this batch
+            const double *r = rb + 8 * u, *t = tb + 8 * u;
             const double R0 = r[0] - x0, R1 = r[1] - x1, R2 = r[2] - x2;
             const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
             const double ri = rsqrt_f64(r2);
             const double ri2 = ri * ri, ri3 = ri2 * ri, ri5 = ri3 * ri2;
             const double RN = __builtin_fma(R0, r[3], __builtin_fma(R1, r[4], R2 * r[5]));
-            const double D = st[2 * q] - phi_i, E = st[2 * q + 1];
+            const double D = sb[2 * q] - phi_i, E = sb[2 * q + 1];
             const double t1 = -D * ri3;
             const double t2 = __builtin_fma(3.0 * D * RN, ri5, E * r[6] * ri3);
             const double d0 = t[0] - e0, d1 = t[1] - e1, d2 = t[2] - e2;  // dR
@@ -490,6 +531,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const double Rd = __builtin_fma(R0, d0, __builtin_fma(R1, d1, R2 * d2));
             const double RdN = __builtin_fma(R0, t[3], __builtin_fma(R1, t[4], R2 * t[5]));
             g += t1 * (Nd + RdN) + t2 * Rd - E * ri * t[6];
+          }
         }
         const bool sing = (rc == cc.x) | (rc == cc.y) | (rc == cc.z) | (rc == cc.w);
         if (!sing) acc += g;
