@@ -881,13 +881,18 @@ static void apply_constrained_multi(wb_ctx *c, int mode, int k, const void *X, s
 {
     require(!c->pure_neumann, WB_E_UNSUPPORTED,
             "several right-hand sides with all-Neumann masks are not supported");
-    const int is_float = mode == 1;
+    const int is_float = mode == 1 || mode == 3;
     const size_t es = is_float ? 4 : 8;
     {
-        KernelScope ks(c, mode == 0 ? WB_K_GEMV64 : (mode == 1 ? WB_K_GEMV32 : WB_K_GEMVHL),
+        KernelScope ks(c,
+                       mode == 0 ? WB_K_GEMV64
+                                 : (mode == 1 ? WB_K_GEMV32 : (mode == 3 ? WB_K_GEMV16 : WB_K_GEMVHL)),
                        (long long)c->n_free);
         double *part = multi_part(c, k);
-        launch_gemv_sel_multi(mode, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld, c->segs.as<int>(),
+        // mode 3 reads the FP16 operator (A16) and its row scales in place of G and H
+        const void *Gm = mode == 3 ? c->A16.p : c->G.p;
+        const void *Hm = mode == 3 ? c->rs16.p : c->H.p;
+        launch_gemv_sel_multi(mode, Gm, Hm, c->Glo.p, c->Hlo.p, c->ld, c->segs.as<int>(),
                               c->n_segs, c->gmask.as<uint8_t>(), 0, 1.0, -1.0, k, X, ldx,
                               c->diag_op.as<double>(), X, c->free_rows.as<int>(), c->n_free, Y,
                               ldy, part, (c->n + 3) / 4, c->stream);
@@ -929,7 +934,7 @@ static std::vector<int> gmres_multi(wb_ctx *c, int is_float, int k, const void *
     }
     void *Z = is_float ? M.Z32.p : M.Z.p;
     void *W = is_float ? M.W32.p : M.W.p;
-    const int mode = is_float ? 1 : (c->cfg.precision == WB_PREC_FP64 ? 0 : 2);
+    const int mode = is_float ? c->float_mode : (c->cfg.precision == WB_PREC_FP64 ? 0 : 2);
     const int parts = krylov_partials_blocks(n);
     const size_t sH = 132;
     auto basis = [&](int i) { return (void *)((char *)V.p + es * vl * i); };
@@ -1156,6 +1161,11 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
         HIPCHK(hipMemcpyAsync(M.R.p, M.B.p, d8, hipMemcpyDeviceToDevice, c->stream));
         int sweeps = 0, total = 0;
         std::vector<int> done(k, 0);
+        // inner solves on the FP16 operator (ir_inner_format = 1) or the FP32 one
+        const int fmode = c->cfg.ir_inner_format == 1 ? 3 : 1;
+        if (fmode == 3) ensure_h16(c);
+        FloatMode fm(c, fmode);
+        std::vector<double> prev(k, 0.0);
         for (;;) {
             precond_multi(c, 0, k, M.R.p, M.T.p);
             launch_norm2_partials(0, M.T.p, n, M.Pn.as<double>(), c->stream, k, vl);
@@ -1175,6 +1185,12 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
             if (all || bad || sweeps >= c->cfg.ir_max_sweeps ||
                 total >= c->cfg.solver_max_steps)
                 break;
+            // an FP16 sweep that did not halve some residual hands the remaining sweeps of
+            // every right-hand side to the FP32 operator
+            if (c->float_mode == 3 && sweeps > 0)
+                for (int v = 0; v < k; ++v)
+                    if (!done[v] && !(resid[v] < 0.5 * prev[v])) c->float_mode = 1;
+            prev = resid;
             launch_vec_ops(1, (int)(vl * k), M.R.p, nullptr, M.R32.p, 0, c->stream);
             launch_vec_ops(7, (int)(vl * k), nullptr, nullptr, M.D32.p, 0, c->stream);
             std::vector<int> st_in(k);

@@ -147,6 +147,83 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_f32(
     }
 }
 
+// Row-scaled FP16 operator (A16, the vmult selection and signs included, row scales rs),
+// float vectors.  The FP16 entries are widened to FP32 in registers and contracted on the
+// FP32 matrix cores, so the right-hand sides keep FP32 precision.  Lane (row i, k-slot q)
+// loads the 8 contiguous columns cb + 8q .. cb + 8q + 7 of its row (16 bytes) per 32-column
+// step; step s contracts column cb + 8q + s, for A and B alike.  Sums move into FP64 every
+// 128 columns, as in k_gemm_sel_f32; the row scale is applied to the FP64 chunk sums.
+typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
+typedef float fvec8 __attribute__((ext_vector_type(8)));
+
+__global__ __launch_bounds__(64 * MW) void k_gemm_h16(
+    const _Float16 *__restrict__ A16, const float *__restrict__ rs, size_t ld, int n, int k,
+    const float *__restrict__ X, size_t ldx, const int *__restrict__ rows, int n_rows,
+    int cols_per_chunk, double *__restrict__ part)
+{
+    const int c0 = blockIdx.y * cols_per_chunk, c1 = min(n, c0 + cols_per_chunk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int base = (blockIdx.x * MW + wave) * 16 * RT;
+    const int i = lane & 15, q = lane >> 4;  // row in tile, k-slot
+    size_t roff[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) roff[t] = (size_t)tile_row(base + 16 * t + i, rows, n_rows) * ld;
+    const int v = lane & 15;  // right-hand side fed by this lane's B operand
+    const bool vok = v < k;
+    const float *Xv = X + (size_t)(vok ? v : 0) * ldx;
+    fvec4 c[RT];
+    double cd[RT][4];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        c[t] = fvec4{0.f, 0.f, 0.f, 0.f};
+        cd[t][0] = cd[t][1] = cd[t][2] = cd[t][3] = 0.0;
+    }
+    int since = 0;
+    for (int cb = c0; cb < c1; cb += 32) {
+        if (++since == 4) {
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cd[t][r] += (double)c[t][r];
+                c[t] = fvec4{0.f, 0.f, 0.f, 0.f};
+            }
+            since = 0;
+        }
+        const int cc = cb + 8 * q;  // this lane's 8 columns (A16 is zero past n up to ld)
+        fvec8 a[RT], b = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < RT; ++t) a[t] = fvec8{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (cc < c1) {
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+                a[t] = __builtin_convertvector(
+                    __builtin_nontemporal_load((const hvec8 *)(A16 + roff[t] + cc)), fvec8);
+            if (vok) {
+                const fvec4 x0 = *((const fvec4 *)(Xv + cc)), x1 = *((const fvec4 *)(Xv + cc) + 1);
+                b = fvec8{x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            }
+        }
+#pragma unroll
+        for (int st = 0; st < 8; ++st)
+#pragma unroll
+            for (int t = 0; t < RT; ++t)
+                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t][st], b[st], c[t], 0, 0, 0);
+    }
+    if (vok) {
+        double *pp = part + ((size_t)blockIdx.y * k + v) * n_rows;
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int s2 = base + 16 * t + 4 * q + r;
+                if (s2 < n_rows) {
+                    const int rr = rows ? rows[s2] : s2;
+                    pp[s2] = (cd[t][r] + (double)c[t][r]) * (double)rs[rr];
+                }
+            }
+    }
+}
+
 // FP64 accumulation on the FP64 matrix cores: MODE 0 reads the FP64 matrices, MODE 2 the
 // FP32 hi + bf16 lo pair (hi + lo formed in FP64 on load); FP64 vectors.  Lane streams 4
 // columns (one 4-column group) of its row of each tile.
@@ -292,7 +369,13 @@ void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *G
     const int chunks = multi_chunks(n_rows, n_groups);
     const int gpc = (n_groups + chunks - 1) / chunks;
     dim3 grid((n_rows + RPB - 1) / RPB, chunks), block(64 * MW);
-    if (mode == 1)
+    if (mode == 3) {
+        // G holds A16 and H the row scales; chunks of whole 32-column steps
+        const int n = 4 * n_groups;
+        const int cpc = ((n + chunks - 1) / chunks + 31) / 32 * 32;
+        hipLaunchKernelGGL(k_gemm_h16, grid, block, 0, s, (const _Float16 *)G, (const float *)H,
+                           ld, n, k, (const float *)X, ldx, rows, n_rows, cpc, part);
+    } else if (mode == 1)
         hipLaunchKernelGGL(k_gemm_sel_f32, grid, block, 0, s, (const float *)G, (const float *)H,
                            ld, (const int4 *)segs, n_segs, gmask, flip, (float)s_sel,
                            (float)s_unsel, k, (const float *)X, ldx, rows, n_rows, gpc, part);
@@ -305,7 +388,7 @@ void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *G
                            (const int4 *)segs, n_segs, gmask, flip, s_sel, s_unsel, k,
                            (const double *)X, ldx, rows, n_rows, gpc, part);
     dim3 rg((n_rows + 255) / 256, k);
-    if (mode == 1)
+    if (mode == 1 || mode == 3)
         hipLaunchKernelGGL(k_multi_reduce<float>, rg, dim3(256), 0, s, part, chunks, k, rows,
                            n_rows, diag, (const float *)Xd, ldx, (float *)Y, ldy);
     else
