@@ -132,13 +132,14 @@ struct wb_ctx {
     // several right-hand sides at once
     struct Multi {
         int k = 0;
-        DevBuf V, V32, Z, W, Z32, W32, P1, P2, Pn, H, hred, X, B, R, T, D32, R32, act, inh, part;
+        DevBuf V, V32, Z, W, Z32, W32, P1, P2, Pn, H, hred, X, B, R, T, D32, R32, act, inh, part,
+            xs;
         double *hpin = nullptr;  // pinned k x 132 coefficients
         int *apin = nullptr;     // pinned k active flags
         std::vector<DevBuf *> all()
         {
             return {&V, &V32, &Z, &W, &Z32, &W32, &P1, &P2, &Pn, &H, &hred,
-                    &X, &B, &R, &T, &D32, &R32, &act, &inh, &part};
+                    &X, &B, &R, &T, &D32, &R32, &act, &inh, &part, &xs};
         }
     } mw;
 };
@@ -868,10 +869,10 @@ static void solve_system_dev(wb_ctx *c, const double *tmp_dev_internal, wb_solve
 // ------------------------------------------------------------------ several right-hand sides
 // Y[:, v] = ConstrainedOperator(A) X[:, v] for the k columns (column strides ldx / ldy).
 // chunk partials of the batched product for k vectors over the unconstrained rows
-static double *multi_part(wb_ctx *c, int k)
+static double *multi_part(wb_ctx *c, int mode, int k)
 {
     const size_t need =
-        sizeof(double) * (size_t)multi_chunks(c->n_free, (c->n + 3) / 4) * k * c->n_free;
+        sizeof(double) * (size_t)multi_chunks(mode, c->n_free, (c->n + 3) / 4) * k * c->n_free;
     c->mw.part.ensure(std::max<size_t>(need, 8));
     return c->mw.part.as<double>();
 }
@@ -888,11 +889,14 @@ static void apply_constrained_multi(wb_ctx *c, int mode, int k, const void *X, s
                        mode == 0 ? WB_K_GEMV64
                                  : (mode == 1 ? WB_K_GEMV32 : (mode == 3 ? WB_K_GEMV16 : WB_K_GEMVHL)),
                        (long long)c->n_free);
-        double *part = multi_part(c, k);
-        // mode 3 reads the FP16 operator (A16) and its row scales in place of G and H
+        double *part = multi_part(c, mode, k);
+        // mode 3 reads the FP16 operator (A16) and its row scales in place of G and H, and
+        // takes the vector-scale and split-vector workspace in place of Glo
         const void *Gm = mode == 3 ? c->A16.p : c->G.p;
         const void *Hm = mode == 3 ? c->rs16.p : c->H.p;
-        launch_gemv_sel_multi(mode, Gm, Hm, c->Glo.p, c->Hlo.p, c->ld, c->segs.as<int>(),
+        if (mode == 3) c->mw.xs.ensure(256 + 4 * (size_t)k * c->ld);
+        const void *Glm = mode == 3 ? c->mw.xs.p : c->Glo.p;
+        launch_gemv_sel_multi(mode, Gm, Hm, Glm, c->Hlo.p, c->ld, c->segs.as<int>(),
                               c->n_segs, c->gmask.as<uint8_t>(), 0, 1.0, -1.0, k, X, ldx,
                               c->diag_op.as<double>(), X, c->free_rows.as<int>(), c->n_free, Y,
                               ldy, part, (c->n + 3) / 4, c->stream);
@@ -1131,7 +1135,7 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
     double t0 = now_ms();
     {
         KernelScope ks(c, mode64 == 0 ? WB_K_GEMV64 : WB_K_GEMVHL, (long long)c->n_free);
-        double *part = multi_part(c, k);
+        double *part = multi_part(c, mode64, k);
         launch_gemv_sel_multi(mode64, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld,
                               c->segs.as<int>(), c->n_segs, c->gmask.as<uint8_t>(), 1, -1.0, 1.0,
                               k, M.T.p, vl, c->diag_rhs.as<double>(), M.T.p,

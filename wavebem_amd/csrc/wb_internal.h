@@ -112,9 +112,11 @@ void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, co
                      double s_sel, double s_unsel, const void *x, const double *diag,
                      const void *xd, const int *rows, int n_rows, void *y, hipStream_t s);
 // k <= 16 vectors at once (column v at X + v*ldx, Xd + v*ldx, Y + v*ldy) on the matrix
-// cores; the columns are split into multi_chunks(n_rows, n_groups) chunks whose partials
-// (part: chunks * k * n_rows doubles) are added in order
-int multi_chunks(int n_rows, int n_groups);
+// cores; the columns are split into multi_chunks(mode, n_rows, n_groups) chunks whose
+// partials (part: chunks * k * n_rows doubles) are added in order.  Mode 3 (FP16 operator)
+// takes G = A16, H = its row scales and Glo = a workspace of 256 + 4 k ld bytes (the vector
+// scales, then the FP16 hi and lo halves of the vectors).
+int multi_chunks(int mode, int n_rows, int n_groups);
 // FP16 inner operator (wb_gemv.hip): A16 (n x ld halfs, row scales rs) from the FP32
 // column-selected vmult operator, and y = diag(rs) A16 x + diag .* xd over the listed rows
 void launch_build_h16(const float *G, const float *H, size_t ld, int n, const int *segs,

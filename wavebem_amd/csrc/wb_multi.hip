@@ -15,6 +15,9 @@
 // not wait on one another's results.  The four lanes of a row (k-slots) read adjacent
 // 4-column groups, so a row segment of 64 (FP32) or 128 (FP64) bytes comes from one load
 // instruction.
+#include <algorithm>
+#include <atomic>
+
 #include "wb_internal.h"
 
 namespace wb {
@@ -148,18 +151,80 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_f32(
 }
 
 // Row-scaled FP16 operator (A16, the vmult selection and signs included, row scales rs),
-// float vectors.  The FP16 entries are widened to FP32 in registers and contracted on the
-// FP32 matrix cores, so the right-hand sides keep FP32 precision.  Lane (row i, k-slot q)
-// loads the 8 contiguous columns cb + 8q .. cb + 8q + 7 of its row (16 bytes) per 32-column
-// step; step s contracts column cb + 8q + s, for A and B alike.  Sums move into FP64 every
-// 128 columns, as in k_gemm_sel_f32; the row scale is applied to the FP64 chunk sums.
+// float vectors, on the FP16 matrix cores (v_mfma_f32_16x16x16_f16; layout checked on the
+// device: A[i][kk] in lane i + 16 (kk/4), element kk % 4, B likewise with j for i, D as for
+// the FP32 instruction).  Each right-hand side is scaled by a power of two xs[v] (its
+// largest entry to about 2^13) and split, once per product, into FP16 hi and
+// lo = (x - hi) 2^11 (k_vec_split), contracted in two accumulator chains, so the vectors
+// keep about 22 bits.  Lane (row i, k-slot q) loads
+// the 8 contiguous columns cb + 8q .. cb + 8q + 7 (16 bytes) of each 32-column sub-step;
+// WB_MULTI16_SUB sub-steps are loaded together, so more bytes are in flight per lane.
+// Matrix-core pass m of a sub-step contracts its columns 8q + 4m + e (e < 4) for A and B
+// alike.  Sums move into FP64 every 4 steps; the row scale and 1/xs[v] are applied to the
+// FP64 chunk sums.
 typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
-typedef float fvec8 __attribute__((ext_vector_type(8)));
+typedef _Float16 hvec4 __attribute__((ext_vector_type(4)));
+#ifndef WB_MULTI16_SUB
+#define WB_MULTI16_SUB 2
+#endif
+
+// xs[v] = 2^(13 - e), e the exponent of max |X[:, v]| (1 for a zero vector); one block
+// per vector, fixed reduction order
+__global__ __launch_bounds__(1024) void k_vec_scale(const float *__restrict__ X, size_t ldx,
+                                                   int n, float *__restrict__ xs)
+{
+    __shared__ float red[16];
+    const float *x = X + (size_t)blockIdx.x * ldx;
+    float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
+    int e = threadIdx.x;
+    for (; e + 3 * 1024 < n; e += 4 * 1024) {
+        m0 = fmaxf(m0, fabsf(x[e]));
+        m1 = fmaxf(m1, fabsf(x[e + 1024]));
+        m2 = fmaxf(m2, fabsf(x[e + 2048]));
+        m3 = fmaxf(m3, fabsf(x[e + 3072]));
+    }
+    for (; e < n; e += 1024) m0 = fmaxf(m0, fabsf(x[e]));
+    float m = fmaxf(fmaxf(m0, m1), fmaxf(m2, m3));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = 0.0f;
+        for (int w = 0; w < 16; ++w) m = fmaxf(m, red[w]);
+        int ex = 0;
+        if (m > 0.0f) (void)frexpf(m, &ex);
+        xs[blockIdx.x] = m > 0.0f ? ldexpf(1.0f, 13 - ex) : 1.0f;
+    }
+}
+
+// XS[v][c] = fp16(xs[v] X[v][c]) and XS[k + v][c] = fp16((xs[v] X[v][c] - XS[v][c]) 2^11)
+// for c < lds (zero past n); one thread per 8 columns
+__global__ __launch_bounds__(256) void k_vec_split(const float *__restrict__ X, size_t ldx,
+                                                  int n, int k, const float *__restrict__ xs,
+                                                  _Float16 *__restrict__ XS, size_t lds)
+{
+    const int v = blockIdx.y;
+    const size_t c = 8 * ((size_t)blockIdx.x * 256 + threadIdx.x);
+    if (c >= lds) return;
+    const float *x = X + (size_t)v * ldx;
+    const float sx = xs[v];
+    hvec8 hi, lo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float f = c + e < (size_t)n ? x[c + e] * sx : 0.0f;
+        const _Float16 h = (_Float16)f;
+        hi[e] = h;
+        lo[e] = (_Float16)((f - (float)h) * 2048.0f);
+    }
+    *(hvec8 *)(XS + (size_t)v * lds + c) = hi;
+    *(hvec8 *)(XS + (size_t)(k + v) * lds + c) = lo;
+}
 
 __global__ __launch_bounds__(64 * MW) void k_gemm_h16(
     const _Float16 *__restrict__ A16, const float *__restrict__ rs, size_t ld, int n, int k,
-    const float *__restrict__ X, size_t ldx, const int *__restrict__ rows, int n_rows,
-    int cols_per_chunk, double *__restrict__ part)
+    const _Float16 *__restrict__ XS, const float *__restrict__ xs, const int *__restrict__ rows,
+    int n_rows, int cols_per_chunk, double *__restrict__ part)
 {
     const int c0 = blockIdx.y * cols_per_chunk, c1 = min(n, c0 + cols_per_chunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -170,47 +235,68 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_h16(
     for (int t = 0; t < RT; ++t) roff[t] = (size_t)tile_row(base + 16 * t + i, rows, n_rows) * ld;
     const int v = lane & 15;  // right-hand side fed by this lane's B operand
     const bool vok = v < k;
-    const float *Xv = X + (size_t)(vok ? v : 0) * ldx;
-    fvec4 c[RT];
+    const _Float16 *Xh = XS + (size_t)(vok ? v : 0) * ld;
+    const _Float16 *Xl = XS + (size_t)(vok ? k + v : 0) * ld;
+    fvec4 ch[RT], cl[RT];
     double cd[RT][4];
 #pragma unroll
     for (int t = 0; t < RT; ++t) {
-        c[t] = fvec4{0.f, 0.f, 0.f, 0.f};
+        ch[t] = fvec4{0.f, 0.f, 0.f, 0.f};
+        cl[t] = fvec4{0.f, 0.f, 0.f, 0.f};
         cd[t][0] = cd[t][1] = cd[t][2] = cd[t][3] = 0.0;
     }
+    constexpr float ILO = 1.0f / 2048.0f;
+    constexpr int NS = WB_MULTI16_SUB;  // 32-column sub-steps loaded together
     int since = 0;
-    for (int cb = c0; cb < c1; cb += 32) {
+    for (int cb = c0; cb < c1; cb += 32 * NS) {
         if (++since == 4) {
 #pragma unroll
             for (int t = 0; t < RT; ++t) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) cd[t][r] += (double)c[t][r];
-                c[t] = fvec4{0.f, 0.f, 0.f, 0.f};
+                for (int r = 0; r < 4; ++r)
+                    cd[t][r] += (double)ch[t][r] + (double)cl[t][r] * (double)ILO;
+                ch[t] = fvec4{0.f, 0.f, 0.f, 0.f};
+                cl[t] = fvec4{0.f, 0.f, 0.f, 0.f};
             }
             since = 0;
         }
-        const int cc = cb + 8 * q;  // this lane's 8 columns (A16 is zero past n up to ld)
-        fvec8 a[RT], b = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        hvec8 a[NS][RT], bh[NS], bl[NS];
 #pragma unroll
-        for (int t = 0; t < RT; ++t) a[t] = fvec8{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (cc < c1) {
+        for (int u = 0; u < NS; ++u) {
+            // this lane's 8 columns of sub-step u (A16 and XS are zero past n up to ld)
+            const int cc = cb + 32 * u + 8 * q;
+            bh[u] = hvec8{0, 0, 0, 0, 0, 0, 0, 0};
+            bl[u] = hvec8{0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-            for (int t = 0; t < RT; ++t)
-                a[t] = __builtin_convertvector(
-                    __builtin_nontemporal_load((const hvec8 *)(A16 + roff[t] + cc)), fvec8);
-            if (vok) {
-                const fvec4 x0 = *((const fvec4 *)(Xv + cc)), x1 = *((const fvec4 *)(Xv + cc) + 1);
-                b = fvec8{x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            for (int t = 0; t < RT; ++t) a[u][t] = hvec8{0, 0, 0, 0, 0, 0, 0, 0};
+            if (cc < c1) {
+#pragma unroll
+                for (int t = 0; t < RT; ++t)
+                    a[u][t] = __builtin_nontemporal_load((const hvec8 *)(A16 + roff[t] + cc));
+                if (vok) {
+                    bh[u] = *(const hvec8 *)(Xh + cc);
+                    bl[u] = *(const hvec8 *)(Xl + cc);
+                }
             }
         }
 #pragma unroll
-        for (int st = 0; st < 8; ++st)
+        for (int u = 0; u < NS; ++u)
 #pragma unroll
-            for (int t = 0; t < RT; ++t)
-                c[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t][st], b[st], c[t], 0, 0, 0);
+            for (int m = 0; m < 2; ++m) {
+                const hvec4 bhm = {bh[u][4 * m], bh[u][4 * m + 1], bh[u][4 * m + 2], bh[u][4 * m + 3]};
+                const hvec4 blm = {bl[u][4 * m], bl[u][4 * m + 1], bl[u][4 * m + 2], bl[u][4 * m + 3]};
+#pragma unroll
+                for (int t = 0; t < RT; ++t) {
+                    const hvec4 am = {a[u][t][4 * m], a[u][t][4 * m + 1], a[u][t][4 * m + 2],
+                                      a[u][t][4 * m + 3]};
+                    ch[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(am, bhm, ch[t], 0, 0, 0);
+                    cl[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(am, blm, cl[t], 0, 0, 0);
+                }
+            }
     }
     if (vok) {
         double *pp = part + ((size_t)blockIdx.y * k + v) * n_rows;
+        const double isx = 1.0 / (double)xs[v];
 #pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
@@ -218,7 +304,8 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_h16(
                 const int s2 = base + 16 * t + 4 * q + r;
                 if (s2 < n_rows) {
                     const int rr = rows ? rows[s2] : s2;
-                    pp[s2] = (cd[t][r] + (double)c[t][r]) * (double)rs[rr];
+                    const double tot = cd[t][r] + (double)ch[t][r] + (double)cl[t][r] * (double)ILO;
+                    pp[s2] = tot * (double)rs[rr] * isx;
                 }
             }
     }
@@ -349,9 +436,51 @@ __global__ void k_multi_reduce(const double *__restrict__ part, int n_chunks, in
 
 }  // namespace
 
-// column chunks of the batched product: about 8k wavefronts in flight whatever n is
-int multi_chunks(int n_rows, int n_groups)
+// Column chunks of the batched product.  The matrix-core kernels of modes 0-2 take about
+// 8k wavefronts whatever n is.  The FP16 kernel (mode 3) streams the operator at close to
+// the memory rate, so its grid is sized in whole rounds of resident blocks: the chunk count
+// (1..16) whose block count comes closest to a multiple of what the device holds at once,
+// the smaller count on ties.
+int multi_chunks(int mode, int n_rows, int n_groups)
 {
+    int cap = 0;
+    if (mode == 3) {
+        static std::atomic<int> resident[64];
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64) {
+            // queried once per device; a failed query is kept as -1 and selects the rule of
+            // the other modes below (a grid shape only, the sums are the same), so the
+            // workspace sizing and the launch always agree
+            cap = resident[dev].load();
+            if (cap == 0) {
+                int per_cu = 0, cus = 0;
+                const bool ok =
+                    hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &per_cu, reinterpret_cast<const void *>(&k_gemm_h16), 64 * MW, 0) ==
+                        hipSuccess &&
+                    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
+                        hipSuccess &&
+                    per_cu > 0 && cus > 0;
+                cap = ok ? per_cu * cus : -1;
+                resident[dev].store(cap);
+            }
+        }
+    }
+    if (mode == 3 && cap > 0) {
+        const long long blocks = (n_rows + RPB - 1) / RPB;
+        const int steps = (4 * n_groups + 32 * WB_MULTI16_SUB - 1) / (32 * WB_MULTI16_SUB);
+        int best = 1;
+        double best_fill = 0.0;
+        for (int c = 1; c <= 16 && c <= std::max(1, steps); ++c) {
+            const long long b = blocks * c, rounds = (b + cap - 1) / cap;
+            const double fill = (double)b / (double)(rounds * cap);
+            if (fill > best_fill + 1e-3) {
+                best_fill = fill;
+                best = c;
+            }
+        }
+        return best;
+    }
     const int waves = (n_rows + 16 * RT - 1) / (16 * RT);
     int c = (8192 + waves - 1) / waves;
     c = c < 1 ? 1 : (c > 16 ? 16 : c);
@@ -366,15 +495,24 @@ void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *G
                            int n_groups, hipStream_t s)
 {
     if (n_rows <= 0 || k <= 0) return;
-    const int chunks = multi_chunks(n_rows, n_groups);
+    const int chunks = multi_chunks(mode, n_rows, n_groups);
     const int gpc = (n_groups + chunks - 1) / chunks;
     dim3 grid((n_rows + RPB - 1) / RPB, chunks), block(64 * MW);
     if (mode == 3) {
-        // G holds A16 and H the row scales; chunks of whole 32-column steps
+        // G holds A16 and H the row scales; Glo is the workspace: k vector scales, then the
+        // split vectors (2k x ld halfs from byte 256); chunks of whole
+        // 32 * WB_MULTI16_SUB-column steps
         const int n = 4 * n_groups;
-        const int cpc = ((n + chunks - 1) / chunks + 31) / 32 * 32;
+        constexpr int step = 32 * WB_MULTI16_SUB;
+        const int cpc = ((n + chunks - 1) / chunks + step - 1) / step * step;
+        float *xs = (float *)const_cast<void *>(Glo);
+        _Float16 *XS = (_Float16 *)((char *)xs + 256);
+        hipLaunchKernelGGL(k_vec_scale, dim3(k), dim3(1024), 0, s, (const float *)X, ldx, n, xs);
+        hipLaunchKernelGGL(k_vec_split, dim3((unsigned)((ld / 8 + 255) / 256), k), dim3(256), 0,
+                           s, (const float *)X, ldx, n, k, (const float *)xs, XS, ld);
         hipLaunchKernelGGL(k_gemm_h16, grid, block, 0, s, (const _Float16 *)G, (const float *)H,
-                           ld, n, k, (const float *)X, ldx, rows, n_rows, cpc, part);
+                           ld, n, k, (const _Float16 *)XS, (const float *)xs, rows, n_rows, cpc,
+                           part);
     } else if (mode == 1)
         hipLaunchKernelGGL(k_gemm_sel_f32, grid, block, 0, s, (const float *)G, (const float *)H,
                            ld, (const int4 *)segs, n_segs, gmask, flip, (float)s_sel,
