@@ -12,6 +12,9 @@
 // twice) and reduces with wavefront shuffles.  Constrained rows
 // (ConstrainedOperator, include/constrained_matrix.h:73-86) are skipped through the
 // caller's row list and rewritten by a fix-up kernel.
+#include <mutex>
+#include <vector>
+
 #include "wb_internal.h"
 
 namespace wb {
@@ -329,6 +332,74 @@ __global__ __launch_bounds__(B16_THREADS) void k_build_h16(
     if (threadIdx.x == 0) rs[row] = sc;
 }
 
+// The same build with the row held in registers: one 1024-thread block per row, thread t
+// owns the 4-column groups t + 1024 k (k < K), read once.  The selection comes from the
+// group's mask byte (bit e: column e from H), which covers pure and mixed groups alike.
+// Same scale and same FP16 values as k_build_h16, at one read of the FP32 entries instead
+// of two (the second pass of k_build_h16 misses L2 at these row lengths).
+constexpr int BR_THREADS = 1024;
+template <int K>
+__global__ __launch_bounds__(BR_THREADS) void k_build_h16_reg(
+    const float *__restrict__ G, const float *__restrict__ H, size_t ld, int n,
+    const uint8_t *__restrict__ gmask, _Float16 *__restrict__ A16, float *__restrict__ rs)
+{
+    __shared__ float red[BR_THREADS / 64];
+    const int row = blockIdx.x;
+    const size_t off = (size_t)row * ld;
+    const int ng = (n + 3) / 4;
+    fvec4 v[K];
+    float mx = 0.0f;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int g = (int)threadIdx.x + BR_THREADS * k;
+        v[k] = fvec4{0.f, 0.f, 0.f, 0.f};
+        if (g < ng) {
+            const unsigned m = gmask[g];
+            if (m == 0u) {
+                v[k] = -__builtin_nontemporal_load((const fvec4 *)(G + off) + g);
+            } else if (m == 15u) {
+                v[k] = __builtin_nontemporal_load((const fvec4 *)(H + off) + g);
+            } else {
+                const fvec4 a = *((const fvec4 *)(G + off) + g), b = *((const fvec4 *)(H + off) + g);
+                v[k].x = (m & 1u) ? b.x : -a.x;
+                v[k].y = (m & 2u) ? b.y : -a.y;
+                v[k].z = (m & 4u) ? b.z : -a.z;
+                v[k].w = (m & 8u) ? b.w : -a.w;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[k].x), fabsf(v[k].y)),
+                             fmaxf(fabsf(v[k].z), fabsf(v[k].w))));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = red[0];
+#pragma unroll
+    for (int w = 1; w < BR_THREADS / 64; ++w) mx = fmaxf(mx, red[w]);
+    int e = 0;
+    if (mx > 0.0f) (void)frexpf(mx, &e);  // mx = f 2^e, f in [0.5, 1): 2^e >= mx
+    const float sc = mx > 0.0f ? ldexpf(1.0f, e) : 1.0f, inv = 1.0f / sc;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int g = (int)threadIdx.x + BR_THREADS * k;
+        if (g < ng) {
+            hvec4 h = __builtin_convertvector(v[k] * inv, hvec4);
+            if (4 * g + 3 >= n) {
+                if (4 * g + 1 >= n) h.y = (_Float16)0.0f;
+                if (4 * g + 2 >= n) h.z = (_Float16)0.0f;
+                if (4 * g + 3 >= n) h.w = (_Float16)0.0f;
+            }
+            *((hvec4 *)(A16 + off) + g) = h;
+        }
+    }
+    for (int g = ng + (int)threadIdx.x; g < (int)(ld / 4); g += BR_THREADS)
+        *((hvec4 *)(A16 + off) + g) = hvec4{0, 0, 0, 0};
+    if (threadIdx.x == 0) rs[row] = sc;
+}
+
 // y_i = s_i sum_j A16_ij x_j + diag_i xd_i over the listed rows (FP32 vectors, FP32 sums
 // per lane, FP64 wavefront reduction); 8 columns per lane per step.
 constexpr int H16_RPW = WB_GEMV16_RPW;
@@ -383,11 +454,50 @@ __global__ __launch_bounds__(64 * H16_WAVES) void k_gemv_h16(
 
 }  // namespace
 
+int resident_blocks(const void *kernel, int threads)
+{
+    struct Entry {
+        const void *k;
+        int threads, dev, blocks;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const Entry &e : cache)
+        if (e.k == kernel && e.threads == threads && e.dev == dev) return e.blocks;
+    int per_cu = 0, cus = 0;
+    const bool ok =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess;
+    const int blocks = ok && per_cu > 0 && cus > 0 ? per_cu * cus : 0;
+    cache.push_back({kernel, threads, dev, blocks});
+    return blocks;
+}
+
 void launch_build_h16(const float *G, const float *H, size_t ld, int n, const int *segs,
                       int n_segs, const uint8_t *gmask, void *A16, float *rs, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_build_h16, dim3(n), dim3(B16_THREADS), 0, s, G, H, ld, n,
-                       (const int4 *)segs, n_segs, gmask, (_Float16 *)A16, rs);
+    // rows of up to 64k columns are held in registers (read once); longer rows take the
+    // two-pass kernel
+    const int ng = (n + 3) / 4;
+    _Float16 *A = (_Float16 *)A16;
+    if (ng <= 4 * BR_THREADS)
+        hipLaunchKernelGGL(k_build_h16_reg<4>, dim3(n), dim3(BR_THREADS), 0, s, G, H, ld, n,
+                           gmask, A, rs);
+    else if (ng <= 8 * BR_THREADS)
+        hipLaunchKernelGGL(k_build_h16_reg<8>, dim3(n), dim3(BR_THREADS), 0, s, G, H, ld, n,
+                           gmask, A, rs);
+    else if (ng <= 12 * BR_THREADS)
+        hipLaunchKernelGGL(k_build_h16_reg<12>, dim3(n), dim3(BR_THREADS), 0, s, G, H, ld, n,
+                           gmask, A, rs);
+    else if (ng <= 16 * BR_THREADS)
+        hipLaunchKernelGGL(k_build_h16_reg<16>, dim3(n), dim3(BR_THREADS), 0, s, G, H, ld, n,
+                           gmask, A, rs);
+    else
+        hipLaunchKernelGGL(k_build_h16, dim3(n), dim3(B16_THREADS), 0, s, G, H, ld, n,
+                           (const int4 *)segs, n_segs, gmask, A, rs);
 }
 
 void launch_gemv_h16(const void *A16, const float *rs, size_t ld, int n, const float *x,

@@ -124,6 +124,10 @@ void launch_build_h16(const float *G, const float *H, size_t ld, int n, const in
 void launch_gemv_h16(const void *A16, const float *rs, size_t ld, int n, const float *x,
                      const double *diag, const float *xd, const int *rows, int n_rows, float *y,
                      hipStream_t s);
+// Blocks of `threads` threads of `kernel` the current device holds at once (occupancy
+// query times the CU count), cached per kernel, block size and device; 0 when the runtime
+// cannot say (callers then keep a fixed grid shape).
+int resident_blocks(const void *kernel, int threads);
 void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *Glo,
                            const void *Hlo, size_t ld, const int *segs, int n_segs,
                            const uint8_t *gmask, int flip, double s_sel, double s_unsel, int k,

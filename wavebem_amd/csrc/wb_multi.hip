@@ -16,7 +16,6 @@
 // 4-column groups, so a row segment of 64 (FP32) or 128 (FP64) bytes comes from one load
 // instruction.
 #include <algorithm>
-#include <atomic>
 
 #include "wb_internal.h"
 
@@ -443,29 +442,12 @@ __global__ void k_multi_reduce(const double *__restrict__ part, int n_chunks, in
 // the smaller count on ties.
 int multi_chunks(int mode, int n_rows, int n_groups)
 {
-    int cap = 0;
-    if (mode == 3) {
-        static std::atomic<int> resident[64];
-        int dev = 0;
-        if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < 64) {
-            // queried once per device; a failed query is kept as -1 and selects the rule of
-            // the other modes below (a grid shape only, the sums are the same), so the
-            // workspace sizing and the launch always agree
-            cap = resident[dev].load();
-            if (cap == 0) {
-                int per_cu = 0, cus = 0;
-                const bool ok =
-                    hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &per_cu, reinterpret_cast<const void *>(&k_gemm_h16), 64 * MW, 0) ==
-                        hipSuccess &&
-                    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) ==
-                        hipSuccess &&
-                    per_cu > 0 && cus > 0;
-                cap = ok ? per_cu * cus : -1;
-                resident[dev].store(cap);
-            }
-        }
-    }
+    // the resident-block count is cached (a failed query stays 0 and selects the rule of the
+    // other modes below: a grid shape only, the sums are the same), so the workspace sizing
+    // and the launch always agree
+    const int cap = mode == 3 ? resident_blocks(reinterpret_cast<const void *>(&k_gemm_h16),
+                                                64 * MW)
+                              : 0;
     if (mode == 3 && cap > 0) {
         const long long blocks = (n_rows + RPB - 1) / RPB;
         const int steps = (4 * n_groups + 32 * WB_MULTI16_SUB - 1) / (32 * WB_MULTI16_SUB);
