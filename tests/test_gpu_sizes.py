@@ -1,4 +1,5 @@
-"""GPU parity at the full workload sizes (Wigley half-tank, 41k and 61k dofs).
+"""GPU parity at the full workload sizes (Wigley half-tank, 41k and 61k dofs) and past 64k
+columns (70,669 dofs: n x ld above 2^32 entries, two-pass FP16 operator build).
 
 The dense oracle cannot build these matrices, so parity is shown through
   * whole rows recomputed by the oracle (row-subset assembly, same arithmetic as the full
@@ -196,3 +197,42 @@ def test_projections_40k(gpu_available):
     assert worst < 1e-12
     assert g_err < 1e-12
     assert cos_min > 0.98
+
+
+def test_past_64k_columns_70k(gpu_available):
+    """70,669 dofs: rows of more than 65,536 columns take the two-pass FP16 operator build,
+    and n x ld (5.0e9 entries) is past 2^32, so every matrix offset must be 64-bit.  The
+    last rows of the matrix are recomputed by the oracle, and the FP32-IR solve (FP16
+    inner operator) is checked against the FP64 solve of the same system."""
+    m = M.wigley_preset("70k")
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    n = m.n_dofs
+    assert n > 65536 and n * ((n + 63) // 64 * 64) > 2**32
+    a, b = n - 6, n
+    rows = np.arange(a, b, dtype=np.int32)
+    Ho, Go = O.assemble_rows(m.support_points, m.cell_dofs, m.cell_orient, ptr, idx, a, b)
+    bc, _ = M.wigley_bc(m, seed=3)
+    out = {}
+    for prec in ("fp64", "fp32_ir"):
+        phi = bc * m.surface_nodes
+        dphi = bc * m.other_nodes
+        with BEMProblem(m, Config(precision=prec, solver_tolerance=1e-10),
+                        double_nodes=(ptr, idx)) as p:
+            p.compute_constraints(bc)
+            p.assemble_system()
+            Gd, Hd = p.matrix_rows(rows)
+            st = p.solve_system(phi, dphi, bc)
+        worst = max(max(np.abs(Gd[k] - Go[k]).max() / np.abs(Go[k]).max(),
+                        np.abs(Hd[k] - Ho[k]).max() / np.abs(Ho[k]).max())
+                    for k in range(b - a))
+        print(f"  [{prec}] n={n}: last rows vs oracle {worst:.2e}; {st['iterations']} it, "
+              f"sweeps {st['ir_sweeps']}, true rel res {st['true_rel_residual']:.2e}")
+        assert worst < (1e-10 if prec == "fp64" else 1e-9)
+        out[prec] = (phi, dphi)
+    N = m.other_nodes > 0
+    S = ~N
+    (p64, d64), (p32, d32) = out["fp64"], out["fp32_ir"]
+    e_phi = np.abs(p32[N] - p64[N]).max() / np.abs(p64[N]).max()
+    e_dn = np.abs(d32[S] - d64[S]).max() / np.abs(d64[S]).max()
+    print(f"  FP32-IR vs FP64: phi {e_phi:.2e}, dphi/dn {e_dn:.2e}")
+    assert e_phi < 1e-8 and e_dn < 1e-8

@@ -77,8 +77,9 @@ def test_wigley_boundary_data():
     assert np.array_equal(bc, M.wigley_bc(m, froude=0.3, seed=0)[0])
 
 
-@pytest.mark.parametrize("name", ["2.5k", "10k", "20k", "40k", "60k"])
+@pytest.mark.parametrize("name", ["2.5k", "10k", "20k", "40k", "60k", "70k"])
 def test_presets_sizes(name):
-    target = {"2.5k": 2500, "10k": 10000, "20k": 20000, "40k": 40000, "60k": 60000}[name]
+    target = {"2.5k": 2500, "10k": 10000, "20k": 20000, "40k": 40000, "60k": 60000,
+              "70k": 70000}[name]
     m = M.wigley_preset(name)
     assert 0.9 * target < m.n_dofs < 1.3 * target

@@ -272,6 +272,7 @@ WIGLEY_PRESETS = {
     "20k": dict(nh=70, nt=16, nu=28, nd=40, ny=42, nzd=8),
     "40k": dict(nh=100, nt=22, nu=40, nd=56, ny=60, nzd=10),
     "60k": dict(nh=120, nt=28, nu=50, nd=70, ny=72, nzd=12),
+    "70k": dict(nh=130, nt=30, nu=54, nd=76, ny=78, nzd=12),
 }
 
 
