@@ -473,6 +473,19 @@ def main():
                                             / np.abs(b["phi"][N]).max())
         line["dphidn_rel_err_vs_fp64"] = float(np.abs(a["dphi"][S] - b["dphi"][S]).max()
                                                / np.abs(b["dphi"][S]).max())
+        # hull pressure drag of both solutions (linearised Bernoulli, steady: phi_t = 0),
+        # evaluated after the timed steps
+        import torch
+
+        from wavebem_amd.adjoint import hull_drag
+        hull = mesh.cell_material == M.HULL
+        Xh = torch.from_numpy(mesh.support_points)
+        z = torch.zeros(n, dtype=torch.float64)
+        dr = {k: float(hull_drag(Xh, mesh.cell_dofs[hull], mesh.cell_orient[hull],
+                                 torch.from_numpy(v["phi"]), torch.from_numpy(v["dphi"]), z, U))
+              for k, v in (("fp32_ir", a), ("fp64", b))}
+        line["drag_fp64"] = dr["fp64"]
+        line["drag_rel_err_vs_fp64"] = abs(dr["fp32_ir"] - dr["fp64"]) / abs(dr["fp64"])
     line["phase_ms"] = {k: {"assembly": round(v["assembly_ms"], 3),
                             "solve": round(v["ms_per_step"] - v["assembly_ms"], 3)}
                         for k, v in res.items()}

@@ -127,6 +127,14 @@ wb_status wb_get_masks(wb_ctx *ctx, double *surface_nodes, double *other_nodes);
  * H (neumann_matrix) on the device, plus the solid-angle row sums. */
 wb_status wb_assemble(wb_ctx *ctx);
 
+/* No counterpart in the reference (BEMProblem keeps its FullMatrix members for its
+ * lifetime).  Frees the device copies of G and H (and their FP32_IR lo halves and FP16
+ * operator) so that a caller holding the context between a forward solve and its adjoint
+ * can give the memory back; everything that reads the matrices returns WB_E_STATE until
+ * the next wb_assemble, which allocates them again.  Mesh, masks, constraints and alpha
+ * are kept. */
+wb_status wb_release_matrices(wb_ctx *ctx);
+
 /* Copies G and H back (row-major, n x n, caller dof order), for parity checks; either
  * pointer may be NULL. */
 wb_status wb_get_matrices(wb_ctx *ctx, double *G, double *H);

@@ -214,3 +214,36 @@ def test_solution_jvp_matches_central_differences(gpu_available):
     err = np.abs(jv - fd).max() / np.abs(fd).max()
     print(f"  tangent vs FD: max |fd| {np.abs(fd).max():.3e}, rel err {err:.2e}")
     assert err < 1e-6
+
+
+def test_release_matrices_between_forward_and_backward(gpu_available):
+    """bem_solve(release=True) frees G and H after the forward solve (the device memory
+    drops by the matrix bytes, and matrix reads are refused until the next assembly); the
+    backward assembles them again at the forward's geometry, so the gradient is the same,
+    bit for bit, as with the matrices kept."""
+    m = M.wigley_preset("2.5k")
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    bc, U = M.wigley_bc(m, seed=0)
+    hull = m.cell_material == M.HULL
+    cells, orient = m.cell_dofs[hull], m.cell_orient[hull]
+    phi_t = torch.zeros(m.n_dofs, dtype=torch.float64)
+    n, ld = m.n_dofs, (m.n_dofs + 63) // 64 * 64
+    mat_bytes = 2 * n * ld * 8                  # FP64 G and H (the adjoint is FP64)
+    grads = {}
+    with BEMProblem(m, Config(solver_tolerance=1e-12), double_nodes=(ptr, idx)) as p:
+        p.compute_constraints(bc)
+        for release in (False, True):
+            X = torch.tensor(m.support_points, dtype=torch.float64, requires_grad=True)
+            before = p.device_bytes()
+            phi, dphi = bem_solve(X, p, bc, release=release)
+            if release:
+                # the first pass allocated every workspace; only the matrices go
+                assert before - p.device_bytes() == mat_bytes, (before, p.device_bytes())
+                with pytest.raises(WaveBemError):
+                    p.compute_alpha()
+            J = hull_drag(X, cells, orient, phi, dphi, phi_t, U)
+            J.backward()
+            grads[release] = X.grad.numpy().copy()
+        p.compute_alpha()                       # the backward assembled them again
+    assert np.abs(grads[False]).max() > 0
+    assert np.array_equal(grads[False], grads[True])

@@ -96,6 +96,7 @@ _SIGS = {
     "wb_set_masks": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_set_support_points": (ctypes.c_int, [_vp, _dp]),
     "wb_assemble": (ctypes.c_int, [_vp]),
+    "wb_release_matrices": (ctypes.c_int, [_vp]),
     "wb_get_matrices": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_get_matrix_rows": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _dp, _dp]),
     "wb_solve_adjoint": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_void_p]),

@@ -68,13 +68,16 @@ def hull_drag(support, cells, orient, phi, dphi_dn, phi_t, U, rho=RHO, q_order=4
 
 class _BEMSolve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, support, problem, tmp_rhs):
+    def forward(ctx, support, problem, tmp_rhs, release):
         x = np.ascontiguousarray(support.detach().cpu().numpy(), dtype=np.float64)
         problem.set_support_points(x)
         problem.assemble_system()
         phi = tmp_rhs * problem.surface_nodes
         dphi = tmp_rhs * problem.other_nodes
         problem.solve_system(phi, dphi, tmp_rhs)
+        if release:
+            # the backward assembles G and H again at this geometry
+            problem.release_matrices()
         ctx.problem = problem
         ctx.x = x
         ctx.phi, ctx.dphi = phi, dphi
@@ -90,14 +93,17 @@ class _BEMSolve(torch.autograd.Function):
                + g_dphi.detach().numpy() * p.surface_nodes)
         lam = p.solve_adjoint(g_u)
         grad = -p.residual_vjp(lam, ctx.phi, ctx.dphi, free_rows_only=True)
-        return torch.from_numpy(grad), None, None
+        return torch.from_numpy(grad), None, None, None
 
 
-def bem_solve(support, problem, tmp_rhs):
+def bem_solve(support, problem, tmp_rhs, release=False):
     """(phi, dphi_dn) of solve_system at support points ``support`` ((n, 3) float64 tensor),
     differentiable with respect to ``support`` (the boundary data ``tmp_rhs`` and the
-    installed masks and constraints are held fixed)."""
-    return _BEMSolve.apply(support, problem, np.ascontiguousarray(tmp_rhs, dtype=np.float64))
+    installed masks and constraints are held fixed).  The backward always assembles G and H
+    again at the forward's geometry; with ``release`` the forward frees them once it has
+    solved, so the device memory is free between the forward and the backward."""
+    return _BEMSolve.apply(support, problem, np.ascontiguousarray(tmp_rhs, dtype=np.float64),
+                           bool(release))
 
 
 def bem_solve_jvp(problem, support, tmp_rhs, dx):

@@ -163,6 +163,10 @@ class BEMProblem:
     def assemble_system(self):
         self._check(L.lib().wb_assemble(self._ctx), "wb_assemble")
 
+    def release_matrices(self):
+        """Frees the device G and H (and derived copies) until the next assemble_system."""
+        self._check(L.lib().wb_release_matrices(self._ctx), "wb_release_matrices")
+
     def matrices(self):
         """(G, H) = (dirichlet_matrix, neumann_matrix), caller numbering, float64."""
         G = np.empty((self.n, self.n))

@@ -223,6 +223,7 @@ static void upload(DevBuf &b, const T *src, size_t count, hipStream_t s)
 }
 
 static size_t elem_bytes(const wb_ctx *c) { return c->cfg.precision == WB_PREC_FP64 ? 8 : 4; }
+static void alloc_matrices(wb_ctx *c);
 
 // ------------------------------------------------------------------ vector plumbing
 static void to_internal(const wb_ctx *c, const double *src, std::vector<double> &dst)
@@ -1414,21 +1415,7 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
                              nq, c->qpts.as<double>(), c->qw.as<double>(), c->cellq.as<double>(),
                              s);
         HIPCHK(hipGetLastError());
-        // matrices, zero-filled so the padding columns read as 0
-        const size_t entries = (size_t)n * c->ld;
-        c->G.ensure(entries * elem_bytes(c));
-        c->H.ensure(entries * elem_bytes(c));
-        HIPCHK(hipMemsetAsync(c->G.p, 0, c->G.bytes, s));
-        HIPCHK(hipMemsetAsync(c->H.p, 0, c->H.bytes, s));
-        if (c->cfg.precision == WB_PREC_FP32_IR) {
-            c->Glo.ensure(entries * 2);
-            c->Hlo.ensure(entries * 2);
-            HIPCHK(hipMemsetAsync(c->Glo.p, 0, c->Glo.bytes, s));
-            HIPCHK(hipMemsetAsync(c->Hlo.p, 0, c->Hlo.bytes, s));
-        } else {
-            c->Glo.release();
-            c->Hlo.release();
-        }
+        alloc_matrices(c);
         const int ntg = (p.n_tiles + TILES_PER_WG - 1) / TILES_PER_WG;
         c->hsum_part.ensure(sizeof(double) * (size_t)ntg * n);
         c->hsum_sing.ensure(sizeof(double) * n);
@@ -1500,6 +1487,25 @@ wb_status wb_compute_surface_gradients(wb_ctx *c, const double *tmp_rhs, double 
         project_fields(c, phi.data(), 0, nullptr, grads);
         if (cg_iterations) *cg_iterations = c->proj_iterations;
     });
+}
+
+// G and H (and the lo halves in FP32_IR), zero-filled so the padding columns read as 0
+static void alloc_matrices(wb_ctx *c)
+{
+    const size_t entries = (size_t)c->n * c->ld;
+    c->G.ensure(entries * elem_bytes(c));
+    c->H.ensure(entries * elem_bytes(c));
+    HIPCHK(hipMemsetAsync(c->G.p, 0, c->G.bytes, c->stream));
+    HIPCHK(hipMemsetAsync(c->H.p, 0, c->H.bytes, c->stream));
+    if (c->cfg.precision == WB_PREC_FP32_IR) {
+        c->Glo.ensure(entries * 2);
+        c->Hlo.ensure(entries * 2);
+        HIPCHK(hipMemsetAsync(c->Glo.p, 0, c->Glo.bytes, c->stream));
+        HIPCHK(hipMemsetAsync(c->Hlo.p, 0, c->Hlo.bytes, c->stream));
+    } else {
+        c->Glo.release();
+        c->Hlo.release();
+    }
 }
 
 // comp_dom.surface_nodes / other_nodes: host copies, device mask, column groups
@@ -1646,6 +1652,7 @@ wb_status wb_assemble(wb_ctx *c)
     return guarded(c, [&] {
         require(c->have_mesh, WB_E_STATE, "wb_assemble before wb_set_mesh");
         HIPCHK(hipSetDevice(c->cfg.device));
+        if (!c->G.p) alloc_matrices(c);  // after wb_release_matrices
         const MeshPlan &p = c->plan;
         PhiTable nodes{};
         for (int i = 0; i < c->quad.q_order; ++i) nodes.phi[i] = c->quad.reg_pts[2 * i];
@@ -1678,6 +1685,17 @@ wb_status wb_assemble(wb_ctx *c)
         c->assembled = true;
         c->assembly_ver++;
         if (c->have_masks) refresh_diag(c);
+    });
+}
+
+wb_status wb_release_matrices(wb_ctx *c)
+{
+    return guarded(c, [&] {
+        require(c->have_mesh, WB_E_STATE, "wb_release_matrices before wb_set_mesh");
+        HIPCHK(hipSetDevice(c->cfg.device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (DevBuf *b : {&c->G, &c->H, &c->Glo, &c->Hlo, &c->A16}) b->release();
+        c->assembled = false;
     });
 }
 
