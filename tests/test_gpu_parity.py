@@ -103,6 +103,33 @@ def test_solve_system_matches_dense_oracle(gpu_available, precision):
     assert np.array_equal(dphi[N], bc[N])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_coarse_space_keeps_the_solution(gpu_available, precision):
+    """The coarse-space correction of the block-Jacobi preconditioner (on by default) and
+    plain block Jacobi (precond_coarse = 0) reach the dense solution on the 2.5k tank, the
+    former in fewer GMRES iterations."""
+    m = M.wigley_preset("2.5k")
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    H, G = O.assemble(m.support_points, m.cell_dofs, m.cell_orient, ptr, idx, n_threads=8)
+    bc, _ = M.wigley_bc(m, seed=4)
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, bc)
+    x = T.dense_solution(H, G, m.surface_nodes, m.other_nodes, cons, bc)
+    its = {}
+    for coarse in (0, 128):
+        phi = bc * m.surface_nodes
+        dphi = bc * m.other_nodes
+        with _problem(m, ptr, idx, precision=precision, solver_tolerance=1e-12,
+                      precond_coarse=coarse) as p:
+            p.compute_constraints(bc)
+            st = p.solve_system(phi, dphi, bc)
+        sol = np.where(m.surface_nodes == 1, dphi, phi)
+        err = np.abs(sol - x).max() / np.abs(x).max()
+        print(f"  [{precision} coarse {coarse}] {st['iterations']} it, err {err:.2e}")
+        assert err < 1e-8
+        its[coarse] = st["iterations"]
+    assert its[128] < its[0]
+
+
 def test_fp16_inner_hands_over_to_fp32(gpu_available):
     """An FP16 sweep that does not halve the outer residual (forced here by an inner
     reduction of 0.9 per sweep) switches the remaining sweeps to the FP32 operator; the

@@ -36,6 +36,7 @@ class WbConfig(ctypes.Structure):
         ("assembly_tile", ctypes.c_int),
         ("ir_inner_format", ctypes.c_int),
         ("ir_inner16_reduction", ctypes.c_double),
+        ("precond_coarse", ctypes.c_int),
     ]
 
 

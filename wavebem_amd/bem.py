@@ -62,6 +62,7 @@ class Config:
     assembly_tile: int = 16
     ir_inner_format: int = 1
     ir_inner16_reduction: float = 3e-4
+    precond_coarse: int = 128  # coarse-space unknowns of the block-Jacobi preconditioner
 
     def to_c(self) -> L.WbConfig:
         c = L.WbConfig()
@@ -81,6 +82,7 @@ class Config:
         c.assembly_tile = self.assembly_tile
         c.ir_inner_format = self.ir_inner_format
         c.ir_inner16_reduction = self.ir_inner16_reduction
+        c.precond_coarse = self.precond_coarse
         return c
 
 

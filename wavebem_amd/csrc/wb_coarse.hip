@@ -1,0 +1,255 @@
+// wb_coarse.hip -- coarse-space correction of the block-Jacobi preconditioner on gfx950.
+//
+// Two-level (multiplicative) preconditioner for the GMRES of BEMProblem::solve_system
+// (source/bem_problem.cc:821-895; the reference preconditions with a banded LU,
+// :1107-1149, and any preconditioner leaves the converged solution unchanged):
+//   M^-1 v = P z + B^-1 (v - A P z),   z = Ac^-1 P^T v,   Ac = P^T A P,
+// with B the block-Jacobi blocks and P the indicator of aggregates: runs of `agg`
+// consecutive dofs of the device ordering (compact patches, since the ordering is built
+// from compact column tiles), agg a multiple of 64, at most CS_MAX aggregates so that Ac
+// is inverted in the LDS of one workgroup.  A is the matrix the block-Jacobi blocks are
+// taken from: H(j,i) (+ alpha_i at i == j) in a Neumann column, -G(j,i) in a Dirichlet
+// column, identity rows for constrained dofs.  Block Jacobi handles the near field; the
+// coarse space takes the smooth coupling between patches, which it misses.
+//
+// Per build: AP = A P (one pass over the selected operator), Ac = P^T AP, its inverse,
+// and W = AP Ac^-1 (n x nc).  Per application: C = P^T v (one block per aggregate),
+// t = v - W C, then the block-Jacobi kernel applies B^-1 t and adds z = Ac^-1 C of the
+// row's aggregate.  Every sum runs in a fixed order, so results do not depend on timing.
+#include "wb_device.h"
+#include "wb_internal.h"
+
+namespace wb {
+namespace {
+
+__device__ __forceinline__ double cs_wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+constexpr int CR_THREADS = 256;   // k_coarse_rows: 4 wavefronts per row
+constexpr int CI_THREADS = 1024;  // k_coarse_invert
+
+typedef float cs_fvec4 __attribute__((ext_vector_type(4)));
+typedef double cs_dvec2 __attribute__((ext_vector_type(2)));
+
+// AP[a][j] = sum over the columns i of aggregate a of A(j, i): one row j per block;
+// wavefront w takes the aggregates a = w, w + 4, ..., 64 four-column groups per step
+// (gmask bit e of group g: column 4g + e comes from H).  Padding columns past n are zero
+// in G and H.
+__global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
+    int n, int agg, int nc, const void *__restrict__ Gv, const void *__restrict__ Hv,
+    int precision, size_t ld, const double *__restrict__ alpha,
+    const uint8_t *__restrict__ gmask, const uint8_t *__restrict__ constrained,
+    double *__restrict__ AP, float *__restrict__ AP32, size_t ldap)
+{
+    const int j = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool con = constrained[j] != 0;  // identity row
+    const size_t roff = (size_t)j * ld;
+    for (int a = wave; a < nc; a += CR_THREADS / 64) {
+        const int c0 = a * agg, c1 = min(n, c0 + agg);
+        double s = 0.0;
+        if (con) {
+            s = (j >= c0 && j < c1) ? 1.0 : 0.0;
+        } else {
+            const int g0 = c0 / 4, g1 = (c1 + 3) / 4;
+            for (int gb = g0; gb < g1; gb += 64) {
+                const int g = gb + lane;
+                double v = 0.0;
+                if (g < g1) {
+                    const unsigned m = gmask[g];
+                    double e[4];
+                    if (precision == WB_PREC_FP64) {
+                        const double *Gr = (const double *)Gv + roff + 4 * (size_t)g;
+                        const double *Hr = (const double *)Hv + roff + 4 * (size_t)g;
+                        cs_dvec2 g01 = {0, 0}, g23 = {0, 0}, h01 = {0, 0}, h23 = {0, 0};
+                        if (m != 15u) {
+                            g01 = *(const cs_dvec2 *)Gr;
+                            g23 = *((const cs_dvec2 *)Gr + 1);
+                        }
+                        if (m != 0u) {
+                            h01 = *(const cs_dvec2 *)Hr;
+                            h23 = *((const cs_dvec2 *)Hr + 1);
+                        }
+                        e[0] = (m & 1u) ? h01.x : -g01.x;
+                        e[1] = (m & 2u) ? h01.y : -g01.y;
+                        e[2] = (m & 4u) ? h23.x : -g23.x;
+                        e[3] = (m & 8u) ? h23.y : -g23.y;
+                    } else {
+                        // the FP32 hi part is accurate enough for a preconditioner
+                        cs_fvec4 gv = {0, 0, 0, 0}, hv = {0, 0, 0, 0};
+                        if (m != 15u) gv = *((const cs_fvec4 *)((const float *)Gv + roff) + g);
+                        if (m != 0u) hv = *((const cs_fvec4 *)((const float *)Hv + roff) + g);
+                        e[0] = (m & 1u) ? hv.x : -gv.x;
+                        e[1] = (m & 2u) ? hv.y : -gv.y;
+                        e[2] = (m & 4u) ? hv.z : -gv.z;
+                        e[3] = (m & 8u) ? hv.w : -gv.w;
+                    }
+                    v = (e[0] + e[1]) + (e[2] + e[3]);
+                }
+                s += cs_wave_sum(v);
+            }
+            // the alpha diagonal of a Neumann column
+            if (j >= c0 && j < c1 && ((gmask[j >> 2] >> (j & 3)) & 1u)) s += alpha[j];
+        }
+        if (lane == 0) {
+            AP[(size_t)a * ldap + j] = s;
+            AP32[(size_t)a * ldap + j] = (float)s;
+        }
+    }
+}
+
+// Ac[a][b] = sum over rows j of aggregate a of AP[b][j]: one wavefront per (a, b).
+__global__ __launch_bounds__(256) void k_coarse_matrix(int n, int agg, int nc,
+                                                       const double *__restrict__ AP, size_t ldap,
+                                                       double *__restrict__ Ac)
+{
+    const int lane = threadIdx.x & 63;
+    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pair >= nc * nc) return;  // wave-uniform
+    const int a = pair / nc, b = pair % nc;
+    const int r0 = a * agg, r1 = min(n, r0 + agg);
+    double s = 0.0;
+    for (int rb = r0; rb < r1; rb += 64) {
+        const int j = rb + lane;
+        s += cs_wave_sum(j < r1 ? AP[(size_t)b * ldap + j] : 0.0);
+    }
+    if (lane == 0) Ac[a * nc + b] = s;
+}
+
+// Aci = Ac^-1 (row-major, FP64 and FP32); status[0] = zero pivots + 1 if any entry of the
+// inverse is not finite (0: usable).
+__global__ __launch_bounds__(CI_THREADS) void k_coarse_invert(int nc, const double *__restrict__ Ac,
+                                                              double *__restrict__ Aci,
+                                                              float *__restrict__ Aci32,
+                                                              int *__restrict__ status)
+{
+    extern __shared__ double sm[];
+    double *A = sm;                                      // nc x nc
+    double *colk = sm + (size_t)nc * nc;                 // nc
+    int *piv = (int *)(sm + (size_t)nc * nc + nc);       // nc
+    __shared__ int red_i[1];
+    for (int e = threadIdx.x; e < nc * nc; e += CI_THREADS) A[e] = Ac[e];
+    __syncthreads();
+    const int zp = gj_invert_lds<CI_THREADS>(A, colk, piv, red_i, nc);
+    int bad = 0;
+    for (int e = threadIdx.x; e < nc * nc; e += CI_THREADS) {
+        const double v = A[e];
+        bad |= !isfinite(v);
+        Aci[e] = v;
+        Aci32[e] = (float)v;
+    }
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) status[0] = zp + (bad ? 1 : 0);
+}
+
+// WT[b][i] = (AP Ac^-1)[i][b] = sum_a AP[a][i] Aci[a][b]: thread per row i, 16 columns b
+// per block (blockIdx.y), so AP is read once per 16 columns
+constexpr int CW_TILE = 16;
+__global__ __launch_bounds__(256) void k_coarse_w(int n, int nc, const double *__restrict__ AP,
+                                                  size_t ldap, const double *__restrict__ Aci,
+                                                  double *__restrict__ WT,
+                                                  float *__restrict__ WT32)
+{
+    const int b0 = blockIdx.y * CW_TILE;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc[CW_TILE];
+#pragma unroll
+    for (int t = 0; t < CW_TILE; ++t) acc[t] = 0.0;
+    for (int a = 0; a < nc; ++a) {
+        const double ap = AP[(size_t)a * ldap + i];
+        const double *arow = Aci + (size_t)a * nc + b0;
+#pragma unroll
+        for (int t = 0; t < CW_TILE; ++t)
+            if (b0 + t < nc) acc[t] += ap * arow[t];
+    }
+#pragma unroll
+    for (int t = 0; t < CW_TILE; ++t)
+        if (b0 + t < nc) {
+            WT[(size_t)(b0 + t) * ldap + i] = acc[t];
+            WT32[(size_t)(b0 + t) * ldap + i] = (float)acc[t];
+        }
+}
+
+// C[v][a] = (P^T x_v)[a]: one block per (aggregate, vector); per-thread strided sums,
+// then the 4 wavefront sums in order
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_restrict(int n, int agg, const T *__restrict__ X,
+                                                         size_t ldx, double *__restrict__ C)
+{
+    __shared__ double ws[4];
+    const int a = blockIdx.x, v = blockIdx.y;
+    const T *x = X + (size_t)v * ldx;
+    const int r0 = a * agg, r1 = min(n, r0 + agg);
+    double s = 0.0;
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += 256) s += (double)x[i];
+    s = cs_wave_sum(s);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) C[(size_t)v * CS_MAX + a] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+// t_v = x_v - W C[v] (W stored [b][i]); vector v = blockIdx.y
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_residual(int n, int nc, const T *__restrict__ X,
+                                                         size_t ldx, const T *__restrict__ WT,
+                                                         size_t ldw, const double *__restrict__ C,
+                                                         T *__restrict__ Tout)
+{
+    __shared__ double cs[CS_MAX];
+    const int v = blockIdx.y;
+    for (int b = threadIdx.x; b < nc; b += blockDim.x) cs[b] = C[(size_t)v * CS_MAX + b];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double t = (double)X[(size_t)v * ldx + i];
+    for (int b = 0; b < nc; ++b) t -= (double)WT[(size_t)b * ldw + i] * cs[b];
+    Tout[(size_t)v * ldx + i] = (T)t;
+}
+
+}  // namespace
+
+size_t coarse_invert_lds_bytes(int nc)
+{
+    return sizeof(double) * ((size_t)nc * nc + nc) + sizeof(int) * (size_t)nc;
+}
+
+void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
+                         size_t ld, const double *alpha, const uint8_t *gmask,
+                         const uint8_t *constrained, double *AP, float *AP32, size_t ldap,
+                         double *Ac, double *Aci, float *Aci32, double *WT, float *WT32,
+                         int *status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_coarse_rows, dim3(n), dim3(CR_THREADS), 0, s, n, agg, nc, G, H,
+                       precision, ld, alpha, gmask, constrained, AP, AP32, ldap);
+    hipLaunchKernelGGL(k_coarse_matrix, dim3((nc * nc + 3) / 4), dim3(256), 0, s, n, agg, nc,
+                       (const double *)AP, ldap, Ac);
+    hipLaunchKernelGGL(k_coarse_invert, dim3(1), dim3(CI_THREADS), coarse_invert_lds_bytes(nc),
+                       s, nc, (const double *)Ac, Aci, Aci32, status);
+    hipLaunchKernelGGL(k_coarse_w, dim3((n + 255) / 256, (nc + CW_TILE - 1) / CW_TILE), dim3(256),
+                       0, s, n, nc, (const double *)AP, ldap, (const double *)Aci, WT, WT32);
+}
+
+void launch_coarse_pre(int is_float, int n, int agg, int nc, const void *X, size_t ldx,
+                       const void *WT, size_t ldw, double *C, void *Tout, int k, hipStream_t s)
+{
+    const dim3 cg(nc, k), rg((n + 255) / 256, k);
+    if (is_float) {
+        hipLaunchKernelGGL(k_coarse_restrict<float>, cg, dim3(256), 0, s, n, agg,
+                           (const float *)X, ldx, C);
+        hipLaunchKernelGGL(k_coarse_residual<float>, rg, dim3(256), 0, s, n, nc, (const float *)X,
+                           ldx, (const float *)WT, ldw, (const double *)C, (float *)Tout);
+    } else {
+        hipLaunchKernelGGL(k_coarse_restrict<double>, cg, dim3(256), 0, s, n, agg,
+                           (const double *)X, ldx, C);
+        hipLaunchKernelGGL(k_coarse_residual<double>, rg, dim3(256), 0, s, n, nc,
+                           (const double *)X, ldx, (const double *)WT, ldw, (const double *)C,
+                           (double *)Tout);
+    }
+}
+
+}  // namespace wb

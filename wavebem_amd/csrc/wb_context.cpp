@@ -83,6 +83,10 @@ struct wb_ctx {
     long assembly_ver = 0, mask_ver = 0, cons_struct_ver = 0;
     long pc_assembly_ver = -1, pc_mask_ver = -1, pc_cons_ver = -1;
     int pc_bs = 0;
+    // coarse-space correction of the block-Jacobi preconditioner (wb_coarse.hip)
+    bool cs_on = false;
+    int cs_agg = 0, cs_nc = 0;
+    DevBuf cs_AP, cs_AP32, cs_Ac, cs_Aci, cs_Aci32, cs_WT, cs_WT32, cs_status, cs_C, cs_T;
 
     // device: mesh
     DevBuf tile_start;
@@ -354,11 +358,28 @@ static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
                            c->con_inhom.as<double>(), 0, x, y, c->stream);
 }
 
+// Block Jacobi, with the coarse-space correction (forward operator only):
+// y = P z + B^-1 (x - AP z), z = Ac^-1 P^T x.  x and y may alias.
+static void block_precond(wb_ctx *c, int is_float, int k, const void *x, void *y)
+{
+    const void *binv = is_float ? c->binv32.p : c->binv.p;
+    if (c->cs_on && !c->op_transpose) {
+        launch_coarse_pre(is_float, c->n, c->cs_agg, c->cs_nc, x, c->vlen,
+                          is_float ? c->cs_WT32.p : c->cs_WT.p, c->vlen, c->cs_C.as<double>(),
+                          c->cs_T.p, k, c->stream);
+        launch_block_jacobi_apply(is_float, c->n, c->pc_bs, binv, c->cs_T.p, y, k, c->vlen,
+                                  c->stream, 0, c->cs_C.as<double>(),
+                                  is_float ? c->cs_Aci32.p : c->cs_Aci.p, c->cs_nc, c->cs_agg);
+    } else {
+        launch_block_jacobi_apply(is_float, c->n, c->pc_bs, binv, x, y, k, c->vlen, c->stream,
+                                  c->op_transpose ? 1 : 0);
+    }
+}
+
 static void apply_precond(wb_ctx *c, int is_float, const void *x, void *y)
 {
     if (c->cfg.preconditioner == WB_PRECOND_BLOCK_JACOBI) {
-        launch_block_jacobi_apply(is_float, c->n, c->pc_bs, is_float ? c->binv32.p : c->binv.p, x,
-                                  y, 1, c->vlen, c->stream, c->op_transpose ? 1 : 0);
+        block_precond(c, is_float, 1, x, y);
     } else {
         HIPCHK(hipMemcpyAsync(y, x, (is_float ? 4 : 8) * (size_t)c->n, hipMemcpyDeviceToDevice,
                               c->stream));
@@ -458,6 +479,38 @@ static void ensure_precond(wb_ctx *c)
                               c->constrained.as<uint8_t>(), c->binv.as<double>(),
                               c->binv32.as<float>(), c->stream);
     HIPCHK(hipGetLastError());
+    c->cs_on = false;
+    if (c->cfg.precond_coarse > 0) {
+        // aggregates of a multiple of 64 dofs, at most precond_coarse of them
+        const int maxc = c->cfg.precond_coarse;
+        const int agg = 64 * ((c->n + 64 * maxc - 1) / (64 * maxc));
+        const int nc = (c->n + agg - 1) / agg;
+        c->cs_agg = agg;
+        c->cs_nc = nc;
+        c->cs_AP.ensure(sizeof(double) * (size_t)nc * c->vlen);
+        c->cs_AP32.ensure(sizeof(float) * (size_t)nc * c->vlen);
+        c->cs_Ac.ensure(sizeof(double) * (size_t)nc * nc);
+        c->cs_Aci.ensure(sizeof(double) * (size_t)nc * nc);
+        c->cs_Aci32.ensure(sizeof(float) * (size_t)nc * nc);
+        c->cs_status.ensure(sizeof(int));
+        c->cs_WT.ensure(sizeof(double) * (size_t)nc * c->vlen);
+        c->cs_WT32.ensure(sizeof(float) * (size_t)nc * c->vlen);
+        c->cs_C.ensure(sizeof(double) * CS_MAX * WB_MULTI_MAX);
+        c->cs_T.ensure(sizeof(double) * c->vlen * WB_MULTI_MAX);
+        launch_coarse_build(c->n, agg, nc, c->G.p, c->H.p, c->cfg.precision, c->ld,
+                            c->alpha.as<double>(), c->gmask.as<uint8_t>(),
+                            c->constrained.as<uint8_t>(), c->cs_AP.as<double>(),
+                            c->cs_AP32.as<float>(), c->vlen, c->cs_Ac.as<double>(),
+                            c->cs_Aci.as<double>(), c->cs_Aci32.as<float>(), c->cs_WT.as<double>(),
+                            c->cs_WT32.as<float>(), c->cs_status.as<int>(), c->stream);
+        HIPCHK(hipGetLastError());
+        int st = 1;
+        HIPCHK(hipMemcpyAsync(&st, c->cs_status.p, sizeof(int), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        // a singular or non-finite coarse inverse leaves the blocks alone
+        c->cs_on = st == 0;
+    }
     c->pc_assembly_ver = c->assembly_ver;
     c->pc_mask_ver = c->mask_ver;
     c->pc_cons_ver = c->cons_struct_ver;
@@ -912,8 +965,7 @@ static void apply_constrained_multi(wb_ctx *c, int mode, int k, const void *X, s
 static void precond_multi(wb_ctx *c, int is_float, int k, const void *X, void *Y)
 {
     if (c->cfg.preconditioner == WB_PRECOND_BLOCK_JACOBI)
-        launch_block_jacobi_apply(is_float, c->n, c->pc_bs, is_float ? c->binv32.p : c->binv.p, X,
-                                  Y, k, c->vlen, c->stream);
+        block_precond(c, is_float, k, X, Y);
     else
         HIPCHK(hipMemcpyAsync(Y, X, (is_float ? 4 : 8) * c->vlen * k, hipMemcpyDeviceToDevice,
                               c->stream));
@@ -1266,6 +1318,7 @@ void wb_config_default(wb_config *cfg)
     cfg->assembly_tile = 16;
     cfg->ir_inner_format = 1;
     cfg->ir_inner16_reduction = 3e-4;
+    cfg->precond_coarse = 128;
 }
 
 const char *wb_version(void) { return "wavebem_amd 0.1 (gfx950)"; }
@@ -1289,6 +1342,7 @@ wb_status wb_create(wb_ctx **out, const wb_config *cfg)
         k.assembly_tile < 1 || k.assembly_tile > 16 ||
         (k.ir_inner_format != 0 && k.ir_inner_format != 1) ||
         !(k.ir_inner16_reduction > 0 && k.ir_inner16_reduction < 1) ||
+        k.precond_coarse < 0 || k.precond_coarse > CS_MAX ||
         !(k.solver_tolerance >= 0)) {
         delete c;
         return WB_E_ARG;
@@ -1319,7 +1373,9 @@ void wb_destroy(wb_ctx *c)
                       &c->H, &c->Glo, &c->Hlo, &c->hsum_part, &c->hsum_sing, &c->alpha,
                       &c->surf_d, &c->gmask, &c->segs, &c->diag_op, &c->diag_rhs, &c->con_rows,
                       &c->con_ptr, &c->con_cols, &c->con_w, &c->con_inhom, &c->constrained,
-                      &c->free_rows, &c->binv, &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2,
+                      &c->free_rows, &c->binv, &c->binv32, &c->cs_AP, &c->cs_AP32, &c->cs_Ac, &c->cs_Aci,
+                      &c->cs_Aci32, &c->cs_WT, &c->cs_WT32, &c->cs_status, &c->cs_C,
+                      &c->cs_T, &c->V, &c->z, &c->w, &c->P1, &c->P2,
                       &c->Pn, &c->hbuf, &c->hred, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
                       &c->z32, &c->w32, &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
                       &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,
@@ -2165,7 +2221,9 @@ size_t wb_device_bytes(const wb_ctx *c)
                             &c->segs,
                             &c->diag_op, &c->diag_rhs, &c->con_rows, &c->con_ptr, &c->con_cols,
                             &c->con_w, &c->con_inhom, &c->constrained, &c->free_rows, &c->binv,
-                            &c->binv32, &c->V, &c->z, &c->w, &c->P1, &c->P2, &c->Pn, &c->hbuf, &c->hred,
+                            &c->binv32, &c->cs_AP, &c->cs_AP32, &c->cs_Ac, &c->cs_Aci,
+                            &c->cs_Aci32, &c->cs_WT, &c->cs_WT32, &c->cs_status, &c->cs_C,
+                      &c->cs_T, &c->V, &c->z, &c->w, &c->P1, &c->P2, &c->Pn, &c->hbuf, &c->hred,
                             &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32, &c->z32, &c->w32,
                             &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
                             &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,

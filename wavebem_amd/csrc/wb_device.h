@@ -61,4 +61,78 @@ __device__ __forceinline__ void q1_shape(double xi, double eta, double (&phi)[4]
     deta[3] = xi;
 }
 
+// In-place inverse of the bs x bs row-major matrix A held in LDS by Gauss-Jordan
+// elimination with partial pivoting, by one workgroup of THREADS (>= 64) threads.  colk
+// (bs doubles), piv (bs ints) and red_i (1 int) are LDS scratch.  Returns, in every
+// thread, the number of zero pivots met (0 for a regular matrix).
+template <int THREADS>
+__device__ int gj_invert_lds(double *A, double *colk, int *piv, int *red_i, int bs)
+{
+    const int tid = threadIdx.x;
+    int zero_pivots = 0;
+    for (int k = 0; k < bs; ++k) {
+        // pivot: argmax |A[i][k]|, i >= k (ties to the smaller row), by the first wavefront
+        if (tid < 64) {
+            double best = -1.0;
+            int bi = k;
+            for (int i = k + tid; i < bs; i += 64) {
+                const double a = fabs(A[i * bs + k]);
+                if (a > best) {
+                    best = a;
+                    bi = i;
+                }
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double o = __shfl_xor(best, off, 64);
+                const int oi = __shfl_xor(bi, off, 64);
+                if (o > best || (o == best && oi < bi)) {
+                    best = o;
+                    bi = oi;
+                }
+            }
+            if (tid == 0) red_i[0] = bi;
+        }
+        __syncthreads();
+        const int p = red_i[0];
+        if (tid == 0) piv[k] = p;
+        if (p != k)
+            for (int c = tid; c < bs; c += THREADS) {
+                const double t = A[k * bs + c];
+                A[k * bs + c] = A[p * bs + c];
+                A[p * bs + c] = t;
+            }
+        __syncthreads();
+        const double d = A[k * bs + k];
+        zero_pivots += d == 0.0;
+        const double inv = d != 0.0 ? 1.0 / d : 0.0;
+        __syncthreads();
+        for (int c = tid; c < bs; c += THREADS) A[k * bs + c] = (c == k ? 1.0 : A[k * bs + c]) * inv;
+        for (int i = tid; i < bs; i += THREADS) colk[i] = A[i * bs + k];  // before it changes
+        __syncthreads();
+        for (int e = tid; e < bs * bs; e += THREADS) {
+            const int i = e / bs, c = e % bs;
+            if (i == k) continue;
+            const double f = colk[i];
+            if (c == k)
+                A[e] = -f * A[k * bs + k];
+            else
+                A[e] -= f * A[k * bs + c];
+        }
+        __syncthreads();
+    }
+    // undo the row interchanges as column interchanges, last first
+    for (int k = bs - 1; k >= 0; --k) {
+        const int p = piv[k];
+        if (p != k)
+            for (int i = tid; i < bs; i += THREADS) {
+                const double t = A[i * bs + k];
+                A[i * bs + k] = A[i * bs + p];
+                A[i * bs + p] = t;
+            }
+        __syncthreads();
+    }
+    return zero_pivots;
+}
+
 }  // namespace wb

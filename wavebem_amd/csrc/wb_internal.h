@@ -143,8 +143,29 @@ void launch_block_jacobi_build(int n, int bs, const void *G, const void *H, cons
                                const double *surface,
                                const uint8_t *constrained, double *binv, float *binv32,
                                hipStream_t s);
+// With C (CS_MAX doubles per vector, see launch_coarse_pre): y_i += (Aci C[v])[i / agg],
+// the coarse-space part (Aci: nc x nc row-major, FP32 for is_float).
 void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, const void *x,
-                               void *y, int k, size_t ldx, hipStream_t s, int trans = 0);
+                               void *y, int k, size_t ldx, hipStream_t s, int trans = 0,
+                               const double *C = nullptr, const void *Aci = nullptr, int nc = 0,
+                               int agg = 1);
+
+// ---------------------------------------------------------------- coarse space (wb_coarse.hip)
+// Aggregates of agg consecutive device-order dofs (agg a multiple of 64), nc <= CS_MAX of
+// them.  Build: AP (nc x ldap, [a][j] = sum over aggregate a of the block-Jacobi matrix row
+// j; FP64 matrices, or the FP32 hi part), Ac = P^T AP, Aci = Ac^-1 (nc x nc row-major),
+// WT ([b][j] = (AP Aci)[j][b]), each in FP64 and FP32; status[0] = 0 when the inverse is
+// usable.  Apply (k vectors, column stride ldx): C[v] = P^T x_v (CS_MAX doubles per vector)
+// and Tout_v = x_v - W C[v]; the block-Jacobi apply then adds P Aci C[v].
+constexpr int CS_MAX = 128;
+size_t coarse_invert_lds_bytes(int nc);
+void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
+                         size_t ld, const double *alpha, const uint8_t *gmask,
+                         const uint8_t *constrained, double *AP, float *AP32, size_t ldap,
+                         double *Ac, double *Aci, float *Aci32, double *WT, float *WT32,
+                         int *status, hipStream_t s);
+void launch_coarse_pre(int is_float, int n, int agg, int nc, const void *X, size_t ldx,
+                       const void *WT, size_t ldw, double *C, void *Tout, int k, hipStream_t s);
 
 // ---------------------------------------------------------------- adjoint (wb_adjoint.hip)
 // y = A_sel^T x over the unconstrained rows (FP64 matrices) + diag .* x on those rows;

@@ -10,6 +10,7 @@
 // wavefront shuffle reductions; constrained rows (ConstrainedOperator,
 // include/constrained_matrix.h:73-86) are skipped and rewritten by a small fix-up kernel.
 #include "wb_internal.h"
+#include "wb_device.h"
 
 namespace wb {
 
@@ -88,8 +89,7 @@ __global__ __launch_bounds__(BJ_THREADS) void k_block_jacobi_build(
     double *A = sm;                                // bs x bs, row-major
     double *colk = sm + (size_t)bs * bs;           // bs: column k before elimination
     int *piv = (int *)(sm + (size_t)bs * bs + bs);  // bs
-    __shared__ double red_v[BJ_THREADS];
-    __shared__ int red_i[BJ_THREADS];
+    __shared__ int red_i[1];
     const int b = blockIdx.x;
     const int base = b * bs;
     const int m = min(bs, n - base);
@@ -120,68 +120,7 @@ __global__ __launch_bounds__(BJ_THREADS) void k_block_jacobi_build(
         A[e] = v;
     }
     __syncthreads();
-    for (int k = 0; k < bs; ++k) {
-        // pivot: argmax |A[i][k]|, i >= k
-        double best = -1.0;
-        int bi = k;
-        for (int i = k + tid; i < bs; i += BJ_THREADS) {
-            const double a = fabs(A[i * bs + k]);
-            if (a > best) {
-                best = a;
-                bi = i;
-            }
-        }
-        red_v[tid] = best;
-        red_i[tid] = bi;
-        __syncthreads();
-        for (int off = BJ_THREADS / 2; off > 0; off >>= 1) {
-            if (tid < off) {
-                const double o = red_v[tid + off];
-                const int oi = red_i[tid + off];
-                if (o > red_v[tid] || (o == red_v[tid] && oi < red_i[tid])) {
-                    red_v[tid] = o;
-                    red_i[tid] = oi;
-                }
-            }
-            __syncthreads();
-        }
-        const int p = red_i[0];
-        if (tid == 0) piv[k] = p;
-        if (p != k)
-            for (int c = tid; c < bs; c += BJ_THREADS) {
-                const double t = A[k * bs + c];
-                A[k * bs + c] = A[p * bs + c];
-                A[p * bs + c] = t;
-            }
-        __syncthreads();
-        const double d = A[k * bs + k];
-        const double inv = d != 0.0 ? 1.0 / d : 0.0;
-        __syncthreads();
-        for (int c = tid; c < bs; c += BJ_THREADS) A[k * bs + c] = (c == k ? 1.0 : A[k * bs + c]) * inv;
-        for (int i = tid; i < bs; i += BJ_THREADS) colk[i] = A[i * bs + k];  // before it changes
-        __syncthreads();
-        for (int e = tid; e < bs * bs; e += BJ_THREADS) {
-            const int i = e / bs, c = e % bs;
-            if (i == k) continue;
-            const double f = colk[i];
-            if (c == k)
-                A[e] = -f * A[k * bs + k];
-            else
-                A[e] -= f * A[k * bs + c];
-        }
-        __syncthreads();
-    }
-    // undo the row interchanges as column interchanges, last first
-    for (int k = bs - 1; k >= 0; --k) {
-        const int p = piv[k];
-        if (p != k)
-            for (int i = tid; i < bs; i += BJ_THREADS) {
-                const double t = A[i * bs + k];
-                A[i * bs + k] = A[i * bs + p];
-                A[i * bs + p] = t;
-            }
-        __syncthreads();
-    }
+    (void)gj_invert_lds<BJ_THREADS>(A, colk, piv, red_i, bs);
     double *out = binv + (size_t)b * bs * bs;
     float *out32 = binv32 + (size_t)b * bs * bs;
     for (int e = tid; e < bs * bs; e += BJ_THREADS) {
@@ -204,18 +143,37 @@ void launch_block_jacobi_build(int n, int bs, const void *G, const void *H, cons
 }
 
 // y_blk = Binv_blk x_blk for k vectors (column stride ldx); one thread per block row.
+// With C (the coarse-space part of the two-level preconditioner, CS_MAX entries per
+// vector): y_i += z_a for the aggregate a = i / agg of row i, z_a = sum_b Aci[a][b] C[v][b]
+// (Aci row-major nc x nc), formed by the block for the at most three aggregates its rows
+// touch (bs <= 128, agg >= 64), the partial products reduced in a fixed order.
 template <typename T>
 __global__ void k_block_jacobi_apply(int n, int bs, const T *__restrict__ binv,
                                      const T *__restrict__ x, T *__restrict__ y, int k, size_t ldx,
-                                     int trans)
+                                     int trans, const double *__restrict__ C,
+                                     const T *__restrict__ Aci, int nc, int agg)
 {
     extern __shared__ char smraw[];
     T *xs = (T *)smraw;
+    __shared__ double zred[2][3];
+    __shared__ double zs[3];
     const int b = blockIdx.x, r = threadIdx.x, base = b * bs;
     const int m = min(bs, n - base);
     const T *B = binv + (size_t)b * bs * bs;
+    const int a0 = C ? base / agg : 0, na = C ? (base + m - 1) / agg - a0 + 1 : 0;
     for (int v = 0; v < k; ++v) {
         if (r < bs) xs[r] = (r < m) ? x[(size_t)v * ldx + base + r] : (T)0;
+        for (int q = 0; q < na; ++q) {
+            const T *arow = Aci + (size_t)(a0 + q) * nc;
+            const double *c = C + (size_t)v * CS_MAX;
+            double p = 0.0;
+            for (int bb = r; bb < nc; bb += blockDim.x) p += (double)arow[bb] * c[bb];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
+            if ((r & 63) == 0) zred[r >> 6][q] = p;
+        }
+        __syncthreads();
+        if (r < na) zs[r] = blockDim.x > 64 ? zred[0][r] + zred[1][r] : zred[0][r];
         __syncthreads();
         if (r < m) {
             T acc = 0;
@@ -223,6 +181,7 @@ __global__ void k_block_jacobi_apply(int n, int bs, const T *__restrict__ binv,
                 for (int c = 0; c < m; ++c) acc += B[r * bs + c] * xs[c];
             else
                 for (int c = 0; c < m; ++c) acc += B[c * bs + r] * xs[c];
+            if (C) acc = (T)((double)acc + zs[(base + r) / agg - a0]);
             y[(size_t)v * ldx + base + r] = acc;
         }
         __syncthreads();
@@ -230,18 +189,20 @@ __global__ void k_block_jacobi_apply(int n, int bs, const T *__restrict__ binv,
 }
 
 void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, const void *x,
-                               void *y, int k, size_t ldx, hipStream_t s, int trans)
+                               void *y, int k, size_t ldx, hipStream_t s, int trans,
+                               const double *C, const void *Aci, int nc, int agg)
 {
     const int nb = (n + bs - 1) / bs;
     const int threads = bs <= 64 ? 64 : 128;
     if (is_float)
         hipLaunchKernelGGL(k_block_jacobi_apply<float>, dim3(nb), dim3(threads),
                            sizeof(float) * bs, s, n, bs, (const float *)binv, (const float *)x,
-                           (float *)y, k, ldx, trans);
+                           (float *)y, k, ldx, trans, C, (const float *)Aci, nc, agg);
     else
         hipLaunchKernelGGL(k_block_jacobi_apply<double>, dim3(nb), dim3(threads),
                            sizeof(double) * bs, s, n, bs, (const double *)binv,
-                           (const double *)x, (double *)y, k, ldx, trans);
+                           (const double *)x, (double *)y, k, ldx, trans, C,
+                           (const double *)Aci, nc, agg);
 }
 
 // elementwise ops on n entries:
