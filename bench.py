@@ -33,7 +33,7 @@ def log(*a):
 
 
 def run_precision(mesh, bc, precision, steps, warmup, tol, ir_inner=1e-6, tile=16, pblock=64,
-                  inner_format=1, inner16=3e-4):
+                  inner_format=1, inner16=3e-4, coarse=128):
     import torch
 
     from wavebem_amd.bem import BEMProblem, Config
@@ -41,7 +41,8 @@ def run_precision(mesh, bc, precision, steps, warmup, tol, ir_inner=1e-6, tile=1
     n = mesh.n_dofs
     cfg = Config(precision=precision, solver_tolerance=tol, solver_max_steps=5000,
                  ir_inner_reduction=ir_inner, assembly_tile=tile, precond_block=pblock,
-                 ir_inner_format=inner_format, ir_inner16_reduction=inner16)
+                 ir_inner_format=inner_format, ir_inner16_reduction=inner16,
+                 precond_coarse=coarse)
     p = BEMProblem(mesh, cfg)
     p.compute_constraints(bc)
     bc_d = torch.tensor(bc, dtype=torch.float64, device="cuda")
@@ -354,6 +355,9 @@ def main():
                     help="skip the host BEMFMA timing inside the CPU baseline")
     ap.add_argument("--precond-block", type=int, default=64,
                     help="block-Jacobi block size (rows of the device ordering, <= 128)")
+    ap.add_argument("--coarse", type=int, default=128,
+                    help="coarse-space unknowns of the preconditioner (<= 192, 0 = block Jacobi "
+                         "only)")
     ap.add_argument("--multi", type=int, default=0, metavar="K",
                     help="only time K right-hand sides solved together (default workload "
                          "60k) and exit")
@@ -429,7 +433,7 @@ def main():
             continue
         res[prec] = run_precision(mesh, bc, prec, args.steps, args.warmup, args.tol,
                                   args.ir_inner, args.tile, args.precond_block,
-                                  args.ir_inner_format, args.ir_inner16)
+                                  args.ir_inner_format, args.ir_inner16, args.coarse)
 
     n = mesh.n_dofs
     N = mesh.other_nodes > 0
@@ -452,7 +456,9 @@ def main():
         "config": {"workload": f"wigley_closed_halfbox_{args.workload}", "n_dofs": n,
                    "n_cells": mesh.n_cells, "quadrature": "QGauss 4x4 + QGaussOneOverR(5)",
                    "gmres_tol_abs_preconditioned": args.tol, "gmres_restart": 98,
-                   "preconditioner": f"block_jacobi_{args.precond_block}", "froude": 0.3},
+                   "preconditioner": f"block_jacobi_{args.precond_block}"
+                                     + (f"+coarse_{args.coarse}" if args.coarse else ""),
+                   "froude": 0.3},
     }
     if "fp64" in res:
         line["fp64_ms_per_step"] = round(res["fp64"]["ms_per_step"], 3)

@@ -72,7 +72,8 @@ typedef struct {
     double ir_inner16_reduction;   /* FP32_IR with ir_inner_format = 1: the inner solve
                                       stops at max(this * rho, tol / 2) */
     int precond_coarse;            /* block Jacobi only: at most this many coarse unknowns
-                                      (<= 128; 0 = off) for a coarse-space correction over
+                                      (<= 192, default 128; 0 = off) for a coarse-space
+                                      correction over
                                       aggregates of consecutive device-order dofs, applied
                                       multiplicatively with the blocks in the forward
                                       solves (not in wb_solve_adjoint); changes iteration

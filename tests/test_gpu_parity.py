@@ -130,6 +130,30 @@ def test_coarse_space_keeps_the_solution(gpu_available, precision):
     assert its[128] < its[0]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_coarse_space_192_at_10k(gpu_available, precision):
+    """More than 128 coarse unknowns (the coarse inverse eliminated in FP32 in LDS): at
+    10,951 dofs the 192-unknown coarse space and plain block Jacobi give the same solution
+    to 1e-9 (relative), the former in fewer iterations."""
+    m = M.wigley_preset("10k")
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    bc, _ = M.wigley_bc(m, seed=5)
+    out, its = {}, {}
+    for coarse in (0, 192):
+        phi = bc * m.surface_nodes
+        dphi = bc * m.other_nodes
+        with _problem(m, ptr, idx, precision=precision, solver_tolerance=1e-12,
+                      precond_coarse=coarse) as p:
+            p.compute_constraints(bc)
+            st = p.solve_system(phi, dphi, bc)
+        out[coarse] = np.where(m.surface_nodes == 1, dphi, phi)
+        its[coarse] = st["iterations"]
+    err = np.abs(out[192] - out[0]).max() / np.abs(out[0]).max()
+    print(f"  [{precision}] iterations {its}, solutions differ by {err:.2e}")
+    assert m.n_dofs > 128 * 64 and its[192] < its[0]
+    assert err < 1e-9
+
+
 def test_fp16_inner_hands_over_to_fp32(gpu_available):
     """An FP16 sweep that does not halve the outer residual (forced here by an inner
     reduction of 0.9 per sweep) switches the remaining sweeps to the FP32 operator; the

@@ -6,7 +6,8 @@
 //   M^-1 v = P z + B^-1 (v - A P z),   z = Ac^-1 P^T v,   Ac = P^T A P,
 // with B the block-Jacobi blocks and P the indicator of aggregates: runs of `agg`
 // consecutive dofs of the device ordering (compact patches, since the ordering is built
-// from compact column tiles), agg a multiple of 64, at most CS_MAX aggregates so that Ac
+// from compact column tiles), agg a multiple of 4 and at least 64 (so a block-Jacobi block
+// of at most 128 rows touches at most 3 aggregates), at most CS_MAX aggregates so that Ac
 // is inverted in the LDS of one workgroup.  A is the matrix the block-Jacobi blocks are
 // taken from: H(j,i) (+ alpha_i at i == j) in a Neumann column, -G(j,i) in a Dirichlet
 // column, identity rows for constrained dofs.  Block Jacobi handles the near field; the
@@ -120,24 +121,27 @@ __global__ __launch_bounds__(256) void k_coarse_matrix(int n, int agg, int nc,
     if (lane == 0) Ac[a * nc + b] = s;
 }
 
-// Aci = Ac^-1 (row-major, FP64 and FP32); status[0] = zero pivots + 1 if any entry of the
-// inverse is not finite (0: usable).
+// Aci = Ac^-1 (row-major, FP64 and FP32), eliminated in LDS in type T: FP64 up to 128
+// unknowns, FP32 above (to CS_MAX = 192; the inverse only preconditions, and the FP64 path
+// applies the stored values as a fixed linear operator); status[0] = zero pivots + 1 if
+// any entry of the inverse is not finite (0: usable).
+template <typename T>
 __global__ __launch_bounds__(CI_THREADS) void k_coarse_invert(int nc, const double *__restrict__ Ac,
                                                               double *__restrict__ Aci,
                                                               float *__restrict__ Aci32,
                                                               int *__restrict__ status)
 {
     extern __shared__ double sm[];
-    double *A = sm;                                      // nc x nc
-    double *colk = sm + (size_t)nc * nc;                 // nc
-    int *piv = (int *)(sm + (size_t)nc * nc + nc);       // nc
+    T *A = (T *)sm;                                      // nc x nc
+    T *colk = A + (size_t)nc * nc;                       // nc
+    int *piv = (int *)(colk + nc);                       // nc
     __shared__ int red_i[1];
-    for (int e = threadIdx.x; e < nc * nc; e += CI_THREADS) A[e] = Ac[e];
+    for (int e = threadIdx.x; e < nc * nc; e += CI_THREADS) A[e] = (T)Ac[e];
     __syncthreads();
-    const int zp = gj_invert_lds<CI_THREADS>(A, colk, piv, red_i, nc);
+    const int zp = gj_invert_lds<CI_THREADS, T>(A, colk, piv, red_i, nc);
     int bad = 0;
     for (int e = threadIdx.x; e < nc * nc; e += CI_THREADS) {
-        const double v = A[e];
+        const double v = (double)A[e];
         bad |= !isfinite(v);
         Aci[e] = v;
         Aci32[e] = (float)v;
@@ -215,7 +219,8 @@ __global__ __launch_bounds__(256) void k_coarse_residual(int n, int nc, const T 
 
 size_t coarse_invert_lds_bytes(int nc)
 {
-    return sizeof(double) * ((size_t)nc * nc + nc) + sizeof(int) * (size_t)nc;
+    const size_t es = nc <= CS_MAX64 ? sizeof(double) : sizeof(float);
+    return es * ((size_t)nc * nc + nc) + sizeof(int) * (size_t)nc;
 }
 
 void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
@@ -228,8 +233,14 @@ void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, i
                        precision, ld, alpha, gmask, constrained, AP, AP32, ldap);
     hipLaunchKernelGGL(k_coarse_matrix, dim3((nc * nc + 3) / 4), dim3(256), 0, s, n, agg, nc,
                        (const double *)AP, ldap, Ac);
-    hipLaunchKernelGGL(k_coarse_invert, dim3(1), dim3(CI_THREADS), coarse_invert_lds_bytes(nc),
-                       s, nc, (const double *)Ac, Aci, Aci32, status);
+    if (nc <= CS_MAX64)
+        hipLaunchKernelGGL(k_coarse_invert<double>, dim3(1), dim3(CI_THREADS),
+                           coarse_invert_lds_bytes(nc), s, nc, (const double *)Ac, Aci, Aci32,
+                           status);
+    else
+        hipLaunchKernelGGL(k_coarse_invert<float>, dim3(1), dim3(CI_THREADS),
+                           coarse_invert_lds_bytes(nc), s, nc, (const double *)Ac, Aci, Aci32,
+                           status);
     hipLaunchKernelGGL(k_coarse_w, dim3((n + 255) / 256, (nc + CW_TILE - 1) / CW_TILE), dim3(256),
                        0, s, n, nc, (const double *)AP, ldap, (const double *)Aci, WT, WT32);
 }

@@ -481,9 +481,9 @@ static void ensure_precond(wb_ctx *c)
     HIPCHK(hipGetLastError());
     c->cs_on = false;
     if (c->cfg.precond_coarse > 0) {
-        // aggregates of a multiple of 64 dofs, at most precond_coarse of them
+        // aggregates of a multiple of 4 dofs, at least 64, at most precond_coarse of them
         const int maxc = c->cfg.precond_coarse;
-        const int agg = 64 * ((c->n + 64 * maxc - 1) / (64 * maxc));
+        const int agg = std::max(64, 4 * ((c->n + 4 * maxc - 1) / (4 * maxc)));
         const int nc = (c->n + agg - 1) / agg;
         c->cs_agg = agg;
         c->cs_nc = nc;
@@ -1318,7 +1318,7 @@ void wb_config_default(wb_config *cfg)
     cfg->assembly_tile = 16;
     cfg->ir_inner_format = 1;
     cfg->ir_inner16_reduction = 3e-4;
-    cfg->precond_coarse = 128;
+    cfg->precond_coarse = CS_MAX64;
 }
 
 const char *wb_version(void) { return "wavebem_amd 0.1 (gfx950)"; }

@@ -65,18 +65,18 @@ __device__ __forceinline__ void q1_shape(double xi, double eta, double (&phi)[4]
 // elimination with partial pivoting, by one workgroup of THREADS (>= 64) threads.  colk
 // (bs doubles), piv (bs ints) and red_i (1 int) are LDS scratch.  Returns, in every
 // thread, the number of zero pivots met (0 for a regular matrix).
-template <int THREADS>
-__device__ int gj_invert_lds(double *A, double *colk, int *piv, int *red_i, int bs)
+template <int THREADS, typename T = double>
+__device__ int gj_invert_lds(T *A, T *colk, int *piv, int *red_i, int bs)
 {
     const int tid = threadIdx.x;
     int zero_pivots = 0;
     for (int k = 0; k < bs; ++k) {
         // pivot: argmax |A[i][k]|, i >= k (ties to the smaller row), by the first wavefront
         if (tid < 64) {
-            double best = -1.0;
+            T best = (T)-1;
             int bi = k;
             for (int i = k + tid; i < bs; i += 64) {
-                const double a = fabs(A[i * bs + k]);
+                const T a = fabs(A[i * bs + k]);
                 if (a > best) {
                     best = a;
                     bi = i;
@@ -84,7 +84,7 @@ __device__ int gj_invert_lds(double *A, double *colk, int *piv, int *red_i, int 
             }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
-                const double o = __shfl_xor(best, off, 64);
+                const T o = __shfl_xor(best, off, 64);
                 const int oi = __shfl_xor(bi, off, 64);
                 if (o > best || (o == best && oi < bi)) {
                     best = o;
@@ -98,22 +98,22 @@ __device__ int gj_invert_lds(double *A, double *colk, int *piv, int *red_i, int 
         if (tid == 0) piv[k] = p;
         if (p != k)
             for (int c = tid; c < bs; c += THREADS) {
-                const double t = A[k * bs + c];
+                const T t = A[k * bs + c];
                 A[k * bs + c] = A[p * bs + c];
                 A[p * bs + c] = t;
             }
         __syncthreads();
-        const double d = A[k * bs + k];
-        zero_pivots += d == 0.0;
-        const double inv = d != 0.0 ? 1.0 / d : 0.0;
+        const T d = A[k * bs + k];
+        zero_pivots += d == (T)0;
+        const T inv = d != (T)0 ? (T)1 / d : (T)0;
         __syncthreads();
-        for (int c = tid; c < bs; c += THREADS) A[k * bs + c] = (c == k ? 1.0 : A[k * bs + c]) * inv;
+        for (int c = tid; c < bs; c += THREADS) A[k * bs + c] = (c == k ? (T)1 : A[k * bs + c]) * inv;
         for (int i = tid; i < bs; i += THREADS) colk[i] = A[i * bs + k];  // before it changes
         __syncthreads();
         for (int e = tid; e < bs * bs; e += THREADS) {
             const int i = e / bs, c = e % bs;
             if (i == k) continue;
-            const double f = colk[i];
+            const T f = colk[i];
             if (c == k)
                 A[e] = -f * A[k * bs + k];
             else
@@ -126,7 +126,7 @@ __device__ int gj_invert_lds(double *A, double *colk, int *piv, int *red_i, int 
         const int p = piv[k];
         if (p != k)
             for (int i = tid; i < bs; i += THREADS) {
-                const double t = A[i * bs + k];
+                const T t = A[i * bs + k];
                 A[i * bs + k] = A[i * bs + p];
                 A[i * bs + p] = t;
             }
