@@ -6,14 +6,15 @@
 //   M^-1 v = P z + B^-1 (v - A P z),   z = Ac^-1 P^T v,   Ac = P^T A P,
 // with B the block-Jacobi blocks and P the indicator of aggregates: runs of `agg`
 // consecutive dofs of the device ordering (compact patches, since the ordering is built
-// from compact column tiles), agg a multiple of 4 and at least 64 (so a block-Jacobi block
+// from compact column tiles), agg a multiple of 8 and at least 64 (so a block-Jacobi block
 // of at most 128 rows touches at most 3 aggregates), at most CS_MAX aggregates so that Ac
 // is inverted in the LDS of one workgroup.  A is the matrix the block-Jacobi blocks are
 // taken from: H(j,i) (+ alpha_i at i == j) in a Neumann column, -G(j,i) in a Dirichlet
 // column, identity rows for constrained dofs.  Block Jacobi handles the near field; the
 // coarse space takes the smooth coupling between patches, which it misses.
 //
-// Per build: AP = A P (one pass over the selected operator), Ac = P^T AP, its inverse,
+// Per build: AP = A P (one pass over the selected operator: the FP64 matrices, or in
+// FP32-IR with the FP16 inner format its row-scaled FP16 copy), Ac = P^T AP, its inverse,
 // and W = AP Ac^-1 (n x nc).  Per application: C = P^T v (one block per aggregate),
 // t = v - W C, then the block-Jacobi kernel applies B^-1 t and adds z = Ac^-1 C of the
 // row's aggregate.  Every sum runs in a fixed order, so results do not depend on timing.
@@ -36,15 +37,19 @@ constexpr int CI_THREADS = 1024;  // k_coarse_invert
 typedef float cs_fvec4 __attribute__((ext_vector_type(4)));
 typedef double cs_dvec2 __attribute__((ext_vector_type(2)));
 
+typedef _Float16 cs_hvec8 __attribute__((ext_vector_type(8)));
+
 // AP[a][j] = sum over the columns i of aggregate a of A(j, i): one row j per block;
 // wavefront w takes the aggregates a = w, w + 4, ..., 64 four-column groups per step
-// (gmask bit e of group g: column 4g + e comes from H).  Padding columns past n are zero
-// in G and H.
+// (gmask bit e of group g: column 4g + e comes from H), or, given the row-scaled FP16
+// copy of the selected operator (A16, row scales rs16), 64 eight-column groups of it per
+// step.  Padding columns past n are zero in G, H and A16.
 __global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
     int n, int agg, int nc, const void *__restrict__ Gv, const void *__restrict__ Hv,
-    int precision, size_t ld, const double *__restrict__ alpha,
-    const uint8_t *__restrict__ gmask, const uint8_t *__restrict__ constrained,
-    double *__restrict__ AP, float *__restrict__ AP32, size_t ldap)
+    int precision, size_t ld, const _Float16 *__restrict__ A16, const float *__restrict__ rs16,
+    const double *__restrict__ alpha, const uint8_t *__restrict__ gmask,
+    const uint8_t *__restrict__ constrained, double *__restrict__ AP, float *__restrict__ AP32,
+    size_t ldap)
 {
     const int j = blockIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -55,6 +60,22 @@ __global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
         double s = 0.0;
         if (con) {
             s = (j >= c0 && j < c1) ? 1.0 : 0.0;
+        } else if (A16) {
+            // FP16 operator: 8 columns per lane (agg is a multiple of 8)
+            const int h0 = c0 / 8, h1 = (c1 + 7) / 8;
+            float f = 0.0f;
+            for (int hb = h0; hb < h1; hb += 64) {
+                const int h = hb + lane;
+                float v = 0.0f;
+                if (h < h1) {
+                    const cs_hvec8 q = *((const cs_hvec8 *)(A16 + roff) + h);
+                    v = (((float)q[0] + (float)q[1]) + ((float)q[2] + (float)q[3])) +
+                        (((float)q[4] + (float)q[5]) + ((float)q[6] + (float)q[7]));
+                }
+                f += (float)cs_wave_sum((double)v);
+            }
+            s = (double)f * (double)rs16[j];
+            if (j >= c0 && j < c1 && ((gmask[j >> 2] >> (j & 3)) & 1u)) s += alpha[j];
         } else {
             const int g0 = c0 / 4, g1 = (c1 + 3) / 4;
             for (int gb = g0; gb < g1; gb += 64) {
@@ -224,13 +245,14 @@ size_t coarse_invert_lds_bytes(int nc)
 }
 
 void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
-                         size_t ld, const double *alpha, const uint8_t *gmask,
-                         const uint8_t *constrained, double *AP, float *AP32, size_t ldap,
-                         double *Ac, double *Aci, float *Aci32, double *WT, float *WT32,
-                         int *status, hipStream_t s)
+                         size_t ld, const void *A16, const float *rs16, const double *alpha,
+                         const uint8_t *gmask, const uint8_t *constrained, double *AP,
+                         float *AP32, size_t ldap, double *Ac, double *Aci, float *Aci32,
+                         double *WT, float *WT32, int *status, hipStream_t s)
 {
     hipLaunchKernelGGL(k_coarse_rows, dim3(n), dim3(CR_THREADS), 0, s, n, agg, nc, G, H,
-                       precision, ld, alpha, gmask, constrained, AP, AP32, ldap);
+                       precision, ld, (const _Float16 *)A16, rs16, alpha, gmask, constrained,
+                       AP, AP32, ldap);
     hipLaunchKernelGGL(k_coarse_matrix, dim3((nc * nc + 3) / 4), dim3(256), 0, s, n, agg, nc,
                        (const double *)AP, ldap, Ac);
     if (nc <= CS_MAX64)

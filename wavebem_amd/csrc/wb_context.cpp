@@ -228,6 +228,7 @@ static void upload(DevBuf &b, const T *src, size_t count, hipStream_t s)
 
 static size_t elem_bytes(const wb_ctx *c) { return c->cfg.precision == WB_PREC_FP64 ? 8 : 4; }
 static void alloc_matrices(wb_ctx *c);
+static void ensure_h16(wb_ctx *c);
 
 // ------------------------------------------------------------------ vector plumbing
 static void to_internal(const wb_ctx *c, const double *src, std::vector<double> &dst)
@@ -481,9 +482,9 @@ static void ensure_precond(wb_ctx *c)
     HIPCHK(hipGetLastError());
     c->cs_on = false;
     if (c->cfg.precond_coarse > 0) {
-        // aggregates of a multiple of 4 dofs, at least 64, at most precond_coarse of them
+        // aggregates of a multiple of 8 dofs, at least 64, at most precond_coarse of them
         const int maxc = c->cfg.precond_coarse;
-        const int agg = std::max(64, 4 * ((c->n + 4 * maxc - 1) / (4 * maxc)));
+        const int agg = std::max(64, 8 * ((c->n + 8 * maxc - 1) / (8 * maxc)));
         const int nc = (c->n + agg - 1) / agg;
         c->cs_agg = agg;
         c->cs_nc = nc;
@@ -497,8 +498,14 @@ static void ensure_precond(wb_ctx *c)
         c->cs_WT32.ensure(sizeof(float) * (size_t)nc * c->vlen);
         c->cs_C.ensure(sizeof(double) * CS_MAX * WB_MULTI_MAX);
         c->cs_T.ensure(sizeof(double) * c->vlen * WB_MULTI_MAX);
+        // FP32-IR with the FP16 inner operator: A P from its FP16 copy (a third of the
+        // bytes of the FP32 hi part; a preconditioner needs no more)
+        const bool from16 = c->cfg.precision == WB_PREC_FP32_IR && c->cfg.ir_inner_format == 1;
+        if (from16) ensure_h16(c);
         launch_coarse_build(c->n, agg, nc, c->G.p, c->H.p, c->cfg.precision, c->ld,
-                            c->alpha.as<double>(), c->gmask.as<uint8_t>(),
+                            from16 ? c->A16.p : nullptr,
+                            from16 ? c->rs16.as<float>() : nullptr, c->alpha.as<double>(),
+                            c->gmask.as<uint8_t>(),
                             c->constrained.as<uint8_t>(), c->cs_AP.as<double>(),
                             c->cs_AP32.as<float>(), c->vlen, c->cs_Ac.as<double>(),
                             c->cs_Aci.as<double>(), c->cs_Aci32.as<float>(), c->cs_WT.as<double>(),

@@ -151,19 +151,20 @@ void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, co
                                int agg = 1);
 
 // ---------------------------------------------------------------- coarse space (wb_coarse.hip)
-// Aggregates of agg consecutive device-order dofs (agg a multiple of 4, at least 64),
+// Aggregates of agg consecutive device-order dofs (agg a multiple of 8, at least 64),
 // nc <= CS_MAX of them (the inverse is eliminated in FP64 up to CS_MAX64, in FP32 above).  Build: AP (nc x ldap, [a][j] = sum over aggregate a of the block-Jacobi matrix row
-// j; FP64 matrices, or the FP32 hi part), Ac = P^T AP, Aci = Ac^-1 (nc x nc row-major),
+// j; FP64 matrices, the FP32 hi part, or with A16 != nullptr the row-scaled FP16 copy),
+// Ac = P^T AP, Aci = Ac^-1 (nc x nc row-major),
 // WT ([b][j] = (AP Aci)[j][b]), each in FP64 and FP32; status[0] = 0 when the inverse is
 // usable.  Apply (k vectors, column stride ldx): C[v] = P^T x_v (CS_MAX doubles per vector)
 // and Tout_v = x_v - W C[v]; the block-Jacobi apply then adds P Aci C[v].
 constexpr int CS_MAX = 192, CS_MAX64 = 128;
 size_t coarse_invert_lds_bytes(int nc);
 void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
-                         size_t ld, const double *alpha, const uint8_t *gmask,
-                         const uint8_t *constrained, double *AP, float *AP32, size_t ldap,
-                         double *Ac, double *Aci, float *Aci32, double *WT, float *WT32,
-                         int *status, hipStream_t s);
+                         size_t ld, const void *A16, const float *rs16, const double *alpha,
+                         const uint8_t *gmask, const uint8_t *constrained, double *AP,
+                         float *AP32, size_t ldap, double *Ac, double *Aci, float *Aci32,
+                         double *WT, float *WT32, int *status, hipStream_t s);
 void launch_coarse_pre(int is_float, int n, int agg, int nc, const void *X, size_t ldx,
                        const void *WT, size_t ldw, double *C, void *Tout, int k, hipStream_t s);
 
