@@ -130,6 +130,29 @@ def test_coarse_space_keeps_the_solution(gpu_available, precision):
     assert its[128] < its[0]
 
 
+@pytest.mark.parametrize("block", [32, 100, 128])
+def test_coarse_space_with_other_block_sizes(gpu_available, block):
+    """Block-Jacobi blocks of 32, 100 and 128 rows with the coarse space (a block of up to
+    128 rows meets up to three 64-dof aggregates; 128-row blocks run two wavefronts):
+    the FP32-IR solve reaches the dense solution of the 2.5k tank."""
+    m = M.wigley_preset("2.5k")
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    H, G = O.assemble(m.support_points, m.cell_dofs, m.cell_orient, ptr, idx, n_threads=8)
+    bc, _ = M.wigley_bc(m, seed=6)
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, bc)
+    x = T.dense_solution(H, G, m.surface_nodes, m.other_nodes, cons, bc)
+    phi = bc * m.surface_nodes
+    dphi = bc * m.other_nodes
+    with _problem(m, ptr, idx, precision="fp32_ir", solver_tolerance=1e-12,
+                  precond_block=block) as p:
+        p.compute_constraints(bc)
+        st = p.solve_system(phi, dphi, bc)
+    sol = np.where(m.surface_nodes == 1, dphi, phi)
+    err = np.abs(sol - x).max() / np.abs(x).max()
+    print(f"  [block {block}] {st['iterations']} it, err {err:.2e}")
+    assert err < 1e-8
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
 def test_coarse_space_192_at_10k(gpu_available, precision):
     """More than 128 coarse unknowns (the coarse inverse eliminated in FP32 in LDS): at
