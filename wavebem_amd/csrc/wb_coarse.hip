@@ -13,8 +13,9 @@
 // column, identity rows for constrained dofs.  Block Jacobi handles the near field; the
 // coarse space takes the smooth coupling between patches, which it misses.
 //
-// Per build: AP = A P (one pass over the selected operator: the FP64 matrices, or in
-// FP32-IR with the FP16 inner format its row-scaled FP16 copy), Ac = P^T AP, its inverse,
+// Per build: AP = A P (one pass over the selected operator: the FP64 matrices; in FP32-IR
+// its row-scaled FP16 copy, or the FP32 hi part with the FP32 inner format), Ac = P^T AP,
+// its inverse,
 // and W = AP Ac^-1 (n x nc).  Per application: C = P^T v (one block per aggregate),
 // t = v - W C, then the block-Jacobi kernel applies B^-1 t and adds z = Ac^-1 C of the
 // row's aggregate.  Every sum runs in a fixed order, so results do not depend on timing.
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
         } else if (A16) {
             // FP16 operator: 8 columns per lane (agg is a multiple of 8)
             const int h0 = c0 / 8, h1 = (c1 + 7) / 8;
-            float f = 0.0f;
+            double f = 0.0;
             for (int hb = h0; hb < h1; hb += 64) {
                 const int h = hb + lane;
                 float v = 0.0f;
@@ -72,9 +73,9 @@ __global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
                     v = (((float)q[0] + (float)q[1]) + ((float)q[2] + (float)q[3])) +
                         (((float)q[4] + (float)q[5]) + ((float)q[6] + (float)q[7]));
                 }
-                f += (float)cs_wave_sum((double)v);
+                f += cs_wave_sum((double)v);
             }
-            s = (double)f * (double)rs16[j];
+            s = f * (double)rs16[j];
             if (j >= c0 && j < c1 && ((gmask[j >> 2] >> (j & 3)) & 1u)) s += alpha[j];
         } else {
             const int g0 = c0 / 4, g1 = (c1 + 3) / 4;
