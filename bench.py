@@ -371,8 +371,8 @@ def main():
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(row-sharded design described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU only; the path would shard by rows of G and H "
+              "across GPUs, which is not implemented", file=sys.stderr)
         sys.exit(2)
 
     import torch
