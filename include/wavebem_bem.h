@@ -75,9 +75,9 @@ typedef struct {
                                       (<= 192, default 128; 0 = off) for a coarse-space
                                       correction over
                                       aggregates of consecutive device-order dofs, applied
-                                      multiplicatively with the blocks in the forward
-                                      solves (not in wb_solve_adjoint); changes iteration
-                                      counts, not the solution */
+                                      multiplicatively with the blocks (transposed in
+                                      wb_solve_adjoint); changes iteration counts, not the
+                                      solution */
 } wb_config;
 
 typedef struct {

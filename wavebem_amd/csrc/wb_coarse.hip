@@ -18,7 +18,9 @@
 // its inverse,
 // and W = AP Ac^-1 (n x nc).  Per application: C = P^T v (one block per aggregate),
 // t = v - W C, then the block-Jacobi kernel applies B^-1 t and adds z = Ac^-1 C of the
-// row's aggregate.  Every sum runs in a fixed order, so results do not depend on timing.
+// row's aggregate.  The transposed system (the adjoint solve) uses the same aggregates:
+// its coarse matrix is Ac^T, and W = A^T P Ac^-T needs one more pass over A for P^T A.
+// Every sum runs in a fixed order, so results do not depend on timing.
 #include "wb_device.h"
 #include "wb_internal.h"
 
@@ -172,12 +174,13 @@ __global__ __launch_bounds__(CI_THREADS) void k_coarse_invert(int nc, const doub
     if (threadIdx.x == 0) status[0] = zp + (bad ? 1 : 0);
 }
 
-// WT[b][i] = (AP Ac^-1)[i][b] = sum_a AP[a][i] Aci[a][b]: thread per row i, 16 columns b
-// per block (blockIdx.y), so AP is read once per 16 columns
+// WT[b][i] = (AP Ac^-1)[i][b] = sum_a AP[a][i] Aci[a][b] (with trans: Aci[b][a], i.e.
+// Ac^-T, for the transposed system): thread per row i, 16 columns b per block
+// (blockIdx.y), so AP is read once per 16 columns; WT32 may be null
 constexpr int CW_TILE = 16;
 __global__ __launch_bounds__(256) void k_coarse_w(int n, int nc, const double *__restrict__ AP,
                                                   size_t ldap, const double *__restrict__ Aci,
-                                                  double *__restrict__ WT,
+                                                  int trans, double *__restrict__ WT,
                                                   float *__restrict__ WT32)
 {
     const int b0 = blockIdx.y * CW_TILE;
@@ -188,17 +191,44 @@ __global__ __launch_bounds__(256) void k_coarse_w(int n, int nc, const double *_
     for (int t = 0; t < CW_TILE; ++t) acc[t] = 0.0;
     for (int a = 0; a < nc; ++a) {
         const double ap = AP[(size_t)a * ldap + i];
-        const double *arow = Aci + (size_t)a * nc + b0;
 #pragma unroll
         for (int t = 0; t < CW_TILE; ++t)
-            if (b0 + t < nc) acc[t] += ap * arow[t];
+            if (b0 + t < nc)
+                acc[t] += ap * (trans ? Aci[(size_t)(b0 + t) * nc + a] : Aci[(size_t)a * nc + b0 + t]);
     }
 #pragma unroll
     for (int t = 0; t < CW_TILE; ++t)
         if (b0 + t < nc) {
             WT[(size_t)(b0 + t) * ldap + i] = acc[t];
-            WT32[(size_t)(b0 + t) * ldap + i] = (float)acc[t];
+            if (WT32) WT32[(size_t)(b0 + t) * ldap + i] = (float)acc[t];
         }
+}
+
+// For the transposed system: PTA[a][i] = sum over the rows j of aggregate a of A(j, i),
+// i.e. row i of A^T P (FP64 matrices): thread per column i, rows in order.
+__global__ __launch_bounds__(256) void k_coarse_cols(
+    int n, int agg, const double *__restrict__ G, const double *__restrict__ H, size_t ld,
+    const double *__restrict__ alpha, const uint8_t *__restrict__ gmask,
+    const uint8_t *__restrict__ constrained, double *__restrict__ PTA, size_t ldap)
+{
+    const int a = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool neu = (gmask[i >> 2] >> (i & 3)) & 1u;
+    const double *M = neu ? H : G;
+    const int r0 = a * agg, r1 = min(n, r0 + agg);
+    double s = 0.0;
+    for (int j = r0; j < r1; ++j) {
+        double v;
+        if (constrained[j])
+            v = (i == j) ? 1.0 : 0.0;
+        else {
+            const double m = M[(size_t)j * ld + i];
+            v = neu ? m + (i == j ? alpha[i] : 0.0) : -m;
+        }
+        s += v;
+    }
+    PTA[(size_t)a * ldap + i] = s;
 }
 
 // C[v][a] = (P^T x_v)[a]: one block per (aggregate, vector); per-thread strided sums,
@@ -265,7 +295,18 @@ void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, i
                            coarse_invert_lds_bytes(nc), s, nc, (const double *)Ac, Aci, Aci32,
                            status);
     hipLaunchKernelGGL(k_coarse_w, dim3((n + 255) / 256, (nc + CW_TILE - 1) / CW_TILE), dim3(256),
-                       0, s, n, nc, (const double *)AP, ldap, (const double *)Aci, WT, WT32);
+                       0, s, n, nc, (const double *)AP, ldap, (const double *)Aci, 0, WT, WT32);
+}
+
+void launch_coarse_build_t(int n, int agg, int nc, const double *G, const double *H, size_t ld,
+                           const double *alpha, const uint8_t *gmask, const uint8_t *constrained,
+                           const double *Aci, double *PTA, size_t ldap, double *WTT,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_coarse_cols, dim3((n + 255) / 256, nc), dim3(256), 0, s, n, agg, G, H,
+                       ld, alpha, gmask, constrained, PTA, ldap);
+    hipLaunchKernelGGL(k_coarse_w, dim3((n + 255) / 256, (nc + CW_TILE - 1) / CW_TILE), dim3(256),
+                       0, s, n, nc, (const double *)PTA, ldap, Aci, 1, WTT, (float *)nullptr);
 }
 
 void launch_coarse_pre(int is_float, int n, int agg, int nc, const void *X, size_t ldx,

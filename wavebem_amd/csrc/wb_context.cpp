@@ -85,8 +85,10 @@ struct wb_ctx {
     int pc_bs = 0;
     // coarse-space correction of the block-Jacobi preconditioner (wb_coarse.hip)
     bool cs_on = false;
+    bool cs_t_on = false;  // the transposed-system data below match the current build
     int cs_agg = 0, cs_nc = 0;
     DevBuf cs_AP, cs_AP32, cs_Ac, cs_Aci, cs_Aci32, cs_WT, cs_WT32, cs_status, cs_C, cs_T;
+    DevBuf cs_PTA, cs_WTT;  // transposed system (adjoint solve), FP64
 
     // device: mesh
     DevBuf tile_start;
@@ -364,7 +366,13 @@ static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
 static void block_precond(wb_ctx *c, int is_float, int k, const void *x, void *y)
 {
     const void *binv = is_float ? c->binv32.p : c->binv.p;
-    if (c->cs_on && !c->op_transpose) {
+    if (c->cs_on && c->op_transpose && c->cs_t_on && !is_float) {
+        // the transposed system: y = P z + B^-T (x - A^T P z), z = Ac^-T P^T x
+        launch_coarse_pre(0, c->n, c->cs_agg, c->cs_nc, x, c->vlen, c->cs_WTT.p, c->vlen,
+                          c->cs_C.as<double>(), c->cs_T.p, k, c->stream);
+        launch_block_jacobi_apply(0, c->n, c->pc_bs, binv, c->cs_T.p, y, k, c->vlen, c->stream,
+                                  1, c->cs_C.as<double>(), c->cs_Aci.p, c->cs_nc, c->cs_agg);
+    } else if (c->cs_on && !c->op_transpose) {
         launch_coarse_pre(is_float, c->n, c->cs_agg, c->cs_nc, x, c->vlen,
                           is_float ? c->cs_WT32.p : c->cs_WT.p, c->vlen, c->cs_C.as<double>(),
                           c->cs_T.p, k, c->stream);
@@ -481,6 +489,7 @@ static void ensure_precond(wb_ctx *c)
                               c->binv32.as<float>(), c->stream);
     HIPCHK(hipGetLastError());
     c->cs_on = false;
+    c->cs_t_on = false;
     if (c->cfg.precond_coarse > 0) {
         // aggregates of a multiple of 8 dofs, at least 64, at most precond_coarse of them
         const int maxc = c->cfg.precond_coarse;
@@ -1382,7 +1391,7 @@ void wb_destroy(wb_ctx *c)
                       &c->con_ptr, &c->con_cols, &c->con_w, &c->con_inhom, &c->constrained,
                       &c->free_rows, &c->binv, &c->binv32, &c->cs_AP, &c->cs_AP32, &c->cs_Ac, &c->cs_Aci,
                       &c->cs_Aci32, &c->cs_WT, &c->cs_WT32, &c->cs_status, &c->cs_C,
-                      &c->cs_T, &c->V, &c->z, &c->w, &c->P1, &c->P2,
+                      &c->cs_T, &c->cs_PTA, &c->cs_WTT, &c->V, &c->z, &c->w, &c->P1, &c->P2,
                       &c->Pn, &c->hbuf, &c->hred, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
                       &c->z32, &c->w32, &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
                       &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,
@@ -2016,6 +2025,18 @@ wb_status wb_solve_adjoint(wb_ctx *c, const double *rhs, double *lambda, wb_solv
         upload_vec(c, c->vb, rhs);
         double t0 = now_ms();
         ensure_precond(c);
+        if (c->cs_on && !c->cs_t_on) {
+            // the coarse space of the transposed system, from the same aggregates and Ac
+            c->cs_PTA.ensure(sizeof(double) * (size_t)c->cs_nc * c->vlen);
+            c->cs_WTT.ensure(sizeof(double) * (size_t)c->cs_nc * c->vlen);
+            launch_coarse_build_t(c->n, c->cs_agg, c->cs_nc, c->G.as<double>(), c->H.as<double>(),
+                                  c->ld, c->alpha.as<double>(), c->gmask.as<uint8_t>(),
+                                  c->constrained.as<uint8_t>(), c->cs_Aci.as<double>(),
+                                  c->cs_PTA.as<double>(), c->vlen, c->cs_WTT.as<double>(),
+                                  c->stream);
+            HIPCHK(hipGetLastError());
+            c->cs_t_on = true;
+        }
         HIPCHK(hipStreamSynchronize(c->stream));
         double t1 = now_ms();
         launch_vec_ops(6, c->n, nullptr, nullptr, c->vx.p, 0, c->stream);
@@ -2230,7 +2251,7 @@ size_t wb_device_bytes(const wb_ctx *c)
                             &c->con_w, &c->con_inhom, &c->constrained, &c->free_rows, &c->binv,
                             &c->binv32, &c->cs_AP, &c->cs_AP32, &c->cs_Ac, &c->cs_Aci,
                             &c->cs_Aci32, &c->cs_WT, &c->cs_WT32, &c->cs_status, &c->cs_C,
-                      &c->cs_T, &c->V, &c->z, &c->w, &c->P1, &c->P2, &c->Pn, &c->hbuf, &c->hred,
+                      &c->cs_T, &c->cs_PTA, &c->cs_WTT, &c->V, &c->z, &c->w, &c->P1, &c->P2, &c->Pn, &c->hbuf, &c->hred,
                             &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32, &c->z32, &c->w32,
                             &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
                             &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,

@@ -144,7 +144,7 @@ void launch_block_jacobi_build(int n, int bs, const void *G, const void *H, cons
                                const uint8_t *constrained, double *binv, float *binv32,
                                hipStream_t s);
 // With C (CS_MAX doubles per vector, see launch_coarse_pre): y_i += (Aci C[v])[i / agg],
-// the coarse-space part (Aci: nc x nc row-major, FP32 for is_float).
+// the coarse-space part (Aci: nc x nc row-major, FP32 for is_float; with trans, Aci^T C).
 void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, const void *x,
                                void *y, int k, size_t ldx, hipStream_t s, int trans = 0,
                                const double *C = nullptr, const void *Aci = nullptr, int nc = 0,
@@ -167,6 +167,13 @@ void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, i
                          double *WT, float *WT32, int *status, hipStream_t s);
 void launch_coarse_pre(int is_float, int n, int agg, int nc, const void *X, size_t ldx,
                        const void *WT, size_t ldw, double *C, void *Tout, int k, hipStream_t s);
+// The transposed system (FP64): PTA = P^T A (nc x ldap, [a][i]) and WTT ([b][i] =
+// (A^T P Ac^-T)[i][b]) from the same aggregates and Aci; the block-Jacobi apply then adds
+// Ac^-T C (launch_block_jacobi_apply with trans = 1).
+void launch_coarse_build_t(int n, int agg, int nc, const double *G, const double *H, size_t ld,
+                           const double *alpha, const uint8_t *gmask, const uint8_t *constrained,
+                           const double *Aci, double *PTA, size_t ldap, double *WTT,
+                           hipStream_t s);
 
 // ---------------------------------------------------------------- adjoint (wb_adjoint.hip)
 // y = A_sel^T x over the unconstrained rows (FP64 matrices) + diag .* x on those rows;

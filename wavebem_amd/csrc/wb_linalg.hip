@@ -145,8 +145,9 @@ void launch_block_jacobi_build(int n, int bs, const void *G, const void *H, cons
 // y_blk = Binv_blk x_blk for k vectors (column stride ldx); one thread per block row.
 // With C (the coarse-space part of the two-level preconditioner, CS_MAX entries per
 // vector): y_i += z_a for the aggregate a = i / agg of row i, z_a = sum_b Aci[a][b] C[v][b]
-// (Aci row-major nc x nc), formed by the block for the at most three aggregates its rows
-// touch (bs <= 128, agg >= 64), the partial products reduced in a fixed order.
+// (Aci row-major nc x nc; with trans, Aci^T for the transposed system), formed by the
+// block for the at most three aggregates its rows touch (bs <= 128, agg >= 64), the
+// partial products reduced in a fixed order.
 template <typename T>
 __global__ void k_block_jacobi_apply(int n, int bs, const T *__restrict__ binv,
                                      const T *__restrict__ x, T *__restrict__ y, int k, size_t ldx,
@@ -164,10 +165,12 @@ __global__ void k_block_jacobi_apply(int n, int bs, const T *__restrict__ binv,
     for (int v = 0; v < k; ++v) {
         if (r < bs) xs[r] = (r < m) ? x[(size_t)v * ldx + base + r] : (T)0;
         for (int q = 0; q < na; ++q) {
-            const T *arow = Aci + (size_t)(a0 + q) * nc;
+            const int a = a0 + q;
             const double *c = C + (size_t)v * CS_MAX;
             double p = 0.0;
-            for (int bb = r; bb < nc; bb += blockDim.x) p += (double)arow[bb] * c[bb];
+            // row a of Ac^-1, or with trans (the adjoint) column a, i.e. row a of Ac^-T
+            for (int bb = r; bb < nc; bb += blockDim.x)
+                p += (double)(trans ? Aci[(size_t)bb * nc + a] : Aci[(size_t)a * nc + bb]) * c[bb];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
             if ((r & 63) == 0) zred[r >> 6][q] = p;
