@@ -7,7 +7,8 @@ solve_system (rhs, constraints, preconditioner, GMRES, scatter) with the boundar
 already resident in HBM.  Both precisions run back to back on the same mesh and boundary
 data; the FP32-IR solution is compared with the FP64 one.
 
-Prints ONE JSON line (rank 0).  value = FP32-IR milliseconds per step.
+Prints ONE JSON line (rank 0).  value = FP32-IR milliseconds per step.  Each precision's
+run starts after the same idle pause (--settle), so neither inherits the other's heat.
 """
 from __future__ import annotations
 
@@ -355,6 +356,10 @@ def main():
                     help="skip the host BEMFMA timing inside the CPU baseline")
     ap.add_argument("--precond-block", type=int, default=64,
                     help="block-Jacobi block size (rows of the device ordering, <= 128)")
+    ap.add_argument("--settle", type=float, default=8.0, metavar="S",
+                    help="idle seconds before each precision's run, so that both start from "
+                         "the same state (the second run otherwise inherits the first one's "
+                         "heat: about 2%% on its GEMVs); not part of any timed region")
     ap.add_argument("--coarse", type=int, default=128,
                     help="coarse-space unknowns of the preconditioner (<= 192, 0 = block Jacobi "
                          "only)")
@@ -431,6 +436,8 @@ def main():
     for prec in ("fp64", "fp32_ir"):
         if args.only and prec != args.only:
             continue
+        if args.settle > 0:
+            time.sleep(args.settle)  # same idle time before each precision's run
         res[prec] = run_precision(mesh, bc, prec, args.steps, args.warmup, args.tol,
                                   args.ir_inner, args.tile, args.precond_block,
                                   args.ir_inner_format, args.ir_inner16, args.coarse)
@@ -458,7 +465,7 @@ def main():
                    "gmres_tol_abs_preconditioned": args.tol, "gmres_restart": 98,
                    "preconditioner": f"block_jacobi_{args.precond_block}"
                                      + (f"+coarse_{args.coarse}" if args.coarse else ""),
-                   "froude": 0.3},
+                   "froude": 0.3, "settle_s_before_each_precision": args.settle},
     }
     if "fp64" in res:
         line["fp64_ms_per_step"] = round(res["fp64"]["ms_per_step"], 3)
