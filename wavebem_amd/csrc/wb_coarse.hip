@@ -127,6 +127,27 @@ __global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
     }
 }
 
+// The same rows of A P from the aggregate sums S of the FP16 copy (written by its build):
+// AP[a][j] = S[a][j] plus the alpha diagonal of a Neumann column; identity rows for
+// constrained dofs.  One thread per (j, a).
+__global__ __launch_bounds__(256) void k_coarse_rows_sums(
+    int n, int agg, int nc, const double *__restrict__ S, const double *__restrict__ alpha,
+    const uint8_t *__restrict__ gmask, const uint8_t *__restrict__ constrained,
+    double *__restrict__ AP, float *__restrict__ AP32, size_t ldap)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x, a = blockIdx.y;
+    if (j >= n) return;
+    const bool mine = j >= a * agg && j < min(n, (a + 1) * agg);
+    const size_t o = (size_t)a * ldap + j;
+    double s;
+    if (constrained[j] != 0)
+        s = mine ? 1.0 : 0.0;
+    else
+        s = S[o] + (mine && ((gmask[j >> 2] >> (j & 3)) & 1u) ? alpha[j] : 0.0);
+    AP[o] = s;
+    AP32[o] = (float)s;
+}
+
 // Ac[a][b] = sum over rows j of aggregate a of AP[b][j]: one wavefront per (a, b).
 __global__ __launch_bounds__(256) void k_coarse_matrix(int n, int agg, int nc,
                                                        const double *__restrict__ AP, size_t ldap,
@@ -278,12 +299,16 @@ size_t coarse_invert_lds_bytes(int nc)
 void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
                          size_t ld, const void *A16, const float *rs16, const double *alpha,
                          const uint8_t *gmask, const uint8_t *constrained, double *AP,
-                         float *AP32, size_t ldap, double *Ac, double *Aci, float *Aci32,
-                         double *WT, float *WT32, int *status, hipStream_t s)
+                         float *AP32, size_t ldap, const double *S, double *Ac, double *Aci,
+                         float *Aci32, double *WT, float *WT32, int *status, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_coarse_rows, dim3(n), dim3(CR_THREADS), 0, s, n, agg, nc, G, H,
-                       precision, ld, (const _Float16 *)A16, rs16, alpha, gmask, constrained,
-                       AP, AP32, ldap);
+    if (S)
+        hipLaunchKernelGGL(k_coarse_rows_sums, dim3((n + 255) / 256, nc), dim3(256), 0, s, n,
+                           agg, nc, S, alpha, gmask, constrained, AP, AP32, ldap);
+    else
+        hipLaunchKernelGGL(k_coarse_rows, dim3(n), dim3(CR_THREADS), 0, s, n, agg, nc, G, H,
+                           precision, ld, (const _Float16 *)A16, rs16, alpha, gmask,
+                           constrained, AP, AP32, ldap);
     hipLaunchKernelGGL(k_coarse_matrix, dim3((nc * nc + 3) / 4), dim3(256), 0, s, n, agg, nc,
                        (const double *)AP, ldap, Ac);
     if (nc <= CS_MAX64)

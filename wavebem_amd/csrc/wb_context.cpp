@@ -89,6 +89,7 @@ struct wb_ctx {
     int cs_agg = 0, cs_nc = 0;
     DevBuf cs_AP, cs_AP32, cs_Ac, cs_Aci, cs_Aci32, cs_WT, cs_WT32, cs_status, cs_C, cs_T;
     DevBuf cs_PTA, cs_WTT;  // transposed system (adjoint solve), FP64
+    DevBuf cs_S;            // aggregate row sums of the FP16 copy (nc x vlen, FP64)
 
     // device: mesh
     DevBuf tile_start;
@@ -116,6 +117,8 @@ struct wb_ctx {
     // row-scaled FP16 copy of the FP32 vmult operator for the inner solves
     DevBuf A16, rs16;
     long h16_assembly_ver = -1, h16_mask_ver = -1;
+    int h16_agg = 0;        // aggregate size the FP16 copy was built for (0: none)
+    bool h16_sums = false;  // cs_S holds its aggregate sums
     int float_mode = 1;  // operator mode of the FP32-vector solves: 1 = FP32 hi, 3 = FP16
     // device: solver work
     DevBuf V, z, w, P1, P2, Pn, hbuf, hred, vx, vb, vr, vt, vs;   // double / raw
@@ -231,6 +234,14 @@ static void upload(DevBuf &b, const T *src, size_t count, hipStream_t s)
 static size_t elem_bytes(const wb_ctx *c) { return c->cfg.precision == WB_PREC_FP64 ? 8 : 4; }
 static void alloc_matrices(wb_ctx *c);
 static void ensure_h16(wb_ctx *c);
+
+// coarse aggregates: a multiple of 8 dofs, at least 64, at most precond_coarse of them
+static void coarse_shape(const wb_ctx *c, int &agg, int &nc)
+{
+    const int maxc = c->cfg.precond_coarse;
+    agg = std::max(64, 8 * ((c->n + 8 * maxc - 1) / (8 * maxc)));
+    nc = (c->n + agg - 1) / agg;
+}
 
 // ------------------------------------------------------------------ vector plumbing
 static void to_internal(const wb_ctx *c, const double *src, std::vector<double> &dst)
@@ -491,10 +502,8 @@ static void ensure_precond(wb_ctx *c)
     c->cs_on = false;
     c->cs_t_on = false;
     if (c->cfg.precond_coarse > 0) {
-        // aggregates of a multiple of 8 dofs, at least 64, at most precond_coarse of them
-        const int maxc = c->cfg.precond_coarse;
-        const int agg = std::max(64, 8 * ((c->n + 8 * maxc - 1) / (8 * maxc)));
-        const int nc = (c->n + agg - 1) / agg;
+        int agg = 0, nc = 0;
+        coarse_shape(c, agg, nc);
         c->cs_agg = agg;
         c->cs_nc = nc;
         c->cs_AP.ensure(sizeof(double) * (size_t)nc * c->vlen);
@@ -516,7 +525,9 @@ static void ensure_precond(wb_ctx *c)
                             from16 ? c->rs16.as<float>() : nullptr, c->alpha.as<double>(),
                             c->gmask.as<uint8_t>(),
                             c->constrained.as<uint8_t>(), c->cs_AP.as<double>(),
-                            c->cs_AP32.as<float>(), c->vlen, c->cs_Ac.as<double>(),
+                            c->cs_AP32.as<float>(), c->vlen,
+                            from16 && c->h16_sums ? c->cs_S.as<double>() : nullptr,
+                            c->cs_Ac.as<double>(),
                             c->cs_Aci.as<double>(), c->cs_Aci32.as<float>(), c->cs_WT.as<double>(),
                             c->cs_WT32.as<float>(), c->cs_status.as<int>(), c->stream);
         HIPCHK(hipGetLastError());
@@ -660,19 +671,28 @@ static double ir_inner_target(const wb_ctx *c, double rho)
     return std::max(red * rho, 0.5 * c->cfg.solver_tolerance);
 }
 
-// The FP16 inner operator for the current assembly and masks.
+// The FP16 inner operator for the current assembly and masks; with a coarse space
+// configured, also the sums of its rows over the coarse aggregates (cs_S) where the build
+// kernel holds whole rows in registers.
 static void ensure_h16(wb_ctx *c)
 {
-    if (c->h16_assembly_ver == c->assembly_ver && c->h16_mask_ver == c->mask_ver) return;
+    int agg = 0, nc = 0;
+    if (c->cfg.precond_coarse > 0) coarse_shape(c, agg, nc);
+    if (c->h16_assembly_ver == c->assembly_ver && c->h16_mask_ver == c->mask_ver &&
+        c->h16_agg == agg)
+        return;
     const size_t entries = (size_t)c->n * c->ld;
     c->A16.ensure(entries * 2);
     c->rs16.ensure(sizeof(float) * c->vlen);
-    launch_build_h16(c->G.as<float>(), c->H.as<float>(), c->ld, c->n, c->segs.as<int>(),
-                     c->n_segs, c->gmask.as<uint8_t>(), c->A16.p, c->rs16.as<float>(),
-                     c->stream);
+    if (agg > 0) c->cs_S.ensure(sizeof(double) * (size_t)nc * c->vlen);
+    c->h16_sums = launch_build_h16(c->G.as<float>(), c->H.as<float>(), c->ld, c->n,
+                                   c->segs.as<int>(), c->n_segs, c->gmask.as<uint8_t>(),
+                                   c->A16.p, c->rs16.as<float>(), agg, nc,
+                                   agg > 0 ? c->cs_S.as<double>() : nullptr, c->vlen, c->stream);
     HIPCHK(hipGetLastError());
     c->h16_assembly_ver = c->assembly_ver;
     c->h16_mask_ver = c->mask_ver;
+    c->h16_agg = agg;
 }
 
 // selects the operator of the FP32-vector solves for the lifetime of the guard
@@ -1391,8 +1411,8 @@ void wb_destroy(wb_ctx *c)
                       &c->con_ptr, &c->con_cols, &c->con_w, &c->con_inhom, &c->constrained,
                       &c->free_rows, &c->binv, &c->binv32, &c->cs_AP, &c->cs_AP32, &c->cs_Ac, &c->cs_Aci,
                       &c->cs_Aci32, &c->cs_WT, &c->cs_WT32, &c->cs_status, &c->cs_C,
-                      &c->cs_T, &c->cs_PTA, &c->cs_WTT, &c->V, &c->z, &c->w, &c->P1, &c->P2,
-                      &c->Pn, &c->hbuf, &c->hred, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
+                      &c->cs_T, &c->cs_PTA, &c->cs_WTT, &c->cs_S, &c->V, &c->z, &c->w, &c->P1,
+                      &c->P2, &c->Pn, &c->hbuf, &c->hred, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
                       &c->z32, &c->w32, &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
                       &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,
                       &c->vjp_rows, &c->vjp_cstate, &c->vjp_p1, &c->vjp_p2, &c->vjp_xg,
@@ -1766,7 +1786,7 @@ wb_status wb_release_matrices(wb_ctx *c)
         require(c->have_mesh, WB_E_STATE, "wb_release_matrices before wb_set_mesh");
         HIPCHK(hipSetDevice(c->cfg.device));
         HIPCHK(hipStreamSynchronize(c->stream));
-        for (DevBuf *b : {&c->G, &c->H, &c->Glo, &c->Hlo, &c->A16}) b->release();
+        for (DevBuf *b : {&c->G, &c->H, &c->Glo, &c->Hlo, &c->A16, &c->cs_S}) b->release();
         c->assembled = false;
     });
 }
@@ -2251,7 +2271,8 @@ size_t wb_device_bytes(const wb_ctx *c)
                             &c->con_w, &c->con_inhom, &c->constrained, &c->free_rows, &c->binv,
                             &c->binv32, &c->cs_AP, &c->cs_AP32, &c->cs_Ac, &c->cs_Aci,
                             &c->cs_Aci32, &c->cs_WT, &c->cs_WT32, &c->cs_status, &c->cs_C,
-                      &c->cs_T, &c->cs_PTA, &c->cs_WTT, &c->V, &c->z, &c->w, &c->P1, &c->P2, &c->Pn, &c->hbuf, &c->hred,
+                            &c->cs_T, &c->cs_PTA, &c->cs_WTT, &c->cs_S, &c->V, &c->z, &c->w,
+                            &c->P1, &c->P2, &c->Pn, &c->hbuf, &c->hred,
                             &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32, &c->z32, &c->w32,
                             &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
                             &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,

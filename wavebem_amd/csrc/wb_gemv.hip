@@ -338,12 +338,17 @@ __global__ __launch_bounds__(B16_THREADS) void k_build_h16(
 // Same scale and same FP16 values as k_build_h16, at one read of the FP32 entries instead
 // of two (the second pass of k_build_h16 misses L2 at these row lengths).
 constexpr int BR_THREADS = 1024;
+// With S non-null it also writes the row's sums over aggregates of agg4 consecutive
+// 4-column groups, S[a ldS + row] = rs_row * sum of the FP16 entries of aggregate a (the
+// rows of A P for a coarse space of nc aggregates), from the FP16 values in registers.
 template <int K>
 __global__ __launch_bounds__(BR_THREADS) void k_build_h16_reg(
     const float *__restrict__ G, const float *__restrict__ H, size_t ld, int n,
-    const uint8_t *__restrict__ gmask, _Float16 *__restrict__ A16, float *__restrict__ rs)
+    const uint8_t *__restrict__ gmask, _Float16 *__restrict__ A16, float *__restrict__ rs,
+    int agg4, int nc, double *__restrict__ S, size_t ldS)
 {
     __shared__ float red[BR_THREADS / 64];
+    __shared__ float gsum[K * BR_THREADS];
     const int row = blockIdx.x;
     const size_t off = (size_t)row * ld;
     const int ng = (n + 3) / 4;
@@ -393,11 +398,22 @@ __global__ __launch_bounds__(BR_THREADS) void k_build_h16_reg(
                 if (4 * g + 3 >= n) h.w = (_Float16)0.0f;
             }
             *((hvec4 *)(A16 + off) + g) = h;
+            if (S) gsum[g] = ((float)h.x + (float)h.y) + ((float)h.z + (float)h.w);
         }
     }
     for (int g = ng + (int)threadIdx.x; g < (int)(ld / 4); g += BR_THREADS)
         *((hvec4 *)(A16 + off) + g) = hvec4{0, 0, 0, 0};
     if (threadIdx.x == 0) rs[row] = sc;
+    if (!S) return;  // block-uniform
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    for (int a = (int)threadIdx.x >> 6; a < nc; a += BR_THREADS / 64) {
+        const int g0 = a * agg4, g1 = min(ng, g0 + agg4);
+        double f = 0.0;
+        for (int g = g0 + lane; g < g1; g += 64) f += (double)gsum[g];
+        f = wsum(f);
+        if (lane == 0) S[(size_t)a * ldS + row] = f * (double)sc;
+    }
 }
 
 // y_i = s_i sum_j A16_ij x_j + diag_i xd_i over the listed rows (FP32 vectors, FP32 sums
@@ -476,28 +492,33 @@ int resident_blocks(const void *kernel, int threads)
     return blocks;
 }
 
-void launch_build_h16(const float *G, const float *H, size_t ld, int n, const int *segs,
-                      int n_segs, const uint8_t *gmask, void *A16, float *rs, hipStream_t s)
+bool launch_build_h16(const float *G, const float *H, size_t ld, int n, const int *segs,
+                      int n_segs, const uint8_t *gmask, void *A16, float *rs, int agg, int nc,
+                      double *S, size_t ldS, hipStream_t s)
 {
+    const int agg4 = agg / 4;
     // rows of up to 64k columns are held in registers (read once); longer rows take the
     // two-pass kernel
     const int ng = (n + 3) / 4;
     _Float16 *A = (_Float16 *)A16;
     if (ng <= 4 * BR_THREADS)
         hipLaunchKernelGGL(k_build_h16_reg<4>, dim3(n), dim3(BR_THREADS), 0, s, G, H, ld, n,
-                           gmask, A, rs);
+                           gmask, A, rs, agg4, nc, S, ldS);
     else if (ng <= 8 * BR_THREADS)
         hipLaunchKernelGGL(k_build_h16_reg<8>, dim3(n), dim3(BR_THREADS), 0, s, G, H, ld, n,
-                           gmask, A, rs);
+                           gmask, A, rs, agg4, nc, S, ldS);
     else if (ng <= 12 * BR_THREADS)
         hipLaunchKernelGGL(k_build_h16_reg<12>, dim3(n), dim3(BR_THREADS), 0, s, G, H, ld, n,
-                           gmask, A, rs);
+                           gmask, A, rs, agg4, nc, S, ldS);
     else if (ng <= 16 * BR_THREADS)
         hipLaunchKernelGGL(k_build_h16_reg<16>, dim3(n), dim3(BR_THREADS), 0, s, G, H, ld, n,
-                           gmask, A, rs);
-    else
+                           gmask, A, rs, agg4, nc, S, ldS);
+    else {
         hipLaunchKernelGGL(k_build_h16, dim3(n), dim3(B16_THREADS), 0, s, G, H, ld, n,
                            (const int4 *)segs, n_segs, gmask, A, rs);
+        return false;  // no aggregate sums
+    }
+    return S != nullptr;
 }
 
 void launch_gemv_h16(const void *A16, const float *rs, size_t ld, int n, const float *x,

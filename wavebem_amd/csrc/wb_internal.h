@@ -118,9 +118,12 @@ void launch_gemv_sel(int mode, const void *G, const void *H, const void *Glo, co
 // scales, then the FP16 hi and lo halves of the vectors).
 int multi_chunks(int mode, int n_rows, int n_groups);
 // FP16 inner operator (wb_gemv.hip): A16 (n x ld halfs, row scales rs) from the FP32
-// column-selected vmult operator, and y = diag(rs) A16 x + diag .* xd over the listed rows
-void launch_build_h16(const float *G, const float *H, size_t ld, int n, const int *segs,
-                      int n_segs, const uint8_t *gmask, void *A16, float *rs, hipStream_t s);
+// column-selected vmult operator, and y = diag(rs) A16 x + diag .* xd over the listed rows.
+// With S non-null (agg a multiple of 4) the build may also write S[a ldS + i] = sum of row
+// i of diag(rs) A16 over the columns of aggregate a (a < nc); it returns whether it did.
+bool launch_build_h16(const float *G, const float *H, size_t ld, int n, const int *segs,
+                      int n_segs, const uint8_t *gmask, void *A16, float *rs, int agg, int nc,
+                      double *S, size_t ldS, hipStream_t s);
 void launch_gemv_h16(const void *A16, const float *rs, size_t ld, int n, const float *x,
                      const double *diag, const float *xd, const int *rows, int n_rows, float *y,
                      hipStream_t s);
@@ -152,8 +155,10 @@ void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, co
 
 // ---------------------------------------------------------------- coarse space (wb_coarse.hip)
 // Aggregates of agg consecutive device-order dofs (agg a multiple of 8, at least 64),
-// nc <= CS_MAX of them (the inverse is eliminated in FP64 up to CS_MAX64, in FP32 above).  Build: AP (nc x ldap, [a][j] = sum over aggregate a of the block-Jacobi matrix row
-// j; FP64 matrices, the FP32 hi part, or with A16 != nullptr the row-scaled FP16 copy),
+// nc <= CS_MAX of them (the inverse is eliminated in FP64 up to CS_MAX64, in FP32 above).
+// Build: AP (nc x ldap, [a][j] = sum over aggregate a of the block-Jacobi matrix row j;
+// FP64 matrices, the FP32 hi part, with A16 != nullptr the row-scaled FP16 copy, or with
+// S != nullptr the FP16 copy's aggregate sums written by launch_build_h16),
 // Ac = P^T AP, Aci = Ac^-1 (nc x nc row-major),
 // WT ([b][j] = (AP Aci)[j][b]), each in FP64 and FP32; status[0] = 0 when the inverse is
 // usable.  Apply (k vectors, column stride ldx): C[v] = P^T x_v (CS_MAX doubles per vector)
@@ -163,8 +168,8 @@ size_t coarse_invert_lds_bytes(int nc);
 void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
                          size_t ld, const void *A16, const float *rs16, const double *alpha,
                          const uint8_t *gmask, const uint8_t *constrained, double *AP,
-                         float *AP32, size_t ldap, double *Ac, double *Aci, float *Aci32,
-                         double *WT, float *WT32, int *status, hipStream_t s);
+                         float *AP32, size_t ldap, const double *S, double *Ac, double *Aci,
+                         float *Aci32, double *WT, float *WT32, int *status, hipStream_t s);
 void launch_coarse_pre(int is_float, int n, int agg, int nc, const void *X, size_t ldx,
                        const void *WT, size_t ldw, double *C, void *Tout, int k, hipStream_t s);
 // The transposed system (FP64): PTA = P^T A (nc x ldap, [a][i]) and WTT ([b][i] =
