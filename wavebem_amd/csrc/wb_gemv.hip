@@ -406,14 +406,18 @@ __global__ __launch_bounds__(BR_THREADS) void k_build_h16_reg(
     if (threadIdx.x == 0) rs[row] = sc;
     if (!S) return;  // block-uniform
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    for (int a = (int)threadIdx.x >> 6; a < nc; a += BR_THREADS / 64) {
-        const int g0 = a * agg4, g1 = min(ng, g0 + agg4);
-        double f = 0.0;
-        for (int g = g0 + lane; g < g1; g += 64) f += (double)gsum[g];
-        f = wsum(f);
-        if (lane == 0) S[(size_t)a * ldS + row] = f * (double)sc;
+    // all aggregates at once: tpa consecutive threads per aggregate (a power of two with
+    // nc tpa <= BR_THREADS), a strided partial sum each, then log2(tpa) shuffle steps
+    int tpa = 1;
+    while (tpa < 64 && 2 * tpa * nc <= BR_THREADS) tpa *= 2;
+    const int a = (int)threadIdx.x / tpa, q = (int)threadIdx.x % tpa;
+    double f = 0.0;
+    if (a < nc) {
+        const int g1 = min(ng, (a + 1) * agg4);
+        for (int g = a * agg4 + q; g < g1; g += tpa) f += (double)gsum[g];
     }
+    for (int o = 1; o < tpa; o <<= 1) f += __shfl_xor(f, o, 64);
+    if (a < nc && q == 0) S[(size_t)a * ldS + row] = f * (double)sc;
 }
 
 // y_i = s_i sum_j A16_ij x_j + diag_i xd_i over the listed rows (FP32 vectors, FP32 sums
@@ -496,6 +500,7 @@ bool launch_build_h16(const float *G, const float *H, size_t ld, int n, const in
                       int n_segs, const uint8_t *gmask, void *A16, float *rs, int agg, int nc,
                       double *S, size_t ldS, hipStream_t s)
 {
+    if (nc > BR_THREADS || agg % 4 != 0) S = nullptr;  // the sums cover one thread per aggregate
     const int agg4 = agg / 4;
     // rows of up to 64k columns are held in registers (read once); longer rows take the
     // two-pass kernel
