@@ -62,13 +62,16 @@ __device__ __forceinline__ void q1_shape(double xi, double eta, double (&phi)[4]
 }
 
 // In-place inverse of the bs x bs row-major matrix A held in LDS by Gauss-Jordan
-// elimination with partial pivoting, by one workgroup of THREADS (>= 64) threads.  colk
+// elimination with partial pivoting, by one workgroup of THREADS (>= 64, >= bs) threads.  colk
 // (bs doubles), piv (bs ints) and red_i (1 int) are LDS scratch.  Returns, in every
 // thread, the number of zero pivots met (0 for a regular matrix).
 template <int THREADS, typename T = double>
 __device__ int gj_invert_lds(T *A, T *colk, int *piv, int *red_i, int bs)
 {
     const int tid = threadIdx.x;
+    // elimination layout (bs <= THREADS): thread tid owns column ec = tid % bs of the rows
+    // ei, ei + rpp, ... (rpp = THREADS / bs rows per pass); the rest idle there
+    const int rpp = THREADS / bs, ec = tid % bs, ei = tid / bs;
     int zero_pivots = 0;
     for (int k = 0; k < bs; ++k) {
         // pivot: argmax |A[i][k]|, i >= k (ties to the smaller row), by the first wavefront
@@ -110,14 +113,16 @@ __device__ int gj_invert_lds(T *A, T *colk, int *piv, int *red_i, int bs)
         for (int c = tid; c < bs; c += THREADS) A[k * bs + c] = (c == k ? (T)1 : A[k * bs + c]) * inv;
         for (int i = tid; i < bs; i += THREADS) colk[i] = A[i * bs + k];  // before it changes
         __syncthreads();
-        for (int e = tid; e < bs * bs; e += THREADS) {
-            const int i = e / bs, c = e % bs;
-            if (i == k) continue;
-            const T f = colk[i];
-            if (c == k)
-                A[e] = -f * A[k * bs + k];
-            else
-                A[e] -= f * A[k * bs + c];
+        if (ei < rpp) {
+            const T rk = A[k * bs + ec];  // the scaled pivot row (1/d in column k)
+            for (int i = ei; i < bs; i += rpp) {
+                if (i == k) continue;
+                const T f = colk[i];
+                if (ec == k)
+                    A[i * bs + ec] = -f * rk;
+                else
+                    A[i * bs + ec] -= f * rk;
+            }
         }
         __syncthreads();
     }
