@@ -196,9 +196,13 @@ __global__ __launch_bounds__(CI_THREADS) void k_coarse_invert(int nc, const doub
 }
 
 // WT[b][i] = (AP Ac^-1)[i][b] = sum_a AP[a][i] Aci[a][b] (with trans: Aci[b][a], i.e.
-// Ac^-T, for the transposed system): thread per row i, 16 columns b per block
-// (blockIdx.y), so AP is read once per 16 columns; WT32 may be null
-constexpr int CW_TILE = 16;
+// Ac^-T, for the transposed system): thread per row i, CW_TILE columns b per block
+// (blockIdx.y), so AP is read once per CW_TILE columns; WT32 may be null.  At 41k rows and
+// 128 columns, 8 columns per block took 0.25 ms, 16 took 0.30 ms and 32 took 0.34 ms.
+#ifndef WB_CW_TILE
+#define WB_CW_TILE 8
+#endif
+constexpr int CW_TILE = WB_CW_TILE;
 __global__ __launch_bounds__(256) void k_coarse_w(int n, int nc, const double *__restrict__ AP,
                                                   size_t ldap, const double *__restrict__ Aci,
                                                   int trans, double *__restrict__ WT,
