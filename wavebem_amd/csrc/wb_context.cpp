@@ -1412,8 +1412,8 @@ void wb_destroy(wb_ctx *c)
                       &c->free_rows, &c->binv, &c->binv32, &c->cs_AP, &c->cs_AP32, &c->cs_Ac, &c->cs_Aci,
                       &c->cs_Aci32, &c->cs_WT, &c->cs_WT32, &c->cs_status, &c->cs_C,
                       &c->cs_T, &c->cs_PTA, &c->cs_WTT, &c->cs_S, &c->V, &c->z, &c->w, &c->P1,
-                      &c->P2, &c->Pn, &c->hbuf, &c->hred, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32,
-                      &c->z32, &c->w32, &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
+                      &c->P2, &c->Pn, &c->hbuf, &c->hred, &c->vx, &c->vb, &c->vr, &c->vt,
+                      &c->vs, &c->V32, &c->z32, &c->w32, &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
                       &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,
                       &c->vjp_rows, &c->vjp_cstate, &c->vjp_p1, &c->vjp_p2, &c->vjp_xg,
                       &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart, &c->inc_ptr, &c->inc_item,
