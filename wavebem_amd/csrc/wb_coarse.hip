@@ -34,7 +34,7 @@ __device__ __forceinline__ double cs_wave_sum(double v)
     return v;
 }
 
-constexpr int CR_THREADS = 256;   // k_coarse_rows: 4 wavefronts per row
+constexpr int CR_THREADS = 1024;  // k_coarse_rows: one row per block
 constexpr int CI_THREADS = 1024;  // k_coarse_invert
 
 typedef float cs_fvec4 __attribute__((ext_vector_type(4)));
@@ -42,11 +42,12 @@ typedef double cs_dvec2 __attribute__((ext_vector_type(2)));
 
 typedef _Float16 cs_hvec8 __attribute__((ext_vector_type(8)));
 
-// AP[a][j] = sum over the columns i of aggregate a of A(j, i): one row j per block;
-// wavefront w takes the aggregates a = w, w + 4, ..., 64 four-column groups per step
-// (gmask bit e of group g: column 4g + e comes from H), or, given the row-scaled FP16
-// copy of the selected operator (A16, row scales rs16), 64 eight-column groups of it per
-// step.  Padding columns past n are zero in G, H and A16.
+// AP[a][j] = sum over the columns i of aggregate a of A(j, i): one row j per block, tpa
+// consecutive threads per aggregate (a power of two with nc tpa <= CR_THREADS), each
+// summing every tpa-th four-column group of the aggregate (gmask bit e of group g: column
+// 4g + e comes from H) or, given the row-scaled FP16 copy of the selected operator (A16,
+// row scales rs16), every tpa-th eight-column group of it; log2(tpa) shuffle steps join
+// them.  Padding columns past n are zero in G, H and A16.
 __global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
     int n, int agg, int nc, const void *__restrict__ Gv, const void *__restrict__ Hv,
     int precision, size_t ld, const _Float16 *__restrict__ A16, const float *__restrict__ rs16,
@@ -55,75 +56,68 @@ __global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
     size_t ldap)
 {
     const int j = blockIdx.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const bool con = constrained[j] != 0;  // identity row
     const size_t roff = (size_t)j * ld;
-    for (int a = wave; a < nc; a += CR_THREADS / 64) {
-        const int c0 = a * agg, c1 = min(n, c0 + agg);
-        double s = 0.0;
-        if (con) {
-            s = (j >= c0 && j < c1) ? 1.0 : 0.0;
-        } else if (A16) {
-            // FP16 operator: 8 columns per lane (agg is a multiple of 8)
-            const int h0 = c0 / 8, h1 = (c1 + 7) / 8;
-            double f = 0.0;
-            for (int hb = h0; hb < h1; hb += 64) {
-                const int h = hb + lane;
-                float v = 0.0f;
-                if (h < h1) {
-                    const cs_hvec8 q = *((const cs_hvec8 *)(A16 + roff) + h);
-                    v = (((float)q[0] + (float)q[1]) + ((float)q[2] + (float)q[3])) +
-                        (((float)q[4] + (float)q[5]) + ((float)q[6] + (float)q[7]));
-                }
-                f += cs_wave_sum((double)v);
+    int tpa = 1;
+    while (tpa < 64 && 2 * tpa * nc <= CR_THREADS) tpa *= 2;
+    const int a = (int)threadIdx.x / tpa, q = (int)threadIdx.x % tpa;
+    const int c0 = a * agg, c1 = min(n, c0 + agg);
+    double s = 0.0;
+    if (a < nc && !con) {
+        if (A16) {
+            const int h1 = (c1 + 7) / 8;  // agg is a multiple of 8
+            for (int h = c0 / 8 + q; h < h1; h += tpa) {
+                const cs_hvec8 v = *((const cs_hvec8 *)(A16 + roff) + h);
+                s += (double)((((float)v[0] + (float)v[1]) + ((float)v[2] + (float)v[3])) +
+                              (((float)v[4] + (float)v[5]) + ((float)v[6] + (float)v[7])));
             }
-            s = f * (double)rs16[j];
-            if (j >= c0 && j < c1 && ((gmask[j >> 2] >> (j & 3)) & 1u)) s += alpha[j];
         } else {
-            const int g0 = c0 / 4, g1 = (c1 + 3) / 4;
-            for (int gb = g0; gb < g1; gb += 64) {
-                const int g = gb + lane;
-                double v = 0.0;
-                if (g < g1) {
-                    const unsigned m = gmask[g];
-                    double e[4];
-                    if (precision == WB_PREC_FP64) {
-                        const double *Gr = (const double *)Gv + roff + 4 * (size_t)g;
-                        const double *Hr = (const double *)Hv + roff + 4 * (size_t)g;
-                        cs_dvec2 g01 = {0, 0}, g23 = {0, 0}, h01 = {0, 0}, h23 = {0, 0};
-                        if (m != 15u) {
-                            g01 = *(const cs_dvec2 *)Gr;
-                            g23 = *((const cs_dvec2 *)Gr + 1);
-                        }
-                        if (m != 0u) {
-                            h01 = *(const cs_dvec2 *)Hr;
-                            h23 = *((const cs_dvec2 *)Hr + 1);
-                        }
-                        e[0] = (m & 1u) ? h01.x : -g01.x;
-                        e[1] = (m & 2u) ? h01.y : -g01.y;
-                        e[2] = (m & 4u) ? h23.x : -g23.x;
-                        e[3] = (m & 8u) ? h23.y : -g23.y;
-                    } else {
-                        // the FP32 hi part is accurate enough for a preconditioner
-                        cs_fvec4 gv = {0, 0, 0, 0}, hv = {0, 0, 0, 0};
-                        if (m != 15u) gv = *((const cs_fvec4 *)((const float *)Gv + roff) + g);
-                        if (m != 0u) hv = *((const cs_fvec4 *)((const float *)Hv + roff) + g);
-                        e[0] = (m & 1u) ? hv.x : -gv.x;
-                        e[1] = (m & 2u) ? hv.y : -gv.y;
-                        e[2] = (m & 4u) ? hv.z : -gv.z;
-                        e[3] = (m & 8u) ? hv.w : -gv.w;
+            const int g1 = (c1 + 3) / 4;
+            for (int g = c0 / 4 + q; g < g1; g += tpa) {
+                const unsigned m = gmask[g];
+                double e[4];
+                if (precision == WB_PREC_FP64) {
+                    const double *Gr = (const double *)Gv + roff + 4 * (size_t)g;
+                    const double *Hr = (const double *)Hv + roff + 4 * (size_t)g;
+                    cs_dvec2 g01 = {0, 0}, g23 = {0, 0}, h01 = {0, 0}, h23 = {0, 0};
+                    if (m != 15u) {
+                        g01 = *(const cs_dvec2 *)Gr;
+                        g23 = *((const cs_dvec2 *)Gr + 1);
                     }
-                    v = (e[0] + e[1]) + (e[2] + e[3]);
+                    if (m != 0u) {
+                        h01 = *(const cs_dvec2 *)Hr;
+                        h23 = *((const cs_dvec2 *)Hr + 1);
+                    }
+                    e[0] = (m & 1u) ? h01.x : -g01.x;
+                    e[1] = (m & 2u) ? h01.y : -g01.y;
+                    e[2] = (m & 4u) ? h23.x : -g23.x;
+                    e[3] = (m & 8u) ? h23.y : -g23.y;
+                } else {
+                    // the FP32 hi part is accurate enough for a preconditioner
+                    cs_fvec4 gv = {0, 0, 0, 0}, hv = {0, 0, 0, 0};
+                    if (m != 15u) gv = *((const cs_fvec4 *)((const float *)Gv + roff) + g);
+                    if (m != 0u) hv = *((const cs_fvec4 *)((const float *)Hv + roff) + g);
+                    e[0] = (m & 1u) ? hv.x : -gv.x;
+                    e[1] = (m & 2u) ? hv.y : -gv.y;
+                    e[2] = (m & 4u) ? hv.z : -gv.z;
+                    e[3] = (m & 8u) ? hv.w : -gv.w;
                 }
-                s += cs_wave_sum(v);
+                s += (e[0] + e[1]) + (e[2] + e[3]);
             }
+        }
+    }
+    for (int o = 1; o < tpa; o <<= 1) s += __shfl_xor(s, o, 64);
+    if (a < nc && q == 0) {
+        const bool mine = j >= c0 && j < c1;
+        if (con) {
+            s = mine ? 1.0 : 0.0;
+        } else {
+            if (A16) s *= (double)rs16[j];
             // the alpha diagonal of a Neumann column
-            if (j >= c0 && j < c1 && ((gmask[j >> 2] >> (j & 3)) & 1u)) s += alpha[j];
+            if (mine && ((gmask[j >> 2] >> (j & 3)) & 1u)) s += alpha[j];
         }
-        if (lane == 0) {
-            AP[(size_t)a * ldap + j] = s;
-            AP32[(size_t)a * ldap + j] = (float)s;
-        }
+        AP[(size_t)a * ldap + j] = s;
+        AP32[(size_t)a * ldap + j] = (float)s;
     }
 }
 
