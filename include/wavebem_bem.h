@@ -78,6 +78,10 @@ typedef struct {
                                       multiplicatively with the blocks (transposed in
                                       wb_solve_adjoint); changes iteration counts, not the
                                       solution */
+    int assembly_strips;           /* 1 (default): the regular-cell assembly sweeps the
+                                      logically structured patches of the mesh as strips of
+                                      grid lines (every cell evaluated about once); 0: column
+                                      tiles everywhere.  Same matrices either way */
 } wb_config;
 
 typedef struct {
@@ -329,6 +333,18 @@ int wb_host_mesh_plan(int n_dofs, int n_cells, const double *support_points,
                       const int32_t *cell_dofs, const int32_t *cell_material, int32_t *perm,
                       int32_t *dn_ptr, int32_t *dn_idx, int dn_cap, int32_t *n_tiles,
                       int32_t *n_tile_cells, int32_t *n_singular_pairs);
+
+/* No counterpart in the reference (its assembly loops over cells serially).  The strip
+ * schedule of the regular-cell assembly for inspection: sizes[0..4] = tasks, gaps, cell
+ * evaluations, strip dofs, column tiles.  With non-NULL buffers (of at least those sizes):
+ * tasks (2 ints each: first gap, gaps), gaps (4 ints: cell mask, first device column of
+ * the line completed by the gap, its position mask, 0), gap_cells (9 ints per gap: cell at
+ * each position or -1), cell_int_s (4 ints per cell: device vertex indices in the
+ * canonical order) and perm (dof -> device index).  strips = 0 plans column tiles only. */
+int wb_host_strip_plan(int n_dofs, int n_cells, const double *support_points,
+                       const int32_t *cell_dofs, const int32_t *cell_material, int strips,
+                       int32_t *sizes, int32_t *tasks, int32_t *gaps, int32_t *gap_cells,
+                       int32_t *cell_int_s, int32_t *perm);
 
 /* BEMProblem::compute_constraints (source/bem_problem.cc:990-1105) rules, closed as by
  * ConstraintMatrix::close; returns the number of lines, or minus a wb_status. */

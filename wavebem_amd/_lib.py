@@ -37,6 +37,7 @@ class WbConfig(ctypes.Structure):
         ("ir_inner_format", ctypes.c_int),
         ("ir_inner16_reduction", ctypes.c_double),
         ("precond_coarse", ctypes.c_int),
+        ("assembly_strips", ctypes.c_int),
     ]
 
 
@@ -125,6 +126,9 @@ _SIGS = {
     "wb_host_quadrature": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
     "wb_host_mesh_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _i32p, _i32p, _i32p,
                                          _i32p, _i32p, ctypes.c_int, _i32p, _i32p, _i32p]),
+    "wb_host_strip_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _i32p, _i32p,
+                                          ctypes.c_int, _i32p, _i32p, _i32p, _i32p, _i32p,
+                                          _i32p]),
     "wb_host_compute_constraints": (ctypes.c_int, [ctypes.c_int, _i32p, _i32p, _dp, _dp, _dp,
                                                    _dp, ctypes.c_int, ctypes.c_int, _i32p, _i32p,
                                                    _i32p, _dp, _dp]),

@@ -63,6 +63,7 @@ class Config:
     ir_inner_format: int = 1
     ir_inner16_reduction: float = 3e-4
     precond_coarse: int = 128  # coarse-space unknowns of the block-Jacobi preconditioner
+    assembly_strips: int = 1  # strip sweeps on structured patches (0: column tiles only)
 
     def to_c(self) -> L.WbConfig:
         c = L.WbConfig()
@@ -83,6 +84,7 @@ class Config:
         c.ir_inner_format = self.ir_inner_format
         c.ir_inner16_reduction = self.ir_inner16_reduction
         c.precond_coarse = self.precond_coarse
+        c.assembly_strips = self.assembly_strips
         return c
 
 
@@ -395,6 +397,31 @@ def host_quadrature(q_order=4, s_order=5):
     if st != L.WB_OK:
         raise WaveBemError(st, "wb_host_quadrature")
     return rp, rw, sp, sw
+
+
+def host_strip_plan(mesh: SurfaceMesh, strips: int = 1):
+    """The strip schedule of the regular-cell assembly (wb_host_strip_plan)."""
+    n, nc = mesh.n_dofs, mesh.n_cells
+    X = _f64(mesh.support_points).reshape(-1)
+    cd = np.ascontiguousarray(mesh.cell_dofs, dtype=np.int32).reshape(-1)
+    mat = np.ascontiguousarray(mesh.cell_material, dtype=np.int32)
+    sizes = np.zeros(5, dtype=np.int32)
+    st = L.lib().wb_host_strip_plan(n, nc, _d(X), _i(cd), _i(mat), strips, _i(sizes), None, None,
+                                    None, None, None)
+    if st != L.WB_OK:
+        raise WaveBemError(st, "wb_host_strip_plan")
+    nt, ng = int(sizes[0]), int(sizes[1])
+    tasks = np.zeros((max(nt, 1), 2), dtype=np.int32)
+    gaps = np.zeros((max(ng, 1), 4), dtype=np.int32)
+    gcells = np.zeros((max(ng, 1), 9), dtype=np.int32)
+    cis = np.zeros((nc, 4), dtype=np.int32)
+    perm = np.zeros(n, dtype=np.int32)
+    st = L.lib().wb_host_strip_plan(n, nc, _d(X), _i(cd), _i(mat), strips, _i(sizes), _i(tasks),
+                                    _i(gaps), _i(gcells), _i(cis), _i(perm))
+    if st != L.WB_OK:
+        raise WaveBemError(st, "wb_host_strip_plan")
+    return dict(tasks=tasks[:nt], gaps=gaps[:ng], gap_cells=gcells[:ng], cell_int_s=cis,
+                perm=perm, cells=int(sizes[2]), strip_dofs=int(sizes[3]), n_tiles=int(sizes[4]))
 
 
 def host_mesh_plan(mesh: SurfaceMesh):

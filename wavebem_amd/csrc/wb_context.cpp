@@ -95,6 +95,9 @@ struct wb_ctx {
     DevBuf tile_start;
     DevBuf X, cell_int, orient, cell_cls, row_cls, tile_ptr, tile_cell, tile_cols, sing_ptr,
         sing_cell, sing_vtx, qpts, qw, srule, cellq, perm_d, iperm_d;
+    // strip schedule of the regular-cell assembly and its cell records in gap order
+    // (canonical vertex order of the structured patches)
+    DevBuf st_task, st_gap, st_gcell, cell_int_s, orient_s, cellq_s;
     // device: operator
     DevBuf G, H, Glo, Hlo, hsum_part, hsum_sing, alpha;
     DevBuf surf_d, gmask, diag_op, diag_rhs, segs;
@@ -151,6 +154,26 @@ struct wb_ctx {
                     &X, &B, &R, &T, &D32, &R32, &act, &inh, &part, &xs};
         }
     } mw;
+
+    // every device buffer of the context
+    std::vector<DevBuf *> all_bufs()
+    {
+        std::vector<DevBuf *> v = {
+            &X, &cell_int, &orient, &cell_cls, &row_cls, &tile_ptr, &tile_cell, &tile_cols,
+            &tile_start, &sing_ptr, &sing_cell, &sing_vtx, &qpts, &qw, &srule, &cellq, &perm_d,
+            &iperm_d, &st_task, &st_gap, &st_gcell, &cell_int_s, &orient_s, &cellq_s, &G, &H,
+            &Glo, &Hlo, &hsum_part, &hsum_sing, &alpha, &surf_d, &gmask, &segs, &diag_op,
+            &diag_rhs, &con_rows, &con_ptr, &con_cols, &con_w, &con_inhom, &constrained,
+            &free_rows, &binv, &binv32, &cs_AP, &cs_AP32, &cs_Ac, &cs_Aci, &cs_Aci32, &cs_WT,
+            &cs_WT32, &cs_status, &cs_C, &cs_T, &cs_PTA, &cs_WTT, &cs_S, &V, &z, &w, &P1, &P2,
+            &Pn, &hbuf, &hred, &vx, &vb, &vr, &vt, &vs, &V32, &z32, &w32, &r32, &d32, &ct_tgt,
+            &ct_ptr, &ct_src, &ct_coef, &gt_part, &vjp_w, &vjp_phi, &vjp_psi, &vjp_rows,
+            &vjp_cstate, &vjp_p1, &vjp_p2, &vjp_xg, &vjp_vbar, &vjp_xs, &vjp_spart, &inc_ptr,
+            &inc_item, &pj_emass, &pj_erhs, &pj_u, &pj_B, &pj_X, &pj_R, &pj_Z, &pj_P, &pj_Q,
+            &pj_dinv, &pj_part, &pj_part2, &pj_state, &A16, &rs16};
+        for (DevBuf *b : mw.all()) v.push_back(b);
+        return v;
+    }
 };
 
 // Brackets one launch with a pair of events when profiling is on.
@@ -232,6 +255,32 @@ static void upload(DevBuf &b, const T *src, size_t count, hipStream_t s)
 }
 
 static size_t elem_bytes(const wb_ctx *c) { return c->cfg.precision == WB_PREC_FP64 ? 8 : 4; }
+
+// Row-sum partials of H written by the regular-cell kernels: one per strip task, then one
+// per group of TILES_PER_WG column tiles.
+static int hsum_parts(const wb_ctx *c)
+{
+    return c->plan.n_tasks + (c->plan.n_tiles + TILES_PER_WG - 1) / TILES_PER_WG;
+}
+
+// Quadrature-point records at the current support points: cellq for every cell in its
+// own vertex order (singular correction, adjoint, tangent) and cellq_s for every gap
+// position of the strip schedule, in the canonical vertex order.
+static void cell_records(wb_ctx *c)
+{
+    const int nc = c->plan.nc, nq = c->quad.q_order * c->quad.q_order;
+    launch_cell_geometry(c->X.as<double>(), c->cell_int.as<int>(), c->orient.as<int8_t>(), nc,
+                         nq, c->qpts.as<double>(), c->qw.as<double>(), c->cellq.as<double>(),
+                         c->stream);
+    HIPCHK(hipGetLastError());
+    if (c->plan.n_tasks > 0) {
+        launch_cell_geometry(c->X.as<double>(), c->cell_int_s.as<int>(),
+                             c->orient_s.as<int8_t>(), (int)c->plan.st_gcell.size(), nq,
+                             c->qpts.as<double>(), c->qw.as<double>(), c->cellq_s.as<double>(),
+                             c->stream, c->st_gcell.as<int>());
+        HIPCHK(hipGetLastError());
+    }
+}
 static void alloc_matrices(wb_ctx *c);
 static void ensure_h16(wb_ctx *c);
 
@@ -1355,6 +1404,7 @@ void wb_config_default(wb_config *cfg)
     cfg->ir_inner_format = 1;
     cfg->ir_inner16_reduction = 3e-4;
     cfg->precond_coarse = CS_MAX64;
+    cfg->assembly_strips = 1;
 }
 
 const char *wb_version(void) { return "wavebem_amd 0.1 (gfx950)"; }
@@ -1379,6 +1429,7 @@ wb_status wb_create(wb_ctx **out, const wb_config *cfg)
         (k.ir_inner_format != 0 && k.ir_inner_format != 1) ||
         !(k.ir_inner16_reduction > 0 && k.ir_inner16_reduction < 1) ||
         k.precond_coarse < 0 || k.precond_coarse > CS_MAX ||
+        (k.assembly_strips != 0 && k.assembly_strips != 1) ||
         !(k.solver_tolerance >= 0)) {
         delete c;
         return WB_E_ARG;
@@ -1402,26 +1453,7 @@ void wb_destroy(wb_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->X, &c->cell_int, &c->orient, &c->cell_cls, &c->row_cls, &c->tile_ptr,
-                      &c->tile_cell, &c->tile_cols, &c->tile_start, &c->sing_ptr, &c->sing_cell,
-                      &c->sing_vtx,
-                      &c->qpts, &c->qw, &c->srule, &c->cellq, &c->perm_d, &c->iperm_d, &c->G,
-                      &c->H, &c->Glo, &c->Hlo, &c->hsum_part, &c->hsum_sing, &c->alpha,
-                      &c->surf_d, &c->gmask, &c->segs, &c->diag_op, &c->diag_rhs, &c->con_rows,
-                      &c->con_ptr, &c->con_cols, &c->con_w, &c->con_inhom, &c->constrained,
-                      &c->free_rows, &c->binv, &c->binv32, &c->cs_AP, &c->cs_AP32, &c->cs_Ac, &c->cs_Aci,
-                      &c->cs_Aci32, &c->cs_WT, &c->cs_WT32, &c->cs_status, &c->cs_C,
-                      &c->cs_T, &c->cs_PTA, &c->cs_WTT, &c->cs_S, &c->V, &c->z, &c->w, &c->P1,
-                      &c->P2, &c->Pn, &c->hbuf, &c->hred, &c->vx, &c->vb, &c->vr, &c->vt,
-                      &c->vs, &c->V32, &c->z32, &c->w32, &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
-                      &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,
-                      &c->vjp_rows, &c->vjp_cstate, &c->vjp_p1, &c->vjp_p2, &c->vjp_xg,
-                      &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart, &c->inc_ptr, &c->inc_item,
-                      &c->pj_emass, &c->pj_erhs, &c->pj_u, &c->pj_B, &c->pj_X, &c->pj_R,
-                      &c->pj_Z, &c->pj_P, &c->pj_Q, &c->pj_dinv, &c->pj_part, &c->pj_part2,
-                      &c->pj_state, &c->A16, &c->rs16};
-    for (DevBuf *b : bufs) b->release();
-    for (DevBuf *b : c->mw.all()) b->release();
+    for (DevBuf *b : c->all_bufs()) b->release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->hpin) (void)hipHostFree(c->hpin);
     if (c->mw.hpin) (void)hipHostFree(c->mw.hpin);
@@ -1447,7 +1479,7 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         HIPCHK(hipSetDevice(c->cfg.device));
         MeshPlan P;
         std::string e = plan_mesh(n, nc, support, cell_dofs, cell_material, dn_ptr, dn_idx,
-                                  c->cfg.assembly_tile, P);
+                                  c->cfg.assembly_tile, c->cfg.assembly_strips, P);
         require(e.empty(), WB_E_ARG, e.c_str());
         c->have_mesh = c->have_masks = c->assembled = c->have_cons = false;
         c->plan = std::move(P);
@@ -1473,6 +1505,17 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         upload(c->tile_cell, p.tile_cell.data(), p.tile_cell.size(), s);
         upload(c->tile_cols, p.tile_cols.data(), p.tile_cols.size(), s);
         upload(c->tile_start, p.tile_start.data(), p.tile_start.size(), s);
+        if (p.n_tasks > 0) {
+            upload(c->st_task, p.st_task.data(), p.st_task.size(), s);
+            upload(c->st_gap, p.st_gap.data(), p.st_gap.size(), s);
+            upload(c->st_gcell, p.st_gcell.data(), p.st_gcell.size(), s);
+        }
+        upload(c->cell_int_s, p.cell_int_s.data(), p.cell_int_s.size(), s);
+        {
+            std::vector<int8_t> os(nc);
+            for (int k = 0; k < nc; ++k) os[k] = (int8_t)(cell_orient[k] * p.canon_sign[k]);
+            upload(c->orient_s, os.data(), os.size(), s);
+        }
         upload(c->sing_ptr, p.sing_ptr.data(), p.sing_ptr.size(), s);
         upload(c->sing_cell, p.sing_cell.data(), p.sing_cell.size(), s);
         upload(c->sing_vtx, p.sing_vtx.data(), p.sing_vtx.size(), s);
@@ -1503,13 +1546,10 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         upload(c->srule, rule.data(), rule.size(), s);
         const int nq = q.q_order * q.q_order;
         c->cellq.ensure(sizeof(double) * 8 * (size_t)nq * nc);
-        launch_cell_geometry(c->X.as<double>(), c->cell_int.as<int>(), c->orient.as<int8_t>(), nc,
-                             nq, c->qpts.as<double>(), c->qw.as<double>(), c->cellq.as<double>(),
-                             s);
-        HIPCHK(hipGetLastError());
+        c->cellq_s.ensure(sizeof(double) * 8 * (size_t)nq * (p.st_gcell.size() + 1));
+        cell_records(c);
         alloc_matrices(c);
-        const int ntg = (p.n_tiles + TILES_PER_WG - 1) / TILES_PER_WG;
-        c->hsum_part.ensure(sizeof(double) * (size_t)ntg * n);
+        c->hsum_part.ensure(sizeof(double) * (size_t)hsum_parts(c) * n);
         c->hsum_sing.ensure(sizeof(double) * n);
         c->alpha.ensure(sizeof(double) * c->vlen);
         HIPCHK(hipMemsetAsync(c->alpha.p, 0, c->alpha.bytes, s));
@@ -1547,11 +1587,7 @@ wb_status wb_set_support_points(wb_ctx *c, const double *support)
         for (int d = 0; d < n; ++d)
             for (int k = 0; k < 3; ++k) Xi[3 * c->plan.perm[d] + k] = support[3 * d + k];
         upload(c->X, Xi.data(), Xi.size(), c->stream);
-        const int nq = c->quad.q_order * c->quad.q_order;
-        launch_cell_geometry(c->X.as<double>(), c->cell_int.as<int>(), c->orient.as<int8_t>(), nc,
-                             nq, c->qpts.as<double>(), c->qw.as<double>(), c->cellq.as<double>(),
-                             c->stream);
-        HIPCHK(hipGetLastError());
+        cell_records(c);
         HIPCHK(hipStreamSynchronize(c->stream));
         c->assembled = false;
     });
@@ -1750,13 +1786,20 @@ wb_status wb_assemble(wb_ctx *c)
         for (int i = 0; i < c->quad.q_order; ++i) nodes.phi[i] = c->quad.reg_pts[2 * i];
         {
             KernelScope ks(c, WB_K_ASSEMBLE, c->n);
-            launch_assemble_tiles(c->cfg.precision, c->quad.q_order, p.tc, c->X.as<double>(),
-                                  c->row_cls.as<int>(), c->cellq.as<double>(),
-                                  c->cell_cls.as<int>(), c->tile_ptr.as<int>(),
-                                  c->tile_cell.as<int>(), c->tile_cols.as<int>(),
-                                  c->tile_start.as<int>(), c->n,
-                                  p.n_tiles, c->ld, nodes, c->G.p, c->H.p, c->Glo.p, c->Hlo.p,
-                                  c->hsum_part.as<double>(), c->stream);
+            launch_assemble_strips(c->cfg.precision, c->quad.q_order, c->X.as<double>(),
+                                   c->cellq_s.as<double>(), c->st_task.as<int>(),
+                                   c->st_gap.as<int>(), c->n, p.n_tasks, c->ld, nodes, c->G.p,
+                                   c->H.p, c->Glo.p, c->Hlo.p, c->hsum_part.as<double>(),
+                                   c->stream);
+            HIPCHK(hipGetLastError());
+            if (p.n_tiles > 0)
+                launch_assemble_tiles(c->cfg.precision, c->quad.q_order, p.tc, c->X.as<double>(),
+                                      c->cellq.as<double>(), c->tile_ptr.as<int>(),
+                                      c->tile_cell.as<int>(), c->tile_cols.as<int>(),
+                                      c->tile_start.as<int>(), c->n, p.n_tiles, c->ld, nodes,
+                                      c->G.p, c->H.p, c->Glo.p, c->Hlo.p,
+                                      c->hsum_part.as<double>() + (size_t)p.n_tasks * c->n,
+                                      c->stream);
             HIPCHK(hipGetLastError());
         }
         const int ns = 2 * c->quad.s_order * c->quad.s_order;
@@ -1765,13 +1808,14 @@ wb_status wb_assemble(wb_ctx *c)
             launch_assemble_singular(c->cfg.precision, c->X.as<double>(), c->cell_int.as<int>(),
                                      c->orient.as<int8_t>(), c->sing_ptr.as<int>(),
                                      c->sing_cell.as<int>(), c->sing_vtx.as<int>(), c->n, ns,
-                                     c->srule.as<double>(), c->ld, c->G.p, c->H.p, c->Glo.p,
-                                     c->Hlo.p, c->hsum_sing.as<double>(), c->stream);
+                                     c->srule.as<double>(), c->cellq.as<double>(),
+                                     c->quad.q_order * c->quad.q_order, c->qpts.as<double>(),
+                                     c->ld, c->G.p, c->H.p, c->Glo.p, c->Hlo.p,
+                                     c->hsum_sing.as<double>(), c->stream);
             HIPCHK(hipGetLastError());
         }
-        const int ntg = (p.n_tiles + TILES_PER_WG - 1) / TILES_PER_WG;
-        launch_alpha_reduce(c->hsum_part.as<double>(), ntg, c->hsum_sing.as<double>(), c->n,
-                            c->alpha.as<double>(), c->stream);
+        launch_alpha_reduce(c->hsum_part.as<double>(), hsum_parts(c), c->hsum_sing.as<double>(),
+                            c->n, c->alpha.as<double>(), c->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
         c->assembled = true;
@@ -2260,29 +2304,7 @@ size_t wb_device_bytes(const wb_ctx *c)
 {
     if (!c) return 0;
     size_t t = 0;
-    const DevBuf *bufs[] = {&c->X, &c->cell_int, &c->orient, &c->cell_cls, &c->row_cls,
-                            &c->tile_ptr, &c->tile_cell, &c->tile_cols, &c->tile_start,
-                            &c->sing_ptr,
-                            &c->sing_cell, &c->sing_vtx, &c->qpts, &c->qw, &c->srule, &c->cellq,
-                            &c->perm_d, &c->iperm_d, &c->G, &c->H, &c->Glo, &c->Hlo,
-                            &c->hsum_part, &c->hsum_sing, &c->alpha, &c->surf_d, &c->gmask,
-                            &c->segs,
-                            &c->diag_op, &c->diag_rhs, &c->con_rows, &c->con_ptr, &c->con_cols,
-                            &c->con_w, &c->con_inhom, &c->constrained, &c->free_rows, &c->binv,
-                            &c->binv32, &c->cs_AP, &c->cs_AP32, &c->cs_Ac, &c->cs_Aci,
-                            &c->cs_Aci32, &c->cs_WT, &c->cs_WT32, &c->cs_status, &c->cs_C,
-                            &c->cs_T, &c->cs_PTA, &c->cs_WTT, &c->cs_S, &c->V, &c->z, &c->w,
-                            &c->P1, &c->P2, &c->Pn, &c->hbuf, &c->hred,
-                            &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V32, &c->z32, &c->w32,
-                            &c->r32, &c->d32, &c->ct_tgt, &c->ct_ptr, &c->ct_src,
-                            &c->ct_coef, &c->gt_part, &c->vjp_w, &c->vjp_phi, &c->vjp_psi,
-                            &c->vjp_rows, &c->vjp_cstate, &c->vjp_p1, &c->vjp_p2, &c->vjp_xg,
-                            &c->vjp_vbar, &c->vjp_xs, &c->vjp_spart, &c->inc_ptr,
-                            &c->inc_item, &c->pj_emass, &c->pj_erhs, &c->pj_u, &c->pj_B,
-                            &c->pj_X, &c->pj_R, &c->pj_Z, &c->pj_P, &c->pj_Q, &c->pj_dinv,
-                            &c->pj_part, &c->pj_part2, &c->pj_state, &c->A16, &c->rs16};
-    for (const DevBuf *b : bufs) t += b->bytes;
-    for (DevBuf *b : const_cast<wb_ctx *>(c)->mw.all()) t += b->bytes;
+    for (const DevBuf *b : const_cast<wb_ctx *>(c)->all_bufs()) t += b->bytes;
     return t;
 }
 

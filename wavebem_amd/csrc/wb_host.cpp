@@ -30,7 +30,8 @@ int wb_host_mesh_plan(int n, int nc, const double *support, const int32_t *cell_
 {
     if (n <= 0 || nc <= 0 || !support || !cell_dofs) return WB_E_ARG;
     wb::MeshPlan P;
-    std::string e = wb::plan_mesh(n, nc, support, cell_dofs, cell_material, nullptr, nullptr, wb::TILE_COLS, P);
+    std::string e = wb::plan_mesh(n, nc, support, cell_dofs, cell_material, nullptr, nullptr,
+                                  wb::TILE_COLS, 1, P);
     if (!e.empty()) return WB_E_ARG;
     if (perm) std::copy(P.perm.begin(), P.perm.end(), perm);
     if (dn_ptr) std::copy(P.dn_ptr.begin(), P.dn_ptr.end(), dn_ptr);
@@ -38,9 +39,31 @@ int wb_host_mesh_plan(int n, int nc, const double *support, const int32_t *cell_
         if ((int)P.dn_idx.size() > dn_cap) return WB_E_ARG;
         std::copy(P.dn_idx.begin(), P.dn_idx.end(), dn_idx);
     }
-    if (n_tiles) *n_tiles = P.n_tiles;
-    if (n_tile_cells) *n_tile_cells = (int32_t)P.tile_cell.size();
+    if (n_tiles) *n_tiles = P.n_tasks + P.n_tiles;
+    if (n_tile_cells) *n_tile_cells = (int32_t)(P.strip_cells + (long)P.tile_cell.size());
     if (n_singular_pairs) *n_singular_pairs = (int32_t)P.sing_cell.size();
+    return WB_OK;
+}
+
+int wb_host_strip_plan(int n, int nc, const double *support, const int32_t *cell_dofs,
+                       const int32_t *cell_material, int strips, int32_t *sizes, int32_t *tasks,
+                       int32_t *gaps, int32_t *gap_cells, int32_t *cell_int_s, int32_t *perm)
+{
+    if (n <= 0 || nc <= 0 || !support || !cell_dofs || !sizes) return WB_E_ARG;
+    wb::MeshPlan P;
+    std::string e = wb::plan_mesh(n, nc, support, cell_dofs, cell_material, nullptr, nullptr,
+                                  wb::TILE_COLS, strips ? 1 : 0, P);
+    if (!e.empty()) return WB_E_ARG;
+    sizes[0] = P.n_tasks;
+    sizes[1] = (int32_t)(P.st_gap.size() / 4);
+    sizes[2] = (int32_t)P.strip_cells;
+    sizes[3] = P.strip_dofs;
+    sizes[4] = P.n_tiles;
+    if (tasks) std::copy(P.st_task.begin(), P.st_task.end(), tasks);
+    if (gaps) std::copy(P.st_gap.begin(), P.st_gap.end(), gaps);
+    if (gap_cells) std::copy(P.st_gcell.begin(), P.st_gcell.end(), gap_cells);
+    if (cell_int_s) std::copy(P.cell_int_s.begin(), P.cell_int_s.end(), cell_int_s);
+    if (perm) std::copy(P.perm.begin(), P.perm.end(), perm);
     return WB_OK;
 }
 

@@ -44,6 +44,18 @@ struct Quadrature {
 void build_quadrature(int q_order, int s_order, Quadrature &q);
 void gauss_legendre_01(int n, double *x, double *w);
 
+// Strip schedule of the regular-cell assembly on logically structured patches.  A line
+// is at most STRIP_W dofs of one grid row, at consecutive positions; a strip is the lines
+// of one band of positions over consecutive grid rows ("levels"); a task is a run of
+// levels of one strip, swept by one wavefront per 64 collocation rows.  The cells of a
+// task lie between consecutive levels, so two lines at a time hold accumulators: the
+// ring of 2 STRIP_W slots, line l in half l & 1.
+constexpr int STRIP_W = 8;
+#ifndef WB_STRIP_TASK_ENTRIES
+#define WB_STRIP_TASK_ENTRIES 640
+#endif
+constexpr int STRIP_TASK_ENTRIES = WB_STRIP_TASK_ENTRIES;  // cells per task, about
+
 struct MeshPlan {
     int n = 0, nc = 0;
     size_t ld = 0;
@@ -56,11 +68,24 @@ struct MeshPlan {
     int n_tiles = 0;
     int tc = TILE_COLS;                    // columns per tile
     std::vector<int> dn_ptr, dn_idx;       // caller-numbered double node sets
+    // strips (structured patches): tasks {first gap, gaps}; gap t (task-local -1 .. L-1,
+    // L lines) = the cells between task lines t and t + 1 (STRIP_W + 1 positions: cell k
+    // touches line positions k - 1 and k) as {cell mask, first column of line t, position
+    // mask of line t (0 for t = -1), 0}, and their cell ids st_gcell (STRIP_W + 1 per gap,
+    // -1 = none).  Cells of structured patches have their vertices in the canonical order
+    // cell_int_s (position fastest, then level), canon_sign = the orientation change of
+    // that reordering; elsewhere cell_int_s = cell_int, canon_sign = 1.
+    int n_tasks = 0;
+    long strip_cells = 0;                  // cell evaluations of all tasks
+    std::vector<int> st_task, st_gap, st_gcell;
+    std::vector<int> cell_int_s;
+    std::vector<int8_t> canon_sign;
+    int strip_dofs = 0;                    // dofs covered by strips (device order [0, this))
 };
 
 std::string plan_mesh(int n, int nc, const double *support, const int32_t *cell_dofs,
                       const int32_t *cell_material, const int32_t *dn_ptr, const int32_t *dn_idx,
-                      int tile_cols, MeshPlan &plan);
+                      int tile_cols, int strips, MeshPlan &plan);
 
 // Constraint rules of BEMProblem::compute_constraints, caller numbering, closed.
 struct ConstraintSet {
@@ -72,19 +97,30 @@ std::string build_constraints(int n, const int *dn_ptr, const int *dn_idx, const
                               ConstraintSet &out);
 
 // ---------------------------------------------------------------- kernel launchers
+// Records of n_slots cells: slot s holds cell s, or cell slot_cell[s] when slot_cell is
+// given (slots with -1 are left untouched).
 void launch_cell_geometry(const double *support_int, const int *cell_int, const int8_t *orient,
-                          int nc, int nq, const double *qpts, const double *qw, double *cellq,
-                          hipStream_t s);
+                          int n_slots, int nq, const double *qpts, const double *qw,
+                          double *cellq, hipStream_t s, const int *slot_cell = nullptr);
 void launch_assemble_tiles(int precision, int q_order, int tc, const double *xrow,
-                           const int *row_cls,
-                           const double *cellq, const int *cell_cls, const int *tile_ptr,
-                           const int *tile_cell, const int *tile_cols, const int *tile_start,
-                           int n, int n_tiles,
+                           const double *cellq, const int *tile_ptr, const int *tile_cell,
+                           const int *tile_cols, const int *tile_start, int n, int n_tiles,
                            size_t ld, const PhiTable &phi, void *G, void *H, void *Glo,
                            void *Hlo, double *hsum_part, hipStream_t s);
+// Strip tasks (MeshPlan::st_*): recs holds the cell records of every gap position
+// (gap g, position k at record g * (STRIP_W + 1) + k), built from cell_int_s;
+// hsum_part rows [0, n_tasks) receive the tasks' row sums of H.
+void launch_assemble_strips(int precision, int q_order, const double *xrow, const double *recs,
+                            const int *tasks, const int *gaps, int n, int n_tasks, size_t ld,
+                            const PhiTable &phi, void *G, void *H, void *Glo, void *Hlo,
+                            double *hsum_part, hipStream_t s);
+// The regular kernels add every cell with the regular rule, singular ones included; this
+// kernel replaces, per singular (row, cell) pair, that regular part (recomputed from the
+// records cellq, nq points at the reference points qpts) by the QGaussOneOverR integral.
 void launch_assemble_singular(int precision, const double *support_int, const int *cell_int,
                               const int8_t *orient, const int *sing_ptr, const int *sing_cell,
                               const int *sing_vtx, int n, int ns, const double *sing_rule,
+                              const double *cellq, int nq, const double *qpts,
                               size_t ld, void *G, void *H, void *Glo, void *Hlo,
                               double *hsum_sing, hipStream_t s);
 // L2 projections onto the Q1 space (wb_project.hip); vectors [n][6]: normal x,y,z and

@@ -9,6 +9,8 @@
 //  * the constraint rules of BEMProblem::compute_constraints
 //    (source/bem_problem.cc:990-1105) closed like deal.II's ConstraintMatrix::close.
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -178,8 +180,197 @@ static void double_nodes(int n, const double *X, std::vector<int> &ptr, std::vec
     }
 }
 
+// ------------------------------------------------------------------- strip planning
+namespace {
+
+// Integer grid coordinates (u, v) of the dofs of one mesh component whose cells form a
+// logically structured quad grid: no cell repeats a dof, an edge belongs to at most two
+// cells, the coordinates carried across shared edges agree at every vertex, every cell is
+// reached through edges and is a unit square.  Returns false otherwise (irregular
+// vertices, a grid closing on itself, cells meeting only at a corner).
+bool grid_coords(const std::vector<int> &cells, const int32_t *cd, std::vector<int> &gu,
+                 std::vector<int> &gv, std::vector<int> &mark, int stamp)
+{
+    static const int cyc[4] = {0, 1, 3, 2};  // cyclic vertex order of a deal.II quad
+    std::unordered_map<uint64_t, std::pair<int, int>> edges;
+    edges.reserve(4 * cells.size());
+    auto ekey = [](int a, int b) {
+        if (a > b) std::swap(a, b);
+        return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+    };
+    for (int c : cells) {
+        const int32_t *v = cd + 4 * (size_t)c;
+        for (int i = 0; i < 4; ++i)
+            for (int j = i + 1; j < 4; ++j)
+                if (v[i] == v[j]) return false;
+        for (int i = 0; i < 4; ++i) {
+            auto it = edges.emplace(ekey(v[cyc[i]], v[cyc[(i + 1) & 3]]), std::make_pair(c, -1));
+            if (!it.second) {
+                if (it.first->second.second >= 0) return false;
+                it.first->second.second = c;
+            }
+        }
+    }
+    auto set = [&](int d, int u, int w) {
+        if (gu[d] == INT_MIN) {
+            gu[d] = u;
+            gv[d] = w;
+            return true;
+        }
+        return gu[d] == u && gv[d] == w;
+    };
+    const int c0 = cells[0];
+    {
+        const int32_t *v = cd + 4 * (size_t)c0;
+        set(v[0], 0, 0);
+        set(v[1], 1, 0);
+        set(v[2], 0, 1);
+        set(v[3], 1, 1);
+    }
+    std::vector<int> queue{c0};
+    mark[c0] = stamp;
+    for (size_t qi = 0; qi < queue.size(); ++qi) {
+        const int c = queue[qi];
+        const int32_t *v = cd + 4 * (size_t)c;
+        for (int i = 0; i < 4; ++i) {
+            // edge (p, q) of c; s is the other neighbour of p in c, so s - p points into c
+            const int p = v[cyc[i]], q = v[cyc[(i + 1) & 3]], s = v[cyc[(i + 3) & 3]];
+            const auto &e = edges[ekey(p, q)];
+            const int o = e.first == c ? e.second : e.first;
+            if (o < 0 || mark[o] == stamp) continue;
+            const int eu = gu[s] - gu[p], ev = gv[s] - gv[p];
+            const int32_t *x = cd + 4 * (size_t)o;
+            int j = 0;
+            while (j < 4 && x[cyc[j]] != p) ++j;
+            if (j == 4) return false;
+            int pa, qa;  // neighbours of p and q in o across the edge
+            if (x[cyc[(j + 1) & 3]] == q) {
+                pa = x[cyc[(j + 3) & 3]];
+                qa = x[cyc[(j + 2) & 3]];
+            } else if (x[cyc[(j + 3) & 3]] == q) {
+                pa = x[cyc[(j + 1) & 3]];
+                qa = x[cyc[(j + 2) & 3]];
+            } else {
+                return false;
+            }
+            if (!set(pa, gu[p] - eu, gv[p] - ev) || !set(qa, gu[q] - eu, gv[q] - ev)) return false;
+            mark[o] = stamp;
+            queue.push_back(o);
+        }
+    }
+    if (queue.size() != cells.size()) return false;
+    for (int c : cells) {
+        const int32_t *v = cd + 4 * (size_t)c;
+        const int au = gu[v[1]] - gu[v[0]], av = gv[v[1]] - gv[v[0]];
+        const int bu = gu[v[2]] - gu[v[0]], bv = gv[v[2]] - gv[v[0]];
+        if (std::abs(au) + std::abs(av) != 1 || std::abs(bu) + std::abs(bv) != 1 ||
+            au * bu + av * bv != 0)
+            return false;
+        if (gu[v[3]] != gu[v[1]] + bu || gv[v[3]] != gv[v[1]] + bv) return false;
+    }
+    return true;
+}
+
+struct StripBuild {
+    std::vector<int> order;                  // the component's dofs in device order
+    std::vector<std::array<int, 3>> tasks;   // {first gap, gaps, cells}
+    std::vector<std::array<int, 3>> gaps;    // {cell mask, first column (index into order), line mask}
+    std::vector<std::array<int, STRIP_W + 1>> gcells;  // cells of a gap by position, -1: none
+    std::vector<std::array<int, 5>> canon;   // {cell, canonical vertices 0..3 (dofs)}
+    long cells = 0;                          // cell evaluations of the schedule
+};
+
+// Strips of one structured component, positions along u (o = 0) or v (o = 1), tasks of
+// about `target` entries.  False if two dofs or two cells share a grid point.
+bool build_strips(const std::vector<int> &cells, const std::vector<int> &dofs,
+                  const int32_t *cd, const std::vector<int> &gu, const std::vector<int> &gv,
+                  int o, int target, StripBuild &B)
+{
+    B = StripBuild();
+    auto P = [&](int d) { return o ? gv[d] : gu[d]; };
+    auto A = [&](int d) { return o ? gu[d] : gv[d]; };
+    int pmin = INT_MAX, pmax = INT_MIN, amin = INT_MAX, amax = INT_MIN;
+    for (int d : dofs) {
+        pmin = std::min(pmin, P(d));
+        pmax = std::max(pmax, P(d));
+        amin = std::min(amin, A(d));
+        amax = std::max(amax, A(d));
+    }
+    const int Np = pmax - pmin + 1, Na = amax - amin + 1;
+    if ((double)Np * Na > 64.0 * (double)dofs.size() + 1024) return false;  // too sparse a grid
+    std::vector<int> node((size_t)Np * Na, -1), cellg((size_t)Np * Na, -1);
+    for (int d : dofs) {
+        int &slot = node[(size_t)(P(d) - pmin) + (size_t)Np * (A(d) - amin)];
+        if (slot >= 0) return false;
+        slot = d;
+    }
+    for (int c : cells) {
+        int p0 = INT_MAX, a0 = INT_MAX;
+        for (int k = 0; k < 4; ++k) {
+            p0 = std::min(p0, P(cd[4 * (size_t)c + k]) - pmin);
+            a0 = std::min(a0, A(cd[4 * (size_t)c + k]) - amin);
+        }
+        int &slot = cellg[(size_t)p0 + (size_t)Np * a0];
+        if (slot >= 0) return false;
+        slot = c;
+        std::array<int, 5> cn{c, node[(size_t)p0 + (size_t)Np * a0],
+                              node[(size_t)p0 + 1 + (size_t)Np * a0],
+                              node[(size_t)p0 + (size_t)Np * (a0 + 1)],
+                              node[(size_t)p0 + 1 + (size_t)Np * (a0 + 1)]};
+        B.canon.push_back(cn);
+    }
+    const int ns = (Np + STRIP_W - 1) / STRIP_W;
+    for (int s = 0; s < ns; ++s) {
+        const int b0 = (int)((long)s * Np / ns), w = (int)((long)(s + 1) * Np / ns) - b0;
+        auto cell_at = [&](int g, int k) {
+            const int p0 = b0 + k - 1;
+            if (p0 < 0 || p0 > Np - 2 || g < 0 || g > Na - 2) return -1;
+            return cellg[(size_t)p0 + (size_t)Np * g];
+        };
+        long total = 0;
+        for (int g = 0; g + 1 < Na; ++g)
+            for (int k = 0; k <= w; ++k) total += cell_at(g, k) >= 0;
+        int np = std::max(1, (int)std::lround((double)total / std::max(1, target)));
+        np = std::min(np, Na);
+        for (int r = 0; r < np; ++r) {
+            // levels [a0, a1): gap t (task-local, -1 .. L-1) holds the cells between lines
+            // t and t + 1; after it, line t is complete
+            const int a0 = (int)((long)r * Na / np), a1 = (int)((long)(r + 1) * Na / np);
+            const int L = a1 - a0;
+            const long cells_before = B.cells;
+            B.tasks.push_back({(int)B.gaps.size(), L + 1, 0});
+            for (int t = -1; t < L; ++t) {
+                std::array<int, STRIP_W + 1> gc;
+                int cmask = 0;
+                for (int k = 0; k <= STRIP_W; ++k) {
+                    gc[k] = k <= w ? cell_at(a0 + t, k) : -1;
+                    if (gc[k] >= 0) {
+                        cmask |= 1 << k;
+                        ++B.cells;
+                    }
+                }
+                int first = -1, lmask = 0;
+                if (t >= 0)
+                    for (int p = 0; p < w; ++p) {
+                        const int d = node[(size_t)(b0 + p) + (size_t)Np * (a0 + t)];
+                        if (d < 0) continue;
+                        if (first < 0) first = (int)B.order.size();
+                        lmask |= 1 << p;
+                        B.order.push_back(d);
+                    }
+                B.gaps.push_back({cmask, first < 0 ? 0 : first, lmask});
+                B.gcells.push_back(gc);
+            }
+            B.tasks.back()[2] = (int)(B.cells - cells_before);
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
 std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const int32_t *cmat,
-                      const int32_t *dnp, const int32_t *dni, int tc, MeshPlan &P)
+                      const int32_t *dnp, const int32_t *dni, int tc, int strips, MeshPlan &P)
 {
     P = MeshPlan();
     if (tc < 1 || tc > TILE_COLS) return "tile width must be 1..16";
@@ -270,7 +461,74 @@ std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const i
     std::vector<int> stamp(nc, -1);
     std::vector<int> order;
     order.reserve(n);
-    std::vector<int> tstart{0};
+    // Structured patches first: each connected component of the cell graph whose cells
+    // form a logically structured grid is laid out as strips (STRIP_W-wide bands of grid
+    // positions, level by level), which the strip kernel sweeps with every cell evaluated
+    // once per strip it touches.  Components are taken in material / Morton order.
+    P.cell_int_s.assign(cd, cd + 4 * (size_t)nc);  // caller numbering until perm is known
+    P.canon_sign.assign(nc, 1);
+    std::vector<std::array<int, 3>> all_tasks;
+    if (strips) {
+        UnionFind cu(n);
+        for (int c = 0; c < nc; ++c)
+            for (int k = 1; k < 4; ++k) cu.unite(cd[4 * c], cd[4 * c + k]);
+        std::vector<int> root_rank(n, -1), comp_of(n);
+        int n_comp = 0;
+        for (int s : seeds) {
+            const int r = cu.find(s);
+            if (root_rank[r] < 0) root_rank[r] = n_comp++;
+        }
+        std::vector<std::vector<int>> ccells(n_comp), cdofs(n_comp);
+        for (int i = 0; i < n; ++i) cdofs[root_rank[cu.find(i)]].push_back(i);
+        for (int c = 0; c < nc; ++c) ccells[root_rank[cu.find(cd[4 * c])]].push_back(c);
+        std::vector<int> gu(n, INT_MIN), gv(n, INT_MIN), mark(nc, -1);
+        StripBuild B0, B1;
+        for (int k = 0; k < n_comp; ++k) {
+            if (ccells[k].empty()) continue;
+            if (!grid_coords(ccells[k], cd, gu, gv, mark, k)) continue;
+            const bool ok0 = build_strips(ccells[k], cdofs[k], cd, gu, gv, 0, STRIP_TASK_ENTRIES, B0);
+            const bool ok1 = build_strips(ccells[k], cdofs[k], cd, gu, gv, 1, STRIP_TASK_ENTRIES, B1);
+            if (!ok0 && !ok1) continue;
+            const StripBuild &B = (ok0 && (!ok1 || B0.cells <= B1.cells)) ? B0 : B1;
+            const int base = (int)order.size();
+            const int goff = (int)P.st_gap.size() / 4;
+            for (int d : B.order) {
+                order.push_back(d);
+                assigned[d] = 1;
+            }
+            for (const auto &g : B.gaps) {
+                P.st_gap.insert(P.st_gap.end(), {g[0], g[2] ? base + g[1] : 0, g[2], 0});
+            }
+            for (const auto &gc : B.gcells) P.st_gcell.insert(P.st_gcell.end(), gc.begin(), gc.end());
+            for (const auto &t : B.tasks) all_tasks.push_back({t[0] + goff, t[1], t[2]});
+            for (const auto &cn : B.canon) {
+                const int c = cn[0];
+                int kk[4];
+                for (int i = 0; i < 4; ++i) {
+                    P.cell_int_s[4 * (size_t)c + i] = cn[1 + i];
+                    kk[i] = 0;
+                    while (cd[4 * (size_t)c + kk[i]] != cn[1 + i]) ++kk[i];
+                }
+                // orientation of the canonical reference frame in the cell's own
+                const int e1x = (kk[1] & 1) - (kk[0] & 1), e1y = (kk[1] >> 1) - (kk[0] >> 1);
+                const int e2x = (kk[2] & 1) - (kk[0] & 1), e2y = (kk[2] >> 1) - (kk[0] >> 1);
+                P.canon_sign[c] = (int8_t)(e1x * e2y - e1y * e2x);
+            }
+        }
+    }
+    P.strip_dofs = (int)order.size();
+    // longest tasks first, so that the short ones fill the end of the grid
+    std::stable_sort(all_tasks.begin(), all_tasks.end(), [](const std::array<int, 3> &a,
+                                                            const std::array<int, 3> &b) {
+        return a[2] > b[2];
+    });
+    P.n_tasks = (int)all_tasks.size();
+    P.strip_cells = 0;
+    for (const auto &t : all_tasks) {
+        P.st_task.insert(P.st_task.end(), {t[0], t[1]});
+        P.strip_cells += t[2];
+    }
+    std::vector<int> tstart{(int)order.size()};
     std::vector<int> prev, tile, front;
     size_t seed_ptr = 0;
     auto free_degree = [&](int v) {
@@ -369,11 +627,12 @@ std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const i
             const int d = cd[4 * c + k];
             P.cell_int[4 * c + k] = P.perm[d];
             P.cell_cls[4 * c + k] = P.perm[uf.find(d)];
+            P.cell_int_s[4 * c + k] = P.perm[P.cell_int_s[4 * c + k]];
         }
     // column tiles: [tile_start[t], tile_start[t+1]) in the device order, <= tc wide
     P.n_tiles = (int)tstart.size() - 1;
     P.tile_start = tstart;
-    std::vector<int> tile_of(n);
+    std::vector<int> tile_of(n, -1);  // -1: a strip column
     for (int t = 0; t < P.n_tiles; ++t)
         for (int i = tstart[t]; i < tstart[t + 1]; ++i) tile_of[i] = t;
     std::vector<std::vector<int>> tl(P.n_tiles), tpk(P.n_tiles);
@@ -381,7 +640,7 @@ std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const i
         int tiles[4];
         for (int k = 0; k < 4; ++k) tiles[k] = tile_of[P.cell_int[4 * c + k]];
         for (int k = 0; k < 4; ++k) {
-            bool dup = false;
+            bool dup = tiles[k] < 0;
             for (int j = 0; j < k; ++j) dup |= tiles[j] == tiles[k];
             if (dup) continue;
             const int t = tiles[k];

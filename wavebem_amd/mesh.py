@@ -350,6 +350,59 @@ def cube_sphere_mesh(n=4, radius=1.0, dirichlet_upper=True) -> SurfaceMesh:
                        dict(kind="cube_sphere", n=n, radius=radius))
 
 
+def tri_block_mesh(n=4, curvature=0.2, centre=(0.0, 0.0, 0.0), radius=1.0) -> SurfaceMesh:
+    """An open curved patch that is not a structured grid: a triangle cut into three
+    quadrilateral blocks (n x n cells each) that share their edges, so the centre vertex
+    has three cells around it.  z = curvature (x^2 + y^2) over the plane; all dofs are
+    Dirichlet.  Used to exercise the column-tile assembly of unstructured parts."""
+    c = np.asarray(centre, dtype=np.float64)
+    ang = np.pi / 2 + np.array([0.0, 2 * np.pi / 3, 4 * np.pi / 3])
+    V = [np.array([radius * np.cos(a), radius * np.sin(a)]) for a in ang]
+    O = np.zeros(2)
+    mid = [(V[k] + V[(k + 1) % 3]) / 2 for k in range(3)]
+    blocks = [(V[0], mid[0], mid[2], O), (V[1], mid[1], mid[0], O), (V[2], mid[2], mid[1], O)]
+    t = np.linspace(0.0, 1.0, n + 1)
+    keyed, pts, cells = {}, [], []
+
+    def dof(p):
+        k = (round(p[0], 12), round(p[1], 12))
+        if k not in keyed:
+            keyed[k] = len(pts)
+            pts.append(np.array([p[0], p[1], curvature * (p[0] ** 2 + p[1] ** 2)]) + c)
+        return keyed[k]
+
+    for (p00, p10, p01, p11) in blocks:
+        ids = [[dof((1 - a) * (1 - b) * p00 + a * (1 - b) * p10 + (1 - a) * b * p01 + a * b * p11)
+                for b in t] for a in t]
+        for j in range(n):
+            for i in range(n):
+                cells.append([ids[i][j], ids[i + 1][j], ids[i][j + 1], ids[i + 1][j + 1]])
+    pts = np.array(pts)
+    cells = np.asarray(cells, dtype=np.int32)
+    orient = np.empty(len(cells), dtype=np.int8)
+    for k, cv in enumerate(cells):
+        p0, p1, p2, p3 = pts[cv]
+        nrm = np.cross(0.5 * ((p1 - p0) + (p3 - p2)), 0.5 * ((p2 - p0) + (p3 - p1)))
+        orient[k] = 1 if nrm[2] > 0 else -1
+    mat = np.full(len(cells), BOX, dtype=np.int32)
+    dmat = np.full(len(pts), BOX, dtype=np.int32)
+    surf = np.ones(len(pts))
+    return SurfaceMesh(pts, cells, orient, mat, surf, 1.0 - surf, dmat,
+                       dict(kind="tri_block", n=n))
+
+
+def merge_meshes(*meshes: SurfaceMesh) -> SurfaceMesh:
+    """Disjoint union of meshes (dofs and cells renumbered consecutively)."""
+    off, cells = 0, []
+    for m in meshes:
+        cells.append(m.cell_dofs + off)
+        off += m.n_dofs
+    cat = lambda f: np.concatenate([getattr(m, f) for m in meshes])  # noqa: E731
+    return SurfaceMesh(cat("support_points"), np.concatenate(cells).astype(np.int32),
+                       cat("cell_orient"), cat("cell_material"), cat("surface_nodes"),
+                       cat("other_nodes"), cat("dof_material"), dict(kind="merged"))
+
+
 def double_nodes_csr(points: np.ndarray, tol: float = 1e-8):
     """Sets of geometrically coincident dofs (each set contains the dof itself).
 
