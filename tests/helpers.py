@@ -30,6 +30,12 @@ def oracle_assembly(kind: str):
         m = M.box_mesh(n=6, dirichlet_top=True)
     elif kind == "sphere4":
         m = M.cube_sphere_mesh(n=4)
+    elif kind == "tri_block":
+        m = M.tri_block_mesh(n=5)
+    elif kind == "mixed":
+        # a structured box (strip kernel) beside three blocks meeting at a valence-3
+        # vertex (no structured grid: column-tile kernel), one system
+        m = M.merge_meshes(M.box_mesh(n=5), M.tri_block_mesh(n=4, centre=(3.0, 0, 0)))
     elif kind.startswith("wigley_"):
         m = wigley_preset(kind.split("_", 1)[1])
     else:

@@ -37,6 +37,27 @@ def test_assembly_matches_oracle(gpu_available, kind):
     assert ea < 1e-12
 
 
+@pytest.mark.parametrize("kind,strips,precision", [
+    ("mixed", 1, "fp64"), ("mixed", 1, "fp32_ir"), ("tri_block", 1, "fp64"),
+    ("wigley_small", 0, "fp64"), ("wigley_small", 0, "fp32_ir")])
+def test_assembly_strip_and_tile_kernels(gpu_available, kind, strips, precision):
+    """The strip kernel (structured patches), the column-tile kernel (the rest) and both
+    in one launch sequence, against the oracle."""
+    m, ptr, idx, H, G = T.oracle_assembly(kind)
+    print(f"[assembly kernels] {kind} strips={strips} {precision} n={m.n_dofs}", flush=True)
+    bar = 1e-10 if precision == "fp64" else 1e-9
+    with _problem(m, ptr, idx, assembly_strips=strips, precision=precision) as p:
+        Gd, Hd = p.matrices()
+        alpha = p.compute_alpha()
+    eg = T.row_scaled_err(Gd, G)
+    eh = T.row_scaled_err(Hd, H)
+    ea = np.abs(alpha - O.compute_alpha(H)).max()
+    print(f"  G {eg:.2e} H {eh:.2e} alpha {ea:.2e}")
+    assert eg < bar
+    assert eh < bar
+    assert ea < 1e-12
+
+
 def test_assembly_fp32_ir_storage(gpu_available):
     m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
     with _problem(m, ptr, idx, precision="fp32_ir") as p:
