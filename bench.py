@@ -277,13 +277,28 @@ def profiled_traffic(workload, kernel):
     return None
 
 
-def cpu_baseline(mesh, iters64, threads, rows_sample, gemv_n, fma=True):
-    """Oracle (CPU restatement of the reference dense path) on a bounded sample."""
+def host_cpu():
+    """CPU model and core counts of the host the baseline runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity_cores": len(os.sched_getaffinity(0))}
+
+
+def cpu_leg(mesh, ptr, idx, iters64, threads, rows_sample, gemv_n):
+    """One timed leg of the oracle dense path: assemble_system on `rows_sample` rows (all
+    rows when it is n), scaled by n / rows, plus FullMatrix::vmult timed at n = gemv_n and
+    scaled by (n / gemv_n)^2, twice per GMRES step (the reference reads G and H)."""
     import oracle as O
-    from wavebem_amd.mesh import double_nodes_csr
 
     n = mesh.n_dofs
-    ptr, idx = double_nodes_csr(mesh.support_points)
     rows = min(rows_sample, n)
     r0 = (n - rows) // 2
     t0 = time.perf_counter()
@@ -293,14 +308,26 @@ def cpu_baseline(mesh, iters64, threads, rows_sample, gemv_n, fma=True):
     t_asm = t_rows * n / rows
     gn = min(gemv_n, n)
     t_gemv = O.gemv_time(gn, reps=3, n_threads=threads) * (n / gn) ** 2
-    # the reference's GMRES operator reads both G and H (two FullMatrix::vmult per step)
     t_step = t_asm + iters64 * 2 * t_gemv
-    out = dict(value=t_step * 1e3, unit="ms", cores=threads, kind="port",
-               sample=(f"oracle assemble_system rows [{r0},{r0 + rows}) of {n} ({t_rows:.1f} s, "
-                       f"scaled by n/rows) + FullMatrix::vmult timed at n={gn} "
-                       f"(scaled by (n/{gn})^2) x 2 per GMRES step x {iters64} steps "
-                       f"(the FP64 GPU iteration count); preconditioner not counted"),
-               assembly_ms=t_asm * 1e3, gemv_ms=t_gemv * 1e3)
+    scaled = "unscaled" if rows == n else f"scaled by n/rows"
+    gscaled = "" if gn == n else f" (scaled by (n/{gn})^2)"
+    return dict(value=t_step * 1e3, unit="ms", cores=threads, kind="port",
+                sample=(f"oracle assemble_system rows [{r0},{r0 + rows}) of {n} "
+                        f"({t_rows:.1f} s, {scaled}) + FullMatrix::vmult timed at n={gn}"
+                        f"{gscaled} x 2 per GMRES step x {iters64} steps (the FP64 GPU "
+                        f"iteration count); preconditioner not counted"),
+                assembly_ms=t_asm * 1e3, gemv_ms=t_gemv * 1e3)
+
+
+def cpu_baseline(mesh, iters64, threads, rows_sample, gemv_n, fma=True, serial_rows=2000):
+    """Oracle (CPU restatement of the reference dense path) on bounded samples: all the
+    cores this process may use, and one thread (the reference is serial)."""
+    from wavebem_amd.mesh import double_nodes_csr
+
+    ptr, idx = double_nodes_csr(mesh.support_points)
+    out = cpu_leg(mesh, ptr, idx, iters64, threads, rows_sample, gemv_n)
+    out["host"] = host_cpu()
+    out["serial"] = cpu_leg(mesh, ptr, idx, iters64, 1, serial_rows, min(gemv_n, 4096))
     if fma:
         out["bemfma"] = bemfma_baseline(mesh, ptr, idx, iters64)
     return out
@@ -342,7 +369,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-rows", type=int, default=24000)
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="rows of the all-cores assembly sample (0: all rows up to 21,000 "
+                         "dofs, else 24,000)")
     ap.add_argument("--cpu-gemv-n", type=int, default=8192)
     ap.add_argument("--only", choices=["fp64", "fp32_ir"], default=None)
     ap.add_argument("--ir-inner16", type=float, default=3e-4,
@@ -503,8 +532,25 @@ def main():
                             "solve": round(v["ms_per_step"] - v["assembly_ms"], 3)}
                         for k, v in res.items()}
 
-    # roofline of the dominant kernel of the headline path (column-selected GEMV)
+    # roofline of the kernel with the most time per step of the headline path: the
+    # regular-cell assembly (FP64 vector ALU), then the inner-solve GEMV (HBM)
     ks = head["kernels"]
+    hp = "fp32_ir" if "fp32_ir" in res else "fp64"
+    a = ks["assemble"]
+    asm_ms = kernel_avg_ms(ks, "assemble")
+    asm_flops = ASM_FLOPS_PER_ROW_QP * n * 16 * mesh.n_cells
+    asm_tf = asm_flops / (asm_ms * 1e-3) / 1e12
+    asm_kernel = "k_assemble_strips<4, 1>" if hp == "fp32_ir" else "k_assemble_strips<4, 0>"
+    line["roofline"] = {"kernel": asm_kernel, "bound": "mfma",
+                        "pipe": "fp64 vector ALU (no matrix-core work; MI355X's FP64 vector "
+                                "and matrix peaks are both 78.6 TFLOP/s)",
+                        "achieved": round(asm_tf, 2), "peak": FP64_VALU_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(asm_tf / FP64_VALU_PEAK_TFLOPS, 4),
+                        "traffic": profiled_traffic(args.workload, asm_kernel),
+                        "avg_launch_ms": round(asm_ms, 4), "launches": a["launches"],
+                        "algorithmic_flops_per_launch": int(asm_flops),
+                        "flops_per_row_point": ASM_FLOPS_PER_ROW_QP,
+                        "share_of_step": round(asm_ms / head["ms_per_step"], 3)}
     if "fp32_ir" not in res:
         gname, esz, xsz, kname = "gemv64", 8, 8, "k_gemv_f64"
     elif args.ir_inner_format == 1:
@@ -519,12 +565,14 @@ def main():
     if esz == 2:
         bytes_per_launch += rows_per_launch * 4
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    line["roofline"] = {"kernel": kname, "bound": "hbm",
-                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": profiled_traffic(args.workload, kname),
-                        "avg_launch_ms": round(avg_ms, 4), "launches": g["launches"],
-                        "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    line["roofline_gemv"] = {"kernel": kname, "bound": "hbm",
+                             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                             "traffic": profiled_traffic(args.workload, kname),
+                             "avg_launch_ms": round(avg_ms, 4), "launches": g["launches"],
+                             "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                             "share_of_step": round(avg_ms * g["launches"] / max(args.steps, 1)
+                                                    / head["ms_per_step"], 3)}
     kern = {}
     for prec, r in res.items():
         for name, k in r["kernels"].items():
@@ -549,8 +597,9 @@ def main():
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         iters = res["fp64"]["stats"]["iterations"] if "fp64" in res else 100
+        rows = args.cpu_rows or (n if n <= 21000 else 24000)
         t = time.perf_counter()
-        line["cpu_baseline"] = cpu_baseline(mesh, iters, threads, args.cpu_rows, args.cpu_gemv_n,
+        line["cpu_baseline"] = cpu_baseline(mesh, iters, threads, rows, args.cpu_gemv_n,
                                             fma=not args.no_fma_baseline)
         log(f"cpu baseline {time.perf_counter() - t:.1f} s")
     print(json.dumps(line), flush=True)

@@ -156,6 +156,14 @@ wb_status wb_get_matrices(wb_ctx *ctx, double *G, double *H);
 wb_status wb_get_matrix_rows(wb_ctx *ctx, int n_rows, const int32_t *rows, double *G,
                              double *H);
 
+/* No counterpart in the reference (its preconditioner is a band LU).  The coarse operator
+ * Ac = P^T A P of the last preconditioner build (FP64, nc x nc row-major), A being the
+ * block-Jacobi matrix (the vmult operator with identity rows for constrained dofs) in
+ * the device dof order and P the indicator of the aggregates: device indices
+ * [a agg, min(n, (a + 1) agg)) for a < nc.  For inspection; WB_E_STATE when no coarse
+ * space is in use.  Ac may be NULL to query nc and agg. */
+wb_status wb_get_coarse_matrix(wb_ctx *ctx, int *nc, int *agg, double *Ac);
+
 /* BEMProblem::compute_alpha (source/bem_problem.cc:594-618): alpha = H * (-1). */
 wb_status wb_compute_alpha(wb_ctx *ctx, double *alpha);
 

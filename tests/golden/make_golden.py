@@ -52,25 +52,12 @@ def integrand(v, orient, x, xi, eta, k, which):
     return ph[k] * rn / (-4 * mp.pi * r ** 3)
 
 
-def integrate(v, orient, x, k, which, singular_vertex=None):
+def integrate(v, orient, x, k, which):
+    """Regular-cell integral by mpmath's adaptive quadrature (singular cells use
+    polar_gauss_legendre)."""
     v = [[mp.mpf(t) for t in p] for p in v]
     x = [mp.mpf(t) for t in x]
-    if singular_vertex is None:
-        return mp.quad(lambda xi, eta: integrand(v, orient, x, xi, eta, k, which), [0, 1], [0, 1])
-    # map the singular vertex to (0,0) of the reference square
-    sv = singular_vertex
-
-    def to_ref(s, t):
-        # (s,t) measured from the singular vertex
-        xi = s if sv in (0, 2) else 1 - s
-        eta = t if sv in (0, 1) else 1 - t
-        return xi, eta
-
-    def f(s, t):
-        xi, eta = to_ref(s, t)
-        return integrand(v, orient, x, xi, eta, k, which)
-
-    raise NotImplementedError("singular cells use polar_gauss_legendre")
+    return mp.quad(lambda xi, eta: integrand(v, orient, x, xi, eta, k, which), [0, 1], [0, 1])
 
 
 def polar_gauss_legendre(v, orient, sv, k, which, panels=16, npts=48):

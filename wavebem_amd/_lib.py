@@ -101,6 +101,8 @@ _SIGS = {
     "wb_release_matrices": (ctypes.c_int, [_vp]),
     "wb_get_matrices": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_get_matrix_rows": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _dp, _dp]),
+    "wb_get_coarse_matrix": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_int), _dp]),
     "wb_solve_adjoint": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_void_p]),
     "wb_residual_vjp": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, _dp]),
     "wb_residual_jvp": (ctypes.c_int, [_vp, _dp, _dp, _dp, ctypes.c_int, _dp]),

@@ -187,6 +187,17 @@ class BEMProblem:
                     "wb_get_matrix_rows")
         return G, H
 
+    def coarse_matrix(self):
+        """(Ac, agg): the coarse operator P^T A P of the last preconditioner build
+        (device dof order aggregates of agg dofs), for inspection."""
+        nc, agg = ctypes.c_int(), ctypes.c_int()
+        self._check(L.lib().wb_get_coarse_matrix(self._ctx, ctypes.byref(nc), ctypes.byref(agg),
+                                                 None), "wb_get_coarse_matrix")
+        Ac = np.empty((nc.value, nc.value))
+        self._check(L.lib().wb_get_coarse_matrix(self._ctx, ctypes.byref(nc), ctypes.byref(agg),
+                                                 _d(Ac)), "wb_get_coarse_matrix")
+        return Ac, agg.value
+
     @property
     def dirichlet_matrix(self):
         return self.matrices()[0]
@@ -298,8 +309,16 @@ class BEMProblem:
         return self.last_stats
 
     def solve(self, phi, dphi_dn, tmp_rhs):
-        self.assemble_system()
-        return self.solve_system(phi, dphi_dn, tmp_rhs)
+        """``BEMProblem::solve`` (Direct branch): assemble_system + solve_system in one call
+        (wb_solve); phi / dphi_dn are updated in place."""
+        t = _f64(tmp_rhs, self.n)
+        self._inplace_ok(phi)
+        self._inplace_ok(dphi_dn)
+        st = L.WbSolveStats()
+        status = L.lib().wb_solve(self._ctx, _d(t), _d(phi), _d(dphi_dn), ctypes.byref(st))
+        self.last_stats = st.as_dict()
+        self._check(status, "wb_solve")
+        return self.last_stats
 
     def residual(self, phi, dphi_dn):
         p = _f64(phi, self.n)

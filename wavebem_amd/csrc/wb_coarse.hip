@@ -52,8 +52,7 @@ __global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
     int n, int agg, int nc, const void *__restrict__ Gv, const void *__restrict__ Hv,
     int precision, size_t ld, const _Float16 *__restrict__ A16, const float *__restrict__ rs16,
     const double *__restrict__ alpha, const uint8_t *__restrict__ gmask,
-    const uint8_t *__restrict__ constrained, double *__restrict__ AP, float *__restrict__ AP32,
-    size_t ldap)
+    const uint8_t *__restrict__ constrained, double *__restrict__ AP, size_t ldap)
 {
     const int j = blockIdx.x;
     const bool con = constrained[j] != 0;  // identity row
@@ -117,7 +116,6 @@ __global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
             if (mine && ((gmask[j >> 2] >> (j & 3)) & 1u)) s += alpha[j];
         }
         AP[(size_t)a * ldap + j] = s;
-        AP32[(size_t)a * ldap + j] = (float)s;
     }
 }
 
@@ -127,7 +125,7 @@ __global__ __launch_bounds__(CR_THREADS) void k_coarse_rows(
 __global__ __launch_bounds__(256) void k_coarse_rows_sums(
     int n, int agg, int nc, const double *__restrict__ S, const double *__restrict__ alpha,
     const uint8_t *__restrict__ gmask, const uint8_t *__restrict__ constrained,
-    double *__restrict__ AP, float *__restrict__ AP32, size_t ldap)
+    double *__restrict__ AP, size_t ldap)
 {
     const int j = blockIdx.x * 256 + threadIdx.x, a = blockIdx.y;
     if (j >= n) return;
@@ -139,7 +137,6 @@ __global__ __launch_bounds__(256) void k_coarse_rows_sums(
     else
         s = S[o] + (mine && ((gmask[j >> 2] >> (j & 3)) & 1u) ? alpha[j] : 0.0);
     AP[o] = s;
-    AP32[o] = (float)s;
 }
 
 // Ac[a][b] = sum over rows j of aggregate a of AP[b][j]: one wavefront per (a, b).
@@ -297,16 +294,16 @@ size_t coarse_invert_lds_bytes(int nc)
 void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
                          size_t ld, const void *A16, const float *rs16, const double *alpha,
                          const uint8_t *gmask, const uint8_t *constrained, double *AP,
-                         float *AP32, size_t ldap, const double *S, double *Ac, double *Aci,
+                         size_t ldap, const double *S, double *Ac, double *Aci,
                          float *Aci32, double *WT, float *WT32, int *status, hipStream_t s)
 {
     if (S)
         hipLaunchKernelGGL(k_coarse_rows_sums, dim3((n + 255) / 256, nc), dim3(256), 0, s, n,
-                           agg, nc, S, alpha, gmask, constrained, AP, AP32, ldap);
+                           agg, nc, S, alpha, gmask, constrained, AP, ldap);
     else
         hipLaunchKernelGGL(k_coarse_rows, dim3(n), dim3(CR_THREADS), 0, s, n, agg, nc, G, H,
                            precision, ld, (const _Float16 *)A16, rs16, alpha, gmask,
-                           constrained, AP, AP32, ldap);
+                           constrained, AP, ldap);
     hipLaunchKernelGGL(k_coarse_matrix, dim3((nc * nc + 3) / 4), dim3(256), 0, s, n, agg, nc,
                        (const double *)AP, ldap, Ac);
     if (nc <= CS_MAX64)

@@ -87,7 +87,7 @@ struct wb_ctx {
     bool cs_on = false;
     bool cs_t_on = false;  // the transposed-system data below match the current build
     int cs_agg = 0, cs_nc = 0;
-    DevBuf cs_AP, cs_AP32, cs_Ac, cs_Aci, cs_Aci32, cs_WT, cs_WT32, cs_status, cs_C, cs_T;
+    DevBuf cs_AP, cs_Ac, cs_Aci, cs_Aci32, cs_WT, cs_WT32, cs_status, cs_C, cs_T;
     DevBuf cs_PTA, cs_WTT;  // transposed system (adjoint solve), FP64
     DevBuf cs_S;            // aggregate row sums of the FP16 copy (nc x vlen, FP64)
 
@@ -164,7 +164,7 @@ struct wb_ctx {
             &iperm_d, &st_task, &st_gap, &st_gcell, &cell_int_s, &orient_s, &cellq_s, &G, &H,
             &Glo, &Hlo, &hsum_part, &hsum_sing, &alpha, &surf_d, &gmask, &segs, &diag_op,
             &diag_rhs, &con_rows, &con_ptr, &con_cols, &con_w, &con_inhom, &constrained,
-            &free_rows, &binv, &binv32, &cs_AP, &cs_AP32, &cs_Ac, &cs_Aci, &cs_Aci32, &cs_WT,
+            &free_rows, &binv, &binv32, &cs_AP, &cs_Ac, &cs_Aci, &cs_Aci32, &cs_WT,
             &cs_WT32, &cs_status, &cs_C, &cs_T, &cs_PTA, &cs_WTT, &cs_S, &V, &z, &w, &P1, &P2,
             &Pn, &hbuf, &hred, &vx, &vb, &vr, &vt, &vs, &V32, &z32, &w32, &r32, &d32, &ct_tgt,
             &ct_ptr, &ct_src, &ct_coef, &gt_part, &vjp_w, &vjp_phi, &vjp_psi, &vjp_rows,
@@ -556,7 +556,6 @@ static void ensure_precond(wb_ctx *c)
         c->cs_agg = agg;
         c->cs_nc = nc;
         c->cs_AP.ensure(sizeof(double) * (size_t)nc * c->vlen);
-        c->cs_AP32.ensure(sizeof(float) * (size_t)nc * c->vlen);
         c->cs_Ac.ensure(sizeof(double) * (size_t)nc * nc);
         c->cs_Aci.ensure(sizeof(double) * (size_t)nc * nc);
         c->cs_Aci32.ensure(sizeof(float) * (size_t)nc * nc);
@@ -573,8 +572,7 @@ static void ensure_precond(wb_ctx *c)
                             from16 ? c->A16.p : nullptr,
                             from16 ? c->rs16.as<float>() : nullptr, c->alpha.as<double>(),
                             c->gmask.as<uint8_t>(),
-                            c->constrained.as<uint8_t>(), c->cs_AP.as<double>(),
-                            c->cs_AP32.as<float>(), c->vlen,
+                            c->constrained.as<uint8_t>(), c->cs_AP.as<double>(), c->vlen,
                             from16 && c->h16_sums ? c->cs_S.as<double>() : nullptr,
                             c->cs_Ac.as<double>(),
                             c->cs_Aci.as<double>(), c->cs_Aci32.as<float>(), c->cs_WT.as<double>(),
@@ -1491,6 +1489,13 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         c->n = n;
         c->ld = c->plan.ld;
         c->vlen = c->ld;
+        // vectors and Krylov bases kept from an earlier mesh take the new layout: clear
+        // them whole, so that the padding [n, vlen) of every vector reads zero
+        for (DevBuf *b : {&c->z, &c->w, &c->vx, &c->vb, &c->vr, &c->vt, &c->vs, &c->V, &c->V32,
+                          &c->z32, &c->w32, &c->r32, &c->d32})
+            if (b->p) HIPCHK(hipMemsetAsync(b->p, 0, b->bytes, c->stream));
+        for (DevBuf *b : c->mw.all())
+            if (b->p) HIPCHK(hipMemsetAsync(b->p, 0, b->bytes, c->stream));
         const MeshPlan &p = c->plan;
         std::vector<double> Xi(3 * (size_t)n);
         for (int d = 0; d < n; ++d)
@@ -1885,6 +1890,21 @@ wb_status wb_get_matrix_rows(wb_ctx *c, int n_rows, const int32_t *rows, double 
     return guarded(c, [&] {
         require(n_rows >= 0 && (rows || n_rows == 0), WB_E_ARG, "bad row list");
         copy_rows(c, n_rows, rows, G, H);
+    });
+}
+
+wb_status wb_get_coarse_matrix(wb_ctx *c, int *nc, int *agg, double *Ac)
+{
+    return guarded(c, [&] {
+        require(nc && agg, WB_E_ARG, "null output");
+        require(c->cs_on, WB_E_STATE, "no coarse space in use (build one with a solve)");
+        *nc = c->cs_nc;
+        *agg = c->cs_agg;
+        if (Ac) {
+            HIPCHK(hipMemcpyAsync(Ac, c->cs_Ac.p, sizeof(double) * c->cs_nc * c->cs_nc,
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
     });
 }
 

@@ -204,7 +204,7 @@ size_t coarse_invert_lds_bytes(int nc);
 void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
                          size_t ld, const void *A16, const float *rs16, const double *alpha,
                          const uint8_t *gmask, const uint8_t *constrained, double *AP,
-                         float *AP32, size_t ldap, const double *S, double *Ac, double *Aci,
+                         size_t ldap, const double *S, double *Ac, double *Aci,
                          float *Aci32, double *WT, float *WT32, int *status, hipStream_t s);
 void launch_coarse_pre(int is_float, int n, int agg, int nc, const void *X, size_t ldx,
                        const void *WT, size_t ldw, double *C, void *Tout, int k, hipStream_t s);
