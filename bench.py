@@ -34,7 +34,7 @@ def log(*a):
 
 
 def run_precision(mesh, bc, precision, steps, warmup, tol, ir_inner=1e-6, tile=16, pblock=64,
-                  inner_format=1, inner16=3e-4, coarse=128):
+                  inner_format=2, inner16=3e-4, coarse=128):
     import torch
 
     from wavebem_amd.bem import BEMProblem, Config
@@ -74,6 +74,8 @@ def run_precision(mesh, bc, precision, steps, warmup, tol, ir_inner=1e-6, tile=1
         f"iterations {stats[-1]['iterations']}, sweeps {stats[-1]['ir_sweeps']}")
     out = dict(ms_per_step=ms, assembly_ms=t_asm * 1e3 / steps, stats=stats[-1], kernels=ks,
                phi=phi_d.cpu().numpy(), dphi=dphi_d.cpu().numpy(), device_bytes=p.device_bytes())
+    if precision == "fp32_ir" and inner_format == 2:
+        out["inner_info"] = p.inner_operator_info()
     p.close()
     del bc_d, phi_d, dphi_d
     torch.cuda.empty_cache()
@@ -376,8 +378,9 @@ def main():
     ap.add_argument("--only", choices=["fp64", "fp32_ir"], default=None)
     ap.add_argument("--ir-inner16", type=float, default=3e-4,
                     help="FP32-IR with the FP16 inner operator: inner reduction per sweep")
-    ap.add_argument("--ir-inner-format", type=int, default=1, choices=(0, 1),
-                    help="FP32-IR inner Krylov operator: 0 FP32 (hi), 1 row-scaled FP16 copy")
+    ap.add_argument("--ir-inner-format", type=int, default=2, choices=(0, 1, 2),
+                    help="FP32-IR inner Krylov operator: 0 FP32 (hi), 1 row-scaled FP16 copy, "
+                         "2 compressed FP16 copy (dense near blocks, low-rank far blocks)")
     ap.add_argument("--ir-inner", type=float, default=1e-7,
                     help="FP32-IR inner GMRES relative residual reduction per sweep")
     ap.add_argument("--tile", type=int, default=16, help="max columns per assembly tile (1..16)")
@@ -486,7 +489,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": (("fp16+fp32/bf16+fp64-ir" if args.ir_inner_format == 1 else "fp32+fp64-ir")
+        "dtype": (("fp32+fp64-ir" if args.ir_inner_format == 0 else "fp16+fp32/bf16+fp64-ir")
                   if "fp32_ir" in res else "f64"),
         "data": "synthetic",
         "config": {"workload": f"wigley_closed_halfbox_{args.workload}", "n_dofs": n,
@@ -494,12 +497,19 @@ def main():
                    "gmres_tol_abs_preconditioned": args.tol, "gmres_restart": 98,
                    "preconditioner": f"block_jacobi_{args.precond_block}"
                                      + (f"+coarse_{args.coarse}" if args.coarse else ""),
-                   "froude": 0.3, "settle_s_before_each_precision": args.settle},
+                   "froude": 0.3, "settle_s_before_each_precision": args.settle,
+                   "fp32_ir_inner_operator": {0: "fp32", 1: "fp16_row_scaled",
+                                              2: "compressed_fp16_lowrank"}[args.ir_inner_format]},
     }
     if "fp64" in res:
         line["fp64_ms_per_step"] = round(res["fp64"]["ms_per_step"], 3)
         line["fp64_iterations"] = res["fp64"]["stats"]["iterations"]
         line["fp64_true_rel_residual"] = res["fp64"]["stats"]["true_rel_residual"]
+    if "fp32_ir" in res and "inner_info" in res["fp32_ir"]:
+        info = res["fp32_ir"]["inner_info"]
+        line["fp32_ir_inner_operator"] = dict(
+            info, fp16_copy_bytes=2 * n * n,
+            compression_vs_fp16=round(2 * n * n / max(info["bytes_per_product"], 1), 2))
     if "fp32_ir" in res:
         st = res["fp32_ir"]["stats"]
         line["fp32_ir_inner_iterations"] = st["iterations"]
@@ -553,6 +563,8 @@ def main():
                         "share_of_step": round(asm_ms / head["ms_per_step"], 3)}
     if "fp32_ir" not in res:
         gname, esz, xsz, kname = "gemv64", 8, 8, "k_gemv_f64"
+    elif args.ir_inner_format == 2:
+        gname, esz, xsz, kname = "gemvc", 0, 4, "k_hm_gemv_t + k_hm_gemv + k_hm_finish"
     elif args.ir_inner_format == 1:
         gname, esz, xsz, kname = "gemv16", 2, 4, "k_gemv_h16"
     else:
@@ -560,10 +572,14 @@ def main():
     g = ks[gname]
     avg_ms = kernel_avg_ms(ks, gname)
     rows_per_launch = g["rows"] / max(g["launches"], 1)
-    # matrix rows + x read + y written (the FP16 rows also read one 4-byte row scale)
-    bytes_per_launch = rows_per_launch * n * esz + 2 * n * xsz
-    if esz == 2:
-        bytes_per_launch += rows_per_launch * 4
+    if gname == "gemvc":
+        # the compressed operator's dense blocks and low-rank factors, x read, y written
+        bytes_per_launch = head["inner_info"]["bytes_per_product"] + 2 * n * xsz
+    else:
+        # matrix rows + x read + y written (the FP16 rows also read one 4-byte row scale)
+        bytes_per_launch = rows_per_launch * n * esz + 2 * n * xsz
+        if esz == 2:
+            bytes_per_launch += rows_per_launch * 4
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     line["roofline_gemv"] = {"kernel": kname, "bound": "hbm",
                              "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -64,12 +64,17 @@ typedef struct {
                                       max(this * rho, tol / 2), rho = outer residual */
     int assembly_tile;             /* max columns per assembly column tile (1..16) */
     int ir_inner_format;           /* FP32_IR: the inner Krylov solves run on 0 = the FP32
-                                      (hi) operator, 1 (default) = a row-scaled FP16 copy
-                                      of it (2 bytes per entry; the refinement residual
-                                      keeps the FP32 hi + bf16 lo operator; a sweep that
-                                      fails to halve the residual switches the remaining
-                                      sweeps to the FP32 operator) */
-    double ir_inner16_reduction;   /* FP32_IR with ir_inner_format = 1: the inner solve
+                                      (hi) operator, 1 = a row-scaled FP16 copy of it (2
+                                      bytes per entry), 2 (default) = a compressed copy:
+                                      blocks of
+                                      clusters of <= 256 dofs, FP16, dense where the
+                                      clusters touch, else low rank (Q B, rank <= 12) where
+                                      that meets ir_inner_lr_tol.  The refinement residual
+                                      keeps the FP32 hi + bf16 lo operator whatever the
+                                      format; with 1 or 2 a sweep that fails to halve the
+                                      residual switches the remaining sweeps to the FP32
+                                      operator */
+    double ir_inner16_reduction;   /* FP32_IR with ir_inner_format = 1 or 2: the inner solve
                                       stops at max(this * rho, tol / 2) */
     int precond_coarse;            /* block Jacobi only: at most this many coarse unknowns
                                       (<= 192, default 128; 0 = off) for a coarse-space
@@ -82,6 +87,9 @@ typedef struct {
                                       logically structured patches of the mesh as strips of
                                       grid lines (every cell evaluated about once); 0: column
                                       tiles everywhere.  Same matrices either way */
+    double ir_inner_lr_tol;        /* ir_inner_format = 2: a block is stored low rank when the
+                                      range finder's estimate of its entry RMS error is at
+                                      most this times the block's largest entry (1e-4) */
 } wb_config;
 
 typedef struct {
@@ -232,6 +240,19 @@ wb_status wb_compute_constraints(wb_ctx *ctx, const double *tmp_rhs, const doubl
  * replaced. */
 wb_status wb_constrained_vmult(wb_ctx *ctx, double *dst, const double *src);
 
+/* No counterpart in the reference (it solves in FP64 only).  The operator of the inner
+ * Krylov solves of an FP32_IR context (per ir_inner_format: the FP32 hi operator, its
+ * row-scaled FP16 copy or its compressed copy) applied to src in FP32 vectors, constrained
+ * rows as in wb_constrained_vmult; src and dst in caller numbering.  For inspection of the
+ * inner operator's error; WB_E_UNSUPPORTED in FP64 contexts. */
+wb_status wb_inner_vmult(wb_ctx *ctx, double *dst, const double *src);
+
+/* No counterpart in the reference.  The compressed inner operator (ir_inner_format 2) of
+ * the current assembly and masks, built if needed: info[0] clusters, [1] dense blocks,
+ * [2] low-rank blocks, [3] zero blocks, [4] sum of the low-rank blocks' ranks, [5] bytes
+ * of factors and dense blocks one product reads.  WB_E_UNSUPPORTED for other formats. */
+wb_status wb_inner_operator_info(wb_ctx *ctx, int64_t *info);
+
 /* BEMProblem::solve_system (source/bem_problem.cc:821-895): alpha, rhs from tmp_rhs,
  * distribute_rhs, preconditioned GMRES (sol = 0 start), then phi(i) = sol(i) where
  * surface_nodes(i) == 0 else dphi_dn(i) = sol(i).  Uses the installed constraints.
@@ -311,7 +332,9 @@ typedef enum {
     WB_K_GEMV32 = 3,    /* column-selected GEMV, FP32 (hi) matrices, inner solver */
     WB_K_GEMVHL = 4,    /* column-selected GEMV, FP32 hi + bf16 lo, FP64 accumulation */
     WB_K_GEMV16 = 5,    /* GEMV of the row-scaled FP16 operator, inner solver */
-    WB_K_COUNT = 6
+    WB_K_GEMVC = 6,     /* product of the compressed operator (ir_inner_format 2) */
+    WB_K_INNER = 7,     /* build of the inner operator (FP16 copy or compressed copy) */
+    WB_K_COUNT = 8
 } wb_kernel_kind;
 
 typedef struct {
@@ -353,6 +376,16 @@ int wb_host_strip_plan(int n_dofs, int n_cells, const double *support_points,
                        const int32_t *cell_dofs, const int32_t *cell_material, int strips,
                        int32_t *sizes, int32_t *tasks, int32_t *gaps, int32_t *gap_cells,
                        int32_t *cell_int_s, int32_t *perm);
+
+/* No counterpart in the reference.  The cluster plan of the compressed inner operator
+ * (ir_inner_format 2) for inspection: *n_clusters always; with non-NULL buffers (of at
+ * least that many clusters) cl_size, cl_idx (256 device indices per cluster, -1 past its
+ * size), near (n_clusters^2 flags: the block is dense by geometry) and perm (dof ->
+ * device index). */
+int wb_host_cluster_plan(int n_dofs, int n_cells, const double *support_points,
+                         const int32_t *cell_dofs, const int32_t *cell_material,
+                         int32_t *n_clusters, int32_t *cl_size, int32_t *cl_idx, uint8_t *near,
+                         int32_t *perm);
 
 /* BEMProblem::compute_constraints (source/bem_problem.cc:990-1105) rules, closed as by
  * ConstraintMatrix::close; returns the number of lines, or minus a wb_status. */

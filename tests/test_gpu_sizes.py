@@ -79,14 +79,16 @@ def test_solid_angles_40k(gpu_available):
 
 
 def test_fp32_ir_matches_fp64_40k(gpu_available):
-    """phi and dphi/dn of the FP32-IR solves (inner Krylov on the row-scaled FP16 copy of the
-    operator, the default, and on the FP32 operator) within 1e-8 (relative to the largest value) of the FP64 solve of
-    the same constrained system."""
+    """phi and dphi/dn of the FP32-IR solves (inner Krylov on the FP32 operator, on its
+    row-scaled FP16 copy and on the default inner operator) within 1e-8 (relative to the
+    largest value) of the FP64 solve of the same constrained system."""
     m, ptr, idx = _mesh("40k")
     bc, _ = M.wigley_bc(m, seed=0)
     out = {}
     for name, kw in (("fp64", dict(precision="fp64")),
                      ("fp32_ir_inner32", dict(precision="fp32_ir", ir_inner_format=0)),
+                     ("fp32_ir_inner16", dict(precision="fp32_ir", ir_inner_format=1)),
+                     ("fp32_ir_compressed", dict(precision="fp32_ir", ir_inner_format=2)),
                      ("fp32_ir", dict(precision="fp32_ir"))):
         phi = bc * m.surface_nodes
         dphi = bc * m.other_nodes
@@ -101,7 +103,7 @@ def test_fp32_ir_matches_fp64_40k(gpu_available):
     N = m.other_nodes > 0
     S = ~N
     p64, d64 = out["fp64"]
-    for name in ("fp32_ir_inner32", "fp32_ir"):
+    for name in ("fp32_ir_inner32", "fp32_ir_inner16", "fp32_ir_compressed", "fp32_ir"):
         p32, d32 = out[name]
         e_phi = np.abs(p32[N] - p64[N]).max() / np.abs(p64[N]).max()
         e_dn = np.abs(d32[S] - d64[S]).max() / np.abs(d64[S]).max()

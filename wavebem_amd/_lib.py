@@ -38,6 +38,7 @@ class WbConfig(ctypes.Structure):
         ("ir_inner16_reduction", ctypes.c_double),
         ("precond_coarse", ctypes.c_int),
         ("assembly_strips", ctypes.c_int),
+        ("ir_inner_lr_tol", ctypes.c_double),
     ]
 
 
@@ -64,14 +65,15 @@ class WbSolveStats(ctypes.Structure):
         return d
 
 
-WB_K_NAMES = ["assemble", "singular", "gemv64", "gemv32", "gemvhl", "gemv16"]
+WB_K_NAMES = ["assemble", "singular", "gemv64", "gemv32", "gemvhl", "gemv16", "gemvc",
+              "inner_build"]
 
 
 class WbKernelStats(ctypes.Structure):
     _fields_ = [
-        ("launches", ctypes.c_int * 6),
-        ("total_ms", ctypes.c_double * 6),
-        ("rows", ctypes.c_longlong * 6),
+        ("launches", ctypes.c_int * len(WB_K_NAMES)),
+        ("total_ms", ctypes.c_double * len(WB_K_NAMES)),
+        ("rows", ctypes.c_longlong * len(WB_K_NAMES)),
     ]
 
     def as_dict(self):
@@ -101,6 +103,11 @@ _SIGS = {
     "wb_release_matrices": (ctypes.c_int, [_vp]),
     "wb_get_matrices": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_get_matrix_rows": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _dp, _dp]),
+    "wb_inner_vmult": (ctypes.c_int, [_vp, _dp, _dp]),
+    "wb_inner_operator_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "wb_host_cluster_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _i32p, _i32p, _i32p,
+                                            _i32p, _i32p, ctypes.POINTER(ctypes.c_uint8),
+                                            _i32p]),
     "wb_get_coarse_matrix": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int), _dp]),
     "wb_solve_adjoint": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_void_p]),

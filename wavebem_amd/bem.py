@@ -60,10 +60,11 @@ class Config:
     ir_max_sweeps: int = 10
     ir_inner_reduction: float = 1e-7
     assembly_tile: int = 16
-    ir_inner_format: int = 1
+    ir_inner_format: int = 2  # 0 FP32 hi, 1 row-scaled FP16 copy, 2 compressed FP16 copy
     ir_inner16_reduction: float = 3e-4
     precond_coarse: int = 128  # coarse-space unknowns of the block-Jacobi preconditioner
     assembly_strips: int = 1  # strip sweeps on structured patches (0: column tiles only)
+    ir_inner_lr_tol: float = 1e-4  # low-rank tolerance of the compressed inner operator
 
     def to_c(self) -> L.WbConfig:
         c = L.WbConfig()
@@ -85,6 +86,7 @@ class Config:
         c.ir_inner16_reduction = self.ir_inner16_reduction
         c.precond_coarse = self.precond_coarse
         c.assembly_strips = self.assembly_strips
+        c.ir_inner_lr_tol = self.ir_inner_lr_tol
         return c
 
 
@@ -186,6 +188,21 @@ class BEMProblem:
         self._check(L.lib().wb_get_matrix_rows(self._ctx, int(r.size), _i(r), _d(G), _d(H)),
                     "wb_get_matrix_rows")
         return G, H
+
+    def inner_vmult(self, src):
+        """The inner Krylov operator of an FP32-IR context applied to src (FP32 vectors)."""
+        s = _f64(src, self.n)
+        dst = np.empty(self.n)
+        self._check(L.lib().wb_inner_vmult(self._ctx, _d(dst), _d(s)), "wb_inner_vmult")
+        return dst
+
+    def inner_operator_info(self):
+        """Blocks and bytes of the compressed inner operator (ir_inner_format 2)."""
+        info = (ctypes.c_int64 * 6)()
+        self._check(L.lib().wb_inner_operator_info(self._ctx, info), "wb_inner_operator_info")
+        keys = ("clusters", "dense_blocks", "lowrank_blocks", "zero_blocks", "rank_sum",
+                "bytes_per_product")
+        return dict(zip(keys, [int(v) for v in info]))
 
     def coarse_matrix(self):
         """(Ac, agg): the coarse operator P^T A P of the last preconditioner build
@@ -441,6 +458,30 @@ def host_strip_plan(mesh: SurfaceMesh, strips: int = 1):
         raise WaveBemError(st, "wb_host_strip_plan")
     return dict(tasks=tasks[:nt], gaps=gaps[:ng], gap_cells=gcells[:ng], cell_int_s=cis,
                 perm=perm, cells=int(sizes[2]), strip_dofs=int(sizes[3]), n_tiles=int(sizes[4]))
+
+
+def host_cluster_plan(mesh: SurfaceMesh):
+    """The cluster plan of the compressed inner operator (wb_host_cluster_plan)."""
+    n, nc = mesh.n_dofs, mesh.n_cells
+    X = _f64(mesh.support_points).reshape(-1)
+    cd = np.ascontiguousarray(mesh.cell_dofs, dtype=np.int32).reshape(-1)
+    mat = np.ascontiguousarray(mesh.cell_material, dtype=np.int32)
+    ncl = ctypes.c_int32()
+    st = L.lib().wb_host_cluster_plan(n, nc, _d(X), _i(cd), _i(mat), ctypes.byref(ncl), None,
+                                      None, None, None)
+    if st != L.WB_OK:
+        raise WaveBemError(st, "wb_host_cluster_plan")
+    k = ncl.value
+    size = np.zeros(k, dtype=np.int32)
+    idx = np.zeros((k, 256), dtype=np.int32)
+    near = np.zeros((k, k), dtype=np.uint8)
+    perm = np.zeros(n, dtype=np.int32)
+    st = L.lib().wb_host_cluster_plan(n, nc, _d(X), _i(cd), _i(mat), ctypes.byref(ncl), _i(size),
+                                      _i(idx), near.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                      _i(perm))
+    if st != L.WB_OK:
+        raise WaveBemError(st, "wb_host_cluster_plan")
+    return dict(size=size, idx=idx, near=near.astype(bool), perm=perm)
 
 
 def host_mesh_plan(mesh: SurfaceMesh):

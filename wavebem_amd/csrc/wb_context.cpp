@@ -122,7 +122,16 @@ struct wb_ctx {
     long h16_assembly_ver = -1, h16_mask_ver = -1;
     int h16_agg = 0;        // aggregate size the FP16 copy was built for (0: none)
     bool h16_sums = false;  // cs_S holds its aggregate sums
-    int float_mode = 1;  // operator mode of the FP32-vector solves: 1 = FP32 hi, 3 = FP16
+    // compressed copy of the FP32 vmult operator (ir_inner_format 2, wb_hmat.hip): the
+    // cluster plan of the mesh, then per assembly and masks the block ranks, scales and
+    // factors
+    HmatPlan hmp;
+    DevBuf hm_idx, hm_size, hm_seg, hm_near, hm_rank, hm_scale, hm_Y, hm_Mx, hm_Q, hm_B, hm_D,
+        hm_T, hm_part;
+    long hm_assembly_ver = -1, hm_mask_ver = -1;
+    int hm_agg = 0;         // aggregate size of the coarse sums written by the build (0: none)
+    int float_mode = 1;  // operator mode of the FP32-vector solves: 1 = FP32 hi, 3 = FP16,
+                         // 4 = compressed
     // device: solver work
     DevBuf V, z, w, P1, P2, Pn, hbuf, hred, vx, vb, vr, vt, vs;   // double / raw
     DevBuf V32, z32, w32, r32, d32;
@@ -170,7 +179,8 @@ struct wb_ctx {
             &ct_ptr, &ct_src, &ct_coef, &gt_part, &vjp_w, &vjp_phi, &vjp_psi, &vjp_rows,
             &vjp_cstate, &vjp_p1, &vjp_p2, &vjp_xg, &vjp_vbar, &vjp_xs, &vjp_spart, &inc_ptr,
             &inc_item, &pj_emass, &pj_erhs, &pj_u, &pj_B, &pj_X, &pj_R, &pj_Z, &pj_P, &pj_Q,
-            &pj_dinv, &pj_part, &pj_part2, &pj_state, &A16, &rs16};
+            &pj_dinv, &pj_part, &pj_part2, &pj_state, &A16, &rs16, &hm_idx, &hm_size, &hm_seg,
+            &hm_near, &hm_rank, &hm_scale, &hm_Y, &hm_Mx, &hm_Q, &hm_B, &hm_D, &hm_T, &hm_part};
         for (DevBuf *b : mw.all()) v.push_back(b);
         return v;
     }
@@ -283,6 +293,7 @@ static void cell_records(wb_ctx *c)
 }
 static void alloc_matrices(wb_ctx *c);
 static void ensure_h16(wb_ctx *c);
+static void ensure_hm(wb_ctx *c);
 
 // coarse aggregates: a multiple of 8 dofs, at least 64, at most precond_coarse of them
 static void coarse_shape(const wb_ctx *c, int &agg, int &nc)
@@ -363,6 +374,16 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
     const double s_sel = which == 0 ? 1.0 : -1.0, s_uns = which == 0 ? -1.0 : 1.0;
     const int *rows = free_rows_only ? c->free_rows.as<int>() : nullptr;
     const int nr = free_rows_only ? c->n_free : c->n;
+    if (mode == 4) {
+        require(which == 0, WB_E_STATE, "the compressed operator holds the vmult selection only");
+        KernelScope ks(c, WB_K_GEMVC, c->n);
+        launch_hm_gemv(c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hmp.ncl,
+                       c->hm_rank.as<uint8_t>(), c->hm_scale.as<float>(), c->hm_Q.p, c->hm_B.p,
+                       c->hm_D.p, c->n, (const float *)x, diag, (const float *)xd,
+                       c->hm_T.as<float>(), c->hm_part.as<float>(), c->vlen, (float *)y,
+                       c->stream);
+        return;
+    }
     if (mode == 3) {
         require(which == 0, WB_E_STATE, "the FP16 operator holds the vmult selection only");
         KernelScope ks(c, WB_K_GEMV16, nr);
@@ -377,10 +398,10 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
 }
 
 // y = ConstrainedOperator(A) x in FP64 (mode 0 or 2) or FP32 vectors (mode 1: FP32
-// operator, mode 3: FP16 operator)
+// operator, mode 3: FP16 operator, mode 4: compressed operator)
 static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
 {
-    const int is_float = mode == 1 || mode == 3;
+    const int is_float = mode == 1 || mode == 3 || mode == 4;
     if (c->op_transpose) {
         // A_c^T x: transposed product over the unconstrained rows (the diagonal alpha term
         // of those rows included), then the transposed constraint lines
@@ -567,13 +588,15 @@ static void ensure_precond(wb_ctx *c)
         // FP32-IR with the FP16 inner operator: A P from its FP16 copy (a third of the
         // bytes of the FP32 hi part; a preconditioner needs no more)
         const bool from16 = c->cfg.precision == WB_PREC_FP32_IR && c->cfg.ir_inner_format == 1;
+        const bool fromhm = c->cfg.precision == WB_PREC_FP32_IR && c->cfg.ir_inner_format == 2;
         if (from16) ensure_h16(c);
+        if (fromhm) ensure_hm(c);
         launch_coarse_build(c->n, agg, nc, c->G.p, c->H.p, c->cfg.precision, c->ld,
                             from16 ? c->A16.p : nullptr,
                             from16 ? c->rs16.as<float>() : nullptr, c->alpha.as<double>(),
                             c->gmask.as<uint8_t>(),
                             c->constrained.as<uint8_t>(), c->cs_AP.as<double>(), c->vlen,
-                            from16 && c->h16_sums ? c->cs_S.as<double>() : nullptr,
+                            (from16 && c->h16_sums) || fromhm ? c->cs_S.as<double>() : nullptr,
                             c->cs_Ac.as<double>(),
                             c->cs_Aci.as<double>(), c->cs_Aci32.as<float>(), c->cs_WT.as<double>(),
                             c->cs_WT32.as<float>(), c->cs_status.as<int>(), c->stream);
@@ -714,7 +737,7 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
 static double ir_inner_target(const wb_ctx *c, double rho)
 {
     const double red =
-        c->float_mode == 3 ? c->cfg.ir_inner16_reduction : c->cfg.ir_inner_reduction;
+        c->float_mode >= 3 ? c->cfg.ir_inner16_reduction : c->cfg.ir_inner_reduction;
     return std::max(red * rho, 0.5 * c->cfg.solver_tolerance);
 }
 
@@ -732,14 +755,53 @@ static void ensure_h16(wb_ctx *c)
     c->A16.ensure(entries * 2);
     c->rs16.ensure(sizeof(float) * c->vlen);
     if (agg > 0) c->cs_S.ensure(sizeof(double) * (size_t)nc * c->vlen);
-    c->h16_sums = launch_build_h16(c->G.as<float>(), c->H.as<float>(), c->ld, c->n,
-                                   c->segs.as<int>(), c->n_segs, c->gmask.as<uint8_t>(),
-                                   c->A16.p, c->rs16.as<float>(), agg, nc,
-                                   agg > 0 ? c->cs_S.as<double>() : nullptr, c->vlen, c->stream);
-    HIPCHK(hipGetLastError());
+    {
+        KernelScope ks(c, WB_K_INNER, c->n);
+        c->h16_sums = launch_build_h16(c->G.as<float>(), c->H.as<float>(), c->ld, c->n,
+                                       c->segs.as<int>(), c->n_segs, c->gmask.as<uint8_t>(),
+                                       c->A16.p, c->rs16.as<float>(), agg, nc,
+                                       agg > 0 ? c->cs_S.as<double>() : nullptr, c->vlen,
+                                       c->stream);
+        HIPCHK(hipGetLastError());
+    }
     c->h16_assembly_ver = c->assembly_ver;
     c->h16_mask_ver = c->mask_ver;
     c->h16_agg = agg;
+}
+
+// The compressed copy of the FP32 vmult operator for the current assembly and masks.
+static void ensure_hm(wb_ctx *c)
+{
+    int agg = 0, nc = 0;
+    if (c->cfg.precond_coarse > 0) coarse_shape(c, agg, nc);
+    if (c->hm_assembly_ver == c->assembly_ver && c->hm_mask_ver == c->mask_ver &&
+        c->hm_agg == agg)
+        return;
+    require(c->hmp.ncl > 0, WB_E_STATE, "no cluster plan (FP32_IR contexts plan at wb_set_mesh)");
+    const size_t ncl = (size_t)c->hmp.ncl, nb = ncl * ncl;
+    c->hm_rank.ensure(nb);
+    c->hm_scale.ensure(sizeof(float) * nb);
+    c->hm_Y.ensure(sizeof(float) * (size_t)c->n * ncl * HM_SK);  // [J][row][k]
+    c->hm_Mx.ensure(sizeof(float) * (size_t)c->n * ncl);
+    c->hm_Q.ensure(2 * nb * HM_RMAX * HM_CL);
+    c->hm_B.ensure(2 * nb * HM_RMAX * HM_CL);
+    c->hm_D.ensure(2 * nb * HM_CL * HM_CL);
+    c->hm_T.ensure(sizeof(float) * nb * HM_RMAX);
+    c->hm_part.ensure(sizeof(float) * c->vlen * hm_chunks(c->hmp.ncl));
+    if (agg > 0) c->cs_S.ensure(sizeof(double) * (size_t)nc * c->vlen);
+    {
+        KernelScope ks(c, WB_K_INNER, c->n);
+        launch_hm_build(c->G.as<float>(), c->H.as<float>(), c->ld, c->n, c->gmask.as<uint8_t>(),
+                        c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hm_seg.as<int>(),
+                        c->hmp.ncl, c->hm_near.as<uint8_t>(), (float)c->cfg.ir_inner_lr_tol,
+                        c->hm_Y.as<float>(), c->hm_Mx.as<float>(), c->hm_rank.as<uint8_t>(),
+                        c->hm_scale.as<float>(), c->hm_Q.p, c->hm_B.p, c->hm_D.p, agg, nc,
+                        agg > 0 ? c->cs_S.as<double>() : nullptr, c->vlen, c->stream);
+        HIPCHK(hipGetLastError());
+    }
+    c->hm_assembly_ver = c->assembly_ver;
+    c->hm_mask_ver = c->mask_ver;
+    c->hm_agg = agg;
 }
 
 // selects the operator of the FP32-vector solves for the lifetime of the guard
@@ -776,8 +838,9 @@ static void run_solver(wb_ctx *c, wb_solve_stats *st)
     } else {
         // FP64 iterative refinement around an FP32-vector GMRES on the FP32 (hi) operator
         // or on its row-scaled FP16 copy.
-        const int fmode = c->cfg.ir_inner_format == 1 ? 3 : 1;
+        const int fmode = c->cfg.ir_inner_format == 2 ? 4 : (c->cfg.ir_inner_format == 1 ? 3 : 1);
         if (fmode == 3) ensure_h16(c);
+        if (fmode == 4) ensure_hm(c);
         FloatMode fm(c, fmode);
         double rho_prev = 0.0;
         const size_t vb4 = sizeof(float) * c->vlen;
@@ -806,7 +869,7 @@ static void run_solver(wb_ctx *c, wb_solve_stats *st)
                 break;
             // an FP16 sweep that no longer halves the residual hands over to the FP32
             // operator for the remaining sweeps
-            if (c->float_mode == 3 && sweeps > 0 && !(rho < 0.5 * rho_prev)) c->float_mode = 1;
+            if (c->float_mode >= 3 && sweeps > 0 && !(rho < 0.5 * rho_prev)) c->float_mode = 1;
             rho_prev = rho;
             // inner: A32 d = r from d = 0, down to the larger of ir_inner_reduction * rho and
             // what the outer tolerance still asks for (half of it, leaving room for the
@@ -1301,8 +1364,9 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
         HIPCHK(hipMemcpyAsync(M.R.p, M.B.p, d8, hipMemcpyDeviceToDevice, c->stream));
         int sweeps = 0, total = 0;
         std::vector<int> done(k, 0);
-        // inner solves on the FP16 operator (ir_inner_format = 1) or the FP32 one
-        const int fmode = c->cfg.ir_inner_format == 1 ? 3 : 1;
+        // inner solves on the FP16 operator (ir_inner_format 1; also for 2, whose compressed
+        // copy has no several-vector product) or the FP32 one
+        const int fmode = c->cfg.ir_inner_format >= 1 ? 3 : 1;
         if (fmode == 3) ensure_h16(c);
         FloatMode fm(c, fmode);
         std::vector<double> prev(k, 0.0);
@@ -1399,10 +1463,11 @@ void wb_config_default(wb_config *cfg)
     cfg->ir_max_sweeps = 10;
     cfg->ir_inner_reduction = 1e-7;
     cfg->assembly_tile = 16;
-    cfg->ir_inner_format = 1;
+    cfg->ir_inner_format = 2;
     cfg->ir_inner16_reduction = 3e-4;
     cfg->precond_coarse = CS_MAX64;
     cfg->assembly_strips = 1;
+    cfg->ir_inner_lr_tol = 1e-4;
 }
 
 const char *wb_version(void) { return "wavebem_amd 0.1 (gfx950)"; }
@@ -1424,7 +1489,8 @@ wb_status wb_create(wb_ctx **out, const wb_config *cfg)
         (k.precision != WB_PREC_FP64 && k.precision != WB_PREC_FP32_IR) ||
         k.precond_block < 1 || k.precond_block > 128 || k.solver_max_steps < 0 ||
         k.assembly_tile < 1 || k.assembly_tile > 16 ||
-        (k.ir_inner_format != 0 && k.ir_inner_format != 1) ||
+        k.ir_inner_format < 0 || k.ir_inner_format > 2 ||
+        !(k.ir_inner_lr_tol > 0 && k.ir_inner_lr_tol < 1) ||
         !(k.ir_inner16_reduction > 0 && k.ir_inner16_reduction < 1) ||
         k.precond_coarse < 0 || k.precond_coarse > CS_MAX ||
         (k.assembly_strips != 0 && k.assembly_strips != 1) ||
@@ -1510,6 +1576,16 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         upload(c->tile_cell, p.tile_cell.data(), p.tile_cell.size(), s);
         upload(c->tile_cols, p.tile_cols.data(), p.tile_cols.size(), s);
         upload(c->tile_start, p.tile_start.data(), p.tile_start.size(), s);
+        // cluster plan of the compressed inner operator (FP32-IR contexts)
+        c->hmp = HmatPlan();
+        if (c->cfg.precision == WB_PREC_FP32_IR) {
+            plan_hmat(p, Xi.data(), c->hmp);
+            upload(c->hm_idx, c->hmp.cl_idx.data(), c->hmp.cl_idx.size(), s);
+            upload(c->hm_size, c->hmp.cl_size.data(), c->hmp.cl_size.size(), s);
+            upload(c->hm_near, c->hmp.near.data(), c->hmp.near.size(), s);
+        }
+        c->hm_assembly_ver = c->hm_mask_ver = -1;
+        c->hm_agg = 0;
         if (p.n_tasks > 0) {
             upload(c->st_task, p.st_task.data(), p.st_task.size(), s);
             upload(c->st_gap, p.st_gap.data(), p.st_gap.size(), s);
@@ -1683,6 +1759,20 @@ static void install_masks(wb_ctx *c, const double *surface, const double *other)
     }
     c->n_segs = (int)sg.size() / 4;
     upload(c->segs, sg.data(), sg.size(), c->stream);
+    // segments of the compressed operator's clusters: {first column, length, 0 = all
+    // from G, 1 = all from H, 2 = mixed, 0}
+    if (c->hmp.ncl > 0) {
+        std::vector<int> hs((size_t)c->hmp.ncl * HM_SEGS * 4, 0);
+        for (size_t q = 0; q < (size_t)c->hmp.ncl * HM_SEGS; ++q) {
+            const int a = c->hmp.cl_seg[2 * q], len = c->hmp.cl_seg[2 * q + 1];
+            int nh = 0;
+            for (int j = a; j < a + len; ++j) nh += oi[j] != 0.0;
+            hs[4 * q] = a;
+            hs[4 * q + 1] = len;
+            hs[4 * q + 2] = nh == 0 ? 0 : (nh == len ? 1 : 2);
+        }
+        upload(c->hm_seg, hs.data(), hs.size(), c->stream);
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_masks = true;
     c->mask_ver++;
@@ -1997,6 +2087,67 @@ wb_status wb_constrained_vmult(wb_ctx *c, double *dst, const double *src)
         ensure_work(c);
         upload_vec(c, c->vs, src);
         apply_constrained(c, c->cfg.precision == WB_PREC_FP64 ? 0 : 2, c->vs.p, c->vr.p);
+        download_vec(c, c->vr, dst);
+    });
+}
+
+wb_status wb_inner_operator_info(wb_ctx *c, int64_t *info)
+{
+    return guarded(c, [&] {
+        check_ready(c, true);
+        require(info != nullptr, WB_E_ARG, "null output");
+        require(c->cfg.precision == WB_PREC_FP32_IR && c->cfg.ir_inner_format == 2,
+                WB_E_UNSUPPORTED, "FP32_IR contexts with ir_inner_format 2 only");
+        ensure_hm(c);
+        const int ncl = c->hmp.ncl;
+        std::vector<uint8_t> rk((size_t)ncl * ncl);
+        HIPCHK(hipMemcpyAsync(rk.data(), c->hm_rank.p, rk.size(), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int64_t nd = 0, nl = 0, nz = 0, rs = 0, bytes = 0;
+        for (int I = 0; I < ncl; ++I)
+            for (int J = 0; J < ncl; ++J) {
+                const int r = rk[(size_t)I * ncl + J];
+                const int64_t a = c->hmp.cl_size[I], b = c->hmp.cl_size[J];
+                if (r == HM_DENSE) {
+                    ++nd;
+                    bytes += 2 * a * b;
+                } else if (r == 0) {
+                    ++nz;
+                } else {
+                    ++nl;
+                    rs += r;
+                    bytes += 2 * (int64_t)r * (a + b);
+                }
+            }
+        const int64_t v[6] = {ncl, nd, nl, nz, rs, bytes};
+        std::copy(v, v + 6, info);
+    });
+}
+
+wb_status wb_inner_vmult(wb_ctx *c, double *dst, const double *src)
+{
+    return guarded(c, [&] {
+        check_ready(c, true);
+        require(dst && src, WB_E_ARG, "null vector");
+        require(c->cfg.precision == WB_PREC_FP32_IR, WB_E_UNSUPPORTED,
+                "the inner operator exists in FP32_IR contexts only");
+        ensure_work(c);
+        const int fmode = c->cfg.ir_inner_format == 2 ? 4 : (c->cfg.ir_inner_format == 1 ? 3 : 1);
+        if (fmode == 3) ensure_h16(c);
+        if (fmode == 4) ensure_hm(c);
+        const size_t vb4 = sizeof(float) * c->vlen;
+        for (DevBuf *b : {&c->z32, &c->w32}) {
+            if (b->bytes < vb4) {
+                b->ensure(vb4);
+                HIPCHK(hipMemsetAsync(b->p, 0, vb4, c->stream));
+            }
+        }
+        upload_vec(c, c->vs, src);
+        launch_vec_ops(1, c->n, c->vs.p, nullptr, c->z32.p, 0, c->stream);
+        apply_constrained(c, fmode, c->z32.p, c->w32.p);
+        launch_vec_ops(6, c->n, nullptr, nullptr, c->vr.p, 0, c->stream);
+        launch_vec_ops(2, c->n, c->w32.p, nullptr, c->vr.p, 0, c->stream);
         download_vec(c, c->vr, dst);
     });
 }

@@ -67,6 +67,28 @@ int wb_host_strip_plan(int n, int nc, const double *support, const int32_t *cell
     return WB_OK;
 }
 
+int wb_host_cluster_plan(int n, int nc, const double *support, const int32_t *cell_dofs,
+                         const int32_t *cell_material, int32_t *n_clusters, int32_t *cl_size,
+                         int32_t *cl_idx, uint8_t *near, int32_t *perm)
+{
+    if (n <= 0 || nc <= 0 || !support || !cell_dofs || !n_clusters) return WB_E_ARG;
+    wb::MeshPlan P;
+    std::string e = wb::plan_mesh(n, nc, support, cell_dofs, cell_material, nullptr, nullptr,
+                                  wb::TILE_COLS, 1, P);
+    if (!e.empty()) return WB_E_ARG;
+    std::vector<double> Xi(3 * (size_t)n);
+    for (int d = 0; d < n; ++d)
+        for (int k = 0; k < 3; ++k) Xi[3 * (size_t)P.perm[d] + k] = support[3 * (size_t)d + k];
+    wb::HmatPlan H;
+    wb::plan_hmat(P, Xi.data(), H);
+    *n_clusters = H.ncl;
+    if (cl_size) std::copy(H.cl_size.begin(), H.cl_size.end(), cl_size);
+    if (cl_idx) std::copy(H.cl_idx.begin(), H.cl_idx.end(), cl_idx);
+    if (near) std::copy(H.near.begin(), H.near.end(), near);
+    if (perm) std::copy(P.perm.begin(), P.perm.end(), perm);
+    return WB_OK;
+}
+
 int wb_host_compute_constraints(int n, const int32_t *dn_ptr, const int32_t *dn_idx,
                                 const double *surface, const double *normals,
                                 const double *grads, const double *tmp_rhs, int cap_lines,

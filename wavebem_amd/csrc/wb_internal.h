@@ -81,11 +81,48 @@ struct MeshPlan {
     std::vector<int> cell_int_s;
     std::vector<int8_t> canon_sign;
     int strip_dofs = 0;                    // dofs covered by strips (device order [0, this))
+    std::vector<int> run_start;            // device index where each strip begins
 };
 
 std::string plan_mesh(int n, int nc, const double *support, const int32_t *cell_dofs,
                       const int32_t *cell_material, const int32_t *dn_ptr, const int32_t *dn_idx,
                       int tile_cols, int strips, MeshPlan &plan);
+
+// ---------------------------------------------------------------- compressed inner operator
+// The inner solves of FP32 iterative refinement may run on a compressed copy of the vmult
+// operator.  Its rows and columns are split into clusters of at most HM_CL dofs, each the
+// union of at most HM_SEGS segments (runs of at most HM_SEG consecutive device indices
+// inside one strip, or consecutive column tiles), grouped by geometric bisection; block
+// (I, J) of the cluster partition is stored dense (FP16) when the clusters' bounding boxes
+// touch, and otherwise as Q B with rank <= HM_RMAX when a randomized range finder (HM_SK
+// sign vectors) meets the tolerance, else dense too.
+constexpr int HM_SEG = 64, HM_SEGS = 4, HM_CL = HM_SEG * HM_SEGS, HM_SK = 16, HM_RMAX = 12;
+constexpr int HM_DENSE = 255;
+struct HmatPlan {
+    int ncl = 0;
+    std::vector<int> cl_idx;    // ncl x HM_CL device indices (-1 past the cluster's size)
+    std::vector<int> cl_size;   // ncl
+    std::vector<int> cl_seg;    // ncl x HM_SEGS x {first device index, length} (0, 0 unused)
+                                // (on the device with the column kind of the masks, int4)
+    std::vector<uint8_t> near;  // ncl x ncl: 1 = dense by geometry
+};
+void plan_hmat(const MeshPlan &P, const double *X_internal, HmatPlan &H);
+// Build (wb_hmat.hip): Y (ncl x n x HM_SK) and Mx (ncl x n) are work space; rank and
+// fscale (ncl^2), Q16 / B16 (ncl^2 x HM_RMAX x HM_CL halfs) and D16 (ncl^2 x HM_CL x HM_CL
+// halfs) hold the operator.  With S (agg >= HM_SEG), the build also writes S[a ldS + i] =
+// the sum of row i of the FP32 operator over the columns of aggregate a (a < nagg), the
+// coarse space's A P.  Product: y = A~ x + diag .* xd (FP32 vectors, all n rows); T
+// (ncl^2 x HM_RMAX) and ypart (hm_chunks(ncl) x ldp) floats are work space.
+int hm_chunks(int ncl);
+void launch_hm_build(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
+                     const int *cl_idx, const int *cl_size, const int *cl_seg, int ncl,
+                     const uint8_t *near, float tol, float *Y, float *Mx, uint8_t *rank,
+                     float *fscale, void *Q16, void *B16, void *D16, int agg, int nagg,
+                     double *S, size_t ldS, hipStream_t s);
+void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
+                    const float *fscale, const void *Q16, const void *B16, const void *D16,
+                    int n, const float *x, const double *diag, const float *xd, float *T,
+                    float *ypart, size_t ldp, float *y, hipStream_t s);
 
 // Constraint rules of BEMProblem::compute_constraints, caller numbering, closed.
 struct ConstraintSet {

@@ -277,6 +277,7 @@ struct StripBuild {
     std::vector<std::array<int, 3>> gaps;    // {cell mask, first column (index into order), line mask}
     std::vector<std::array<int, STRIP_W + 1>> gcells;  // cells of a gap by position, -1: none
     std::vector<std::array<int, 5>> canon;   // {cell, canonical vertices 0..3 (dofs)}
+    std::vector<int> strip_first;            // index into order where each strip starts
     long cells = 0;                          // cell evaluations of the schedule
 };
 
@@ -321,6 +322,7 @@ bool build_strips(const std::vector<int> &cells, const std::vector<int> &dofs,
     }
     const int ns = (Np + STRIP_W - 1) / STRIP_W;
     for (int s = 0; s < ns; ++s) {
+        B.strip_first.push_back((int)B.order.size());
         const int b0 = (int)((long)s * Np / ns), w = (int)((long)(s + 1) * Np / ns) - b0;
         auto cell_at = [&](int g, int k) {
             const int p0 = b0 + k - 1;
@@ -492,6 +494,7 @@ std::string plan_mesh(int n, int nc, const double *X, const int32_t *cd, const i
             const StripBuild &B = (ok0 && (!ok1 || B0.cells <= B1.cells)) ? B0 : B1;
             const int base = (int)order.size();
             const int goff = (int)P.st_gap.size() / 4;
+            for (int f : B.strip_first) P.run_start.push_back(base + f);
             for (int d : B.order) {
                 order.push_back(d);
                 assigned[d] = 1;
@@ -694,6 +697,115 @@ struct Line {
     double inhom = 0;
 };
 }  // namespace
+
+// ------------------------------------------------------------------- compressed operator plan
+void plan_hmat(const MeshPlan &P, const double *X, HmatPlan &H)
+{
+    H = HmatPlan();
+    const int n = P.n;
+    // segments: pieces of at most HM_SEG of each strip, then groups of whole column tiles
+    std::vector<int> seg_a, seg_b;
+    std::vector<int> starts = P.run_start;
+    std::sort(starts.begin(), starts.end());
+    starts.push_back(P.strip_dofs);
+    for (size_t k = 0; k + 1 < starts.size(); ++k)
+        for (int a = starts[k]; a < starts[k + 1]; a += HM_SEG) {
+            seg_a.push_back(a);
+            seg_b.push_back(std::min(a + HM_SEG, starts[k + 1]));
+        }
+    {
+        int a = P.strip_dofs;
+        for (int t = 0; t < P.n_tiles; ++t) {
+            const int e = P.tile_start[t + 1];
+            if (e - a > HM_SEG && P.tile_start[t] > a) {
+                seg_a.push_back(a);
+                seg_b.push_back(P.tile_start[t]);
+                a = P.tile_start[t];
+            }
+        }
+        if (a < n) {
+            seg_a.push_back(a);
+            seg_b.push_back(n);
+        }
+    }
+    const int ns = (int)seg_a.size();
+    std::vector<std::array<double, 3>> cen(ns);
+    for (int k = 0; k < ns; ++k) {
+        double c[3] = {0, 0, 0};
+        for (int i = seg_a[k]; i < seg_b[k]; ++i)
+            for (int d = 0; d < 3; ++d) c[d] += X[3 * (size_t)i + d];
+        for (int d = 0; d < 3; ++d) cen[k][d] = c[d] / (seg_b[k] - seg_a[k]);
+    }
+    // recursive bisection of the segments along the longest extent of their centroids;
+    // the left part takes a multiple of HM_SEGS segments, so that most leaves are full
+    std::vector<int> ids(ns);
+    for (int k = 0; k < ns; ++k) ids[k] = k;
+    std::vector<std::vector<int>> leaves;
+    std::vector<std::pair<int, int>> stack{{0, ns}};
+    while (!stack.empty()) {
+        auto [lo, hi] = stack.back();
+        stack.pop_back();
+        const int cnt = hi - lo;
+        if (cnt <= HM_SEGS) {
+            leaves.emplace_back(ids.begin() + lo, ids.begin() + hi);
+            continue;
+        }
+        double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+        for (int k = lo; k < hi; ++k)
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = std::min(mn[d], cen[ids[k]][d]);
+                mx[d] = std::max(mx[d], cen[ids[k]][d]);
+            }
+        int ax = 0;
+        for (int d = 1; d < 3; ++d)
+            if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+        std::stable_sort(ids.begin() + lo, ids.begin() + hi,
+                         [&](int a, int b) { return cen[a][ax] < cen[b][ax]; });
+        const int left = HM_SEGS * std::max(1, (cnt + HM_SEGS) / (2 * HM_SEGS));
+        stack.push_back({lo + left, hi});
+        stack.push_back({lo, lo + left});
+    }
+    const int ncl = (int)leaves.size();
+    H.ncl = ncl;
+    H.cl_idx.assign((size_t)ncl * HM_CL, -1);
+    H.cl_size.assign(ncl, 0);
+    H.cl_seg.assign((size_t)ncl * HM_SEGS * 2, 0);
+    std::vector<std::array<double, 6>> box(ncl);
+    for (int c = 0; c < ncl; ++c) {
+        for (size_t q = 0; q < leaves[c].size(); ++q) {
+            const int k = leaves[c][q];
+            H.cl_seg[((size_t)c * HM_SEGS + q) * 2] = seg_a[k];
+            H.cl_seg[((size_t)c * HM_SEGS + q) * 2 + 1] = seg_b[k] - seg_a[k];
+        }
+        int m = 0;
+        std::array<double, 6> b = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+        for (int k : leaves[c])
+            for (int i = seg_a[k]; i < seg_b[k]; ++i) {
+                H.cl_idx[(size_t)c * HM_CL + m++] = i;
+                for (int d = 0; d < 3; ++d) {
+                    b[d] = std::min(b[d], X[3 * (size_t)i + d]);
+                    b[3 + d] = std::max(b[3 + d], X[3 * (size_t)i + d]);
+                }
+            }
+        H.cl_size[c] = m;
+        box[c] = b;
+    }
+    // dense by geometry: the bounding boxes touch or overlap (gap below a billionth of
+    // the larger box)
+    H.near.assign((size_t)ncl * ncl, 0);
+    for (int I = 0; I < ncl; ++I)
+        for (int J = 0; J < ncl; ++J) {
+            double g2 = 0, d2 = 0;
+            for (int d = 0; d < 3; ++d) {
+                const double g = std::max(0.0, std::max(box[I][d] - box[J][3 + d],
+                                                        box[J][d] - box[I][3 + d]));
+                const double e = std::max(box[I][3 + d] - box[I][d], box[J][3 + d] - box[J][d]);
+                g2 += g * g;
+                d2 += e * e;
+            }
+            H.near[(size_t)I * ncl + J] = (I == J || g2 <= 1e-18 * d2) ? 1 : 0;
+        }
+}
 
 std::string build_constraints(int n, const int *dnp, const int *dni, const double *surface,
                               const double *normals, const double *grads, const double *tmp_rhs,
