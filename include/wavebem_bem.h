@@ -253,6 +253,16 @@ wb_status wb_inner_vmult(wb_ctx *ctx, double *dst, const double *src);
  * of factors and dense blocks one product reads.  WB_E_UNSUPPORTED for other formats. */
 wb_status wb_inner_operator_info(wb_ctx *ctx, int64_t *info);
 
+/* No counterpart in the reference.  wb_constrained_vmult with src and dst device pointers
+ * (caller numbering, n doubles each) that allocates no device memory, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops): its work space is
+ * a fixed static device array per GPU (two vectors of at most 262144 doubles), so
+ * n rounded up to 64 must not exceed 262144 (WB_E_ARG otherwise); the context must be
+ * assembled, with its masks and constraints installed; pure-Neumann operators are
+ * rejected (WB_E_UNSUPPORTED).  Returns after the product is complete.  Calls that
+ * overlap on one GPU share the work space and race. */
+wb_status wb_constrained_vmult_noalloc(wb_ctx *ctx, const double *src_dev, double *dst_dev);
+
 /* BEMProblem::solve_system (source/bem_problem.cc:821-895): alpha, rhs from tmp_rhs,
  * distribute_rhs, preconditioned GMRES (sol = 0 start), then phi(i) = sol(i) where
  * surface_nodes(i) == 0 else dphi_dn(i) = sol(i).  Uses the installed constraints.

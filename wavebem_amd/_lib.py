@@ -104,6 +104,7 @@ _SIGS = {
     "wb_get_matrices": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_get_matrix_rows": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _dp, _dp]),
     "wb_inner_vmult": (ctypes.c_int, [_vp, _dp, _dp]),
+    "wb_constrained_vmult_noalloc": (ctypes.c_int, [_vp, _vp, _vp]),
     "wb_inner_operator_info": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "wb_host_cluster_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _i32p, _i32p, _i32p,
                                             _i32p, _i32p, ctypes.POINTER(ctypes.c_uint8),

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwavebem_bem.so")
 SOURCES = ["wb_setup.cpp", "wb_host.cpp", "wb_assemble.hip", "wb_adjoint.hip", "wb_project.hip",
            "wb_gemv.hip", "wb_multi.hip", "wb_krylov.hip", "wb_linalg.hip", "wb_coarse.hip",
-           "wb_hmat.hip",
+           "wb_hmat.hip", "wb_noalloc.hip",
            "wb_context.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1501,6 +1501,8 @@ wb_status wb_create(wb_ctx **out, const wb_config *cfg)
     wb_status st = guarded(c, [&] {
         HIPCHK(hipSetDevice(k.device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        // the no-allocation entry point's work space on this device
+        HIPCHK(noalloc_setup(k.device));
         build_quadrature(k.quadrature_order, k.singular_quadrature_order, c->quad);
     });
     if (st != WB_OK) {
@@ -2149,6 +2151,37 @@ wb_status wb_inner_vmult(wb_ctx *c, double *dst, const double *src)
         launch_vec_ops(6, c->n, nullptr, nullptr, c->vr.p, 0, c->stream);
         launch_vec_ops(2, c->n, c->w32.p, nullptr, c->vr.p, 0, c->stream);
         download_vec(c, c->vr, dst);
+    });
+}
+
+wb_status wb_constrained_vmult_noalloc(wb_ctx *c, const double *src, double *dst)
+{
+    return guarded(c, [&] {
+        check_ready(c, true);
+        require(src && dst, WB_E_ARG, "null vector");
+        require(c->vlen <= (size_t)NOALLOC_MAX_N, WB_E_ARG,
+                "wb_constrained_vmult_noalloc: more than 262144 dofs (rounded up to 64) "
+                "exceed its fixed work space");
+        require(!c->pure_neumann, WB_E_UNSUPPORTED,
+                "wb_constrained_vmult_noalloc: pure-Neumann operators need a host norm");
+        require(!c->op_transpose, WB_E_STATE, "transposed operator active");
+        double *ws = noalloc_scratch(c->cfg.device);
+        require(ws != nullptr, WB_E_STATE, "no work space on this device");
+        double *xs = ws, *ys = ws + NOALLOC_MAX_N;  // source and product, device order
+        hipStream_t s = c->stream;
+        // the array keeps the last call's contents: clear the source's padding columns
+        launch_vec_ops(6, (int)c->vlen, nullptr, nullptr, xs, 0, s);
+        launch_permute(c->n, c->iperm_d.as<int>(), src, xs, 1, s);
+        const int mode = c->cfg.precision == WB_PREC_FP64 ? 0 : 2;
+        launch_gemv_sel(mode, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld, c->segs.as<int>(),
+                        c->n_segs, c->gmask.as<uint8_t>(), 0, 1.0, -1.0, xs,
+                        c->diag_op.as<double>(), xs, c->free_rows.as<int>(), c->n_free, ys, s);
+        launch_constraint_rows(0, c->n_lines, c->con_rows.as<int>(), c->con_ptr.as<int>(),
+                               c->con_cols.as<int>(), c->con_w.as<double>(),
+                               c->con_inhom.as<double>(), 0, xs, ys, s);
+        launch_permute(c->n, c->perm_d.as<int>(), ys, dst, 1, s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
     });
 }
 

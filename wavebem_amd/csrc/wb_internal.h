@@ -124,6 +124,14 @@ void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_
                     int n, const float *x, const double *diag, const float *xd, float *T,
                     float *ypart, size_t ldp, float *y, hipStream_t s);
 
+// Work space of the entry points that allocate nothing (wb_noalloc.hip): a static device
+// array of 2 NOALLOC_MAX_N doubles per GPU.  noalloc_setup looks its address up on a device
+// (once; called when a context is created there); noalloc_scratch returns it (nullptr
+// before setup).
+constexpr int NOALLOC_MAX_N = 262144;
+hipError_t noalloc_setup(int device);
+double *noalloc_scratch(int device);
+
 // Constraint rules of BEMProblem::compute_constraints, caller numbering, closed.
 struct ConstraintSet {
     std::vector<int> rows, ptr, cols;
