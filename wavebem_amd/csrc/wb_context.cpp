@@ -788,7 +788,7 @@ static void ensure_hm(wb_ctx *c)
     c->hm_D.ensure(2 * nb * HM_CL * HM_CL);
     c->hm_T.ensure(sizeof(float) * nb * HM_RMAX);
     c->hm_part.ensure(sizeof(float) * c->vlen * hm_chunks(c->hmp.ncl));
-    if (agg > 0) c->cs_S.ensure(sizeof(double) * (size_t)nc * c->vlen);
+    if (agg > 0) c->cs_S.ensure(sizeof(double) * HM_SKETCH_PARTS * (size_t)nc * c->vlen);
     {
         KernelScope ks(c, WB_K_INNER, c->n);
         launch_hm_build(c->G.as<float>(), c->H.as<float>(), c->ld, c->n, c->gmask.as<uint8_t>(),

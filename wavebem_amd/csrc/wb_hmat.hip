@@ -97,17 +97,18 @@ __device__ __forceinline__ float pow2_at_least(float m)
 
 }  // namespace
 
-// One wavefront per 16 consecutive device rows, all column clusters J in order, on the
+// One wavefront per 16 consecutive device rows and a range of column clusters J, on the
 // matrix cores (v_mfma_f32_16x16x4f32: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k,
 // D[i][j] in lane j + 16 (i / 4), register i % 4):
 //   Y[J][row][k] = sum over the columns p of cluster J of A(row, col_J(p)) s_k(p),
 //   Mx[J][row]   = max |A(row, col_J(p))|,
-// lane (i, kk) reading row i at the positions q = 16 m + 4 kk + e of each segment.  The
-// segment descriptors {first column, length, kind, 0} say whether the segment reads only G,
-// only H or both (then column by column through gmask), so the loads of a whole cluster
-// depend on nothing and are issued together.  With S, also S[a][row] = the row's sum over
-// the columns of aggregate a (agg consecutive device columns, agg >= HM_SEG, so a segment
-// meets at most two aggregates), summed per segment in a fixed order.
+// lane (i, kk) reading row i at the positions q = 16 m + 4 kk + e of each segment (16 bytes
+// per load).  The segment descriptors {first column, length, kind, unused} say whether a
+// segment reads only G, only H or both (then column by column through gmask): pure
+// segments go to separate G and H accumulators, combined as H - G at the end, so the
+// values need no sign.  With S, also S[a][row] = the row's sum over the columns of
+// aggregate a (agg consecutive device columns, agg >= HM_SEG, so a segment meets at most
+// two aggregates), summed per segment in a fixed order.
 __global__ __launch_bounds__(64) void k_hm_sketch(const float *__restrict__ G,
                                                   const float *__restrict__ H, size_t ld, int n,
                                                   const uint8_t *__restrict__ gmask,
@@ -121,107 +122,119 @@ __global__ __launch_bounds__(64) void k_hm_sketch(const float *__restrict__ G,
     const int row0 = blockIdx.x * 16, row = row0 + i;
     const bool rok = row < n;
     const size_t off = (size_t)(rok ? row : 0) * ld;
-    // the lane's sign of sketch vector i at position 16 g + 4 kk + e: bit 4 g + e of smask
-    uint64_t smask = 0;
+    // clusters [J0, J1) of this wavefront: the row block's clusters are split over gridDim.y
+    // wavefronts, whose aggregate sums go to parts S + y nagg ldS (added in order later)
+    const int J0 = (int)((long)blockIdx.y * ncl / gridDim.y);
+    const int J1 = (int)((long)(blockIdx.y + 1) * ncl / gridDim.y);
+    if (S) S += (size_t)blockIdx.y * nagg * ldS;
+    // the lane's sign of sketch vector i at position 16 g + 4 kk + e (g = 4 q + m)
+    float sg[16][4];
 #pragma unroll
     for (int g = 0; g < 16; ++g)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-            smask |= (uint64_t)((sign_bits((uint32_t)(16 * g + 4 * kk + e)) >> i) & 1u)
-                     << (4 * g + e);
-    uint32_t sm0 = (uint32_t)smask, sm1 = (uint32_t)(smask >> 32);
-    auto sgn_of = [&](int bit) {
-        const uint32_t w = bit < 32 ? sm0 : sm1;
-        return __uint_as_float(0xbf800000u ^ (((w >> (bit & 31)) & 1u) << 31));  // bit ? 1 : -1
-    };
+            sg[g][e] = ((sign_bits((uint32_t)(16 * g + 4 * kk + e)) >> i) & 1u) ? 1.0f : -1.0f;
     if (S)
         for (int t = lane; t < 16 * CS_MAX; t += 64) sacc[t / CS_MAX][t % CS_MAX] = 0.0f;
     __syncthreads();
-    for (int J = 0; J < ncl; ++J) {
-        // keep the 64 signs out of the registers: recomputed per cluster from the mask
-        asm volatile("" : "+v"(sm0), "+v"(sm1));
-        int4 sd[HM_SEGS];
-#pragma unroll
-        for (int q = 0; q < HM_SEGS; ++q) {
-            sd[q] = cl_seg[J * HM_SEGS + q];
-            sd[q].x = __builtin_amdgcn_readfirstlane(sd[q].x);
-            sd[q].y = __builtin_amdgcn_readfirstlane(sd[q].y);
-            sd[q].z = __builtin_amdgcn_readfirstlane(sd[q].z);
-        }
-        // all loads of the cluster first: lane (i, kk) reads 16 contiguous bytes of row i
-        // at segment positions 16 m + 4 kk .. + 3 (dword-aligned 16-byte loads); a tail
-        // past the segment's length is read element by element, so no load leaves the row
-        float a[HM_SEGS][16];
-#pragma unroll
-        for (int q = 0; q < HM_SEGS; ++q) {
-            const int start = sd[q].x, len = sd[q].y, kind = sd[q].z;
-            const float *src = (kind == 1 ? H : G) + off + start;
-            const float sgn = kind == 1 ? 1.0f : -1.0f;
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const int p = 16 * m + 4 * kk;
-                fvec4 v = {0.f, 0.f, 0.f, 0.f};
-                if (rok && p < len) {
-                    if (kind != 2) {
-                        if (p + 4 <= len) {
-                            __builtin_memcpy(&v, src + p, 16);
-                        } else {
-#pragma unroll
-                            for (int e = 0; e < 4; ++e)
-                                if (p + e < len) v[e] = src[p + e];
-                        }
-                        v *= sgn;
-                    } else {
-#pragma unroll
-                        for (int e = 0; e < 4; ++e)
-                            if (p + e < len) {
-                                const int col = start + p + e;
-                                const bool h = (gmask[col >> 2] >> (col & 3)) & 1u;
-                                const float w = (h ? H : G)[off + col];
-                                v[e] = h ? w : -w;
-                            }
-                    }
-                }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) a[q][4 * m + e] = v[e];
-            }
-        }
-        fvec4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    for (int J = J0; J < J1; ++J) {
+        fvec4 aG0 = {0.f, 0.f, 0.f, 0.f}, aG1 = aG0, aH0 = aG0, aH1 = aG0;
         float mx = 0.0f;
 #pragma unroll
-        for (int q = 0; q < HM_SEGS; ++q) {
-            float slo = 0.0f, shi = 0.0f;
-            const int abound = S ? (sd[q].x / agg + 1) * agg : 0;
+        for (int q0 = 0; q0 < HM_SEGS; q0 += 2) {
+            // two segments' loads in flight together
+            int4 sd[2];
+            fvec4 v[2][4];
 #pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                const float v = a[q][t];
-                mx = fmaxf(mx, fabsf(v));
-                if (S) {
-                    const int col = sd[q].x + 16 * (t >> 2) + 4 * kk + (t & 3);
-                    if (col < abound)
-                        slo += v;
-                    else
-                        shi += v;
+            for (int h = 0; h < 2; ++h) {
+                sd[h] = cl_seg[J * HM_SEGS + q0 + h];
+                sd[h].x = __builtin_amdgcn_readfirstlane(sd[h].x);
+                sd[h].y = __builtin_amdgcn_readfirstlane(sd[h].y);
+                sd[h].z = __builtin_amdgcn_readfirstlane(sd[h].z);
+                const int start = sd[h].x, len = sd[h].y, kind = sd[h].z;
+                const float *src = (kind == 1 ? H : G) + off + start;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int p = 16 * m + 4 * kk;
+                    fvec4 w = {0.f, 0.f, 0.f, 0.f};
+                    if (rok && p < len) {
+                        if (kind != 2) {
+                            if (p + 4 <= len) {
+                                __builtin_memcpy(&w, src + p, 16);
+                            } else {
+#pragma unroll
+                                for (int e = 0; e < 4; ++e)
+                                    if (p + e < len) w[e] = src[p + e];
+                            }
+                        } else {
+                            // mixed: the operator's value (sign included), column by column
+#pragma unroll
+                            for (int e = 0; e < 4; ++e)
+                                if (p + e < len) {
+                                    const int col = start + p + e;
+                                    const bool hh = (gmask[col >> 2] >> (col & 3)) & 1u;
+                                    const float x = (hh ? H : G)[off + col];
+                                    w[e] = hh ? x : -x;
+                                }
+                        }
+                    }
+                    v[h][m] = w;
                 }
-                const float sb = sgn_of(4 * (4 * q + (t >> 2)) + (t & 3));
-                if (t & 1)
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(v, sb, acc1, 0, 0, 0);
-                else
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(v, sb, acc0, 0, 0, 0);
             }
-            if (S && sd[q].y > 0) {
-                slo += __shfl_xor(slo, 16, 64);
-                slo += __shfl_xor(slo, 32, 64);
-                shi += __shfl_xor(shi, 16, 64);
-                shi += __shfl_xor(shi, 32, 64);
-                if (kk == 0) {
-                    const int a0 = sd[q].x / agg;
-                    sacc[i][a0] += slo;
-                    if (a0 + 1 < nagg) sacc[i][a0 + 1] += shi;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = q0 + h;
+                const int len = sd[h].y, kind = sd[h].z;
+                if (len == 0) continue;
+                float tot = 0.0f, lo = 0.0f;
+                // columns below qb (segment position) belong to the segment's first aggregate
+                const int a0 = S ? sd[h].x / agg : 0;
+                const int qb = S ? (a0 + 1) * agg - sd[h].x : HM_SEG;
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float x = v[h][m][e];
+                        mx = fmaxf(mx, fabsf(x));
+                        if (S) tot += x;
+                        if (qb < HM_SEG) lo += (16 * m + 4 * kk + e < qb) ? x : 0.0f;
+                    }
+                // kind 0 reads G, whose operator entries are -G: to the G sums; kind 1 and
+                // mixed segments (values with their sign) to the H sums
+                if (kind == 0) {
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        aG0 = __builtin_amdgcn_mfma_f32_16x16x4f32(v[h][m][0], sg[4 * q + m][0], aG0, 0, 0, 0);
+                        aG1 = __builtin_amdgcn_mfma_f32_16x16x4f32(v[h][m][1], sg[4 * q + m][1], aG1, 0, 0, 0);
+                        aG0 = __builtin_amdgcn_mfma_f32_16x16x4f32(v[h][m][2], sg[4 * q + m][2], aG0, 0, 0, 0);
+                        aG1 = __builtin_amdgcn_mfma_f32_16x16x4f32(v[h][m][3], sg[4 * q + m][3], aG1, 0, 0, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        aH0 = __builtin_amdgcn_mfma_f32_16x16x4f32(v[h][m][0], sg[4 * q + m][0], aH0, 0, 0, 0);
+                        aH1 = __builtin_amdgcn_mfma_f32_16x16x4f32(v[h][m][1], sg[4 * q + m][1], aH1, 0, 0, 0);
+                        aH0 = __builtin_amdgcn_mfma_f32_16x16x4f32(v[h][m][2], sg[4 * q + m][2], aH0, 0, 0, 0);
+                        aH1 = __builtin_amdgcn_mfma_f32_16x16x4f32(v[h][m][3], sg[4 * q + m][3], aH1, 0, 0, 0);
+                    }
+                }
+                if (S) {
+                    if (qb >= HM_SEG) lo = tot;
+                    const float sgn = kind == 0 ? -1.0f : 1.0f;
+                    float hi = (tot - lo) * sgn;
+                    lo *= sgn;
+                    lo += __shfl_xor(lo, 16, 64);
+                    lo += __shfl_xor(lo, 32, 64);
+                    hi += __shfl_xor(hi, 16, 64);
+                    hi += __shfl_xor(hi, 32, 64);
+                    if (kk == 0) {
+                        sacc[i][a0] += lo;
+                        if (a0 + 1 < nagg) sacc[i][a0 + 1] += hi;
+                    }
                 }
             }
         }
-        const fvec4 acc = acc0 + acc1;
+        const fvec4 acc = (aH0 + aH1) - (aG0 + aG1);
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             const int r = row0 + 4 * kk + g;
@@ -372,14 +385,16 @@ __global__ __launch_bounds__(64) void k_hm_fill(const float *__restrict__ G,
     const int r = rank[b];
     if (r == 0) return;
     const int szI = cl_size[I], szJ = cl_size[J];
-    int rowv[HM_CL / 64];
+    // the lane's rows l, l + 64, ...: element offsets row * ld, read back with readlane
+    uint64_t rowo[HM_CL / 64];
 #pragma unroll
     for (int u = 0; u < HM_CL / 64; ++u) {
         const int li = lane + 64 * u;
-        rowv[u] = li < szI ? cl_idx[(size_t)I * HM_CL + li] : 0;
+        rowo[u] = (uint64_t)(li < szI ? cl_idx[(size_t)I * HM_CL + li] : 0) * ld;
     }
     const float *ptr[4];
     float sgn[4];
+    bool same = true;  // four consecutive columns of one matrix: one 16-byte load per row
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const int p = 4 * lane + c;
@@ -389,37 +404,48 @@ __global__ __launch_bounds__(64) void k_hm_fill(const float *__restrict__ G,
             ptr[c] = (h ? H : G) + col;
             sgn[c] = h ? 1.0f : -1.0f;
         } else {
-            ptr[c] = G;
+            ptr[c] = G + c;  // a valid address of the row, weighted 0
             sgn[c] = 0.0f;
         }
+        if (c > 0) same = same && ptr[c] == ptr[c - 1] + 1 && sgn[c] == sgn[0];
     }
-    auto row_off = [&](int i) {
-        int v = rowv[0];
+    auto load4 = [&](size_t off, float (&v)[4]) {
+        if (same) {
+            fvec4 w;
+            __builtin_memcpy(&w, ptr[0] + off, 16);
 #pragma unroll
-        for (int u = 1; u < HM_CL / 64; ++u)
-            if ((i >> 6) == u) v = rowv[u];
-        return (size_t)__builtin_amdgcn_readlane(v, i & 63) * ld;
+            for (int c = 0; c < 4; ++c) v[c] = w[c] * sgn[0];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] = ptr[c][off] * sgn[c];
+        }
+    };
+    auto row_off_u = [&](int u, int il) {
+        const uint64_t v = rowo[u];
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, il);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), il);
+        return (size_t)(((uint64_t)hi << 32) | lo);
     };
     if (r == HM_DENSE) {
         const float inv = 1.0f / fscale[b];
-        for (int i0 = 0; i0 < szI; i0 += 8) {
-            float v[8][4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int i = min(i0 + u, szI - 1);
-                const size_t off = row_off(i);
+        for (int uq = 0; uq < HM_CL / 64; ++uq)
+            for (int j0 = 0; j0 < 64 && 64 * uq + j0 < szI; j0 += 8) {
+                float v[8][4];
 #pragma unroll
-                for (int c = 0; c < 4; ++c) v[u][c] = ptr[c][off] * sgn[c];
-            }
+                for (int u = 0; u < 8; ++u)
+                    load4(row_off_u(uq, min(j0 + u, szI - 1 - 64 * uq)), v[u]);
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (i0 + u < szI) {
-                    hvec4 h;
+                for (int u = 0; u < 8; ++u) {
+                    const int i = 64 * uq + j0 + u;
+                    if (i < szI) {
+                        hvec4 h;
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) h[c] = (_Float16)(v[u][c] * inv);
-                    *(hvec4 *)(D16 + ((size_t)b * HM_CL + i0 + u) * HM_CL + 4 * lane) = h;
+                        for (int c = 0; c < 4; ++c) h[c] = (_Float16)(v[u][c] * inv);
+                        *(hvec4 *)(D16 + ((size_t)b * HM_CL + i) * HM_CL + 4 * lane) = h;
+                    }
                 }
-        }
+            }
         return;
     }
     for (int e = lane; e < r * HM_CL; e += 64) {
@@ -432,27 +458,35 @@ __global__ __launch_bounds__(64) void k_hm_fill(const float *__restrict__ G,
     for (int k = 0; k < HM_RMAX; ++k)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[k][c] = 0.0f;
-    for (int i0 = 0; i0 < szI; i0 += 8) {
-        float v[8][4];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int i = min(i0 + u, szI - 1);
-            const size_t off = row_off(i);
+    for (int uq = 0; uq < HM_CL / 64; ++uq)
+        for (int j0 = 0; j0 < 64 && 64 * uq + j0 < szI; j0 += 8) {
+            float v[8][4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) v[u][c] = ptr[c][off] * sgn[c];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if (i0 + u >= szI) break;
+            for (int u = 0; u < 8; ++u)
+                load4(row_off_u(uq, min(j0 + u, szI - 1 - 64 * uq)), v[u]);
+            const int i0 = 64 * uq + j0;
 #pragma unroll
             for (int k = 0; k < HM_RMAX; ++k)
                 if (k < r) {
-                    const float q = Qs[k][i0 + u];
+                    // Q rows i0 .. i0 + 7 (zero past the block's rows)
+                    const fvec4 q0 = *(const fvec4 *)&Qs[k][i0];
+                    const fvec4 q1 = *(const fvec4 *)&Qs[k][i0 + 4];
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) acc[k][c] = fmaf(q, v[u][c], acc[k][c]);
+                    for (int c = 0; c < 4; ++c) {
+                        float t = acc[k][c];
+                        t = fmaf(q0[0], v[0][c], t);
+                        t = fmaf(q0[1], v[1][c], t);
+                        t = fmaf(q0[2], v[2][c], t);
+                        t = fmaf(q0[3], v[3][c], t);
+                        t = fmaf(q1[0], v[4][c], t);
+                        t = fmaf(q1[1], v[5][c], t);
+                        t = fmaf(q1[2], v[6][c], t);
+                        t = fmaf(q1[3], v[7][c], t);
+                        acc[k][c] = t;
+                    }
                 }
         }
-    }
     float mx = 0.0f;
 #pragma unroll
     for (int k = 0; k < HM_RMAX; ++k)
@@ -615,15 +649,28 @@ __global__ __launch_bounds__(256) void k_hm_finish(const float *__restrict__ ypa
     y[i] = (float)(s + diag[i] * (double)xd[i]);
 }
 
-int hm_chunks(int ncl)
+// S[a][r] += the other sketch parts' sums (parts 1 .. nparts - 1, in order)
+__global__ __launch_bounds__(256) void k_hm_sum_parts(int n, int nagg, int nparts, size_t ldS,
+                                                      double *__restrict__ S)
 {
-    // about four waves per SIMD over the whole chip (256 CUs x 4 SIMDs) in the row phase
-    const int c = (4096 + ncl - 1) / ncl;
-    return std::max(1, std::min(64, c));
+    const int r = blockIdx.x * 256 + threadIdx.x, a = blockIdx.y;
+    if (r >= n) return;
+    double v = S[(size_t)a * ldS + r];
+    for (int p = 1; p < nparts; ++p) v += S[((size_t)p * nagg + a) * ldS + r];
+    S[(size_t)a * ldS + r] = v;
 }
 
-// row-cluster chunks of the T phase: about four waves per SIMD
-static int hm_tchunks(int ncl) { return std::max(1, std::min(64, (4096 + ncl - 1) / ncl)); }
+// wavefronts of each product phase (1024 SIMDs: WB_HM_WAVES / 1024 per SIMD)
+#ifndef WB_HM_WAVES
+#define WB_HM_WAVES 16384
+#endif
+int hm_chunks(int ncl)
+{
+    const int c = (WB_HM_WAVES + ncl - 1) / ncl;
+    return std::max(1, std::min(128, c));
+}
+
+static int hm_tchunks(int ncl) { return hm_chunks(ncl); }
 
 void launch_hm_build(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
                      const int *cl_idx, const int *cl_size, const int *cl_seg, int ncl,
@@ -631,8 +678,11 @@ void launch_hm_build(const float *G, const float *H, size_t ld, int n, const uin
                      float *fscale, void *Q16, void *B16, void *D16, int agg, int nagg,
                      double *S, size_t ldS, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_hm_sketch, dim3((n + 15) / 16), dim3(64), 0, s, G, H, ld, n, gmask,
-                       (const int4 *)cl_seg, ncl, Y, Mx, agg, nagg, S, ldS);
+    hipLaunchKernelGGL(k_hm_sketch, dim3((n + 15) / 16, HM_SKETCH_PARTS), dim3(64), 0, s, G, H,
+                       ld, n, gmask, (const int4 *)cl_seg, ncl, Y, Mx, agg, nagg, S, ldS);
+    if (S && HM_SKETCH_PARTS > 1)
+        hipLaunchKernelGGL(k_hm_sum_parts, dim3((n + 255) / 256, nagg), dim3(256), 0, s, n, nagg,
+                           HM_SKETCH_PARTS, ldS, S);
     const unsigned nb = (unsigned)ncl * (unsigned)ncl;
     hipLaunchKernelGGL(k_hm_qr, dim3(nb), dim3(64), 0, s, Y, Mx, n, cl_idx, cl_size, ncl, near,
                        tol, rank, fscale, (_Float16 *)Q16);

@@ -98,6 +98,7 @@ std::string plan_mesh(int n, int nc, const double *support, const int32_t *cell_
 // sign vectors) meets the tolerance, else dense too.
 constexpr int HM_SEG = 64, HM_SEGS = 4, HM_CL = HM_SEG * HM_SEGS, HM_SK = 16, HM_RMAX = 12;
 constexpr int HM_DENSE = 255;
+constexpr int HM_SKETCH_PARTS = 2;  // wavefronts per 16-row block of the sketch (cluster ranges)
 struct HmatPlan {
     int ncl = 0;
     std::vector<int> cl_idx;    // ncl x HM_CL device indices (-1 past the cluster's size)
@@ -109,9 +110,9 @@ struct HmatPlan {
 void plan_hmat(const MeshPlan &P, const double *X_internal, HmatPlan &H);
 // Build (wb_hmat.hip): Y (ncl x n x HM_SK) and Mx (ncl x n) are work space; rank and
 // fscale (ncl^2), Q16 / B16 (ncl^2 x HM_RMAX x HM_CL halfs) and D16 (ncl^2 x HM_CL x HM_CL
-// halfs) hold the operator.  With S (agg >= HM_SEG), the build also writes S[a ldS + i] =
-// the sum of row i of the FP32 operator over the columns of aggregate a (a < nagg), the
-// coarse space's A P.  Product: y = A~ x + diag .* xd (FP32 vectors, all n rows); T
+// halfs) hold the operator.  With S (agg >= HM_SEG; HM_SKETCH_PARTS x nagg x ldS doubles of
+// work space), the build also writes S[a ldS + i] = the sum of row i of the FP32 operator
+// over the columns of aggregate a (a < nagg), the coarse space's A P.  Product: y = A~ x + diag .* xd (FP32 vectors, all n rows); T
 // (ncl^2 x HM_RMAX) and ypart (hm_chunks(ncl) x ldp) floats are work space.
 int hm_chunks(int ncl);
 void launch_hm_build(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
