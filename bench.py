@@ -203,6 +203,65 @@ def run_multi(workload, k, steps):
     return out
 
 
+def run_bdf(workload, steps, dt):
+    """The free-surface time-loop stand-in (wavebem_amd.timeloop): fixed-step BDF2 with
+    Newton iterations whose J v comes from wb_residual_jvp + wb_solve_rhs, every residual
+    re-assembling G and H at the displaced mesh; FP32-IR, then FP64, on the same problem.
+    Returns per-step times, Newton/GMRES counts and the hull-force drift of FP32-IR against
+    FP64."""
+    from wavebem_amd import mesh as M
+    from wavebem_amd import timeloop as TL
+    from wavebem_amd.bem import BEMProblem, Config
+
+    m = M.wigley_preset(workload)
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    bc, U = M.wigley_bc(m, seed=0)
+    out = {"n_dofs": m.n_dofs, "steps": steps, "dt": dt}
+    res = {}
+    for prec in ("fp32_ir", "fp64"):
+        with BEMProblem(m, Config(precision=prec), double_nodes=(ptr, idx)) as p:
+            p.compute_constraints(bc)
+            dae = TL.FreeSurfaceStandIn(p, m, (ptr, idx), bc, t_ramp=10 * dt)
+            it = TL.BDFIntegrator(dae, np.zeros(dae.size), dt)
+            phis, forces, times, states = [np.zeros(m.n_dofs)], [], [], []
+            for s in range(steps):
+                t0 = time.perf_counter()
+                y = it.step()
+                phi, _, _ = dae.bem_state(it.t, y)
+                times.append((time.perf_counter() - t0) * 1e3)
+                phis = (phis + [phi.copy()])[-3:]
+                forces.append(dae.hull_force_x(it.t, y, it.derivative(phis), U))
+                states.append(y.copy())
+                tr = " ".join(f"|F| {a:.2e} d {b:.2e} k {c}" for a, b, c in it.last_trace)
+                log(f"bdf {prec} step {s + 1}/{steps}: {times[-1]:.0f} ms, force "
+                    f"{forces[-1]:.6e}; newton: {tr}")
+            if prec == "fp64":
+                v = np.random.default_rng(1).standard_normal(dae.size)
+                fr, geo = dae.jvp_parts(it.t, y, v)
+                # size of the term the reference's frozen-G/H product leaves out, for a
+                # seeded direction at the last state
+                out["geometric_term_rel_at_last_state"] = float(
+                    np.abs(geo).max() / max(np.abs(fr + geo).max(), 1e-300))
+            c = dae.counters
+            res[prec] = dict(ms=times, forces=np.array(forces), states=np.array(states))
+            out[f"{prec}_ms_per_step"] = round(float(np.mean(times)), 2)
+            out[f"{prec}_newton_per_step"] = round(it.stats["newton"] / steps, 2)
+            out[f"{prec}_gmres_per_step"] = round(it.stats["krylov"] / steps, 2)
+            out[f"{prec}_steps_by_newton_count"] = {
+                str(k): int(v) for k, v in zip(*np.unique(it.stats["newton_per_step"],
+                                                          return_counts=True))}
+            out[f"{prec}_bem_solves_per_step"] = round(
+                (c["solves"] + c["jvp_solves"]) / steps, 2)
+            out[f"{prec}_assemblies_per_step"] = round(c["assemblies"] / steps, 2)
+    f32, f64 = res["fp32_ir"]["forces"], res["fp64"]["forces"]
+    out["force_rel_drift_max"] = float(np.abs(f32 - f64).max() / np.abs(f64).max())
+    s32, s64 = res["fp32_ir"]["states"], res["fp64"]["states"]
+    out["state_rel_drift_max"] = float(np.abs(s32 - s64).max() / np.abs(s64).max())
+    out["force_fp64_first_last"] = [float(f64[0]), float(f64[-1])]
+    out["speedup_vs_fp64"] = round(out["fp64_ms_per_step"] / out["fp32_ir_ms_per_step"], 3)
+    return out
+
+
 def run_moving(workload, steps):
     """Prescribed free-surface motion (no DAE integrator): every step moves the free-surface
     nodes to a travelling wave of 2e-3 L amplitude, re-assembles G and H and solves, in
@@ -398,6 +457,9 @@ def main():
     ap.add_argument("--multi", type=int, default=0, metavar="K",
                     help="only time K right-hand sides solved together (default workload "
                          "60k) and exit")
+    ap.add_argument("--bdf", type=int, default=0, metavar="STEPS",
+                    help="free-surface time-loop stand-in: STEPS BDF steps with Newton J v")
+    ap.add_argument("--dt", type=float, default=0.01, help="time step of --bdf")
     ap.add_argument("--moving", type=int, default=0, metavar="STEPS",
                     help="only run STEPS re-assembled steps of a prescribed free-surface "
                          "motion in FP32-IR and FP64 (drag drift, ms per step) and exit")
@@ -417,6 +479,18 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.bdf:
+        r = run_bdf(args.workload, args.bdf, args.dt)
+        print(json.dumps({"metric": "free-surface BDF step time (Newton-Krylov, re-assembly "
+                          "per residual)", "unit": "ms",
+                          "value": r["fp32_ir_ms_per_step"], "higher_is_better": False,
+                          "dtype": "fp16+fp32/bf16+fp64-ir", "data": "synthetic",
+                          "config": {"workload": f"wigley_closed_halfbox_{args.workload}",
+                                     "n_dofs": r["n_dofs"], "steps": r["steps"],
+                                     "dt": r["dt"], "time_loop": "stand-in (linear free "
+                                     "surface, fixed-step BDF2, full-tangent Newton)"},
+                          "detail": r}), flush=True)
+        return
     if args.moving:
         r = run_moving(args.workload, args.moving)
         print(json.dumps({"metric": "moving-mesh step time (re-assembly + solve)", "unit": "ms",
