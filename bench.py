@@ -230,7 +230,8 @@ def run_bdf(workload, steps, dt):
                 phi, _, _ = dae.bem_state(it.t, y)
                 times.append((time.perf_counter() - t0) * 1e3)
                 phis = (phis + [phi.copy()])[-3:]
-                forces.append(dae.hull_force_x(it.t, y, it.derivative(phis), U))
+                forces.append(float(dae.hull_force(it.t, y, it.derivative(phis), U,
+                                                   it.last_yp[:dae.n_nodes])[0]))
                 states.append(y.copy())
                 tr = " ".join(f"|F| {a:.2e} d {b:.2e} k {c}" for a, b, c in it.last_trace)
                 log(f"bdf {prec} step {s + 1}/{steps}: {times[-1]:.0f} ms, force "

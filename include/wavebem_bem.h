@@ -223,6 +223,13 @@ wb_status wb_prepare_bem_vectors(wb_ctx *ctx, int n_fs_ids, const int32_t *fs_id
 wb_status wb_compute_surface_gradients(wb_ctx *ctx, const double *tmp_rhs, double *grads,
                                        int *cg_iterations);
 
+/* FreeSurface::compute_potential_gradients (source/free_surface.cc:9271-9400): the L2
+ * projection onto the Q1 space of the complete potential gradient n dphi_dn + surface
+ * gradient of phi (full vectors, no masks), solved as in wb_compute_normals, then the mean
+ * over every set of coincident dofs.  grads: n x 3, caller order. */
+wb_status wb_compute_potential_gradients(wb_ctx *ctx, const double *phi, const double *dphi_dn,
+                                         double *grads, int *cg_iterations);
+
 /* BEMProblem::compute_constraints (source/bem_problem.cc:990-1105) for the current masks
  * and double nodes, from tmp_rhs.  The rules read node normals (n x 3,
  * ComputationalDomain::compute_normals) only where two Dirichlet dofs coincide and
@@ -289,9 +296,11 @@ wb_status wb_residual(wb_ctx *ctx, double *res, const double *phi, const double 
 
 /* Solves A_c^T lambda = rhs with the transposed ConstrainedOperator of the installed
  * constraints and masks (the operator of solve_system), preconditioned by the transposed
- * block Jacobi, with the same GMRES settings.  rhs and lambda in the caller's dof
- * numbering.  FP64 contexts only (WB_E_UNSUPPORTED otherwise and for all-Neumann
- * masks); WB_E_NOCONV leaves lambda unchanged. */
+ * block Jacobi (FP64: with the transposed coarse space), with the same GMRES settings.
+ * FP32_IR contexts refine in FP64 with the transposed hi + lo operator around an inner
+ * FP32 GMRES on the transposed hi part (the FP16 and compressed inner operators hold the
+ * forward selection only).  rhs and lambda in the caller's dof numbering.  All-Neumann
+ * masks return WB_E_UNSUPPORTED; WB_E_NOCONV leaves lambda unchanged. */
 wb_status wb_solve_adjoint(wb_ctx *ctx, const double *rhs, double *lambda,
                            wb_solve_stats *stats);
 

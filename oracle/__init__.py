@@ -175,6 +175,61 @@ def surface_gradients(support, cell_dofs, cell_orient, phi, q_order=4):
     return out
 
 
+def potential_gradients(support, cell_dofs, cell_orient, phi, psi, dn_ptr, dn_idx, q_order=4,
+                        average=True):
+    """FreeSurface::compute_potential_gradients (source/free_surface.cc:9271-9400), restated
+    in numpy: on every cell and point of the QGauss rule, the complete gradient
+    g = n psi + surface gradient of the Q1 interpolant of phi (:9325-9344); the L2 system
+    sum_j M_ij x_j = sum_c sum_q N_i g JxW, one scalar mass matrix for the three components
+    (the vector mass matrix of :9346-9358 couples equal components only), solved directly
+    (the reference's SparseDirectUMFPACK, :9374-9376); then every set of coincident dofs
+    takes its mean (:9380-9396).  Returns (n, 3)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    X = _f64(support)
+    cd = np.asarray(cell_dofs, dtype=np.int64)
+    o = np.asarray(cell_orient, dtype=np.float64)
+    n = X.shape[0]
+    pts, w = qgauss_2d(q_order)
+    xi, eta = pts[:, 0], pts[:, 1]
+    sh = np.stack([(1 - xi) * (1 - eta), xi * (1 - eta), (1 - xi) * eta, xi * eta], 1)
+    dxi = np.stack([-(1 - eta), (1 - eta), -eta, eta], 1)
+    deta = np.stack([-(1 - xi), -xi, (1 - xi), xi], 1)
+    V = X[cd]                                            # (c, 4, 3)
+    a = np.einsum("qv,cvd->cqd", dxi, V)
+    b = np.einsum("qv,cvd->cqd", deta, V)
+    cr = np.cross(a, b)
+    ln = np.linalg.norm(cr, axis=2)
+    nrm = o[:, None, None] * cr / ln[:, :, None]
+    g00, g01, g11 = (a * a).sum(2), (a * b).sum(2), (b * b).sum(2)
+    det = g00 * g11 - g01 * g01
+    jxw = np.sqrt(det) * w[None, :]
+    pc = np.asarray(phi, dtype=np.float64)[cd]
+    ux, ue = pc @ dxi.T, pc @ deta.T
+    ca = (g11 * ux - g01 * ue) / det
+    cb = (g00 * ue - g01 * ux) / det
+    psq = np.asarray(psi, dtype=np.float64)[cd] @ sh.T
+    grad = ca[:, :, None] * a + cb[:, :, None] * b + nrm * psq[:, :, None]
+    Me = np.einsum("qi,qj,cq->cij", sh, sh, jxw)
+    re = np.einsum("qi,cqd,cq->cid", sh, grad, jxw)
+    rows = np.repeat(cd, 4, axis=1).ravel()
+    cols = np.tile(cd, (1, 4)).ravel()
+    M = sp.csc_matrix((Me.ravel(), (rows, cols)), shape=(n, n))
+    rhs = np.zeros((n, 3))
+    np.add.at(rhs, cd.ravel(), re.reshape(-1, 3))
+    lu = spla.splu(M)
+    out = np.stack([lu.solve(rhs[:, e]) for e in range(3)], 1)
+    if average:
+        ptr, idx = np.asarray(dn_ptr), np.asarray(dn_idx)
+        avg = out.copy()
+        for i in range(n):
+            if ptr[i + 1] - ptr[i] > 1:
+                avg[i] = out[idx[ptr[i]:ptr[i + 1]]].mean(axis=0)
+        out = avg
+    return out
+
+
 def prepare_bem_vectors(support, cell_dofs, cell_orient, cell_material, dn_ptr, dn_idx,
                         fs_ids, wall_ids, inflow_ids, wind, bem_bc, dphi_dn,
                         inflow_dphi_dn=None, transom=None, prev_dphi_dn=None, q_order=4):

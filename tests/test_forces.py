@@ -74,3 +74,29 @@ def test_force_record_layout(tmp_path):
     assert vals[:4] == [0.0, 1.5, -2.25e-3, 3.0]
     assert vals[16] == 0.25
     assert tok[-1] == "3.12346"                               # 6 significant digits
+
+
+def test_bernoulli_pressure_closed_forms():
+    """free_surface.cc:9662-9674: a still state is hydrostatic; a stagnation point
+    (grad phi = -V_inf) recovers the dynamic pressure; moving nodes see the Lagrangian
+    derivative corrected by v . grad phi; the expansion in grad phi to first order is the
+    linearised form -rho (phi_t + V_inf . grad phi)."""
+    rng = np.random.default_rng(4)
+    n = 7
+    z = rng.uniform(-1.0, 0.0, n)
+    V = np.array([1.3, -0.2, 0.4])
+    zero = np.zeros((n, 3))
+    p = F.bernoulli_pressure(np.zeros(n), zero, z, V)
+    assert np.allclose(p, -F.RHO * F.GRAVITY * z, rtol=0, atol=1e-10)
+    p = F.bernoulli_pressure(np.zeros(n), -np.tile(V, (n, 1)), z, V)
+    assert np.allclose(p, 0.5 * F.RHO * V @ V - F.RHO * F.GRAVITY * z, rtol=0, atol=1e-10)
+    gp = rng.standard_normal((n, 3))
+    vel = rng.standard_normal((n, 3))
+    dt = rng.standard_normal(n)
+    a = F.bernoulli_pressure(dt, gp, z, V, node_vel=vel)
+    b = F.bernoulli_pressure(dt - (vel * gp).sum(1), gp, z, V)
+    assert np.allclose(a, b, rtol=0, atol=1e-9)
+    eps = 1e-6
+    lin = F.bernoulli_pressure(eps * dt, eps * gp, np.zeros(n), V) / eps
+    # the second-order term is rho eps |grad phi|^2 / 2
+    assert np.allclose(lin, -F.RHO * (dt + gp @ V), rtol=0, atol=1e-5 * F.RHO)

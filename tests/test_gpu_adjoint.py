@@ -1,7 +1,8 @@
 """GPU adjoint: the transposed solve, the backward of the assembly, and drag gradients.
 
 Bars: the backward of the assembly within 1e-11 (norm-wise) of the oracle's, itself pinned
-by central differences; the transposed solve within 1e-8 of a dense solve; adjoint drag
+by central differences; the transposed solve (FP64 and FP32-IR) within 1e-8 of a dense
+solve; adjoint drag
 gradients within 1e-6 (relative to the largest checked entry) of central differences of
 the GPU forward solve with h = 1e-6.
 """
@@ -51,13 +52,48 @@ def test_adjoint_solve_matches_dense(gpu_available):
     assert err < 1e-8
 
 
-def test_adjoint_rejects_fp32_ir(gpu_available):
-    m, ptr, idx, H, G = T.oracle_assembly("box6")
-    with BEMProblem(m, Config(precision="fp32_ir"), double_nodes=(ptr, idx)) as p:
-        p.compute_constraints(np.zeros(m.n_dofs))
+def test_adjoint_fp32_ir_matches_dense(gpu_available):
+    """The transposed solve of an FP32-IR context (refinement residuals with the transposed
+    hi + lo operator, inner GMRES on the transposed FP32 hi part) against a dense solve of
+    the oracle's FP64 system: the hi + lo storage is within 2^-33 of the FP64 entries, so
+    the bar is the FP64 one."""
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
+    bc, _ = M.wigley_bc(m, seed=3)
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, bc)
+    A, _ = T.explicit_system(H, G, m.surface_nodes, m.other_nodes, cons, bc)
+    rng = np.random.default_rng(12)
+    g = rng.standard_normal(m.n_dofs)
+    lam_ref = np.linalg.solve(A.T, g)
+    with BEMProblem(m, Config(precision="fp32_ir", solver_tolerance=1e-13),
+                    double_nodes=(ptr, idx)) as p:
+        p.compute_constraints(bc)
         p.assemble_system()
-        with pytest.raises(WaveBemError):
-            p.solve_adjoint(np.ones(m.n_dofs))
+        lam = p.solve_adjoint(g)
+        st = p.last_stats
+    err = np.abs(lam - lam_ref).max() / np.abs(lam_ref).max()
+    print(f"  adjoint fp32_ir: {st['iterations']} inner it, {st['ir_sweeps']} sweeps, "
+          f"rel err {err:.2e}")
+    assert st["ir_sweeps"] >= 1
+    assert err < 1e-8
+
+
+def test_adjoint_fp32_ir_matches_fp64_10k(gpu_available):
+    m = M.wigley_preset("10k")
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    bc, _ = M.wigley_bc(m, seed=4)
+    g = np.random.default_rng(13).standard_normal(m.n_dofs)
+    lam = {}
+    for prec in ("fp64", "fp32_ir"):
+        with BEMProblem(m, Config(precision=prec), double_nodes=(ptr, idx)) as p:
+            p.compute_constraints(bc)
+            p.assemble_system()
+            lam[prec] = p.solve_adjoint(g)
+            st = p.last_stats
+        print(f"  [{prec}] {st['iterations']} it, sweeps {st['ir_sweeps']}, "
+              f"true rel res {st['true_rel_residual']:.2e}", flush=True)
+    e = np.abs(lam["fp32_ir"] - lam["fp64"]).max() / np.abs(lam["fp64"]).max()
+    print(f"  adjoint FP32-IR vs FP64 at n={m.n_dofs}: {e:.2e}")
+    assert e < 1e-8
 
 
 def _class_param(m, ptr, idx, nodes):
@@ -112,16 +148,30 @@ def _drag_value_and_grad(m, ptr, idx, p, bc, U, X0, fs_sel, hull_sel, phi_t):
 
     J = J_of(th_fs, th_hull)
     J.backward()
-    return J_of, th_fs.grad.numpy().copy(), th_hull.grad.numpy().copy(), float(J.detach())
+    g_fs, g_hull = th_fs.grad.numpy().copy(), th_hull.grad.numpy().copy()
+    # the frozen-G/H form: the solution held fixed while the nodes move (the reference's
+    # Jacobian re-solves with the matrices of the current geometry, free_surface.cc:4993-5014,
+    # so the change of G and H with the nodes does not reach the solution)
+    fr_fs = torch.zeros(len(fs_sel), dtype=torch.float64, requires_grad=True)
+    fr_hull = torch.zeros((len(hull_sel), 3), dtype=torch.float64, requires_grad=True)
+    with torch.no_grad():
+        phi0, dphi0 = bem_solve(support(fr_fs, fr_hull), p, bc)
+    X = support(fr_fs, fr_hull)
+    hull_drag(X, cells, orient, phi0, dphi0, torch.tensor(phi_t), U).backward()
+    frozen = (fr_fs.grad.numpy().copy(), fr_hull.grad.numpy().copy())
+    return J_of, g_fs, g_hull, float(J.detach()), frozen
 
 
 @pytest.mark.parametrize("preset,n_fs,n_hull,check_fs,check_hull",
-                         [("2.5k", 6, 4, 6, 4), ("10k", 500, 200, 6, 3)])
+                         [("2.5k", 6, 4, 6, 4), ("10k", 500, 200, 30, 12)])
 def test_drag_gradient_matches_central_differences(gpu_available, preset, n_fs, n_hull,
                                                    check_fs, check_hull):
     """Gradients of the hull drag with respect to n_fs free-surface node elevations and
-    n_hull hull node positions from one adjoint solve; a seeded subset is checked against
-    central differences of the GPU forward solve."""
+    n_hull hull node positions from one adjoint solve; a seeded subset (at 10k: 30
+    elevations and all three components of 12 hull nodes, 66 entries) is checked against
+    central differences of the GPU forward solve.  The frozen-G/H gradient (solution held
+    fixed) is compared with the same differences: what it misses is the term
+    -lambda^T dR/dx of the change of G and H with the nodes, whose size is reported."""
     m = M.wigley_preset(preset)
     ptr, idx = M.double_nodes_csr(m.support_points)
     bc, U = M.wigley_bc(m, seed=0)
@@ -132,8 +182,8 @@ def test_drag_gradient_matches_central_differences(gpu_available, preset, n_fs, 
     with BEMProblem(m, Config(solver_tolerance=1e-13, solver_max_steps=3000),
                     double_nodes=(ptr, idx)) as p:
         p.compute_constraints(bc)
-        J_of, g_fs, g_hull, J0 = _drag_value_and_grad(m, ptr, idx, p, bc, U, X0, fs_sel,
-                                                       hull_sel, phi_t)
+        J_of, g_fs, g_hull, J0, frozen = _drag_value_and_grad(m, ptr, idx, p, bc, U, X0,
+                                                               fs_sel, hull_sel, phi_t)
         h = 1e-6
         fd_fs, fd_hull = np.zeros(check_fs), np.zeros((check_hull, 3))
         zf = torch.zeros(len(fs_sel), dtype=torch.float64)
@@ -149,12 +199,43 @@ def test_drag_gradient_matches_central_differences(gpu_available, preset, n_fs, 
                     e[k, d] = h
                     fd_hull[j, d] = (float(J_of(zf, e)) - float(J_of(zf, -e))) / (2 * h)
     g = np.concatenate([g_fs[kf], g_hull[kh].ravel()])
+    gf = np.concatenate([frozen[0][kf], frozen[1][kh].ravel()])
     fd = np.concatenate([fd_fs, fd_hull.ravel()])
     err = np.abs(g - fd).max() / np.abs(fd).max()
+    err_frozen = np.abs(gf - fd).max() / np.abs(fd).max()
+    geo = np.abs(g - gf).max() / np.abs(g).max()  # the d(G, H)/dx term, relative
     print(f"  [{preset}] n={m.n_dofs} drag {J0:.6e}; {g_fs.size} + {g_hull.size} gradient "
           f"entries; checked {fd.size}: |fd| max {np.abs(fd).max():.3e}, "
-          f"adjoint vs FD max rel err {err:.2e}")
+          f"adjoint vs FD max rel err {err:.2e}; frozen G/H vs FD {err_frozen:.2e} "
+          f"(d(G,H)/dx term {geo:.2e} of the largest entry)")
+    assert fd.size >= (60 if preset == "10k" else 18)
     assert err < 1e-6
+    # the frozen form misses the geometric term, and the term is what the differences show
+    assert err_frozen > 100 * err
+    assert abs(np.abs(g - gf).max() - np.abs(gf - fd).max()) <= 0.05 * np.abs(g - gf).max() + \
+        2 * err * np.abs(fd).max()
+
+
+def test_drag_gradient_fp32_ir_matches_fp64(gpu_available):
+    """The whole adjoint gradient (forward solve, transposed solve, backward of the assembly)
+    in an FP32-IR context against the FP64 context's, at 10k dofs."""
+    m = M.wigley_preset("10k")
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    bc, U = M.wigley_bc(m, seed=0)
+    X0, fs_sel, hull_sel, phi_t = _drag_setup(m, ptr, idx, 7, 300, 100)
+    out = {}
+    for prec in ("fp64", "fp32_ir"):
+        with BEMProblem(m, Config(precision=prec), double_nodes=(ptr, idx)) as p:
+            p.compute_constraints(bc)
+            _, g_fs, g_hull, J0, _ = _drag_value_and_grad(m, ptr, idx, p, bc, U, X0, fs_sel,
+                                                          hull_sel, phi_t)
+        out[prec] = (np.concatenate([g_fs, g_hull.ravel()]), J0)
+    g64, J64 = out["fp64"]
+    g32, J32 = out["fp32_ir"]
+    e = np.abs(g32 - g64).max() / np.abs(g64).max()
+    print(f"  n={m.n_dofs}: {g64.size} gradient entries, FP32-IR vs FP64 {e:.2e}, "
+          f"drag {abs(J32 - J64) / abs(J64):.2e}")
+    assert e < 1e-8 and abs(J32 - J64) < 1e-8 * abs(J64)
 
 
 @pytest.mark.parametrize("kind", ["wigley_small", "sphere4"])

@@ -162,3 +162,22 @@ def test_prepare_bem_vectors_matches_oracle(gpu_available):
     print(f"  bc vs oracle {err:.2e}; solve vs dense {serr:.2e}")
     assert err < 1e-12
     assert serr < 1e-9
+
+
+@pytest.mark.parametrize("kind", ["box_split", "sphere4", "wigley_small"])
+def test_potential_gradients_match_oracle(gpu_available, kind):
+    """wb_compute_potential_gradients (FreeSurface::compute_potential_gradients,
+    source/free_surface.cc:9271-9400) against the oracle's direct solve and double-node
+    mean, for seeded phi and dphi/dn."""
+    m = _mesh(kind)
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    rng = np.random.default_rng(22)
+    phi, psi = rng.standard_normal(m.n_dofs), rng.standard_normal(m.n_dofs)
+    ref = O.potential_gradients(m.support_points, m.cell_dofs, m.cell_orient, phi, psi, ptr,
+                                idx)
+    with BEMProblem(m, Config(), double_nodes=(ptr, idx)) as p:
+        g = p.compute_potential_gradients(phi, psi)
+        it = p.last_projection_iterations
+    e = np.abs(g - ref).max() / np.abs(ref).max()
+    print(f"  [{kind}] n={m.n_dofs}: {it} CG iterations, rel err {e:.2e}")
+    assert e < 1e-12

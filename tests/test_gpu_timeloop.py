@@ -6,8 +6,9 @@
   and the size of the geometric term that the reference's frozen-G/H product leaves out
   (source/free_surface.cc:4993-5014).
 * Five BDF steps with Newton-GMRES (the Schur-reduced Newton systems) in FP32-IR against
-  FP64: state and hull force within 1e-8 of each other; and the frozen Jacobian reaching
-  the same states as the full one.
+  FP64: state and hull force (the reference's Bernoulli pressure from the device-projected
+  potential gradient) within 1e-8 of each other; and the frozen Jacobian reaching the same
+  states as the full one.
 """
 import functools
 
@@ -87,8 +88,9 @@ def _run(precision, jacobian, steps=5, dt=0.02, **newton):
             y = it.step()
             phi, _, _ = dae.bem_state(it.t, y)
             phis.append(phi.copy())
-            phi_t = it.derivative(phis[-3:])
-            forces.append(dae.hull_force_x(it.t, y, phi_t, U))
+            dphi_dt = it.derivative(phis[-3:])
+            eta_t = it.last_yp[:dae.n_nodes]
+            forces.append(dae.hull_force(it.t, y, dphi_dt, U, eta_t)[0])
         return it.y.copy(), np.array(forces), dict(it.stats), dict(dae.counters)
 
 

@@ -129,3 +129,45 @@ def test_prepare_bem_vectors_closed_forms():
     for i in cand[:3]:
         dup = [d for d in idx[ptr[i]:ptr[i + 1]] if d != i]
         assert abs(bc[i] - np.mean(prev[dup])) < 1e-15 and dn[i] == bc[i]
+
+
+def test_potential_gradient_closed_form_on_box():
+    """phi = a.x, dphi/dn = a.n on every face: the complete gradient is a on every cell, so
+    the projection returns a at every dof, before and after the double-node mean."""
+    m = M.box_mesh(4, lo=(-0.2, 0.1, 0.0), hi=(0.7, 0.9, 1.3))
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    a = np.array([0.3, -0.7, 1.1])
+    phi = m.support_points @ a
+    psi = O.node_normals(m.support_points, m.cell_dofs, m.cell_orient) @ a
+    for avg in (False, True):
+        g = O.potential_gradients(m.support_points, m.cell_dofs, m.cell_orient, phi, psi, ptr,
+                                  idx, average=avg)
+        assert np.abs(g - a).max() < 1e-13
+
+
+def test_potential_gradient_without_psi_is_the_surface_gradient():
+    """With dphi/dn = 0 the projection is the surface-gradient projection of the C oracle."""
+    m = M.cube_sphere_mesh(n=4)
+    ptr, idx = M.double_nodes_csr(m.support_points)
+    phi = np.random.default_rng(3).standard_normal(m.n_dofs)
+    g = O.potential_gradients(m.support_points, m.cell_dofs, m.cell_orient, phi,
+                              np.zeros(m.n_dofs), ptr, idx, average=False)
+    ref = O.surface_gradients(m.support_points, m.cell_dofs, m.cell_orient, phi)
+    assert np.abs(g - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_potential_gradient_on_sphere_converges_to_harmonic_gradient():
+    """phi = z (harmonic), dphi/dn = n_z on the unit sphere: the projected complete gradient
+    tends to e_z as the mesh is refined (first order at the nodes in the max norm: the flat
+    cell normals carry the n psi term), and coincident dofs agree after the mean."""
+    errs = []
+    for k in (4, 8):
+        m = M.cube_sphere_mesh(n=k)
+        ptr, idx = M.double_nodes_csr(m.support_points)
+        X = m.support_points
+        nrm = X / np.linalg.norm(X, axis=1)[:, None]
+        g = O.potential_gradients(X, m.cell_dofs, m.cell_orient, X[:, 2], nrm[:, 2], ptr, idx)
+        errs.append(np.abs(g - np.array([0.0, 0.0, 1.0])).max())
+        for i in range(m.n_dofs):
+            assert np.array_equal(g[i], g[idx[ptr[i]]])
+    assert errs[0] < 0.05 and errs[1] < errs[0] / 1.8, errs

@@ -121,6 +121,8 @@ _SIGS = {
     "wb_set_constraints": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, _dp, _dp]),
     "wb_compute_normals": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int)]),
     "wb_compute_surface_gradients": (ctypes.c_int, [_vp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]),
+    "wb_compute_potential_gradients": (ctypes.c_int, [_vp, _dp, _dp, _dp,
+                                                      ctypes.POINTER(ctypes.c_int)]),
     "wb_get_masks": (ctypes.c_int, [_vp, _dp, _dp]),
     "wb_prepare_bem_vectors": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, ctypes.c_int, _i32p,
                                               ctypes.c_int, _i32p, _dp, _dp,

@@ -20,7 +20,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-RHO = 1025.1  # source/free_surface.cc:9418
+RHO = 1025.1  # source/free_surface.cc:9417
 
 
 def _gauss01(q):

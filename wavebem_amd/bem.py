@@ -264,6 +264,17 @@ class BEMProblem:
         self.last_projection_iterations = it.value
         return out
 
+    def compute_potential_gradients(self, phi, dphi_dn):
+        """FreeSurface::compute_potential_gradients: (n, 3) L2 projection of the complete
+        gradient n dphi_dn + surface gradient of phi, averaged over coincident dofs."""
+        out = np.empty((self.n, 3))
+        it = ctypes.c_int(0)
+        self._check(L.lib().wb_compute_potential_gradients(
+            self._ctx, _d(_f64(phi, self.n)), _d(_f64(dphi_dn, self.n)), _d(out),
+            ctypes.byref(it)), "wb_compute_potential_gradients")
+        self.last_projection_iterations = it.value
+        return out
+
     def prepare_bem_vectors(self, free_surface_ids, wall_ids, inflow_ids, wind, bem_bc,
                             dphi_dn, inflow_dphi_dn=None, transom=None, prev_dphi_dn=None):
         """FreeSurface::prepare_bem_vectors: installs the masks of the free-surface cells

@@ -5,7 +5,8 @@
 // :6099-6109).  For a functional J(u) of the solution of A_c u = b (the constrained
 // system of source/bem_problem.cc:821-895), dJ/dtheta = dJ/dtheta|_u - lambda^T dR/dtheta
 // with A_c^T lambda = dJ/du.  This file holds:
-//   * the transposed column-selected product A_sel^T x (kernel k_gemvT_f64): every
+//   * the transposed column-selected product A_sel^T x (kernel k_gemvT, FP64 entries, the
+//     FP32 hi part alone, or hi + bf16 lo accumulated in FP64): every
 //     workgroup owns a strip of 128 columns and a chunk of 512 rows; lanes read whole row
 //     segments (coalesced, like the forward product), four rows per step, and keep their
 //     two columns' sums in registers; the chunk partials are added in a fixed order;
@@ -45,15 +46,62 @@ typedef double dvec2 __attribute__((ext_vector_type(2)));
 constexpr int GT_WAVES = 4;
 constexpr int GT_COLS = 128;  // columns per workgroup: 64 lanes x 2
 
+// The column pair (col, col + 1) of row offset off, as FP64: MODE 0 FP64 entries, MODE 1
+// the FP32 hi part, MODE 2 hi + bf16 lo.  M0/M1 (and L0/L1) are the pair's matrices.
+template <int MODE>
+__device__ __forceinline__ void load_pair(const void *M0, const void *M1, const uint16_t *L0,
+                                          const uint16_t *L1, size_t off, bool same, double &v0,
+                                          double &v1)
+{
+    if (MODE == 0) {
+        if (same) {
+            const dvec2 v = __builtin_nontemporal_load((const dvec2 *)((const double *)M0 + off));
+            v0 = v.x;
+            v1 = v.y;
+        } else {
+            v0 = ((const double *)M0)[off];
+            v1 = ((const double *)M1)[off + 1];
+        }
+    } else {
+        float h0, h1;
+        if (same) {
+            const float2 h = *(const float2 *)((const float *)M0 + off);
+            h0 = h.x;
+            h1 = h.y;
+        } else {
+            h0 = ((const float *)M0)[off];
+            h1 = ((const float *)M1)[off + 1];
+        }
+        v0 = h0;
+        v1 = h1;
+        if (MODE == 2) {
+            uint16_t l0, l1;
+            if (same) {
+                const ushort2 l = *(const ushort2 *)(L0 + off);
+                l0 = l.x;
+                l1 = l.y;
+            } else {
+                l0 = L0[off];
+                l1 = L1[off + 1];
+            }
+            v0 += (double)__uint_as_float((uint32_t)l0 << 16);
+            v1 += (double)__uint_as_float((uint32_t)l1 << 16);
+        }
+    }
+}
+
 // part[chunk][col] = sum over the chunk's unconstrained rows i of
-// sel(col) * A_sel(col)(i, col) * x[i].  Rows are taken in order (constrained rows read
-// with weight 0), four per wavefront step, so the row addresses are affine and the loads
-// of a step are independent.
-__global__ __launch_bounds__(64 * GT_WAVES) void k_gemvT_f64(
-    const double *__restrict__ G, const double *__restrict__ H, size_t ld, int n,
-    const uint8_t *__restrict__ gmask, int flip, double s_sel, double s_uns,
-    const double *__restrict__ x, const uint8_t *__restrict__ constrained, int rows_per_chunk,
-    double *__restrict__ part)
+// sel(col) * A_sel(col)(i, col) * x[i], accumulated in FP64 for every storage MODE (x of
+// type XT: double, or float for the FP32 inner solves).  Rows are taken in order
+// (constrained rows read with weight 0), four per wavefront step, so the row addresses are
+// affine and the loads of a step are independent.  Columns come in pairs, so a pair at an
+// odd-aligned column of FP32 or bf16 storage still loads 8 and 4 bytes.
+template <int MODE, typename XT>
+__global__ __launch_bounds__(64 * GT_WAVES) void k_gemvT(
+    const void *__restrict__ G, const void *__restrict__ H, const uint16_t *__restrict__ Glo,
+    const uint16_t *__restrict__ Hlo, size_t ld, int n, const uint8_t *__restrict__ gmask,
+    int flip, double s_sel, double s_uns, const XT *__restrict__ x,
+    const uint8_t *__restrict__ constrained, int rows_per_chunk, double *__restrict__ part)
 {
     __shared__ double red[GT_WAVES][64][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -65,46 +113,37 @@ __global__ __launch_bounds__(64 * GT_WAVES) void k_gemvT_f64(
         b0 = (gm >> (col & 3)) & 1u;
         b1 = (gm >> ((col + 1) & 3)) & 1u;
     }
-    const double *M0 = b0 ? H : G, *M1 = b1 ? H : G;
+    const void *M0 = b0 ? H : G, *M1 = b1 ? H : G;
+    const uint16_t *L0 = b0 ? Hlo : Glo, *L1 = b1 ? Hlo : Glo;
     const bool same = b0 == b1;
     double a0 = 0.0, a1 = 0.0;
     const int i0 = blockIdx.y * rows_per_chunk;
     const int i1 = min(n, i0 + rows_per_chunk);
     constexpr int U = 4;  // rows per wavefront step
-    auto xw = [&](int i) { return constrained[i] ? 0.0 : x[i]; };
+    auto xw = [&](int i) { return constrained[i] ? 0.0 : (double)x[i]; };
     if (live) {
         const int nfull = (i1 - i0) / U;  // whole groups of U rows; group g to wave g % 4
         for (int g = wave; g < nfull; g += GT_WAVES) {
             const int i = i0 + g * U;
-            double xi[U];
+            double xi[U], v0[U], v1[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) xi[u] = xw(i + u);
-            if (same) {
-                dvec2 v[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u)
-                    v[u] = __builtin_nontemporal_load(
-                        (const dvec2 *)(M0 + (size_t)(i + u) * ld + col));
+            for (int u = 0; u < U; ++u)
+                load_pair<MODE>(M0, M1, L0, L1, (size_t)(i + u) * ld + col, same, v0[u], v1[u]);
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    a0 += v[u].x * xi[u];
-                    a1 += v[u].y * xi[u];
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const size_t off = (size_t)(i + u) * ld + col;
-                    a0 += M0[off] * xi[u];
-                    a1 += M1[off + 1] * xi[u];
-                }
+            for (int u = 0; u < U; ++u) {
+                a0 += v0[u] * xi[u];
+                a1 += v1[u] * xi[u];
             }
         }
         if (wave == 0)  // the last (i1 - i0) % U rows
             for (int i = i0 + nfull * U; i < i1; ++i) {
                 const double xv = xw(i);
-                const size_t off = (size_t)i * ld + col;
-                a0 += M0[off] * xv;
-                a1 += M1[off + 1] * xv;
+                double v0, v1;
+                load_pair<MODE>(M0, M1, L0, L1, (size_t)i * ld + col, false, v0, v1);
+                a0 += v0 * xv;
+                a1 += v1 * xv;
             }
     }
     red[wave][lane][0] = a0 * (b0 ? s_sel : s_uns);
@@ -124,30 +163,32 @@ __global__ __launch_bounds__(64 * GT_WAVES) void k_gemvT_f64(
 }
 
 // y[j] = sum_chunks part[c][j] + (row j unconstrained ? diag[j] x[j] : 0)
+template <typename XT>
 __global__ void k_gemvT_reduce(const double *__restrict__ part, int n_chunks, size_t ld, int n,
                                const double *__restrict__ diag,
                                const uint8_t *__restrict__ constrained,
-                               const double *__restrict__ x, double *__restrict__ y)
+                               const XT *__restrict__ x, XT *__restrict__ y)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     double s = 0.0;
     for (int c = 0; c < n_chunks; ++c) s += part[(size_t)c * ld + j];
-    if (diag && !(constrained && constrained[j])) s += diag[j] * x[j];
-    y[j] = s;
+    if (diag && !(constrained && constrained[j])) s += diag[j] * (double)x[j];
+    y[j] = (XT)s;
 }
 
 // y[tgt[t]] += sum_p coef[p] x[src[p]]  (targets distinct)
+template <typename T>
 __global__ void k_constraint_T(int n_targets, const int *__restrict__ tgt,
                                const int *__restrict__ ptr, const int *__restrict__ src,
-                               const double *__restrict__ coef, const double *__restrict__ x,
-                               double *__restrict__ y)
+                               const double *__restrict__ coef, const T *__restrict__ x,
+                               T *__restrict__ y)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_targets) return;
     double s = y[tgt[t]];
-    for (int p = ptr[t]; p < ptr[t + 1]; ++p) s += coef[p] * x[src[p]];
-    y[tgt[t]] = s;
+    for (int p = ptr[t]; p < ptr[t + 1]; ++p) s += coef[p] * (double)x[src[p]];
+    y[tgt[t]] = (T)s;
 }
 
 // Phi and Psi (Q1 interpolants of phi and psi) at every regular point of every cell
@@ -640,26 +681,46 @@ __global__ void k_sum_chunks(const double *__restrict__ part, int n_chunks, size
 // 512-row chunks: about 4 x n / 128 wavefronts, enough to keep HBM busy at 10k dofs
 int gemvT_chunks(int n) { return max(1, min(128, (n + 511) / 512)); }
 
-void launch_gemvT(const double *G, const double *H, size_t ld, int n, const uint8_t *gmask,
-                  int flip, double s_sel, double s_unsel, const double *x,
-                  const double *diag, const uint8_t *constrained, double *part, double *y,
-                  hipStream_t s)
+void launch_gemvT(int mode, const void *G, const void *H, const void *Glo, const void *Hlo,
+                  size_t ld, int n, const uint8_t *gmask, int flip, double s_sel,
+                  double s_unsel, const void *x, const double *diag,
+                  const uint8_t *constrained, double *part, void *y, hipStream_t s)
 {
     const int chunks = gemvT_chunks(n);
     const int rpc = (n + chunks - 1) / chunks;
     dim3 grid((n + GT_COLS - 1) / GT_COLS, chunks);
-    hipLaunchKernelGGL(k_gemvT_f64, grid, dim3(64 * GT_WAVES), 0, s, G, H, ld, n, gmask, flip,
-                       s_sel, s_unsel, x, constrained, rpc, part);
-    hipLaunchKernelGGL(k_gemvT_reduce, dim3((n + 255) / 256), dim3(256), 0, s, part, chunks, ld,
-                       n, diag, constrained, x, y);
+    const uint16_t *gl = (const uint16_t *)Glo, *hl = (const uint16_t *)Hlo;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_gemvT<0, double>), grid, dim3(64 * GT_WAVES), 0, s, G, H, gl, hl,
+                           ld, n, gmask, flip, s_sel, s_unsel, (const double *)x, constrained,
+                           rpc, part);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_gemvT<1, float>), grid, dim3(64 * GT_WAVES), 0, s, G, H, gl, hl,
+                           ld, n, gmask, flip, s_sel, s_unsel, (const float *)x, constrained,
+                           rpc, part);
+    else
+        hipLaunchKernelGGL((k_gemvT<2, double>), grid, dim3(64 * GT_WAVES), 0, s, G, H, gl, hl,
+                           ld, n, gmask, flip, s_sel, s_unsel, (const double *)x, constrained,
+                           rpc, part);
+    if (mode == 1)
+        hipLaunchKernelGGL(k_gemvT_reduce<float>, dim3((n + 255) / 256), dim3(256), 0, s, part,
+                           chunks, ld, n, diag, constrained, (const float *)x, (float *)y);
+    else
+        hipLaunchKernelGGL(k_gemvT_reduce<double>, dim3((n + 255) / 256), dim3(256), 0, s, part,
+                           chunks, ld, n, diag, constrained, (const double *)x, (double *)y);
 }
 
-void launch_constraint_T(int n_targets, const int *tgt, const int *ptr, const int *src,
-                         const double *coef, const double *x, double *y, hipStream_t s)
+void launch_constraint_T(int is_float, int n_targets, const int *tgt, const int *ptr,
+                         const int *src, const double *coef, const void *x, void *y,
+                         hipStream_t s)
 {
     if (n_targets <= 0) return;
-    hipLaunchKernelGGL(k_constraint_T, dim3((n_targets + 255) / 256), dim3(256), 0, s, n_targets,
-                       tgt, ptr, src, coef, x, y);
+    if (is_float)
+        hipLaunchKernelGGL(k_constraint_T<float>, dim3((n_targets + 255) / 256), dim3(256), 0, s,
+                           n_targets, tgt, ptr, src, coef, (const float *)x, (float *)y);
+    else
+        hipLaunchKernelGGL(k_constraint_T<double>, dim3((n_targets + 255) / 256), dim3(256), 0,
+                           s, n_targets, tgt, ptr, src, coef, (const double *)x, (double *)y);
 }
 
 int vjp_row_chunks(int nc) { return max(1, min(64, (nc + 511) / 512)); }

@@ -112,8 +112,8 @@ struct wb_ctx {
     DevBuf vjp_w, vjp_phi, vjp_psi, vjp_rows, vjp_cstate, vjp_p1, vjp_p2, vjp_xg, vjp_vbar,
         vjp_xs, vjp_spart;
     // L2 projections: node -> (cell, local vertex) incidence, element blocks, CG vectors
-    DevBuf inc_ptr, inc_item, pj_emass, pj_erhs, pj_u, pj_B, pj_X, pj_R, pj_Z, pj_P, pj_Q,
-        pj_dinv, pj_part, pj_part2, pj_state;
+    DevBuf inc_ptr, inc_item, pj_emass, pj_erhs, pj_u, pj_psi, pj_B, pj_X, pj_R, pj_Z, pj_P,
+        pj_Q, pj_dinv, pj_part, pj_part2, pj_state;
     int proj_iterations = 0;
     int n_lines = 0, n_free = 0;
     DevBuf binv, binv32;
@@ -178,8 +178,8 @@ struct wb_ctx {
             &Pn, &hbuf, &hred, &vx, &vb, &vr, &vt, &vs, &V32, &z32, &w32, &r32, &d32, &ct_tgt,
             &ct_ptr, &ct_src, &ct_coef, &gt_part, &vjp_w, &vjp_phi, &vjp_psi, &vjp_rows,
             &vjp_cstate, &vjp_p1, &vjp_p2, &vjp_xg, &vjp_vbar, &vjp_xs, &vjp_spart, &inc_ptr,
-            &inc_item, &pj_emass, &pj_erhs, &pj_u, &pj_B, &pj_X, &pj_R, &pj_Z, &pj_P, &pj_Q,
-            &pj_dinv, &pj_part, &pj_part2, &pj_state, &A16, &rs16, &hm_idx, &hm_size, &hm_seg,
+            &inc_item, &pj_emass, &pj_erhs, &pj_u, &pj_psi, &pj_B, &pj_X, &pj_R, &pj_Z, &pj_P,
+            &pj_Q, &pj_dinv, &pj_part, &pj_part2, &pj_state, &A16, &rs16, &hm_idx, &hm_size, &hm_seg,
             &hm_near, &hm_rank, &hm_scale, &hm_Y, &hm_Mx, &hm_Q, &hm_B, &hm_D, &hm_T, &hm_part};
         for (DevBuf *b : mw.all()) v.push_back(b);
         return v;
@@ -405,17 +405,18 @@ static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
     if (c->op_transpose) {
         // A_c^T x: transposed product over the unconstrained rows (the diagonal alpha term
         // of those rows included), then the transposed constraint lines
-        require(mode == 0, WB_E_UNSUPPORTED, "transposed operator needs FP64 matrices");
+        // (mode 0: FP64 matrices; FP32-IR: mode 1 the hi part on FP32 vectors, mode 2 hi + lo)
+        require(mode <= 2, WB_E_UNSUPPORTED,
+                "the transposed operator has no FP16 or compressed form");
         const int chunks = gemvT_chunks(c->n);
         c->gt_part.ensure(sizeof(double) * c->ld * chunks);
-        KernelScope ks(c, WB_K_GEMV64, (long long)c->n_free);
-        launch_gemvT(c->G.as<double>(), c->H.as<double>(), c->ld, c->n, c->gmask.as<uint8_t>(), 0,
-                     1.0, -1.0, (const double *)x, c->diag_op.as<double>(),
-                     c->constrained.as<uint8_t>(), c->gt_part.as<double>(), (double *)y,
-                     c->stream);
-        launch_constraint_T(c->n_ct, c->ct_tgt.as<int>(), c->ct_ptr.as<int>(),
-                            c->ct_src.as<int>(), c->ct_coef.as<double>(), (const double *)x,
-                            (double *)y, c->stream);
+        KernelScope ks(c, mode == 0 ? WB_K_GEMV64 : (mode == 1 ? WB_K_GEMV32 : WB_K_GEMVHL),
+                       (long long)c->n_free);
+        launch_gemvT(mode, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->ld, c->n,
+                     c->gmask.as<uint8_t>(), 0, 1.0, -1.0, x, c->diag_op.as<double>(),
+                     c->constrained.as<uint8_t>(), c->gt_part.as<double>(), y, c->stream);
+        launch_constraint_T(is_float, c->n_ct, c->ct_tgt.as<int>(), c->ct_ptr.as<int>(),
+                            c->ct_src.as<int>(), c->ct_coef.as<double>(), x, y, c->stream);
         return;
     }
     if (c->pure_neumann) {
@@ -836,9 +837,12 @@ static void run_solver(wb_ctx *c, wb_solve_stats *st)
             throw WbError(WB_E_NOCONV, "GMRES did not converge (SolverControl::NoConvergence)");
         }
     } else {
-        // FP64 iterative refinement around an FP32-vector GMRES on the FP32 (hi) operator
-        // or on its row-scaled FP16 copy.
-        const int fmode = c->cfg.ir_inner_format == 2 ? 4 : (c->cfg.ir_inner_format == 1 ? 3 : 1);
+        // FP64 iterative refinement around an FP32-vector GMRES on the FP32 (hi) operator,
+        // its row-scaled FP16 copy or its compressed copy; the transposed system (adjoint)
+        // runs its inner solves on the transposed hi part
+        const int fmode = c->op_transpose ? 1
+                        : (c->cfg.ir_inner_format == 2 ? 4
+                                                        : (c->cfg.ir_inner_format == 1 ? 3 : 1));
         if (fmode == 3) ensure_h16(c);
         if (fmode == 4) ensure_hm(c);
         FloatMode fm(c, fmode);
@@ -937,7 +941,7 @@ static void build_rhs(wb_ctx *c, const void *tmp_dev)
 // matrix to a relative residual of 1e-14 per column, in place of the reference's direct
 // solve (the mass matrix is SPD and, diagonally scaled, well conditioned).
 static void project_fields(wb_ctx *c, const double *u, int normalize, double *normals,
-                           double *grads)
+                           double *grads, const double *psi = nullptr)
 {
     const int n = c->n, nc = c->plan.nc, nq = c->quad.q_order * c->quad.q_order;
     hipStream_t s = c->stream;
@@ -957,10 +961,17 @@ static void project_fields(wb_ctx *c, const double *u, int normalize, double *no
         upload(c->pj_u, ui.data(), ui.size(), s);
         ud = c->pj_u.as<double>();
     }
+    const double *pd = nullptr;
+    if (psi) {
+        std::vector<double> pi;
+        to_internal(c, psi, pi);
+        upload(c->pj_psi, pi.data(), pi.size(), s);
+        pd = c->pj_psi.as<double>();
+    }
     double *X = c->pj_X.as<double>(), *R = c->pj_R.as<double>(), *Z = c->pj_Z.as<double>(),
            *P = c->pj_P.as<double>(), *Q = c->pj_Q.as<double>(), *st = c->pj_state.as<double>();
-    launch_proj_setup(c->X.as<double>(), c->cell_int.as<int>(), c->orient.as<int8_t>(), ud, nc,
-                      nq, c->qpts.as<double>(), c->qw.as<double>(), c->inc_ptr.as<int>(),
+    launch_proj_setup(c->X.as<double>(), c->cell_int.as<int>(), c->orient.as<int8_t>(), ud, pd,
+                      nc, nq, c->qpts.as<double>(), c->qw.as<double>(), c->inc_ptr.as<int>(),
                       c->inc_item.as<int>(), n, c->pj_emass.as<double>(),
                       c->pj_erhs.as<double>(), c->pj_B.as<double>(), c->pj_dinv.as<double>(), s);
     launch_proj_cg_start(c->pj_B.as<double>(), c->pj_dinv.as<double>(), n, X, R, Z, P,
@@ -1700,6 +1711,29 @@ wb_status wb_compute_surface_gradients(wb_ctx *c, const double *tmp_rhs, double 
     });
 }
 
+wb_status wb_compute_potential_gradients(wb_ctx *c, const double *phi, const double *dphi_dn,
+                                         double *grads, int *cg_iterations)
+{
+    return guarded(c, [&] {
+        require(c->have_mesh, WB_E_STATE, "wb_compute_potential_gradients before wb_set_mesh");
+        require(phi && dphi_dn && grads, WB_E_ARG, "null vector");
+        project_fields(c, phi, 0, nullptr, grads, dphi_dn);
+        // coincident dofs take the mean of their projected gradients
+        const MeshPlan &p = c->plan;
+        std::vector<double> g(grads, grads + 3 * (size_t)c->n);
+        for (int i = 0; i < c->n; ++i) {
+            const int q0 = p.dn_ptr[i], q1 = p.dn_ptr[i + 1];
+            if (q1 - q0 < 2) continue;
+            for (int e = 0; e < 3; ++e) {
+                double a = 0.0;
+                for (int q = q0; q < q1; ++q) a += g[3 * (size_t)p.dn_idx[q] + e];
+                grads[3 * (size_t)i + e] = a / (q1 - q0);
+            }
+        }
+        if (cg_iterations) *cg_iterations = c->proj_iterations;
+    });
+}
+
 // G and H (and the lo halves in FP32_IR), zero-filled so the padding columns read as 0
 static void alloc_matrices(wb_ctx *c)
 {
@@ -2285,12 +2319,27 @@ wb_status wb_solve_adjoint(wb_ctx *c, const double *rhs, double *lambda, wb_solv
     return guarded(c, [&] {
         check_ready(c, true);
         require(rhs && lambda, WB_E_ARG, "null adjoint vector");
-        require(c->cfg.precision == WB_PREC_FP64, WB_E_UNSUPPORTED,
-                "the adjoint solve needs an FP64 context");
         require(!c->pure_neumann, WB_E_UNSUPPORTED,
                 "the adjoint solve does not support all-Neumann masks");
         ensure_work(c);
         upload_vec(c, c->vb, rhs);
+        struct Flip {
+            wb_ctx *c;
+            Flip(wb_ctx *c_) : c(c_) { c->op_transpose = true; }
+            ~Flip() { c->op_transpose = false; }
+        };
+        if (c->cfg.precision == WB_PREC_FP32_IR) {
+            // iterative refinement of the transposed system: residuals with the transposed
+            // hi + lo operator, inner GMRES on the transposed FP32 hi part, transposed block
+            // Jacobi (the transposed coarse space is built from FP64 matrices only)
+            ensure_precond(c);
+            Flip flip(c);
+            wb_solve_stats s{};
+            run_solver(c, &s);
+            if (st) *st = s;
+            download_vec(c, c->vx, lambda);
+            return;
+        }
         double t0 = now_ms();
         ensure_precond(c);
         if (c->cs_on && !c->cs_t_on) {
@@ -2308,11 +2357,7 @@ wb_status wb_solve_adjoint(wb_ctx *c, const double *rhs, double *lambda, wb_solv
         HIPCHK(hipStreamSynchronize(c->stream));
         double t1 = now_ms();
         launch_vec_ops(6, c->n, nullptr, nullptr, c->vx.p, 0, c->stream);
-        struct Flip {
-            wb_ctx *c;
-            Flip(wb_ctx *c_) : c(c_) { c->op_transpose = true; }
-            ~Flip() { c->op_transpose = false; }
-        } flip(c);
+        Flip flip(c);
         GmresResult R = gmres(c, 0, c->vb.p, c->vx.p, c->cfg.solver_tolerance, 0.0,
                               c->cfg.solver_max_steps, true);
         HIPCHK(hipStreamSynchronize(c->stream));

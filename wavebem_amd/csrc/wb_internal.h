@@ -173,9 +173,9 @@ void launch_assemble_singular(int precision, const double *support_int, const in
 // surface gradient x,y,z.  state holds rho[2][6], |b|^2[6], |r|^2[6].
 int proj_blocks(int n);
 void launch_proj_setup(const double *X, const int *cell_int, const int8_t *orient,
-                       const double *u, int nc, int nq, const double *qpts, const double *qw,
-                       const int *inc_ptr, const int *inc_item, int n, double *emass,
-                       double *erhs, double *B, double *dinv, hipStream_t s);
+                       const double *u, const double *psi, int nc, int nq, const double *qpts,
+                       const double *qw, const int *inc_ptr, const int *inc_item, int n,
+                       double *emass, double *erhs, double *B, double *dinv, hipStream_t s);
 void launch_proj_cg_start(const double *B, const double *dinv, int n, double *Xs, double *R,
                           double *Z, double *P, double *part2, double *state, hipStream_t s);
 void launch_proj_cg_iter(const int *inc_ptr, const int *inc_item, const int *cell_int,
@@ -263,15 +263,17 @@ void launch_coarse_build_t(int n, int agg, int nc, const double *G, const double
                            hipStream_t s);
 
 // ---------------------------------------------------------------- adjoint (wb_adjoint.hip)
-// y = A_sel^T x over the unconstrained rows (FP64 matrices) + diag .* x on those rows;
-// part: gemvT_chunks(n) x ld doubles
+// y = A_sel^T x over the unconstrained rows + diag .* x on those rows; mode 0: FP64
+// matrices and vectors, 1: FP32 hi part and FP32 vectors, 2: hi + bf16 lo and FP64 vectors
+// (FP64 sums in every mode); part: gemvT_chunks(n) x ld doubles
 int gemvT_chunks(int n);
-void launch_gemvT(const double *G, const double *H, size_t ld, int n, const uint8_t *gmask,
-                  int flip, double s_sel, double s_unsel, const double *x,
-                  const double *diag, const uint8_t *constrained, double *part, double *y,
-                  hipStream_t s);
-void launch_constraint_T(int n_targets, const int *tgt, const int *ptr, const int *src,
-                         const double *coef, const double *x, double *y, hipStream_t s);
+void launch_gemvT(int mode, const void *G, const void *H, const void *Glo, const void *Hlo,
+                  size_t ld, int n, const uint8_t *gmask, int flip, double s_sel,
+                  double s_unsel, const void *x, const double *diag,
+                  const uint8_t *constrained, double *part, void *y, hipStream_t s);
+void launch_constraint_T(int is_float, int n_targets, const int *tgt, const int *ptr,
+                         const int *src, const double *coef, const void *x, void *y,
+                         hipStream_t s);
 // backward of the assembly; buffer sizes: cstate nc*nq*2, part1 vjp_row_chunks(nc)*n*3,
 // part2 vjp_point_chunks(n)*nc*nq*7, xgrad n*3, vbar nc*12, xs n*3, spart n_sing*12
 int vjp_row_chunks(int nc);

@@ -3,10 +3,11 @@
 // Replaces the UMFPACK mass-matrix solves of
 //   ComputationalDomain::compute_normals        (source/computational_domain.cc:1525-1620),
 //   NumericalTowingTank::compute_normals_at_nodes (source/numerical_towing_tank.cc:3358-3433),
-//   BEMProblem::compute_surface_gradients       (source/bem_problem.cc:1154-1290):
+//   BEMProblem::compute_surface_gradients       (source/bem_problem.cc:1154-1290),
+//   FreeSurface::compute_potential_gradients    (source/free_surface.cc:9271-9400):
 //   sum_j M_ij x_j = sum_c sum_q phi_i(q) f(q) JxW(q),  M_ij = sum_c sum_q phi_i phi_j JxW,
 // with f = the cell normal (normals) or the surface gradient of the Q1 interpolant of u
-// (gradients), one scalar mass matrix for the three components (the reference's vector
+// (gradients; plus n psi for the complete potential gradient), one scalar mass matrix for the three components (the reference's vector
 // mass matrix couples only equal components).
 //
 // The mass matrix stays in element form: ten values per cell (the symmetric 4x4 block),
@@ -32,19 +33,23 @@ __host__ __device__ constexpr int msym(int a, int b)
 }
 
 // one thread per cell: element mass block (10) and element right-hand sides (4 x 6)
+// With psi (dphi/dn at the dofs), the gradient columns take the complete potential
+// gradient n psi + surface gradient of u (FreeSurface::compute_potential_gradients,
+// source/free_surface.cc:9325-9344).
 __global__ void k_proj_cells(const double *__restrict__ X, const int *__restrict__ cell_int,
                              const int8_t *__restrict__ orient, const double *__restrict__ u,
-                             int nc, int nq, const double *__restrict__ qpts,
-                             const double *__restrict__ qw, double *__restrict__ emass,
-                             double *__restrict__ erhs)
+                             const double *__restrict__ psi, int nc, int nq,
+                             const double *__restrict__ qpts, const double *__restrict__ qw,
+                             double *__restrict__ emass, double *__restrict__ erhs)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    double v[4][3], uc[4] = {0, 0, 0, 0};
+    double v[4][3], uc[4] = {0, 0, 0, 0}, pc[4] = {0, 0, 0, 0};
     for (int k = 0; k < 4; ++k) {
         const int d = cell_int[4 * c + k];
         for (int e = 0; e < 3; ++e) v[k][e] = X[3 * d + e];
         if (u) uc[k] = u[d];
+        if (psi) pc[k] = psi[d];
     }
     const double so = orient[c] < 0 ? -1.0 : 1.0;
     double m[10], r[4][NCOL];
@@ -55,7 +60,7 @@ __global__ void k_proj_cells(const double *__restrict__ X, const int *__restrict
         double sh[4], dx[4], de[4];
         q1_shape(qpts[2 * q], qpts[2 * q + 1], sh, dx, de);
         double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
-        double ux = 0.0, ue = 0.0;
+        double ux = 0.0, ue = 0.0, pq = 0.0;
         for (int k = 0; k < 4; ++k) {
             for (int e = 0; e < 3; ++e) {
                 a[e] += dx[k] * v[k][e];
@@ -63,6 +68,7 @@ __global__ void k_proj_cells(const double *__restrict__ X, const int *__restrict
             }
             ux += dx[k] * uc[k];
             ue += de[k] * uc[k];
+            pq += sh[k] * pc[k];
         }
         const double g00 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
         const double g01 = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
@@ -77,7 +83,7 @@ __global__ void k_proj_cells(const double *__restrict__ X, const int *__restrict
         double f[NCOL];
         for (int e = 0; e < 3; ++e) {
             f[e] = nn[e] * ns * jxw;
-            f[3 + e] = (ca * a[e] + cb * b[e]) * jxw;
+            f[3 + e] = (ca * a[e] + cb * b[e] + nn[e] * ns * pq) * jxw;
         }
         for (int i = 0; i < 4; ++i) {
             for (int j = i; j < 4; ++j) m[msym(i, j)] += sh[i] * sh[j] * jxw;
@@ -257,12 +263,12 @@ __global__ void __launch_bounds__(PB) k_proj_init(const double *__restrict__ B,
 int proj_blocks(int n) { return (n + PB - 1) / PB; }
 
 void launch_proj_setup(const double *X, const int *cell_int, const int8_t *orient,
-                       const double *u, int nc, int nq, const double *qpts, const double *qw,
-                       const int *inc_ptr, const int *inc_item, int n, double *emass,
-                       double *erhs, double *B, double *dinv, hipStream_t s)
+                       const double *u, const double *psi, int nc, int nq, const double *qpts,
+                       const double *qw, const int *inc_ptr, const int *inc_item, int n,
+                       double *emass, double *erhs, double *B, double *dinv, hipStream_t s)
 {
     hipLaunchKernelGGL(k_proj_cells, dim3((nc + 127) / 128), dim3(128), 0, s, X, cell_int,
-                       orient, u, nc, nq, qpts, qw, emass, erhs);
+                       orient, u, psi, nc, nq, qpts, qw, emass, erhs);
     hipLaunchKernelGGL(k_proj_rows, dim3(proj_blocks(n)), dim3(PB), 0, s, inc_ptr, inc_item,
                        emass, erhs, n, B, dinv);
 }

@@ -6,9 +6,11 @@ The reference integrates, over the wall (hull) cells and the 4x4 Gauss points of
   press_moment = sum_q p(q) (x_q - b) x n_q JxW_q   (:9599-9604, b the hull barycentre)
 with p(q) the nodal pressure interpolated by the Q1 shape functions and n_q the cell
 normal, adds the ITTC-1957 viscous drag (:10121-10126) and appends one line per time to
-"<output_file_name>_force.txt" (:10128-10139).  The nodal pressure itself comes from the
-DAE state (outside this library); ``linearised_pressure`` gives the single-state form of the
-adjoint configuration, p = -rho (phi_t + U dphi/dx), as used by adjoint.hull_drag.
+"<output_file_name>_force.txt" (:10128-10139).  ``bernoulli_pressure`` is the reference's
+nodal pressure (:9662-9674) from the complete potential gradient
+(BEMProblem.compute_potential_gradients, on the device) and the time derivative of phi;
+``linearised_pressure`` is the single-state form of the adjoint configuration,
+p = -rho (phi_t + U dphi/dx), as used by adjoint.hull_drag.
 """
 from __future__ import annotations
 
@@ -16,7 +18,8 @@ import math
 
 import numpy as np
 
-RHO = 1025.1          # source/free_surface.cc:9418
+RHO = 1025.1          # source/free_surface.cc:9417
+GRAVITY = 9.81        # source/free_surface.cc:9416 (g of compute_pressure)
 NU_WATER = 1.307e-6   # kinematic viscosity at 10 C, source/free_surface.cc:10122
 
 
@@ -56,6 +59,23 @@ def surface_forces(support, cells, orient, p_nodes, barycentre=(0.0, 0.0, 0.0), 
     arm = X - np.asarray(barycentre, dtype=np.float64)[None, None, :]
     moment = np.cross(arm, f).sum(axis=(0, 1))
     return force, moment, float(jxw.sum())
+
+
+def bernoulli_pressure(dphi_dt, grad_phi, z, V_inf, node_vel=None, rho=RHO, g=GRAVITY):
+    """Nodal pressure of FreeSurface::compute_pressure (source/free_surface.cc:9662-9674):
+        p = 1/2 rho |V_inf|^2 - 1/2 rho |V_inf + grad phi|^2 - rho g z
+            - rho (DphiDt - v . grad phi),
+    with grad phi the complete potential gradient at the nodes (n, 3;
+    compute_potential_gradients), DphiDt the time derivative of phi following the node,
+    v the node velocity (n, 3; None = fixed nodes, DphiDt = phi_t) and z the nodes'
+    vertical coordinate."""
+    gp = np.asarray(grad_phi, dtype=np.float64)
+    V = np.asarray(V_inf, dtype=np.float64)
+    tot = V[None, :] + gp
+    p = (0.5 * rho * float(V @ V) - 0.5 * rho * (tot * tot).sum(1)
+         - rho * g * np.asarray(z, dtype=np.float64))
+    conv = 0.0 if node_vel is None else (np.asarray(node_vel, dtype=np.float64) * gp).sum(1)
+    return p - rho * (np.asarray(dphi_dt, dtype=np.float64) - conv)
 
 
 def linearised_pressure(phi_t, dphi_dx, U, rho=RHO):

@@ -214,6 +214,31 @@ class FreeSurfaceStandIn:
         return alpha * v - self.rhs_jvp(t, y, v)
 
     # ---- hull force of the current BEM state
+    def hull_force(self, t, y, dphi_dt, U, eta_t=None):
+        """Pressure force on the hull (FreeSurface::compute_pressure's press_force,
+        source/free_surface.cc:9662-9725): the nodal Bernoulli pressure of
+        forces.bernoulli_pressure with the complete potential gradient projected on the
+        device (wb_compute_potential_gradients), V_inf = (U ramp(t), 0, 0) (the stream whose
+        Neumann data the hull carries), DphiDt = dphi_dt following the nodes and node
+        velocity (0, 0, eta_t) on the moving nodes, integrated over the hull cells at the
+        displaced mesh.  Returns the force (3,)."""
+        from .forces import bernoulli_pressure, surface_forces
+
+        phi, dphi, _ = self.bem_state(t, y)
+        eta, _ = self.split(y)
+        X = self.support(eta)
+        grad = self.p.compute_potential_gradients(phi, dphi)
+        vel = None
+        if eta_t is not None:
+            vel = np.zeros((self.n, 3))
+            vel[self.move_rows, 2] = eta_t[self.move_class]
+        V = np.array([U * smooth_ramp(t, self.t_ramp), 0.0, 0.0])
+        p_nodes = bernoulli_pressure(dphi_dt, grad, X[:, 2], V, node_vel=vel)
+        hull = self.mesh.cell_material == M.HULL
+        force, _, _ = surface_forces(X, self.mesh.cell_dofs[hull], self.mesh.cell_orient[hull],
+                                     p_nodes)
+        return force
+
     def hull_force_x(self, t, y, phi_t, U):
         """x-force of the linearised Bernoulli pressure -rho (phi_t + U ramp(t) dphi/dx)
         over the hull at the displaced mesh (adjoint.hull_drag)."""
@@ -312,6 +337,12 @@ class BDFIntegrator:
     @property
     def y(self):
         return self.hist[-1]
+
+    def restart(self, y, t):
+        """Continues from state y at time t (a restored dump) with an empty history: the
+        next step is of order 1, as IDA's restart from (y, y')."""
+        self.hist = [np.asarray(y, dtype=np.float64).copy()]
+        self.t = float(t)
 
     def yp_of(self, y):
         """y' of the BDF formula for y at t + dt, and alpha = dy'/dy."""
