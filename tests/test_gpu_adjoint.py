@@ -64,7 +64,8 @@ def test_adjoint_fp32_ir_matches_dense(gpu_available):
     rng = np.random.default_rng(12)
     g = rng.standard_normal(m.n_dofs)
     lam_ref = np.linalg.solve(A.T, g)
-    with BEMProblem(m, Config(precision="fp32_ir", solver_tolerance=1e-13),
+    # 1e-12: the refinement's FP64 residual of this system levels off near 1e-13
+    with BEMProblem(m, Config(precision="fp32_ir", solver_tolerance=1e-12),
                     double_nodes=(ptr, idx)) as p:
         p.compute_constraints(bc)
         p.assemble_system()

@@ -16,7 +16,8 @@ GPU path runs inside a real Newton loop with the same call pattern:
 * right-hand side f(t, y) = (w, -g eta), the kinematic and dynamic conditions of linear wave
   theory at zero forward speed, w_c = dphi/dn at the node from the BEM solve on the
   displaced mesh with Dirichlet data phi on the free surface and the Neumann data of the
-  hull ramped in time (``ramp(t) * bc``, a hull starting to move through still water);
+  hull times a factor of time (``forcing(t) * bc``: a ramp, or a ramped oscillation that
+  keeps the waves small against the cells, which linear theory assumes);
 * DAE residual F(t, y, y') = y' - f(t, y) (FreeSurface::residual's role) and Jacobian
   alpha v - f'(t, y) v (FreeSurface::jacobian's role, alpha = dF/dy');
 * f' v in two forms: ``"full"``, the exact tangent, whose free-surface rows come from the
@@ -53,19 +54,23 @@ class FreeSurfaceStandIn:
 
     problem      a BEMProblem built on ``mesh`` with the same double nodes
     double_nodes (ptr, idx) CSR of coincident dofs
-    bc           boundary data at t >= t_ramp: Neumann entries used as given times ramp(t),
-                 Dirichlet entries ignored (they come from phi)
+    bc           boundary data: Neumann entries used as given times forcing(t), Dirichlet
+                 entries ignored (they come from phi)
     jacobian     "full" or "frozen"
+    forcing      time factor of the Neumann data; default the smooth ramp to 1 at t_ramp
     """
 
     def __init__(self, problem, mesh, double_nodes, bc, t_ramp=0.2, g=GRAVITY,
-                 jacobian="full"):
+                 jacobian="full", forcing=None):
         if jacobian not in ("full", "frozen"):
             raise ValueError("jacobian must be 'full' or 'frozen'")
         self.p = problem
         self.mesh = mesh
         self.g = float(g)
         self.t_ramp = float(t_ramp)
+        # time factor of the Neumann data (and of the stream of the hull pressure)
+        self.forcing = forcing if forcing is not None else (
+            lambda t: smooth_ramp(t, self.t_ramp))
         self.jacobian_kind = jacobian
         ptr, idx = double_nodes
         n = mesh.n_dofs
@@ -110,7 +115,7 @@ class FreeSurfaceStandIn:
         return X
 
     def boundary_data(self, t, phi_nodes):
-        tmp = self.bc * smooth_ramp(t, self.t_ramp)
+        tmp = self.bc * self.forcing(t)
         tmp[self.dirichlet] = 0.0
         tmp[self.fs_dofs] = phi_nodes[self.fs_class]
         return tmp
@@ -218,8 +223,8 @@ class FreeSurfaceStandIn:
         """Pressure force on the hull (FreeSurface::compute_pressure's press_force,
         source/free_surface.cc:9662-9725): the nodal Bernoulli pressure of
         forces.bernoulli_pressure with the complete potential gradient projected on the
-        device (wb_compute_potential_gradients), V_inf = (U ramp(t), 0, 0) (the stream whose
-        Neumann data the hull carries), DphiDt = dphi_dt following the nodes and node
+        device (wb_compute_potential_gradients), V_inf = (U forcing(t), 0, 0) (the stream
+        whose Neumann data the hull carries), DphiDt = dphi_dt following the nodes and node
         velocity (0, 0, eta_t) on the moving nodes, integrated over the hull cells at the
         displaced mesh.  Returns the force (3,)."""
         from .forces import bernoulli_pressure, surface_forces
@@ -232,7 +237,7 @@ class FreeSurfaceStandIn:
         if eta_t is not None:
             vel = np.zeros((self.n, 3))
             vel[self.move_rows, 2] = eta_t[self.move_class]
-        V = np.array([U * smooth_ramp(t, self.t_ramp), 0.0, 0.0])
+        V = np.array([U * self.forcing(t), 0.0, 0.0])
         p_nodes = bernoulli_pressure(dphi_dt, grad, X[:, 2], V, node_vel=vel)
         hull = self.mesh.cell_material == M.HULL
         force, _, _ = surface_forces(X, self.mesh.cell_dofs[hull], self.mesh.cell_orient[hull],
@@ -240,7 +245,7 @@ class FreeSurfaceStandIn:
         return force
 
     def hull_force_x(self, t, y, phi_t, U):
-        """x-force of the linearised Bernoulli pressure -rho (phi_t + U ramp(t) dphi/dx)
+        """x-force of the linearised Bernoulli pressure -rho (phi_t + U forcing(t) dphi/dx)
         over the hull at the displaced mesh (adjoint.hull_drag)."""
         import torch
 
@@ -253,7 +258,7 @@ class FreeSurfaceStandIn:
         return float(hull_drag(torch.from_numpy(X), self.mesh.cell_dofs[hull],
                                self.mesh.cell_orient[hull], torch.from_numpy(phi),
                                torch.from_numpy(dphi), torch.from_numpy(phi_t),
-                               U * smooth_ramp(t, self.t_ramp)))
+                               U * self.forcing(t)))
 
 
 def gmres(apply, b, tol, max_iter=40, restart=40):
