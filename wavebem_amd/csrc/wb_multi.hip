@@ -155,17 +155,21 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_f32(
 // the FP32 instruction).  Each right-hand side is scaled by a power of two xs[v] (its
 // largest entry to about 2^13) and split, once per product, into FP16 hi and
 // lo = (x - hi) 2^11 (k_vec_split), contracted in two accumulator chains, so the vectors
-// keep about 22 bits.  Lane (row i, k-slot q) loads
-// the 8 contiguous columns cb + 8q .. cb + 8q + 7 (16 bytes) of each 32-column sub-step;
-// WB_MULTI16_SUB sub-steps are loaded together, so more bytes are in flight per lane.
-// Matrix-core pass m of a sub-step contracts its columns 8q + 4m + e (e < 4) for A and B
-// alike.  Sums move into FP64 every 4 steps; the row scale and 1/xs[v] are applied to the
-// FP64 chunk sums.
+// keep about 22 bits.  The block stages panels of its 128 rows x HP16 columns through LDS:
+// every load instruction of a wavefront reads whole HP16-column row segments (256 bytes at
+// 128 columns), where fragment-shaped loads would read 16 rows x 64 bytes.  Lane (row i,
+// k-slot q) then takes the 8 contiguous columns cb + 8q .. cb + 8q + 7 of each 32-column
+// sub-step from LDS; matrix-core pass m of a sub-step contracts its columns 8q + 4m + e
+// (e < 4) for A and B alike.  Sums move into FP64 every 256 columns; the row scale and
+// 1/xs[v] are applied to the FP64 chunk sums.
 typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
 typedef _Float16 hvec4 __attribute__((ext_vector_type(4)));
-#ifndef WB_MULTI16_SUB
-#define WB_MULTI16_SUB 2
+#ifndef WB_MULTI16_PANEL
+#define WB_MULTI16_PANEL 128
 #endif
+constexpr int HP16 = WB_MULTI16_PANEL;  // columns per staged panel (a multiple of 32)
+constexpr int HU16 = HP16 / 8;          // 16-byte units per panel row
+constexpr int HS16 = HU16 + 1;          // LDS row stride in units
 
 // xs[v] = 2^(13 - e), e the exponent of max |X[:, v]| (1 for a zero vector); one block
 // per vector, fixed reduction order
@@ -225,13 +229,22 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_h16(
     const _Float16 *__restrict__ XS, const float *__restrict__ xs, const int *__restrict__ rows,
     int n_rows, int cols_per_chunk, double *__restrict__ part)
 {
+    // the block's RPB rows x HP columns of A16, staged in LDS with a row stride of HS
+    // 16-byte units (one more than a row holds, so the 16 rows of a fragment read fall in
+    // distinct bank groups)
+    __shared__ hvec8 tile[RPB * HS16];
     const int c0 = blockIdx.y * cols_per_chunk, c1 = min(n, c0 + cols_per_chunk);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int base = (blockIdx.x * MW + wave) * 16 * RT;
-    const int i = lane & 15, q = lane >> 4;  // row in tile, k-slot
-    size_t roff[RT];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int brow = blockIdx.x * RPB;
+    // staging: thread tid moves unit su of rows sr, sr + SR, ...: a wavefront reads
+    // 64 / HU16 whole row segments of HP columns per load instruction
+    constexpr int SR = 64 * MW / HU16;   // rows per staging pass
+    constexpr int SJ = RPB / SR;         // staging passes
+    const int su = tid % HU16, sr = tid / HU16;
+    size_t soff[SJ];
 #pragma unroll
-    for (int t = 0; t < RT; ++t) roff[t] = (size_t)tile_row(base + 16 * t + i, rows, n_rows) * ld;
+    for (int j = 0; j < SJ; ++j) soff[j] = (size_t)tile_row(brow + sr + SR * j, rows, n_rows) * ld;
+    const int i = lane & 15, q = lane >> 4;  // row in tile, k-slot
     const int v = lane & 15;  // right-hand side fed by this lane's B operand
     const bool vok = v < k;
     const _Float16 *Xh = XS + (size_t)(vok ? v : 0) * ld;
@@ -245,10 +258,20 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_h16(
         cd[t][0] = cd[t][1] = cd[t][2] = cd[t][3] = 0.0;
     }
     constexpr float ILO = 1.0f / 2048.0f;
-    constexpr int NS = WB_MULTI16_SUB;  // 32-column sub-steps loaded together
+    constexpr int FLUSH = 256 / HP16 > 0 ? 256 / HP16 : 1;  // panels per FP64 flush
     int since = 0;
-    for (int cb = c0; cb < c1; cb += 32 * NS) {
-        if (++since == 4) {
+    const hvec8 z8 = {0, 0, 0, 0, 0, 0, 0, 0};
+    // A16 is zero past n up to ld; columns of the next chunk read as zero
+    hvec8 st[SJ];
+    auto load_panel = [&](int pc) {
+        const int scol = pc + 8 * su;
+#pragma unroll
+        for (int j = 0; j < SJ; ++j)
+            st[j] = scol < c1 ? __builtin_nontemporal_load((const hvec8 *)(A16 + soff[j] + scol))
+                              : z8;
+    };
+    for (int cb = c0; cb < c1; cb += HP16) {
+        if (++since > FLUSH) {
 #pragma unroll
             for (int t = 0; t < RT; ++t) {
 #pragma unroll
@@ -257,45 +280,43 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_h16(
                 ch[t] = fvec4{0.f, 0.f, 0.f, 0.f};
                 cl[t] = fvec4{0.f, 0.f, 0.f, 0.f};
             }
-            since = 0;
+            since = 1;
         }
-        hvec8 a[NS][RT], bh[NS], bl[NS];
+        load_panel(cb);
+        __syncthreads();  // the previous panel's fragments have been read
 #pragma unroll
-        for (int u = 0; u < NS; ++u) {
-            // this lane's 8 columns of sub-step u (A16 and XS are zero past n up to ld)
+        for (int j = 0; j < SJ; ++j) tile[(sr + SR * j) * HS16 + su] = st[j];
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < HP16 / 32; ++u) {
+            // this lane's 8 columns of sub-step u: cb + 32 u + 8 q .. + 7
             const int cc = cb + 32 * u + 8 * q;
-            bh[u] = hvec8{0, 0, 0, 0, 0, 0, 0, 0};
-            bl[u] = hvec8{0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int t = 0; t < RT; ++t) a[u][t] = hvec8{0, 0, 0, 0, 0, 0, 0, 0};
-            if (cc < c1) {
-#pragma unroll
-                for (int t = 0; t < RT; ++t)
-                    a[u][t] = __builtin_nontemporal_load((const hvec8 *)(A16 + roff[t] + cc));
-                if (vok) {
-                    bh[u] = *(const hvec8 *)(Xh + cc);
-                    bl[u] = *(const hvec8 *)(Xl + cc);
-                }
+            hvec8 bh = z8, bl = z8;
+            if (vok && cc < c1) {
+                bh = *(const hvec8 *)(Xh + cc);
+                bl = *(const hvec8 *)(Xl + cc);
             }
-        }
+            hvec8 a[RT];
 #pragma unroll
-        for (int u = 0; u < NS; ++u)
+            for (int t = 0; t < RT; ++t) a[t] = tile[(wave * 16 * RT + 16 * t + i) * HS16 + 4 * u + q];
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
-                const hvec4 bhm = {bh[u][4 * m], bh[u][4 * m + 1], bh[u][4 * m + 2], bh[u][4 * m + 3]};
-                const hvec4 blm = {bl[u][4 * m], bl[u][4 * m + 1], bl[u][4 * m + 2], bl[u][4 * m + 3]};
+                const hvec4 bhm = {bh[4 * m], bh[4 * m + 1], bh[4 * m + 2], bh[4 * m + 3]};
+                const hvec4 blm = {bl[4 * m], bl[4 * m + 1], bl[4 * m + 2], bl[4 * m + 3]};
 #pragma unroll
                 for (int t = 0; t < RT; ++t) {
-                    const hvec4 am = {a[u][t][4 * m], a[u][t][4 * m + 1], a[u][t][4 * m + 2],
-                                      a[u][t][4 * m + 3]};
+                    const hvec4 am = {a[t][4 * m], a[t][4 * m + 1], a[t][4 * m + 2],
+                                      a[t][4 * m + 3]};
                     ch[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(am, bhm, ch[t], 0, 0, 0);
                     cl[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(am, blm, cl[t], 0, 0, 0);
                 }
             }
+        }
     }
     if (vok) {
         double *pp = part + ((size_t)blockIdx.y * k + v) * n_rows;
         const double isx = 1.0 / (double)xs[v];
+        const int base = brow + wave * 16 * RT;
 #pragma unroll
         for (int t = 0; t < RT; ++t)
 #pragma unroll
@@ -417,6 +438,140 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_d(
     }
 }
 
+// The hi + bf16 lo operator (FP64 vectors) with LDS-staged panels: the block's 128 rows x
+// HL_P columns of the selected hi (FP32) and lo (bf16) entries are staged with whole-row
+// loads (a wavefront reads 4 rows x 256 bytes of hi and 4 rows x 128 bytes of lo per
+// instruction) and contracted on the FP64 matrix cores as in k_gemm_sel_d.  A staging
+// thread owns one 4-column group of the panel: gmask says whether its columns read G or H
+// (both, column by column, for a mixed group); the signs go to the vector operand.
+// LDS row strides of 17 (hi, 16-byte units) and 18 (lo, 8-byte units) put the 16 rows of
+// a fragment read in distinct banks.
+#ifndef WB_MULTI_HL_PANEL
+#define WB_MULTI_HL_PANEL 64
+#endif
+constexpr int HL_P = WB_MULTI_HL_PANEL;  // columns per panel (a multiple of 16)
+constexpr int HL_U = HL_P / 4;      // 4-column groups per panel row
+constexpr int HL_SH = HL_U + 1;     // hi row stride (16-byte units)
+constexpr int HL_SL = HL_U + 2;     // lo row stride (8-byte units)
+
+__global__ __launch_bounds__(64 * MW) void k_gemm_hl_lds(
+    const float *__restrict__ G, const float *__restrict__ H, const uint16_t *__restrict__ Glo,
+    const uint16_t *__restrict__ Hlo, size_t ld, int n, const uint8_t *__restrict__ gmask,
+    int flip, double s_sel, double s_uns, int k, const double *__restrict__ X, size_t ldx,
+    const int *__restrict__ rows, int n_rows, int cols_per_chunk, double *__restrict__ part)
+{
+    __shared__ fvec4 tile_h[RPB * HL_SH];
+    __shared__ uvec4 tile_l[RPB * HL_SL];
+    const int c0 = blockIdx.y * cols_per_chunk, c1 = min(n, c0 + cols_per_chunk);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int brow = blockIdx.x * RPB;
+    constexpr int SR = 64 * MW / HL_U;  // rows per staging pass
+    constexpr int SJ = RPB / SR;        // staging passes
+    const int su = tid % HL_U, sr = tid / HL_U;
+    size_t soff[SJ];
+#pragma unroll
+    for (int j = 0; j < SJ; ++j) soff[j] = (size_t)tile_row(brow + sr + SR * j, rows, n_rows) * ld;
+    const int i = lane & 15, q = lane >> 4;
+    const int v = lane & 15;
+    const bool vok = v < k;
+    const double *Xv = X + (size_t)(vok ? v : 0) * ldx;
+    const unsigned fm = flip ? 0xfu : 0u;
+    dvec4 c[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) c[t] = dvec4{0.0, 0.0, 0.0, 0.0};
+    fvec4 sh[SJ];
+    uvec4 sl[SJ];
+    // stage: group su of the panel at pc for rows sr + SR j
+    auto load_panel = [&](int pc) {
+        const int scol = pc + 4 * su;
+        const fvec4 z4 = {0.f, 0.f, 0.f, 0.f};
+        const uvec4 zl = {0, 0, 0, 0};
+        if (scol < c1) {
+            const unsigned m = (unsigned)gmask[scol >> 2] ^ fm;  // bit e: column e reads H
+            if (m == 0u || m == 0xfu) {
+                const float *M = m ? H : G;
+                const uint16_t *L = m ? Hlo : Glo;
+#pragma unroll
+                for (int j = 0; j < SJ; ++j) {
+                    sh[j] = __builtin_nontemporal_load((const fvec4 *)(M + soff[j] + scol));
+                    sl[j] = __builtin_nontemporal_load((const uvec4 *)(L + soff[j] + scol));
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < SJ; ++j) {
+                    const fvec4 g = *(const fvec4 *)(G + soff[j] + scol);
+                    const fvec4 h = *(const fvec4 *)(H + soff[j] + scol);
+                    const uvec4 gl = *(const uvec4 *)(Glo + soff[j] + scol);
+                    const uvec4 hl = *(const uvec4 *)(Hlo + soff[j] + scol);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const bool he = (m >> e) & 1u;
+                        sh[j][e] = he ? h[e] : g[e];
+                        sl[j][e] = he ? hl[e] : gl[e];
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < SJ; ++j) {
+                sh[j] = z4;
+                sl[j] = zl;
+            }
+        }
+    };
+    for (int cb = c0; cb < c1; cb += HL_P) {
+        load_panel(cb);
+        __syncthreads();  // the previous panel's fragments have been read
+#pragma unroll
+        for (int j = 0; j < SJ; ++j) {
+            tile_h[(sr + SR * j) * HL_SH + su] = sh[j];
+            tile_l[(sr + SR * j) * HL_SL + su] = sl[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < HL_P / 16; ++u) {
+            const int cc = cb + 16 * u + 4 * q;  // this lane's 4 columns
+            double b[4] = {0.0, 0.0, 0.0, 0.0};
+            if (vok && cc < c1) {
+                const unsigned m = (unsigned)gmask[cc >> 2] ^ fm;
+                const dvec2 x0 = *((const dvec2 *)(Xv + cc));
+                const dvec2 x1 = *((const dvec2 *)(Xv + cc) + 1);
+                b[0] = x0.x * ((m & 1u) ? s_sel : s_uns);
+                b[1] = x0.y * ((m & 2u) ? s_sel : s_uns);
+                b[2] = x1.x * ((m & 4u) ? s_sel : s_uns);
+                b[3] = x1.y * ((m & 8u) ? s_sel : s_uns);
+            }
+            double a[RT][4];
+#pragma unroll
+            for (int t = 0; t < RT; ++t) {
+                const int r = wave * 16 * RT + 16 * t + i;
+                const fvec4 h = tile_h[r * HL_SH + 4 * u + q];
+                const uvec4 l = tile_l[r * HL_SL + 4 * u + q];
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    a[t][e] = (double)h[e] + (double)__uint_as_float((uint32_t)l[e] << 16);
+            }
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+                for (int t = 0; t < RT; ++t)
+                    c[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t][s2], b[s2], c[t], 0, 0, 0);
+        }
+    }
+    // FP64 result layout: D[i][j] in lane j + 16 (i % 4), register i / 4
+    if (vok) {
+        double *pp = part + ((size_t)blockIdx.y * k + v) * n_rows;
+        const int base = brow + wave * 16 * RT;
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int s2 = base + 16 * t + q + 4 * r;
+                if (s2 < n_rows) pp[s2] = c[t][r];
+            }
+    }
+}
+
 // Y[v][rows[s]] = sum over column chunks of part[c][v][s] (in order) + diag .* Xd
 template <typename T>
 __global__ void k_multi_reduce(const double *__restrict__ part, int n_chunks, int k,
@@ -450,7 +605,7 @@ int multi_chunks(int mode, int n_rows, int n_groups)
                               : 0;
     if (mode == 3 && cap > 0) {
         const long long blocks = (n_rows + RPB - 1) / RPB;
-        const int steps = (4 * n_groups + 32 * WB_MULTI16_SUB - 1) / (32 * WB_MULTI16_SUB);
+        const int steps = (4 * n_groups + HP16 - 1) / HP16;
         int best = 1;
         double best_fill = 0.0;
         for (int c = 1; c <= 16 && c <= std::max(1, steps); ++c) {
@@ -482,10 +637,9 @@ void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *G
     dim3 grid((n_rows + RPB - 1) / RPB, chunks), block(64 * MW);
     if (mode == 3) {
         // G holds A16 and H the row scales; Glo is the workspace: k vector scales, then the
-        // split vectors (2k x ld halfs from byte 256); chunks of whole
-        // 32 * WB_MULTI16_SUB-column steps
+        // split vectors (2k x ld halfs from byte 256); chunks of whole HP16-column panels
         const int n = 4 * n_groups;
-        constexpr int step = 32 * WB_MULTI16_SUB;
+        constexpr int step = HP16;
         const int cpc = ((n + chunks - 1) / chunks + step - 1) / step * step;
         float *xs = (float *)const_cast<void *>(Glo);
         _Float16 *XS = (_Float16 *)((char *)xs + 256);
@@ -499,10 +653,14 @@ void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *G
         hipLaunchKernelGGL(k_gemm_sel_f32, grid, block, 0, s, (const float *)G, (const float *)H,
                            ld, (const int4 *)segs, n_segs, gmask, flip, (float)s_sel,
                            (float)s_unsel, k, (const float *)X, ldx, rows, n_rows, gpc, part);
-    else if (mode == 2)
-        hipLaunchKernelGGL(k_gemm_sel_d<2>, grid, block, 0, s, G, H, (const uint16_t *)Glo,
-                           (const uint16_t *)Hlo, ld, (const int4 *)segs, n_segs, gmask, flip,
-                           s_sel, s_unsel, k, (const double *)X, ldx, rows, n_rows, gpc, part);
+    else if (mode == 2) {
+        // chunks of whole HL_P-column panels (chunks past the columns write zero sums)
+        const int n = 4 * n_groups;
+        const int cpc = ((n + chunks - 1) / chunks + HL_P - 1) / HL_P * HL_P;
+        hipLaunchKernelGGL(k_gemm_hl_lds, grid, block, 0, s, (const float *)G, (const float *)H,
+                           (const uint16_t *)Glo, (const uint16_t *)Hlo, ld, n, gmask, flip,
+                           s_sel, s_unsel, k, (const double *)X, ldx, rows, n_rows, cpc, part);
+    }
     else
         hipLaunchKernelGGL(k_gemm_sel_d<0>, grid, block, 0, s, G, H, nullptr, nullptr, ld,
                            (const int4 *)segs, n_segs, gmask, flip, s_sel, s_unsel, k,
