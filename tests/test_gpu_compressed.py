@@ -3,7 +3,8 @@ wb_hmat.hip) on the GPU.
 
 * its product against the FP64-accurate operator (hi + lo) of the same context, next to the
   FP32 hi operator and the row-scaled FP16 copy: the compressed copy is a correction-solve
-  operator, so it is held to the FP16 copy's order of accuracy, not to FP64's;
+  operator, so it is held to the FP16 copy's order of accuracy (a bar of 1e-4, five times
+  its measured error), not to FP64's;
 * solutions: the refinement residual always uses the FP64-accurate operator, so the
   solutions meet the same bars as every FP32-IR solve: 1e-8 against dense LU of the
   oracle's system (2.5k dofs) and against the FP64 solve (10k dofs).
@@ -26,7 +27,9 @@ def test_inner_operator_error(gpu_available, kind):
     bc, _ = M.wigley_bc(m, seed=6)
     x = np.random.default_rng(6).standard_normal(m.n_dofs)
     errs = {}
-    for fmt, bar in ((0, 1e-6), (1, 2e-3), (2, 3e-3)):
+    # measured: 3.5e-8 (FP32 hi), 1.2e-5 - 1.4e-5 (FP16 copy), 1.7e-5 - 1.9e-5 (compressed,
+    # rank threshold 1e-4 of each block's largest entry)
+    for fmt, bar in ((0, 2e-7), (1, 5e-5), (2, 1e-4)):
         with BEMProblem(m, Config(precision="fp32_ir", ir_inner_format=fmt),
                         double_nodes=(ptr, idx)) as p:
             p.compute_constraints(bc)
