@@ -190,14 +190,15 @@ def test_wb_solve_matches_assemble_and_solve(gpu_available):
     assert st["iterations"] > 0
 
 
-@pytest.mark.parametrize("precision,inner,bar", [("fp64", 1, 1e-12), ("fp32_ir", 0, 1e-6),
-                                                 ("fp32_ir", 1, 3e-3), ("fp32_ir", 2, 1e-5)])
+@pytest.mark.parametrize("precision,inner,bar", [("fp64", 1, 1e-12), ("fp32_ir", 0, 1e-8),
+                                                 ("fp32_ir", 1, 5e-5), ("fp32_ir", 2, 1e-7)])
 def test_coarse_operator_is_PtAP(gpu_available, precision, inner, bar):
     """The coarse operator of the preconditioner against P^T A P formed from the oracle's
     matrices: A P from the FP64 matrices (FP64), from the FP32 hi part (FP32-IR, FP32 inner
     operator), from the aggregate sums the FP16 copy's build writes (FP32-IR, FP16 inner
     operator; FP16 rounding of the entries, hence the looser bar) and from the FP32 sums the
-    compressed operator's sketch pass writes (FP32-IR, compressed inner operator).  A wrong aggregate map or
+    compressed operator's sketch pass writes (FP32-IR, compressed inner operator).  Measured:
+    9e-15, 5e-10, 7e-6 and 2e-9.  A wrong aggregate map or
     reduction would only cost iterations in a solve; here it fails."""
     from wavebem_amd import bem as B
 
