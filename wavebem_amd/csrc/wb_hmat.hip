@@ -28,6 +28,7 @@ namespace wb {
 namespace {
 
 typedef _Float16 hvec4 __attribute__((ext_vector_type(4)));
+typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 typedef double dvec4 __attribute__((ext_vector_type(4)));
 
@@ -63,16 +64,6 @@ __device__ __forceinline__ float wave_reduce63(float v)
 __device__ __forceinline__ float lane63(float v)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-}
-
-// The vmult operator's FP32 hi entry (row offset off, device column col): H for a Neumann
-// column (gmask bit set), -G for a Dirichlet column.
-__device__ __forceinline__ float sel_entry(const float *__restrict__ G,
-                                           const float *__restrict__ H,
-                                           const uint8_t *__restrict__ gmask, size_t off, int col)
-{
-    const bool h = (gmask[col >> 2] >> (col & 3)) & 1u;
-    return h ? H[off + col] : -G[off + col];
 }
 
 // sign of sketch vector k at cluster-local position p: bit k of a 32-bit integer hash of p
@@ -363,7 +354,7 @@ __global__ __launch_bounds__(64) void k_hm_qr(const float *__restrict__ Y,
     }
 }
 
-// One wavefront per block: dense blocks -> D16 (rows of the block, |J| columns, scaled by
+// One wavefront per block: dense blocks -> D16 (column-major, 256 x 256, scaled by
 // 1 / fscale); low-rank blocks -> B16 = (Q16^T A_IJ) / fscale, fscale replaced by a power
 // of two >= max |B|.  Lane l holds the block columns 4 l .. 4 l + 3 (their matrix and sign
 // chosen once) and the block rows l, l + 64, ... (read back with readlane), so the loads
@@ -427,23 +418,27 @@ __global__ __launch_bounds__(64) void k_hm_fill(const float *__restrict__ G,
         return (size_t)(((uint64_t)hi << 32) | lo);
     };
     if (r == HM_DENSE) {
+        // column-major: D16[b][col][row]; lane l writes its columns 4 l .. 4 l + 3 (zero
+        // past |J|) for 8 rows at a time; rows past |I| are zero as well
         const float inv = 1.0f / fscale[b];
+        _Float16 *Db = D16 + (size_t)b * HM_CL * HM_CL;
 #pragma unroll
         for (int uq = 0; uq < HM_CL / 64; ++uq)
-            for (int j0 = 0; j0 < 64 && 64 * uq + j0 < szI; j0 += 8) {
+            for (int j0 = 0; j0 < 64; j0 += 8) {
                 float v[8][4];
+                if (64 * uq + j0 < szI) {
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    load4(row_off_u(uq, min(j0 + u, szI - 1 - 64 * uq)), v[u]);
+                    for (int u = 0; u < 8; ++u)
+                        load4(row_off_u(uq, min(j0 + u, szI - 1 - 64 * uq)), v[u]);
+                }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int i = 64 * uq + j0 + u;
-                    if (i < szI) {
-                        hvec4 h;
+                for (int c = 0; c < 4; ++c) {
+                    hvec8 h;
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) h[c] = (_Float16)(v[u][c] * inv);
-                        *(hvec4 *)(D16 + ((size_t)b * HM_CL + i) * HM_CL + 4 * lane) = h;
-                    }
+                    for (int u = 0; u < 8; ++u)
+                        h[u] = 64 * uq + j0 + u < szI ? (_Float16)(v[u][c] * inv)
+                                                      : (_Float16)0.0f;
+                    *(hvec8 *)(Db + (size_t)(4 * lane + c) * HM_CL + 64 * uq + j0) = h;
                 }
             }
         return;
@@ -598,7 +593,8 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
         }
         yacc += s2[0] + s2[1];
     }
-    // dense blocks, row by row: lane l holds x_J at the block columns 4 l .. 4 l + 3
+    // dense blocks, column by column (column-major storage): lane l holds the block rows
+    // 4 l .. 4 l + 3 and, for the broadcast, x_J at the block columns 4 l .. 4 l + 3
     for (int J = ch; J < ncl; J += chunks) {
         const size_t b = (size_t)I * ncl + J;
         const int r = __builtin_amdgcn_readfirstlane(rank[b]);
@@ -612,21 +608,23 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
             xv[c] = p < szJ ? x[cl_idx[(size_t)J * HM_CL + p]] : 0.0f;
         }
         const _Float16 *D = D16 + b * HM_CL * HM_CL + 4 * lane;
-        for (int i0 = 0; i0 < szI; i0 += 4) {
-            float t[4];
+        fvec4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0;
+        for (int c0 = 0; c0 < szJ; c0 += 4) {
+            // columns c0 .. c0 + 3 (zero past |J|), their x from lane c0 / 4
+            const int src = c0 >> 2;
+            hvec4 h[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const hvec4 h = *(const hvec4 *)(D + (size_t)min(i0 + u, szI - 1) * HM_CL);
-                t[u] = (float)h[0] * xv[0] + (float)h[1] * xv[1] + (float)h[2] * xv[2] +
-                       (float)h[3] * xv[3];
+            for (int c = 0; c < 4; ++c) h[c] = *(const hvec4 *)(D + (size_t)(c0 + c) * HM_CL);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float xc = __int_as_float(
+                    __builtin_amdgcn_readlane(__float_as_int(xv[c]), src));
+                fvec4 &a = (c & 1) ? a1 : a0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) a[e] = fmaf((float)h[c][e], xc, a[e]);
             }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) t[u] = lane63(wave_reduce63<false>(t[u]));
-            // rows i0 .. i0 + 3 belong to lane i0 / 4
-            if (lane == (i0 >> 2))
-#pragma unroll
-                for (int u = 0; u < 4; ++u) yacc[u] = fmaf(t[u], fs, yacc[u]);
         }
+        yacc += (a0 + a1) * fs;
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -635,18 +633,27 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
     }
 }
 
-// y[i] = sum over chunks of ypart[c][i] (in chunk order) + diag[i] xd[i]
+// y[i] = sum over chunks of ypart[c][i] + diag[i] xd[i]: 64 rows per block, the chunks
+// split into four contiguous quarters summed by four threads of a row (in chunk order),
+// the quarters then added in order, so the sum is the same on every call
 __global__ __launch_bounds__(256) void k_hm_finish(const float *__restrict__ ypart, size_t ldp,
                                                    int chunks, int n,
                                                    const double *__restrict__ diag,
                                                    const float *__restrict__ xd,
                                                    float *__restrict__ y)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    __shared__ double qs[4][64];
+    const int rl = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + rl;
+    const int c0 = (int)((long)q * chunks / 4), c1 = (int)((long)(q + 1) * chunks / 4);
     double s = 0.0;
-    for (int c = 0; c < chunks; ++c) s += (double)ypart[(size_t)c * ldp + i];
-    y[i] = (float)(s + diag[i] * (double)xd[i]);
+    if (i < n)
+        for (int c = c0; c < c1; ++c) s += (double)ypart[(size_t)c * ldp + i];
+    qs[q][rl] = s;
+    __syncthreads();
+    if (q == 0 && i < n)
+        y[i] = (float)((((qs[0][rl] + qs[1][rl]) + qs[2][rl]) + qs[3][rl]) +
+                       diag[i] * (double)xd[i]);
 }
 
 // S[a][r] += the other sketch parts' sums (parts 1 .. nparts - 1, in order)
@@ -702,7 +709,7 @@ void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_
     hipLaunchKernelGGL(k_hm_gemv, dim3(ncl, ch), dim3(64), 0, s, cl_idx, cl_size, ncl, ch,
                        rank, fscale, (const _Float16 *)Q16, (const _Float16 *)D16, T, x, ldp,
                        ypart);
-    hipLaunchKernelGGL(k_hm_finish, dim3((n + 255) / 256), dim3(256), 0, s, ypart, ldp, ch, n,
+    hipLaunchKernelGGL(k_hm_finish, dim3((n + 63) / 64), dim3(256), 0, s, ypart, ldp, ch, n,
                        diag, xd, y);
 }
 
