@@ -265,7 +265,10 @@ __global__ __launch_bounds__(256) void k_coarse_restrict(int n, int agg, const T
     if (threadIdx.x == 0) C[(size_t)v * CS_MAX + a] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
-// t_v = x_v - W C[v] (W stored [b][i]); vector v = blockIdx.y
+// t_v = x_v - W C[v] (W stored [b][i]); vector v = blockIdx.y.  64 rows per block: the
+// aggregates split into four contiguous quarters, each summed by one thread of the row
+// with two alternating accumulators, the parts then added in a fixed order, so the chains
+// are short and the sum is the same on every call.
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse_residual(int n, int nc, const T *__restrict__ X,
                                                          size_t ldx, const T *__restrict__ WT,
@@ -273,14 +276,27 @@ __global__ __launch_bounds__(256) void k_coarse_residual(int n, int nc, const T 
                                                          T *__restrict__ Tout)
 {
     __shared__ double cs[CS_MAX];
+    __shared__ double qs[4][64];
     const int v = blockIdx.y;
     for (int b = threadIdx.x; b < nc; b += blockDim.x) cs[b] = C[(size_t)v * CS_MAX + b];
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double t = (double)X[(size_t)v * ldx + i];
-    for (int b = 0; b < nc; ++b) t -= (double)WT[(size_t)b * ldw + i] * cs[b];
-    Tout[(size_t)v * ldx + i] = (T)t;
+    const int rl = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + rl;
+    const int b0 = q * nc / 4, b1 = (q + 1) * nc / 4;
+    double a0 = 0.0, a1 = 0.0;
+    if (i < n) {
+        int b = b0;
+        for (; b + 1 < b1; b += 2) {
+            a0 = fma((double)WT[(size_t)b * ldw + i], cs[b], a0);
+            a1 = fma((double)WT[(size_t)(b + 1) * ldw + i], cs[b + 1], a1);
+        }
+        if (b < b1) a0 = fma((double)WT[(size_t)b * ldw + i], cs[b], a0);
+    }
+    qs[q][rl] = a0 + a1;
+    __syncthreads();
+    if (q == 0 && i < n)
+        Tout[(size_t)v * ldx + i] =
+            (T)((double)X[(size_t)v * ldx + i] - (((qs[0][rl] + qs[1][rl]) + qs[2][rl]) + qs[3][rl]));
 }
 
 }  // namespace
@@ -332,7 +348,7 @@ void launch_coarse_build_t(int n, int agg, int nc, const double *G, const double
 void launch_coarse_pre(int is_float, int n, int agg, int nc, const void *X, size_t ldx,
                        const void *WT, size_t ldw, double *C, void *Tout, int k, hipStream_t s)
 {
-    const dim3 cg(nc, k), rg((n + 255) / 256, k);
+    const dim3 cg(nc, k), rg((n + 63) / 64, k);
     if (is_float) {
         hipLaunchKernelGGL(k_coarse_restrict<float>, cg, dim3(256), 0, s, n, agg,
                            (const float *)X, ldx, C);

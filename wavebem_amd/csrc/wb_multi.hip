@@ -159,8 +159,9 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_f32(
 // every load instruction of a wavefront reads whole HP16-column row segments (256 bytes at
 // 128 columns), where fragment-shaped loads would read 16 rows x 64 bytes.  Lane (row i,
 // k-slot q) then takes the 8 contiguous columns cb + 8q .. cb + 8q + 7 of each 32-column
-// sub-step from LDS; matrix-core pass m of a sub-step contracts its columns 8q + 4m + e
-// (e < 4) for A and B alike.  Sums move into FP64 every 256 columns; the row scale and
+// sub-step from LDS, and the vector halves from the panel's copy in LDS (loaded once per
+// block, not once per wavefront: 1.35 -> 1.21 ms at 61k, k = 8); matrix-core pass m of a
+// sub-step contracts its columns 8q + 4m + e (e < 4) for A and B alike.  Sums move into FP64 every 256 columns; the row scale and
 // 1/xs[v] are applied to the FP64 chunk sums.
 typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
 typedef _Float16 hvec4 __attribute__((ext_vector_type(4)));
@@ -233,6 +234,9 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_h16(
     // 16-byte units (one more than a row holds, so the 16 rows of a fragment read fall in
     // distinct bank groups)
     __shared__ hvec8 tile[RPB * HS16];
+    // the panel's columns of the split vectors (hi rows 0 .. k - 1, lo rows k .. 2k - 1),
+    // loaded once per block instead of once per wavefront
+    __shared__ hvec8 xtile[2 * WB_MULTI_MAX * HS16];
     const int c0 = blockIdx.y * cols_per_chunk, c1 = min(n, c0 + cols_per_chunk);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int brow = blockIdx.x * RPB;
@@ -247,8 +251,6 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_h16(
     const int i = lane & 15, q = lane >> 4;  // row in tile, k-slot
     const int v = lane & 15;  // right-hand side fed by this lane's B operand
     const bool vok = v < k;
-    const _Float16 *Xh = XS + (size_t)(vok ? v : 0) * ld;
-    const _Float16 *Xl = XS + (size_t)(vok ? k + v : 0) * ld;
     fvec4 ch[RT], cl[RT];
     double cd[RT][4];
 #pragma unroll
@@ -283,18 +285,42 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_h16(
             since = 1;
         }
         load_panel(cb);
+        hvec8 sx[2 * WB_MULTI_MAX * HU16 / (64 * MW) > 0 ? 2 * WB_MULTI_MAX * HU16 / (64 * MW) : 1];
+        {
+            // unit t of the 2k x HU16 vector panel: vector t / HU16, unit t % HU16
+            constexpr int XJ = 2 * WB_MULTI_MAX * HU16 / (64 * MW) > 0
+                                   ? 2 * WB_MULTI_MAX * HU16 / (64 * MW)
+                                   : 1;
+#pragma unroll
+            for (int j = 0; j < XJ; ++j) {
+                const int t = tid + 64 * MW * j, vv = t / HU16, uu = t % HU16;
+                const int col = cb + 8 * uu;
+                sx[j] = (vv < 2 * k && col < c1) ? *(const hvec8 *)(XS + (size_t)vv * ld + col)
+                                                 : z8;
+            }
+        }
         __syncthreads();  // the previous panel's fragments have been read
 #pragma unroll
         for (int j = 0; j < SJ; ++j) tile[(sr + SR * j) * HS16 + su] = st[j];
+        {
+            constexpr int XJ = 2 * WB_MULTI_MAX * HU16 / (64 * MW) > 0
+                                   ? 2 * WB_MULTI_MAX * HU16 / (64 * MW)
+                                   : 1;
+#pragma unroll
+            for (int j = 0; j < XJ; ++j) {
+                const int t = tid + 64 * MW * j;
+                if (t < 2 * WB_MULTI_MAX * HU16) xtile[(t / HU16) * HS16 + t % HU16] = sx[j];
+            }
+        }
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < HP16 / 32; ++u) {
-            // this lane's 8 columns of sub-step u: cb + 32 u + 8 q .. + 7
-            const int cc = cb + 32 * u + 8 * q;
+            // this lane's 8 columns of sub-step u: cb + 32 u + 8 q .. + 7 (zero past the
+            // chunk in both operands)
             hvec8 bh = z8, bl = z8;
-            if (vok && cc < c1) {
-                bh = *(const hvec8 *)(Xh + cc);
-                bl = *(const hvec8 *)(Xl + cc);
+            if (vok) {
+                bh = xtile[v * HS16 + 4 * u + q];
+                bl = xtile[(k + v) * HS16 + 4 * u + q];
             }
             hvec8 a[RT];
 #pragma unroll
@@ -440,14 +466,16 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_d(
 
 // The hi + bf16 lo operator (FP64 vectors) with LDS-staged panels: the block's 128 rows x
 // HL_P columns of the selected hi (FP32) and lo (bf16) entries are staged with whole-row
-// loads (a wavefront reads 4 rows x 256 bytes of hi and 4 rows x 128 bytes of lo per
-// instruction) and contracted on the FP64 matrix cores as in k_gemm_sel_d.  A staging
+// loads (at 32 columns a wavefront reads 8 rows x 128 bytes of hi and 8 rows x 64 bytes of
+// lo per instruction), the panel's vector entries once per block with their signs, and
+// contracted on the FP64 matrix cores as in k_gemm_sel_d.  A staging
 // thread owns one 4-column group of the panel: gmask says whether its columns read G or H
 // (both, column by column, for a mixed group); the signs go to the vector operand.
-// LDS row strides of 17 (hi, 16-byte units) and 18 (lo, 8-byte units) put the 16 rows of
-// a fragment read in distinct banks.
+// LDS row strides of HL_U + 1 (hi, 16-byte units) and HL_U + 2 (lo, 8-byte units) put the
+// 16 rows of a fragment read in distinct banks.  (64-column panels without the vector staging:
+// 5.1-5.9 ms at 61k, k = 8; 32 columns with it: 5.35 ms on the box that measured 5.9.)
 #ifndef WB_MULTI_HL_PANEL
-#define WB_MULTI_HL_PANEL 64
+#define WB_MULTI_HL_PANEL 32
 #endif
 constexpr int HL_P = WB_MULTI_HL_PANEL;  // columns per panel (a multiple of 16)
 constexpr int HL_U = HL_P / 4;      // 4-column groups per panel row
@@ -462,6 +490,8 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_hl_lds(
 {
     __shared__ fvec4 tile_h[RPB * HL_SH];
     __shared__ uvec4 tile_l[RPB * HL_SL];
+    // the panel's vector entries, signs applied: [v][column] (loaded once per block)
+    __shared__ double xtile[WB_MULTI_MAX * (HL_P + 1)];
     const int c0 = blockIdx.y * cols_per_chunk, c1 = min(n, c0 + cols_per_chunk);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int brow = blockIdx.x * RPB;
@@ -474,7 +504,6 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_hl_lds(
     const int i = lane & 15, q = lane >> 4;
     const int v = lane & 15;
     const bool vok = v < k;
-    const double *Xv = X + (size_t)(vok ? v : 0) * ldx;
     const unsigned fm = flip ? 0xfu : 0u;
     dvec4 c[RT];
 #pragma unroll
@@ -521,25 +550,37 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_hl_lds(
     };
     for (int cb = c0; cb < c1; cb += HL_P) {
         load_panel(cb);
+        constexpr int XE = (WB_MULTI_MAX * HL_P + 64 * MW - 1) / (64 * MW);
+        double sx[XE];
+#pragma unroll
+        for (int j = 0; j < XE; ++j) {
+            const int t = tid + 64 * MW * j, vv = t / HL_P, cc = cb + t % HL_P;
+            double xv = 0.0;
+            if (vv < k && cc < c1) {
+                const unsigned m = (unsigned)gmask[cc >> 2] ^ fm;
+                xv = X[(size_t)vv * ldx + cc] * (((m >> (cc & 3)) & 1u) ? s_sel : s_uns);
+            }
+            sx[j] = xv;
+        }
         __syncthreads();  // the previous panel's fragments have been read
 #pragma unroll
         for (int j = 0; j < SJ; ++j) {
             tile_h[(sr + SR * j) * HL_SH + su] = sh[j];
             tile_l[(sr + SR * j) * HL_SL + su] = sl[j];
         }
+#pragma unroll
+        for (int j = 0; j < XE; ++j) {
+            const int t = tid + 64 * MW * j;
+            if (t < WB_MULTI_MAX * HL_P) xtile[(t / HL_P) * (HL_P + 1) + t % HL_P] = sx[j];
+        }
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < HL_P / 16; ++u) {
-            const int cc = cb + 16 * u + 4 * q;  // this lane's 4 columns
+            // this lane's 4 columns cb + 16 u + 4 q .. + 3, signs applied (zero past the chunk)
             double b[4] = {0.0, 0.0, 0.0, 0.0};
-            if (vok && cc < c1) {
-                const unsigned m = (unsigned)gmask[cc >> 2] ^ fm;
-                const dvec2 x0 = *((const dvec2 *)(Xv + cc));
-                const dvec2 x1 = *((const dvec2 *)(Xv + cc) + 1);
-                b[0] = x0.x * ((m & 1u) ? s_sel : s_uns);
-                b[1] = x0.y * ((m & 2u) ? s_sel : s_uns);
-                b[2] = x1.x * ((m & 4u) ? s_sel : s_uns);
-                b[3] = x1.y * ((m & 8u) ? s_sel : s_uns);
+            if (vok) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) b[e] = xtile[v * (HL_P + 1) + 16 * u + 4 * q + e];
             }
             double a[RT][4];
 #pragma unroll
