@@ -127,7 +127,7 @@ struct wb_ctx {
     // factors
     HmatPlan hmp;
     DevBuf hm_idx, hm_size, hm_seg, hm_near, hm_rank, hm_scale, hm_Y, hm_Mx, hm_Q, hm_B, hm_D,
-        hm_T, hm_part;
+        hm_doff, hm_T, hm_part;
     long hm_assembly_ver = -1, hm_mask_ver = -1;
     int hm_agg = 0;         // aggregate size of the coarse sums written by the build (0: none)
     int float_mode = 1;  // operator mode of the FP32-vector solves: 1 = FP32 hi, 3 = FP16,
@@ -180,7 +180,8 @@ struct wb_ctx {
             &vjp_cstate, &vjp_p1, &vjp_p2, &vjp_xg, &vjp_vbar, &vjp_xs, &vjp_spart, &inc_ptr,
             &inc_item, &pj_emass, &pj_erhs, &pj_u, &pj_psi, &pj_B, &pj_X, &pj_R, &pj_Z, &pj_P,
             &pj_Q, &pj_dinv, &pj_part, &pj_part2, &pj_state, &A16, &rs16, &hm_idx, &hm_size, &hm_seg,
-            &hm_near, &hm_rank, &hm_scale, &hm_Y, &hm_Mx, &hm_Q, &hm_B, &hm_D, &hm_T, &hm_part};
+            &hm_near, &hm_rank, &hm_scale, &hm_Y, &hm_Mx, &hm_Q, &hm_B, &hm_D, &hm_doff, &hm_T,
+            &hm_part};
         for (DevBuf *b : mw.all()) v.push_back(b);
         return v;
     }
@@ -379,7 +380,8 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
         KernelScope ks(c, WB_K_GEMVC, c->n);
         launch_hm_gemv(c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hmp.ncl,
                        c->hm_rank.as<uint8_t>(), c->hm_scale.as<float>(), c->hm_Q.p, c->hm_B.p,
-                       c->hm_D.p, c->n, (const float *)x, diag, (const float *)xd,
+                       c->hm_D.p, c->hm_doff.as<int>(), c->n, (const float *)x, diag,
+                       (const float *)xd,
                        c->hm_T.as<float>(), c->hm_part.as<float>(), c->vlen, (float *)y,
                        c->stream);
         return;
@@ -786,18 +788,32 @@ static void ensure_hm(wb_ctx *c)
     c->hm_Mx.ensure(sizeof(float) * (size_t)c->n * ncl);
     c->hm_Q.ensure(2 * nb * HM_RMAX * HM_CL);
     c->hm_B.ensure(2 * nb * HM_RMAX * HM_CL);
-    c->hm_D.ensure(2 * nb * HM_CL * HM_CL);
     c->hm_T.ensure(sizeof(float) * nb * HM_RMAX);
     c->hm_part.ensure(sizeof(float) * c->vlen * hm_chunks(c->hmp.ncl));
+    c->hm_doff.ensure(sizeof(int) * nb);
     if (agg > 0) c->cs_S.ensure(sizeof(double) * HM_SKETCH_PARTS * (size_t)nc * c->vlen);
     {
         KernelScope ks(c, WB_K_INNER, c->n);
-        launch_hm_build(c->G.as<float>(), c->H.as<float>(), c->ld, c->n, c->gmask.as<uint8_t>(),
-                        c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hm_seg.as<int>(),
-                        c->hmp.ncl, c->hm_near.as<uint8_t>(), (float)c->cfg.ir_inner_lr_tol,
-                        c->hm_Y.as<float>(), c->hm_Mx.as<float>(), c->hm_rank.as<uint8_t>(),
-                        c->hm_scale.as<float>(), c->hm_Q.p, c->hm_B.p, c->hm_D.p, agg, nc,
-                        agg > 0 ? c->cs_S.as<double>() : nullptr, c->vlen, c->stream);
+        launch_hm_rank(c->G.as<float>(), c->H.as<float>(), c->ld, c->n, c->gmask.as<uint8_t>(),
+                       c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hm_seg.as<int>(),
+                       c->hmp.ncl, c->hm_near.as<uint8_t>(), (float)c->cfg.ir_inner_lr_tol,
+                       c->hm_Y.as<float>(), c->hm_Mx.as<float>(), c->hm_rank.as<uint8_t>(),
+                       c->hm_scale.as<float>(), c->hm_Q.p, agg, nc,
+                       agg > 0 ? c->cs_S.as<double>() : nullptr, c->vlen, c->stream);
+        HIPCHK(hipGetLastError());
+        // storage for the dense blocks only: the ranks decide which blocks are dense
+        std::vector<uint8_t> rk(nb);
+        HIPCHK(hipMemcpyAsync(rk.data(), c->hm_rank.p, nb, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<int> doff(nb);
+        int nd = 0;
+        for (size_t b = 0; b < nb; ++b) doff[b] = rk[b] == HM_DENSE ? nd++ : -1;
+        upload(c->hm_doff, doff.data(), nb, c->stream);
+        c->hm_D.ensure(2 * (size_t)std::max(nd, 1) * HM_CL * HM_CL);
+        launch_hm_fill(c->G.as<float>(), c->H.as<float>(), c->ld, c->gmask.as<uint8_t>(),
+                       c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hmp.ncl,
+                       c->hm_rank.as<uint8_t>(), c->hm_scale.as<float>(), c->hm_Q.p, c->hm_B.p,
+                       c->hm_D.p, c->hm_doff.as<int>(), c->stream);
         HIPCHK(hipGetLastError());
     }
     c->hm_assembly_ver = c->assembly_ver;

@@ -368,7 +368,8 @@ __global__ __launch_bounds__(64) void k_hm_fill(const float *__restrict__ G,
                                                 float *__restrict__ fscale,
                                                 const _Float16 *__restrict__ Q16,
                                                 _Float16 *__restrict__ B16,
-                                                _Float16 *__restrict__ D16)
+                                                _Float16 *__restrict__ D16,
+                                                const int *__restrict__ doff)
 {
     __shared__ float Qs[HM_RMAX][HM_CL];
     const int b = blockIdx.x, lane = threadIdx.x;
@@ -421,7 +422,7 @@ __global__ __launch_bounds__(64) void k_hm_fill(const float *__restrict__ G,
         // column-major: D16[b][col][row]; lane l writes its columns 4 l .. 4 l + 3 (zero
         // past |J|) for 8 rows at a time; rows past |I| are zero as well
         const float inv = 1.0f / fscale[b];
-        _Float16 *Db = D16 + (size_t)b * HM_CL * HM_CL;
+        _Float16 *Db = D16 + (size_t)doff[b] * HM_CL * HM_CL;
 #pragma unroll
         for (int uq = 0; uq < HM_CL / 64; ++uq)
             for (int j0 = 0; j0 < 64; j0 += 8) {
@@ -554,6 +555,7 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
                                                 const float *__restrict__ fscale,
                                                 const _Float16 *__restrict__ Q16,
                                                 const _Float16 *__restrict__ D16,
+                                                const int *__restrict__ doff,
                                                 const float *__restrict__ T,
                                                 const float *__restrict__ x, size_t ldp,
                                                 float *__restrict__ ypart)
@@ -607,7 +609,7 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
             const int p = 4 * lane + c;
             xv[c] = p < szJ ? x[cl_idx[(size_t)J * HM_CL + p]] : 0.0f;
         }
-        const _Float16 *D = D16 + b * HM_CL * HM_CL + 4 * lane;
+        const _Float16 *D = D16 + (size_t)doff[b] * HM_CL * HM_CL + 4 * lane;
         fvec4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0;
         for (int c0 = 0; c0 < szJ; c0 += 4) {
             // columns c0 .. c0 + 3 (zero past |J|), their x from lane c0 / 4
@@ -679,11 +681,11 @@ int hm_chunks(int ncl)
 
 static int hm_tchunks(int ncl) { return hm_chunks(ncl); }
 
-void launch_hm_build(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
-                     const int *cl_idx, const int *cl_size, const int *cl_seg, int ncl,
-                     const uint8_t *near, float tol, float *Y, float *Mx, uint8_t *rank,
-                     float *fscale, void *Q16, void *B16, void *D16, int agg, int nagg,
-                     double *S, size_t ldS, hipStream_t s)
+void launch_hm_rank(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
+                    const int *cl_idx, const int *cl_size, const int *cl_seg, int ncl,
+                    const uint8_t *near, float tol, float *Y, float *Mx, uint8_t *rank,
+                    float *fscale, void *Q16, int agg, int nagg, double *S, size_t ldS,
+                    hipStream_t s)
 {
     hipLaunchKernelGGL(k_hm_sketch, dim3((n + 15) / 16, HM_SKETCH_PARTS), dim3(64), 0, s, G, H,
                        ld, n, gmask, (const int4 *)cl_seg, ncl, Y, Mx, agg, nagg, S, ldS);
@@ -693,22 +695,31 @@ void launch_hm_build(const float *G, const float *H, size_t ld, int n, const uin
     const unsigned nb = (unsigned)ncl * (unsigned)ncl;
     hipLaunchKernelGGL(k_hm_qr, dim3(nb), dim3(64), 0, s, Y, Mx, n, cl_idx, cl_size, ncl, near,
                        tol, rank, fscale, (_Float16 *)Q16);
+}
+
+void launch_hm_fill(const float *G, const float *H, size_t ld, const uint8_t *gmask,
+                    const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
+                    float *fscale, const void *Q16, void *B16, void *D16, const int *doff,
+                    hipStream_t s)
+{
+    const unsigned nb = (unsigned)ncl * (unsigned)ncl;
     hipLaunchKernelGGL(k_hm_fill, dim3(nb), dim3(64), 0, s, G, H, ld, gmask, cl_idx, cl_size,
                        ncl, rank, fscale, (const _Float16 *)Q16, (_Float16 *)B16,
-                       (_Float16 *)D16);
+                       (_Float16 *)D16, doff);
 }
 
 void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
                     const float *fscale, const void *Q16, const void *B16, const void *D16,
-                    int n, const float *x, const double *diag, const float *xd, float *T,
-                    float *ypart, size_t ldp, float *y, hipStream_t s)
+                    const int *doff, int n, const float *x, const double *diag,
+                    const float *xd, float *T, float *ypart, size_t ldp, float *y,
+                    hipStream_t s)
 {
     const int ch = hm_chunks(ncl), tch = hm_tchunks(ncl);
     hipLaunchKernelGGL(k_hm_gemv_t, dim3(ncl, tch), dim3(64), 0, s, cl_idx, cl_size, ncl, tch,
                        rank, (const _Float16 *)B16, x, T);
     hipLaunchKernelGGL(k_hm_gemv, dim3(ncl, ch), dim3(64), 0, s, cl_idx, cl_size, ncl, ch,
-                       rank, fscale, (const _Float16 *)Q16, (const _Float16 *)D16, T, x, ldp,
-                       ypart);
+                       rank, fscale, (const _Float16 *)Q16, (const _Float16 *)D16, doff, T, x,
+                       ldp, ypart);
     hipLaunchKernelGGL(k_hm_finish, dim3((n + 63) / 64), dim3(256), 0, s, ypart, ldp, ch, n,
                        diag, xd, y);
 }

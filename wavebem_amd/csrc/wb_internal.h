@@ -115,15 +115,24 @@ void plan_hmat(const MeshPlan &P, const double *X_internal, HmatPlan &H);
 // over the columns of aggregate a (a < nagg), the coarse space's A P.  Product: y = A~ x + diag .* xd (FP32 vectors, all n rows); T
 // (ncl^2 x HM_RMAX) and ypart (hm_chunks(ncl) x ldp) floats are work space.
 int hm_chunks(int ncl);
-void launch_hm_build(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
-                     const int *cl_idx, const int *cl_size, const int *cl_seg, int ncl,
-                     const uint8_t *near, float tol, float *Y, float *Mx, uint8_t *rank,
-                     float *fscale, void *Q16, void *B16, void *D16, int agg, int nagg,
-                     double *S, size_t ldS, hipStream_t s);
+// build, in two launches with the host between them: launch_hm_rank (sketch, aggregate
+// sums, rank test, Q) writes rank[]; the caller lays out the dense blocks (doff[b]: the
+// block's slot in D16, HM_CL x HM_CL halfs each; -1 if not dense) and launch_hm_fill
+// writes B and the dense blocks
+void launch_hm_rank(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
+                    const int *cl_idx, const int *cl_size, const int *cl_seg, int ncl,
+                    const uint8_t *near, float tol, float *Y, float *Mx, uint8_t *rank,
+                    float *fscale, void *Q16, int agg, int nagg, double *S, size_t ldS,
+                    hipStream_t s);
+void launch_hm_fill(const float *G, const float *H, size_t ld, const uint8_t *gmask,
+                    const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
+                    float *fscale, const void *Q16, void *B16, void *D16, const int *doff,
+                    hipStream_t s);
 void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
                     const float *fscale, const void *Q16, const void *B16, const void *D16,
-                    int n, const float *x, const double *diag, const float *xd, float *T,
-                    float *ypart, size_t ldp, float *y, hipStream_t s);
+                    const int *doff, int n, const float *x, const double *diag,
+                    const float *xd, float *T, float *ypart, size_t ldp, float *y,
+                    hipStream_t s);
 
 // Work space of the entry points that allocate nothing (wb_noalloc.hip): a static device
 // array of 2 NOALLOC_MAX_N doubles per GPU.  noalloc_setup looks its address up on a device
