@@ -136,6 +136,10 @@ struct wb_ctx {
     DevBuf V, z, w, P1, P2, Pn, hbuf, hred, vx, vb, vr, vt, vs;   // double / raw
     DevBuf V32, z32, w32, r32, d32;
     double *hpin = nullptr;  // pinned host copy of hbuf
+    // coherent pinned host memory the GMRES column is published to: 132 values, then a
+    // sequence number; device-side addresses of both
+    double *hmap = nullptr, *hmap_dev = nullptr;
+    unsigned long long hmap_seq = 0;
     size_t vlen = 0;         // padded vector length
 
     // kernel timing
@@ -351,6 +355,32 @@ static void ensure_work(wb_ctx *c)
     c->hbuf.ensure(sizeof(double) * 132);
     c->hred.ensure(sizeof(double) * 2 * KMAX);
     if (!c->hpin) HIPCHK(hipHostMalloc((void **)&c->hpin, sizeof(double) * 132, 0));
+    if (!c->hmap) {
+        HIPCHK(hipHostMalloc((void **)&c->hmap, sizeof(double) * 134,
+                             hipHostMallocMapped | hipHostMallocCoherent));
+        HIPCHK(hipHostGetDevicePointer((void **)&c->hmap_dev, c->hmap, 0));
+        *(volatile unsigned long long *)(c->hmap + 133) = 0;
+        c->hmap_seq = 0;
+    }
+}
+
+// The device's publication of a GMRES column (k_finalize with hmap): waits until the
+// sequence number reaches seq.  The stream is queried now and then, so an error or an
+// idle stream that never published ends the wait with an exception instead of a hang.
+static void wait_hmap(wb_ctx *c, unsigned long long seq)
+{
+    const unsigned long long *p = (const unsigned long long *)(c->hmap + 133);
+    for (unsigned spins = 1;; ++spins) {
+        if (__atomic_load_n(p, __ATOMIC_ACQUIRE) >= seq) return;
+        if ((spins & 1023u) == 0) {
+            const hipError_t e = hipStreamQuery(c->stream);
+            if (e != hipSuccess && e != hipErrorNotReady) HIPCHK(e);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(p, __ATOMIC_ACQUIRE) >= seq) return;
+                throw WbError(WB_E_HIP, "GMRES column was not published");
+            }
+        }
+    }
 }
 
 static double device_norm(wb_ctx *c, const void *x, int is_float)
@@ -661,12 +691,12 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
             apply_precond(c, is_float, w, z);
         }
         launch_norm2_partials(is_float, z, n, c->Pn.as<double>(), c->stream);
+        const unsigned long long s0 = ++c->hmap_seq;
         launch_finalize(is_float, 0, nullptr, nullptr, c->Pn.as<double>(), z, n, vec(0),
-                        c->hbuf.as<double>(), c->stream);
-        HIPCHK(hipMemcpyAsync(c->hpin, c->hbuf.p, sizeof(double), hipMemcpyDeviceToHost,
-                              c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        double rho = c->hpin[0];
+                        c->hbuf.as<double>(), c->stream, 1, 0, 0, 0, 0, nullptr, c->hmap_dev,
+                        (unsigned long long *)(c->hmap_dev + 133), s0);
+        wait_hmap(c, s0);
+        double rho = c->hmap[0];
         if (first) {
             R.first_residual = rho;
             if (rel_tol > 0) tol = std::max(abs_tol, rel_tol * rho);
@@ -692,12 +722,12 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
             launch_reduce_parts(dim, c->P2.as<double>(), n, h2, c->stream);
             launch_update_dot(is_float, V.p, ldv, dim, h2, z, n, nullptr, c->Pn.as<double>(), 1,
                               c->stream);
+            const unsigned long long sk = ++c->hmap_seq;
             launch_finalize(is_float, dim, h1, h2, c->Pn.as<double>(), z, n, vec(dim),
-                            c->hbuf.as<double>(), c->stream);
-            HIPCHK(hipMemcpyAsync(c->hpin, c->hbuf.p, sizeof(double) * (dim + 1),
-                                  hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-            for (int i = 0; i <= dim; ++i) h[i] = c->hpin[i];
+                            c->hbuf.as<double>(), c->stream, 1, 0, 0, 0, 0, nullptr,
+                            c->hmap_dev, (unsigned long long *)(c->hmap_dev + 133), sk);
+            wait_hmap(c, sk);
+            for (int i = 0; i <= dim; ++i) h[i] = c->hmap[i];
             for (int i = 0; i < inner; ++i) {
                 const double s = si[i], cc = ci[i], d0 = h[i];
                 h[i] = cc * d0 + s * h[i + 1];
@@ -1549,6 +1579,7 @@ void wb_destroy(wb_ctx *c)
     for (DevBuf *b : c->all_bufs()) b->release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->hpin) (void)hipHostFree(c->hpin);
+    if (c->hmap) (void)hipHostFree(c->hmap);
     if (c->mw.hpin) (void)hipHostFree(c->mw.hpin);
     if (c->mw.apin) (void)hipHostFree(c->mw.apin);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -311,10 +311,13 @@ void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, cons
                        void *z, int n, double *Pout, double *normp, int want_norm, hipStream_t s,
                        int batch = 1, size_t sV = 0, size_t sZ = 0, size_t sH = 0,
                        const int *active = nullptr);
+// hmap/hseq/seq: also publish hout (kcount + 1 values, single instance) to coherent pinned
+// host memory, then seq to *hseq
 void launch_finalize(int is_float, int kcount, const double *h1, const double *h2,
                      const double *Pn, void *z, int n, void *vout, double *hout, hipStream_t s,
                      int batch = 1, size_t sZ = 0, size_t sV = 0, size_t sHin = 0,
-                     size_t sH = 0, const int *active = nullptr);
+                     size_t sH = 0, const int *active = nullptr, double *hmap = nullptr,
+                     unsigned long long *hseq = nullptr, unsigned long long seq = 0);
 void launch_axpy_basis(int is_float, const void *V, size_t ldv, int kcount, const double *y,
                        void *x_acc, int x_is_float, int n, hipStream_t s, int batch = 1,
                        size_t sV = 0, size_t sY = 0, size_t sX = 0, const int *active = nullptr);

@@ -147,6 +147,9 @@ __global__ __launch_bounds__(KTHREADS) void k_update_dot(const T *__restrict__ V
 
 // hout[k] = h1[k] + h2[k] (k < kcount), hout[kcount] = sqrt(sum Pn); vout = z / hout[kcount].
 // Every workgroup forms the norm itself, wavefront-parallel and in the same order.
+// With hmap (coherent pinned host memory, single instance), the column also goes to the
+// host: every thread of workgroup 0 writes its entries and fences at system scope, then
+// thread 0 publishes seq after them, so a host that reads seq (acquire) sees the column.
 template <typename T>
 __global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double *__restrict__ h1,
                                                        const double *__restrict__ h2,
@@ -155,7 +158,10 @@ __global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double 
                                                        const T *__restrict__ z, size_t sZ,
                                                        int n, T *__restrict__ vout, size_t sV,
                                                        double *__restrict__ hout, size_t sH,
-                                                       const int *__restrict__ active)
+                                                       const int *__restrict__ active,
+                                                       double *__restrict__ hmap,
+                                                       unsigned long long *__restrict__ hseq,
+                                                       unsigned long long seq)
 {
     __shared__ double snorm;
     const int b = blockIdx.y;
@@ -176,9 +182,21 @@ __global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double 
     __syncthreads();
     const double nv = snorm;
     if (blockIdx.x == 0) {
-        for (int k = threadIdx.x; k < kcount; k += KTHREADS)
-            hout[k] = (h1 ? h1[k] : 0.0) + (h2 ? h2[k] : 0.0);
-        if (threadIdx.x == 0) hout[kcount] = nv;
+        for (int k = threadIdx.x; k < kcount; k += KTHREADS) {
+            const double hk = (h1 ? h1[k] : 0.0) + (h2 ? h2[k] : 0.0);
+            hout[k] = hk;
+            if (hmap) hmap[k] = hk;
+        }
+        if (threadIdx.x == 0) {
+            hout[kcount] = nv;
+            if (hmap) hmap[kcount] = nv;
+        }
+        if (hmap) {
+            __threadfence_system();
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(hseq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
     if (vout) {
         const double inv = nv > 0 ? 1.0 / nv : 0.0;
@@ -279,15 +297,18 @@ void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, cons
 void launch_finalize(int is_float, int kcount, const double *h1, const double *h2,
                      const double *Pn, void *z, int n, void *vout, double *hout, hipStream_t s,
                      int batch, size_t sZ, size_t sV, size_t sHin, size_t sH,
-                     const int *active)
+                     const int *active, double *hmap, unsigned long long *hseq,
+                     unsigned long long seq)
 {
     dim3 grid(krylov_partials_blocks(n), batch);
     if (is_float)
         hipLaunchKernelGGL(k_finalize<float>, grid, dim3(KTHREADS), 0, s, kcount, h1, h2, sHin,
-                           Pn, (const float *)z, sZ, n, (float *)vout, sV, hout, sH, active);
+                           Pn, (const float *)z, sZ, n, (float *)vout, sV, hout, sH, active,
+                           hmap, hseq, seq);
     else
         hipLaunchKernelGGL(k_finalize<double>, grid, dim3(KTHREADS), 0, s, kcount, h1, h2, sHin,
-                           Pn, (const double *)z, sZ, n, (double *)vout, sV, hout, sH, active);
+                           Pn, (const double *)z, sZ, n, (double *)vout, sV, hout, sH, active,
+                           hmap, hseq, seq);
 }
 
 void launch_axpy_basis(int is_float, const void *V, size_t ldv, int kcount, const double *y,
