@@ -26,10 +26,12 @@ namespace wb {
 __device__ __forceinline__ void split_hilo(double v, float &hi, uint16_t &lo)
 {
     hi = (float)v;
-    const float r = (float)(v - (double)hi);
-    uint32_t b = __float_as_uint(r);
-    b += 0x7fffu + ((b >> 16) & 1u);  // round to nearest even on the upper 16 bits
-    lo = (uint16_t)(b >> 16);
+    float r = (float)(v - (double)hi);
+    // r is rounded to FP32 first, then to bf16 (nearest even, v_cvt_pk_bf16_f32 on
+    // gfx950); the empty asm keeps the compiler from merging the two roundings into one
+    // emulated FP64 -> bf16 rounding, which costs eight more instructions
+    asm volatile("" : "+v"(r));
+    lo = __builtin_bit_cast(uint16_t, (__bf16)r);
 }
 
 __device__ __forceinline__ double join_hilo(float hi, uint16_t lo)
@@ -128,7 +130,9 @@ void launch_cell_geometry(const double *X, const int *cell_int, const int8_t *or
 // g_k = sum_q phi_k(q) JxW_q / |y_q - x|, h_k = sum_q phi_k(q) (y_q - x).N_q / |y_q - x|^3
 // (N = n JxW; the 1/(4 pi) factors and the sign of H are applied at the write-out).
 // rec: the cell's NQ records (wave-uniform, read by scalar loads).
-template <int Q>
+// FAST: the values are stored as FP32 hi + bf16 lo, and 1/r takes one Newton step
+// (rsqrt_f64_newton1) instead of the third-order one.
+template <int Q, bool FAST = false>
 __device__ __forceinline__ void cell_local_values(const double *rec, double x0, double x1,
                                                   double x2, const PhiTable &nodes,
                                                   double (&g)[4], double (&h)[4])
@@ -153,7 +157,7 @@ __device__ __forceinline__ void cell_local_values(const double *rec, double x0, 
                 const double *r = rr + 8 * u;
                 const double R0 = r[0] - x0, R1 = r[1] - x1, R2 = r[2] - x2;
                 const double r2 = __builtin_fma(R0, R0, __builtin_fma(R1, R1, R2 * R2));
-                const double ri = rsqrt_f64(r2);
+                const double ri = FAST ? rsqrt_f64_newton1(r2) : rsqrt_f64(r2);
                 sg = __builtin_fma(ri, r[6], sg);   // sum JxW / r
                 sg1 = __builtin_fma(ri, r[7], sg1); // sum xi JxW / r
                 const double dn = __builtin_fma(R0, r[3], __builtin_fma(R1, r[4], R2 * r[5]));
@@ -235,7 +239,7 @@ void k_assemble_tiles(
             const int cols = __builtin_amdgcn_readfirstlane(tile_cols[p]);
             const double *rec = cellq + (size_t)cell * (NQ * 8);
             double gl[4], hl[4];
-            cell_local_values<Q>(rec, x0, x1, x2, nodes, gl, hl);
+            cell_local_values<Q, OUT == 1>(rec, x0, x1, x2, nodes, gl, hl);
             const double g0 = gl[0], g1 = gl[1], g2 = gl[2], g3 = gl[3];
             const double h0 = hl[0], h1 = hl[1], h2 = hl[2], h3 = hl[3];
             const int c0 = (int)(int8_t)(cols & 0xff);
@@ -384,7 +388,7 @@ __device__ __forceinline__ void strip_add(double (&lo)[STRIP_W], double (&up)[ST
 }
 
 // The cells of one gap, positions K .. STRIP_W (present where cmask has the bit).
-template <int Q, int K>
+template <int Q, int K, bool FAST>
 __device__ __forceinline__ void strip_gap_cells(int cmask, const double *grec, double x0,
                                                 double x1, double x2, const PhiTable &nodes,
                                                 double (&lo)[STRIP_W], double (&up)[STRIP_W],
@@ -393,10 +397,10 @@ __device__ __forceinline__ void strip_gap_cells(int cmask, const double *grec, d
     if constexpr (K <= STRIP_W) {
         if (cmask & (1 << K)) {
             double cg[4], ch[4];
-            cell_local_values<Q>(grec + K * (Q * Q * 8), x0, x1, x2, nodes, cg, ch);
+            cell_local_values<Q, FAST>(grec + K * (Q * Q * 8), x0, x1, x2, nodes, cg, ch);
             strip_add<K>(lo, up, hlo, hup, cg, ch, acc_ld);
         }
-        strip_gap_cells<Q, K + 1>(cmask, grec, x0, x1, x2, nodes, lo, up, hlo, hup, acc_ld);
+        strip_gap_cells<Q, K + 1, FAST>(cmask, grec, x0, x1, x2, nodes, lo, up, hlo, hup, acc_ld);
     }
 }
 
@@ -444,7 +448,7 @@ void k_assemble_strips(const double *__restrict__ xrow, const double *__restrict
         double *hlo = lacc + hl * (STRIP_W * ACC_LD) + lane;
         double *hup = lacc + (1 - hl) * (STRIP_W * ACC_LD) + lane;
         const double *grec = recs + (size_t)g * KS * (NQ * 8);
-        strip_gap_cells<Q, 0>(cmask, grec, x0, x1, x2, nodes, lo, up, hlo, hup, ACC_LD);
+        strip_gap_cells<Q, 0, OUT == 1>(cmask, grec, x0, x1, x2, nodes, lo, up, hlo, hup, ACC_LD);
         // line t is complete: row sums, then H and G out through LDS, then clear
         double *hbase = lacc + hl * (STRIP_W * ACC_LD);
 #pragma unroll

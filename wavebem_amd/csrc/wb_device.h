@@ -42,6 +42,17 @@ __device__ __forceinline__ double rsqrt_f64(double r2)
 #endif
 }
 
+// 1/sqrt(r2) with one Newton step from the hardware estimate: relative error (3/2) eps^2,
+// at most 4.1e-15 with the measured eps above (4 operations).  For values stored as FP32
+// hi + bf16 lo (relative rounding 2^-33 = 1.2e-10) this is 30000 times below the storage
+// error; the FP64 stores use rsqrt_f64.
+__device__ __forceinline__ double rsqrt_f64_newton1(double r2)
+{
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double e = __builtin_fma(-0.5 * r2, y * y, 0.5);
+    return __builtin_fma(y, e, y);
+}
+
 // Q1 shape values and reference derivatives at (xi, eta), deal.II vertex order
 // v0=(0,0) v1=(1,0) v2=(0,1) v3=(1,1).
 __device__ __forceinline__ void q1_shape(double xi, double eta, double (&phi)[4],
