@@ -299,6 +299,81 @@ __global__ __launch_bounds__(256) void k_coarse_residual(int n, int nc, const T 
             (T)((double)X[(size_t)v * ldx + i] - (((qs[0][rl] + qs[1][rl]) + qs[2][rl]) + qs[3][rl]));
 }
 
+// The block-Jacobi apply of one vector with the coarse residual formed in the same
+// workgroup (blocks of bs <= 64 rows, 256 threads): t = x - W C for the block's rows, four
+// quarter sums per row as in k_coarse_residual; z_a = (Aci C)_a for the aggregates the rows
+// touch, one wavefront each; then y = B^-1 t + z, the columns of B^-1 split over the four
+// wavefronts and the partial products added in wavefront order.  x and y may alias: every
+// row is read and written by its own workgroup only, the read before the write.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bj_coarse(int n, int bs, const T *__restrict__ binv,
+                                                   const T *x, T *y, const T *__restrict__ WT,
+                                                   size_t ldw, const double *__restrict__ C,
+                                                   const T *__restrict__ Aci, int nc, int agg)
+{
+    __shared__ double cs[CS_MAX];
+    __shared__ double qs[4][64];
+    __shared__ T ts[64];
+    __shared__ T ps[4][64];
+    __shared__ double zs[4];
+    const int tid = threadIdx.x, rl = tid & 63, w = tid >> 6;
+    const int base = blockIdx.x * bs, m = min(bs, n - base);
+    for (int b = tid; b < nc; b += 256) cs[b] = C[b];
+    __syncthreads();
+    const int i = base + rl;
+    const int b0 = w * nc / 4, b1 = (w + 1) * nc / 4;
+    // loads in batches of 16 (independent, so one wait per batch), two accumulators
+    double a0 = 0.0, a1 = 0.0;
+    if (rl < m) {
+        int b = b0;
+        for (; b + 16 <= b1; b += 16) {
+            T wv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) wv[u] = WT[(size_t)(b + u) * ldw + i];
+#pragma unroll
+            for (int u = 0; u < 16; u += 2) {
+                a0 = fma((double)wv[u], cs[b + u], a0);
+                a1 = fma((double)wv[u + 1], cs[b + u + 1], a1);
+            }
+        }
+        for (; b < b1; ++b) a0 = fma((double)WT[(size_t)b * ldw + i], cs[b], a0);
+    }
+    qs[w][rl] = a0 + a1;
+    const int ag0 = base / agg, na = (base + m - 1) / agg - ag0 + 1;
+    if (w < na) {
+        const int a = ag0 + w;
+        double p = 0.0;
+#pragma unroll 3
+        for (int bb = rl; bb < nc; bb += 64) p += (double)Aci[(size_t)a * nc + bb] * cs[bb];
+        p = cs_wave_sum(p);
+        if (rl == 0) zs[w] = p;
+    }
+    __syncthreads();
+    if (w == 0)
+        ts[rl] = rl < m ? (T)((double)x[i] - (((qs[0][rl] + qs[1][rl]) + qs[2][rl]) + qs[3][rl]))
+                        : (T)0;
+    __syncthreads();
+    const int c0 = w * bs / 4, c1 = min(m, (w + 1) * bs / 4);
+    const T *B = binv + (size_t)blockIdx.x * bs * bs;
+    T acc = 0;
+    if (rl < m) {
+        int c = c0;
+        for (; c + 16 <= c1; c += 16) {
+            T bv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) bv[u] = B[(size_t)(c + u) * bs + rl];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) acc += bv[u] * ts[c + u];
+        }
+        for (; c < c1; ++c) acc += B[(size_t)c * bs + rl] * ts[c];
+    }
+    ps[w][rl] = acc;
+    __syncthreads();
+    if (w == 0 && rl < m)
+        y[i] = (T)((double)(((ps[0][rl] + ps[1][rl]) + ps[2][rl]) + ps[3][rl]) +
+                   zs[i / agg - ag0]);
+}
+
 }  // namespace
 
 size_t coarse_invert_lds_bytes(int nc)
@@ -361,6 +436,35 @@ void launch_coarse_pre(int is_float, int n, int agg, int nc, const void *X, size
                            (const double *)X, ldx, (const double *)WT, ldw, (const double *)C,
                            (double *)Tout);
     }
+}
+
+void launch_coarse_restrict(int is_float, int n, int agg, int nc, const void *X, size_t ldx,
+                            double *C, int k, hipStream_t s)
+{
+    const dim3 cg(nc, k);
+    if (is_float)
+        hipLaunchKernelGGL(k_coarse_restrict<float>, cg, dim3(256), 0, s, n, agg,
+                           (const float *)X, ldx, C);
+    else
+        hipLaunchKernelGGL(k_coarse_restrict<double>, cg, dim3(256), 0, s, n, agg,
+                           (const double *)X, ldx, C);
+}
+
+bool bj_coarse_fits(int bs, int agg, int nc) { return bs >= 4 && bs <= 64 && agg >= bs && nc <= CS_MAX; }
+
+void launch_bj_coarse(int is_float, int n, int bs, const void *binv, const void *x, void *y,
+                      const void *WT, size_t ldw, const double *C, const void *Aci, int nc,
+                      int agg, hipStream_t s)
+{
+    const int nb = (n + bs - 1) / bs;
+    if (is_float)
+        hipLaunchKernelGGL(k_bj_coarse<float>, dim3(nb), dim3(256), 0, s, n, bs,
+                           (const float *)binv, (const float *)x, (float *)y, (const float *)WT,
+                           ldw, C, (const float *)Aci, nc, agg);
+    else
+        hipLaunchKernelGGL(k_bj_coarse<double>, dim3(nb), dim3(256), 0, s, n, bs,
+                           (const double *)binv, (const double *)x, (double *)y,
+                           (const double *)WT, ldw, C, (const double *)Aci, nc, agg);
 }
 
 }  // namespace wb

@@ -17,6 +17,9 @@ using namespace wb;
 
 namespace {
 
+constexpr int HMAP_STRIDE = 136;  // doubles per GMRES column slot in pinned memory
+static_assert(KMAX + 1 < HMAP_STRIDE, "a column and its sequence word fit one slot");
+
 struct WbError : std::runtime_error {
     wb_status code;
     WbError(wb_status c, const std::string &m) : std::runtime_error(m), code(c) {}
@@ -127,7 +130,7 @@ struct wb_ctx {
     // factors
     HmatPlan hmp;
     DevBuf hm_idx, hm_size, hm_seg, hm_near, hm_rank, hm_scale, hm_Y, hm_Mx, hm_Q, hm_B, hm_D,
-        hm_doff, hm_T, hm_part;
+        hm_doff, hm_T, hm_part, hm_iptr, hm_items;
     long hm_assembly_ver = -1, hm_mask_ver = -1;
     int hm_agg = 0;         // aggregate size of the coarse sums written by the build (0: none)
     int float_mode = 1;  // operator mode of the FP32-vector solves: 1 = FP32 hi, 3 = FP16,
@@ -136,8 +139,9 @@ struct wb_ctx {
     DevBuf V, z, w, P1, P2, Pn, hbuf, hred, vx, vb, vr, vt, vs;   // double / raw
     DevBuf V32, z32, w32, r32, d32;
     double *hpin = nullptr;  // pinned host copy of hbuf
-    // coherent pinned host memory the GMRES column is published to: 132 values, then a
-    // sequence number; device-side addresses of both
+    // coherent pinned host memory the GMRES columns are published to: two slots of
+    // HMAP_STRIDE doubles (up to KMAX + 1 values, the sequence number in the last word),
+    // publication seq using slot seq & 1; device-side address of the same memory
     double *hmap = nullptr, *hmap_dev = nullptr;
     unsigned long long hmap_seq = 0;
     size_t vlen = 0;         // padded vector length
@@ -185,7 +189,7 @@ struct wb_ctx {
             &inc_item, &pj_emass, &pj_erhs, &pj_u, &pj_psi, &pj_B, &pj_X, &pj_R, &pj_Z, &pj_P,
             &pj_Q, &pj_dinv, &pj_part, &pj_part2, &pj_state, &A16, &rs16, &hm_idx, &hm_size, &hm_seg,
             &hm_near, &hm_rank, &hm_scale, &hm_Y, &hm_Mx, &hm_Q, &hm_B, &hm_D, &hm_doff, &hm_T,
-            &hm_part};
+            &hm_part, &hm_iptr, &hm_items};
         for (DevBuf *b : mw.all()) v.push_back(b);
         return v;
     }
@@ -356,27 +360,40 @@ static void ensure_work(wb_ctx *c)
     c->hred.ensure(sizeof(double) * 2 * KMAX);
     if (!c->hpin) HIPCHK(hipHostMalloc((void **)&c->hpin, sizeof(double) * 132, 0));
     if (!c->hmap) {
-        HIPCHK(hipHostMalloc((void **)&c->hmap, sizeof(double) * 134,
+        HIPCHK(hipHostMalloc((void **)&c->hmap, sizeof(double) * 2 * HMAP_STRIDE,
                              hipHostMallocMapped | hipHostMallocCoherent));
         HIPCHK(hipHostGetDevicePointer((void **)&c->hmap_dev, c->hmap, 0));
-        *(volatile unsigned long long *)(c->hmap + 133) = 0;
+        for (int slot = 0; slot < 2; ++slot)
+            *(volatile unsigned long long *)(c->hmap + slot * HMAP_STRIDE + HMAP_STRIDE - 1) = 0;
         c->hmap_seq = 0;
     }
+}
+
+// Launch of k_finalize that also publishes the column as publication seq
+static void finalize_publish(wb_ctx *c, int is_float, int kcount, const double *h1,
+                             const double *h2, void *z, int n, void *vout,
+                             unsigned long long seq)
+{
+    double *slot = c->hmap_dev + (seq & 1) * HMAP_STRIDE;
+    launch_finalize(is_float, kcount, h1, h2, c->Pn.as<double>(), z, n, vout,
+                    c->hbuf.as<double>(), c->stream, 1, 0, 0, 0, 0, nullptr, slot,
+                    (unsigned long long *)(slot + HMAP_STRIDE - 1), seq);
 }
 
 // The device's publication of a GMRES column (k_finalize with hmap): waits until the
 // sequence number reaches seq.  The stream is queried now and then, so an error or an
 // idle stream that never published ends the wait with an exception instead of a hang.
-static void wait_hmap(wb_ctx *c, unsigned long long seq)
+static const double *wait_hmap(wb_ctx *c, unsigned long long seq)
 {
-    const unsigned long long *p = (const unsigned long long *)(c->hmap + 133);
+    const double *slot = c->hmap + (seq & 1) * HMAP_STRIDE;
+    const unsigned long long *p = (const unsigned long long *)(slot + HMAP_STRIDE - 1);
     for (unsigned spins = 1;; ++spins) {
-        if (__atomic_load_n(p, __ATOMIC_ACQUIRE) >= seq) return;
+        if (__atomic_load_n(p, __ATOMIC_ACQUIRE) >= seq) return slot;
         if ((spins & 1023u) == 0) {
             const hipError_t e = hipStreamQuery(c->stream);
             if (e != hipSuccess && e != hipErrorNotReady) HIPCHK(e);
             if (e == hipSuccess) {
-                if (__atomic_load_n(p, __ATOMIC_ACQUIRE) >= seq) return;
+                if (__atomic_load_n(p, __ATOMIC_ACQUIRE) >= seq) return slot;
                 throw WbError(WB_E_HIP, "GMRES column was not published");
             }
         }
@@ -410,8 +427,8 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
         KernelScope ks(c, WB_K_GEMVC, c->n);
         launch_hm_gemv(c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hmp.ncl,
                        c->hm_rank.as<uint8_t>(), c->hm_scale.as<float>(), c->hm_Q.p, c->hm_B.p,
-                       c->hm_D.p, c->hm_doff.as<int>(), c->n, (const float *)x, diag,
-                       (const float *)xd,
+                       c->hm_D.p, c->hm_doff.as<int>(), c->hm_iptr.as<int>(),
+                       c->hm_items.as<int>(), c->n, (const float *)x, diag, (const float *)xd,
                        c->hm_T.as<float>(), c->hm_part.as<float>(), c->vlen, (float *)y,
                        c->stream);
         return;
@@ -486,6 +503,14 @@ static void block_precond(wb_ctx *c, int is_float, int k, const void *x, void *y
                           c->cs_C.as<double>(), c->cs_T.p, k, c->stream);
         launch_block_jacobi_apply(0, c->n, c->pc_bs, binv, c->cs_T.p, y, k, c->vlen, c->stream,
                                   1, c->cs_C.as<double>(), c->cs_Aci.p, c->cs_nc, c->cs_agg);
+    } else if (c->cs_on && !c->op_transpose && k == 1 &&
+               bj_coarse_fits(c->pc_bs, c->cs_agg, c->cs_nc)) {
+        launch_coarse_restrict(is_float, c->n, c->cs_agg, c->cs_nc, x, c->vlen,
+                               c->cs_C.as<double>(), 1, c->stream);
+        launch_bj_coarse(is_float, c->n, c->pc_bs, binv, x, y,
+                         is_float ? c->cs_WT32.p : c->cs_WT.p, c->vlen, c->cs_C.as<double>(),
+                         is_float ? c->cs_Aci32.p : c->cs_Aci.p, c->cs_nc, c->cs_agg,
+                         c->stream);
     } else if (c->cs_on && !c->op_transpose) {
         launch_coarse_pre(is_float, c->n, c->cs_agg, c->cs_nc, x, c->vlen,
                           is_float ? c->cs_WT32.p : c->cs_WT.p, c->vlen, c->cs_C.as<double>(),
@@ -681,6 +706,7 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
     int state = 0;
     bool first = true;
     double tol = abs_tol;
+    double ratio = 1.0;  // last residual reduction factor of one step
     do {
         // v0 = P^-1 (b - A x)
         if (x_is_zero && first) {
@@ -692,11 +718,8 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
         }
         launch_norm2_partials(is_float, z, n, c->Pn.as<double>(), c->stream);
         const unsigned long long s0 = ++c->hmap_seq;
-        launch_finalize(is_float, 0, nullptr, nullptr, c->Pn.as<double>(), z, n, vec(0),
-                        c->hbuf.as<double>(), c->stream, 1, 0, 0, 0, 0, nullptr, c->hmap_dev,
-                        (unsigned long long *)(c->hmap_dev + 133), s0);
-        wait_hmap(c, s0);
-        double rho = c->hmap[0];
+        finalize_publish(c, is_float, 0, nullptr, nullptr, z, n, vec(0), s0);
+        double rho = wait_hmap(c, s0)[0];
         if (first) {
             R.first_residual = rho;
             if (rel_tol > 0) tol = std::max(abs_tol, rel_tol * rho);
@@ -709,25 +732,37 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
         if (state != 0) break;
         gamma[0] = rho;
         int dim = 0;
+        // Arnoldi step k: v_{k+1} and column k of the Hessenberg matrix, published to the
+        // host as the returned publication number.  The step does not depend on the host's
+        // rotations, so step k + 1 is queued before step k's column is read whenever the
+        // residual trend (rho times the last reduction ratio) stays SPEC_MARGIN above the
+        // tolerance: the launches then overlap the wait.  A step queued for a check that
+        // ends the cycle only writes v_{k+2} and work vectors, which nothing reads.
+        auto enqueue_step = [&](int k) {
+            apply_constrained(c, mode, vec(k), w);
+            apply_precond(c, is_float, w, z);
+            double *h1 = c->hred.as<double>(), *h2 = h1 + KMAX;
+            launch_dot_partials(is_float, V.p, ldv, k + 1, z, n, c->P1.as<double>(), c->stream);
+            launch_update_dot(is_float, V.p, ldv, k + 1, nullptr, z, n, c->P2.as<double>(),
+                              nullptr, 0, c->stream, 1, 0, 0, 0, nullptr, c->P1.as<double>(),
+                              h1);
+            launch_update_dot(is_float, V.p, ldv, k + 1, nullptr, z, n, nullptr,
+                              c->Pn.as<double>(), 1, c->stream, 1, 0, 0, 0, nullptr,
+                              c->P2.as<double>(), h2);
+            const unsigned long long sq = ++c->hmap_seq;
+            finalize_publish(c, is_float, k + 1, h1, h2, z, n, vec(k + 1), sq);
+            return sq;
+        };
+        constexpr double SPEC_MARGIN = 8.0;
+        unsigned long long queued = 0;  // publication number of step inner, if already queued
         for (int inner = 0; inner < m && state == 0; ++inner) {
             ++acc;
-            apply_constrained(c, mode, vec(inner), w);
-            apply_precond(c, is_float, w, z);
+            const unsigned long long sk = queued ? queued : enqueue_step(inner);
+            queued = 0;
+            if (inner + 1 < m && rho * ratio > SPEC_MARGIN * tol) queued = enqueue_step(inner + 1);
             dim = inner + 1;
-            double *h1 = c->hred.as<double>(), *h2 = h1 + KMAX;
-            launch_dot_partials(is_float, V.p, ldv, dim, z, n, c->P1.as<double>(), c->stream);
-            launch_reduce_parts(dim, c->P1.as<double>(), n, h1, c->stream);
-            launch_update_dot(is_float, V.p, ldv, dim, h1, z, n, c->P2.as<double>(), nullptr, 0,
-                              c->stream);
-            launch_reduce_parts(dim, c->P2.as<double>(), n, h2, c->stream);
-            launch_update_dot(is_float, V.p, ldv, dim, h2, z, n, nullptr, c->Pn.as<double>(), 1,
-                              c->stream);
-            const unsigned long long sk = ++c->hmap_seq;
-            launch_finalize(is_float, dim, h1, h2, c->Pn.as<double>(), z, n, vec(dim),
-                            c->hbuf.as<double>(), c->stream, 1, 0, 0, 0, 0, nullptr,
-                            c->hmap_dev, (unsigned long long *)(c->hmap_dev + 133), sk);
-            wait_hmap(c, sk);
-            for (int i = 0; i <= dim; ++i) h[i] = c->hmap[i];
+            const double *col = wait_hmap(c, sk);
+            for (int i = 0; i <= dim; ++i) h[i] = col[i];
             for (int i = 0; i < inner; ++i) {
                 const double s = si[i], cc = ci[i], d0 = h[i];
                 h[i] = cc * d0 + s * h[i + 1];
@@ -740,7 +775,10 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
             gamma[inner + 1] = -si[inner] * gamma[inner];
             gamma[inner] *= ci[inner];
             for (int i = 0; i < dim; ++i) Hh[(size_t)i * m + inner] = h[i];
-            rho = std::fabs(gamma[dim]);
+            const double rho_new = std::fabs(gamma[dim]);
+            if (rho > 0 && rho_new < rho) ratio = rho_new / rho;
+            else ratio = 1.0;
+            rho = rho_new;
             R.residual = rho;
             if (rho <= tol) state = 1;
             else if (std::isnan(rho) || acc >= max_steps) state = -1;
@@ -845,6 +883,11 @@ static void ensure_hm(wb_ctx *c)
                        c->hm_rank.as<uint8_t>(), c->hm_scale.as<float>(), c->hm_Q.p, c->hm_B.p,
                        c->hm_D.p, c->hm_doff.as<int>(), c->stream);
         HIPCHK(hipGetLastError());
+        // the product's work lists, planned while the fill runs
+        std::vector<int> iptr, items;
+        hm_plan_items(c->hmp.ncl, rk.data(), c->hmp.cl_size.data(), iptr, items);
+        upload(c->hm_iptr, iptr.data(), iptr.size(), c->stream);
+        upload(c->hm_items, items.data(), items.size(), c->stream);
     }
     c->hm_assembly_ver = c->assembly_ver;
     c->hm_mask_ver = c->mask_ver;

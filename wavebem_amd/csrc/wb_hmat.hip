@@ -20,6 +20,8 @@
 // Product: k_hm_gemv (one wavefront per 64 rows of a row cluster and a chunk of column
 // clusters) and k_hm_finish (chunk sums, the alpha diagonal).
 #include <algorithm>
+#include <functional>
+#include <vector>
 
 #include "wb_internal.h"
 
@@ -31,6 +33,10 @@ typedef _Float16 hvec4 __attribute__((ext_vector_type(4)));
 typedef _Float16 hvec8 __attribute__((ext_vector_type(8)));
 typedef float fvec4 __attribute__((ext_vector_type(4)));
 typedef double dvec4 __attribute__((ext_vector_type(4)));
+
+#ifndef WB_HM_BATCH
+#define WB_HM_BATCH 16  // dense columns per batch of loads in the product
+#endif
 
 // Wavefront sum / max with DPP (quad permutes, row mirrors, row broadcasts): the total
 // lands in lane 63.
@@ -545,10 +551,13 @@ __global__ __launch_bounds__(64) void k_hm_gemv_t(const int *__restrict__ cl_idx
     }
 }
 
-// Phase 2: one wavefront per (row cluster I, chunk of column clusters), lane l holding the
-// block rows 4 l .. 4 l + 3: ypart[chunk][row] = sum over the chunk's blocks of
-// (A~_IJ x_J)(row), low-rank blocks as fscale Q16 T (8-byte loads of four rows of a Q
-// column), dense ones row by row with wavefront sums.
+// Phase 2: one wavefront per (row cluster I, chunk ch), lane l holding the block rows
+// 4 l .. 4 l + 3: ypart[ch][row] = sum over the chunk's work items of (A~_IJ x_J)(row).
+// The items (built on the host from the ranks, hm_plan_items) are the low-rank blocks of
+// row cluster I, applied as fscale Q16 T (8-byte loads of four rows of a Q column, two
+// blocks at a time), and HM_DCOLS-column parts of its dense blocks, streamed column by
+// column in batches of WB_HM_BATCH columns (independent 512-byte loads per wavefront), so
+// no wavefront walks a whole dense block alone.
 __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
                                                 const int *__restrict__ cl_size, int ncl,
                                                 int chunks, const uint8_t *__restrict__ rank,
@@ -558,72 +567,85 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
                                                 const int *__restrict__ doff,
                                                 const float *__restrict__ T,
                                                 const float *__restrict__ x, size_t ldp,
-                                                float *__restrict__ ypart)
+                                                float *__restrict__ ypart,
+                                                const int *__restrict__ item_ptr,
+                                                const int *__restrict__ items)
 {
     const int lane = threadIdx.x;
     const int I = blockIdx.x, ch = blockIdx.y;
     const int szI = cl_size[I];
-    // column clusters ch, ch + chunks, ...: the dense blocks, which cluster around the
-    // diagonal, are spread over the chunks
+    const int p0 = __builtin_amdgcn_readfirstlane(item_ptr[I * chunks + ch]);
+    const int p1 = __builtin_amdgcn_readfirstlane(item_ptr[I * chunks + ch + 1]);
     fvec4 yacc = {0.f, 0.f, 0.f, 0.f};
-    // low-rank blocks, two at a time (their loads in flight together)
-    for (int J = ch; J < ncl; J += 2 * chunks) {
-        fvec4 s2[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            s2[u] = fvec4{0.f, 0.f, 0.f, 0.f};
-            const int Ju = J + u * chunks;
-            if (Ju >= ncl) continue;
-            const size_t b = (size_t)I * ncl + Ju;
-            const int r = __builtin_amdgcn_readfirstlane(rank[b]);
-            if (r == 0 || r == HM_DENSE) continue;
-            const _Float16 *Q = Q16 + b * HM_RMAX * HM_CL + 4 * lane;
-            const float *t = T + b * HM_RMAX;
-            hvec4 q[HM_RMAX];
-#pragma unroll
-            for (int k = 0; k < HM_RMAX; ++k)
-                if (k < r) q[k] = *(const hvec4 *)(Q + (size_t)k * HM_CL);
-            fvec4 sv = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < HM_RMAX; ++k)
-                if (k < r) {
-                    const float tk = t[k];
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) sv[c] = fmaf((float)q[k][c], tk, sv[c]);
-                }
-            s2[u] = sv * fscale[b];
-        }
-        yacc += s2[0] + s2[1];
-    }
-    // dense blocks, column by column (column-major storage): lane l holds the block rows
-    // 4 l .. 4 l + 3 and, for the broadcast, x_J at the block columns 4 l .. 4 l + 3
-    for (int J = ch; J < ncl; J += chunks) {
+    auto lowrank = [&](int J) {
         const size_t b = (size_t)I * ncl + J;
         const int r = __builtin_amdgcn_readfirstlane(rank[b]);
-        if (r != HM_DENSE) continue;
+        const _Float16 *Q = Q16 + b * HM_RMAX * HM_CL + 4 * lane;
+        const float *t = T + b * HM_RMAX;
+        hvec4 q[HM_RMAX];
+#pragma unroll
+        for (int k = 0; k < HM_RMAX; ++k)
+            if (k < r) q[k] = *(const hvec4 *)(Q + (size_t)k * HM_CL);
+        fvec4 sv = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < HM_RMAX; ++k)
+            if (k < r) {
+                const float tk = t[k];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) sv[c] = fmaf((float)q[k][c], tk, sv[c]);
+            }
+        return sv * fscale[b];
+    };
+    int p = p0;
+    // low-rank items (listed first), two at a time so that their loads are in flight together
+    for (; p < p1; p += 2) {
+        const int it0 = __builtin_amdgcn_readfirstlane(items[p]);
+        if ((it0 & HM_ITEM_MASK) != HM_ITEM_LOWRANK) break;
+        const int it1 = p + 1 < p1 ? __builtin_amdgcn_readfirstlane(items[p + 1]) : 0;
+        const bool two = p + 1 < p1 && (it1 & HM_ITEM_MASK) == HM_ITEM_LOWRANK;
+        const fvec4 s0 = lowrank(it0 >> HM_ITEM_SHIFT);
+        const fvec4 s1 = two ? lowrank(it1 >> HM_ITEM_SHIFT) : fvec4{0.f, 0.f, 0.f, 0.f};
+        yacc += s0 + s1;
+        if (!two) {
+            ++p;
+            break;
+        }
+    }
+    // dense parts: columns HM_DCOLS q .. HM_DCOLS (q + 1) - 1 of block (I, J), column-major
+    // storage; lanes 0 .. HM_DCOLS / 4 - 1 hold x_J at the part's columns 4 l .. 4 l + 3
+    // for the broadcast
+    constexpr int NB = WB_HM_BATCH;
+    static_assert(HM_DCOLS % NB == 0 && NB % 4 == 0 && HM_DCOLS <= 4 * 64, "batch");
+    static_assert(HM_CL / HM_DCOLS < HM_ITEM_LOWRANK, "item kinds");
+    for (; p < p1; ++p) {
+        const int it = __builtin_amdgcn_readfirstlane(items[p]);
+        const int J = it >> HM_ITEM_SHIFT, q = it & HM_ITEM_MASK;
+        const size_t b = (size_t)I * ncl + J;
         const float fs = fscale[b];
         const int szJ = __builtin_amdgcn_readfirstlane(cl_size[J]);
+        const int cb = HM_DCOLS * q, ce = min(cb + HM_DCOLS, szJ);
         fvec4 xv;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const int p = 4 * lane + c;
-            xv[c] = p < szJ ? x[cl_idx[(size_t)J * HM_CL + p]] : 0.0f;
+            const int pc = cb + 4 * lane + c;
+            xv[c] = (lane < HM_DCOLS / 4 && pc < ce) ? x[cl_idx[(size_t)J * HM_CL + pc]] : 0.0f;
         }
         const _Float16 *D = D16 + (size_t)doff[b] * HM_CL * HM_CL + 4 * lane;
         fvec4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0;
-        for (int c0 = 0; c0 < szJ; c0 += 4) {
-            // columns c0 .. c0 + 3 (zero past |J|), their x from lane c0 / 4
-            const int src = c0 >> 2;
-            hvec4 h[4];
+        for (int c0 = cb; c0 < ce; c0 += NB) {
+            // columns c0 .. c0 + NB - 1 (inside the part and the 256 stored columns, zero past
+            // |J|), their x from lanes (c0 - cb) / 4 ..
+            hvec4 h[NB];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) h[c] = *(const hvec4 *)(D + (size_t)(c0 + c) * HM_CL);
+            for (int u = 0; u < NB; ++u) h[u] = *(const hvec4 *)(D + (size_t)(c0 + u) * HM_CL);
+            const int src = (c0 - cb) >> 2;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
+            for (int u = 0; u < NB; ++u) {
                 const float xc = __int_as_float(
-                    __builtin_amdgcn_readlane(__float_as_int(xv[c]), src));
-                fvec4 &a = (c & 1) ? a1 : a0;
+                    __builtin_amdgcn_readlane(__float_as_int(xv[u & 3]), src + (u >> 2)));
+                fvec4 &a = (u & 1) ? a1 : a0;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) a[e] = fmaf((float)h[c][e], xc, a[e]);
+                for (int e = 0; e < 4; ++e) a[e] = fmaf((float)h[u][e], xc, a[e]);
             }
         }
         yacc += (a0 + a1) * fs;
@@ -708,18 +730,78 @@ void launch_hm_fill(const float *G, const float *H, size_t ld, const uint8_t *gm
                        (_Float16 *)D16, doff);
 }
 
+void hm_plan_items(int ncl, const uint8_t *rank, const int *cl_size, std::vector<int> &item_ptr,
+                   std::vector<int> &items)
+{
+    const int chunks = hm_chunks(ncl);
+    item_ptr.assign((size_t)ncl * chunks + 1, 0);
+    items.clear();
+    std::vector<int> row;                 // the row cluster's items: low-rank, then dense
+    std::vector<int> weight, chunk_of, order, count(chunks + 1);
+    std::vector<uint64_t> heap(chunks);  // (load << 8) | chunk, a min-heap
+    const int w_lowrank = 8 + 8 * HM_RMAX, w_dense = 8 * HM_DCOLS;  // weights in 64-byte units
+    const int wmax = std::max(w_lowrank, w_dense);
+    std::vector<int> cnt(wmax + 2);
+    for (int I = 0; I < ncl; ++I) {
+        row.clear();
+        weight.clear();
+        for (int J = 0; J < ncl; ++J) {
+            const int r = rank[(size_t)I * ncl + J];
+            if (r == 0 || r == HM_DENSE) continue;
+            row.push_back((J << HM_ITEM_SHIFT) | HM_ITEM_LOWRANK);
+            weight.push_back(8 + 8 * r);  // r columns of Q (512 bytes each) and the r values of T
+        }
+        for (int J = 0; J < ncl; ++J)
+            if (rank[(size_t)I * ncl + J] == HM_DENSE)
+                for (int q = 0; HM_DCOLS * q < cl_size[J]; ++q) {
+                    row.push_back((J << HM_ITEM_SHIFT) | q);
+                    weight.push_back(w_dense);  // HM_DCOLS columns of 512 bytes
+                }
+        const int ni = (int)row.size();
+        // largest weight first (a stable counting sort over the weights 1 .. wmax)
+        order.resize(ni);
+        {
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (int k = 0; k < ni; ++k) ++cnt[wmax - weight[k] + 1];
+            for (int w = 1; w <= wmax + 1; ++w) cnt[w] += cnt[w - 1];
+            for (int k = 0; k < ni; ++k) order[cnt[wmax - weight[k]]++] = k;
+        }
+        // each to the least loaded chunk, lowest index on ties
+        for (int k = 0; k < chunks; ++k) heap[k] = (uint64_t)k;
+        std::make_heap(heap.begin(), heap.end(), std::greater<uint64_t>());
+        chunk_of.resize(ni);
+        for (int k : order) {
+            std::pop_heap(heap.begin(), heap.end(), std::greater<uint64_t>());
+            uint64_t &top = heap.back();
+            const int c = (int)(top & 0xff);
+            chunk_of[k] = c;
+            top = ((top >> 8) + (uint64_t)weight[k]) << 8 | (uint64_t)c;
+            std::push_heap(heap.begin(), heap.end(), std::greater<uint64_t>());
+        }
+        // grouped by chunk, keeping the row's order (low-rank first, each kind by column)
+        std::fill(count.begin(), count.end(), 0);
+        for (int k = 0; k < ni; ++k) ++count[chunk_of[k] + 1];
+        for (int c = 0; c < chunks; ++c) count[c + 1] += count[c];
+        const size_t base = items.size();
+        items.resize(base + ni);
+        for (int c = 0; c < chunks; ++c) item_ptr[(size_t)I * chunks + c + 1] = (int)(base + count[c + 1]);
+        for (int k = 0; k < ni; ++k) items[base + count[chunk_of[k]]++] = row[k];
+    }
+    if (items.empty()) items.push_back(0);
+}
+
 void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
                     const float *fscale, const void *Q16, const void *B16, const void *D16,
-                    const int *doff, int n, const float *x, const double *diag,
-                    const float *xd, float *T, float *ypart, size_t ldp, float *y,
-                    hipStream_t s)
+                    const int *doff, const int *item_ptr, const int *items, int n,
+                    const float *x, const double *diag, const float *xd, float *T, float *ypart,
+                    size_t ldp, float *y, hipStream_t s)
 {
     const int ch = hm_chunks(ncl), tch = hm_tchunks(ncl);
     hipLaunchKernelGGL(k_hm_gemv_t, dim3(ncl, tch), dim3(64), 0, s, cl_idx, cl_size, ncl, tch,
                        rank, (const _Float16 *)B16, x, T);
     hipLaunchKernelGGL(k_hm_gemv, dim3(ncl, ch), dim3(64), 0, s, cl_idx, cl_size, ncl, ch,
                        rank, fscale, (const _Float16 *)Q16, (const _Float16 *)D16, doff, T, x,
-                       ldp, ypart);
+                       ldp, ypart, item_ptr, items);
     hipLaunchKernelGGL(k_hm_finish, dim3((n + 63) / 64), dim3(256), 0, s, ypart, ldp, ch, n,
                        diag, xd, y);
 }

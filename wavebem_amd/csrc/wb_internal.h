@@ -98,6 +98,13 @@ std::string plan_mesh(int n, int nc, const double *support, const int32_t *cell_
 // sign vectors) meets the tolerance, else dense too.
 constexpr int HM_SEG = 64, HM_SEGS = 4, HM_CL = HM_SEG * HM_SEGS, HM_SK = 16, HM_RMAX = 12;
 constexpr int HM_DENSE = 255;
+// product work items: (J << HM_ITEM_SHIFT) | kind, kind q < HM_CL / HM_DCOLS for columns
+// HM_DCOLS q .. HM_DCOLS (q + 1) - 1 of a dense block, HM_ITEM_LOWRANK for a low-rank block
+#ifndef WB_HM_DCOLS
+#define WB_HM_DCOLS 16
+#endif
+constexpr int HM_DCOLS = WB_HM_DCOLS;
+constexpr int HM_ITEM_SHIFT = 5, HM_ITEM_MASK = 31, HM_ITEM_LOWRANK = 31;
 constexpr int HM_SKETCH_PARTS = 2;  // wavefronts per 16-row block of the sketch (cluster ranges)
 struct HmatPlan {
     int ncl = 0;
@@ -128,11 +135,17 @@ void launch_hm_fill(const float *G, const float *H, size_t ld, const uint8_t *gm
                     const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
                     float *fscale, const void *Q16, void *B16, void *D16, const int *doff,
                     hipStream_t s);
+// Product work lists from the ranks (host copies): for row cluster I and chunk k <
+// hm_chunks(ncl), items[item_ptr[I chunks + k] .. item_ptr[I chunks + k + 1]) are the
+// work items of row cluster I (low-rank blocks and column parts of dense blocks), spread
+// over its chunks by weight (bytes read), low-rank ones first
+void hm_plan_items(int ncl, const uint8_t *rank, const int *cl_size, std::vector<int> &item_ptr,
+                   std::vector<int> &items);
 void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
                     const float *fscale, const void *Q16, const void *B16, const void *D16,
-                    const int *doff, int n, const float *x, const double *diag,
-                    const float *xd, float *T, float *ypart, size_t ldp, float *y,
-                    hipStream_t s);
+                    const int *doff, const int *item_ptr, const int *items, int n,
+                    const float *x, const double *diag, const float *xd, float *T, float *ypart,
+                    size_t ldp, float *y, hipStream_t s);
 
 // Work space of the entry points that allocate nothing (wb_noalloc.hip): a static device
 // array of 2 NOALLOC_MAX_N doubles per GPU.  noalloc_setup looks its address up on a device
@@ -255,6 +268,15 @@ void launch_block_jacobi_apply(int is_float, int n, int bs, const void *binv, co
 // usable.  Apply (k vectors, column stride ldx): C[v] = P^T x_v (CS_MAX doubles per vector)
 // and Tout_v = x_v - W C[v]; the block-Jacobi apply then adds P Aci C[v].
 constexpr int CS_MAX = 192, CS_MAX64 = 128;
+// C = P^T x alone (k vectors)
+void launch_coarse_restrict(int is_float, int n, int agg, int nc, const void *X, size_t ldx,
+                            double *C, int k, hipStream_t s);
+// One vector: y = B^-1 (x - W C) + P Aci C in one kernel (x and y may alias); usable when
+// bj_coarse_fits(bs, agg, nc)
+bool bj_coarse_fits(int bs, int agg, int nc);
+void launch_bj_coarse(int is_float, int n, int bs, const void *binv, const void *x, void *y,
+                      const void *WT, size_t ldw, const double *C, const void *Aci, int nc,
+                      int agg, hipStream_t s);
 size_t coarse_invert_lds_bytes(int nc);
 void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
                          size_t ld, const void *A16, const float *rs16, const double *alpha,
@@ -307,10 +329,13 @@ void launch_dot_partials(int is_float, const void *V, size_t ldv, int kcount, co
 // h[b*sH + k] = sum over chunks of the partials P (k < kcount)
 void launch_reduce_parts(int kcount, const double *P, int n, double *h, hipStream_t s,
                          int batch = 1, size_t sH = 0, const int *active = nullptr);
+// Pin: h is reduced here from the previous pass's chunk partials (h unused), workgroup 0
+// storing it to hsave
 void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, const double *h,
                        void *z, int n, double *Pout, double *normp, int want_norm, hipStream_t s,
                        int batch = 1, size_t sV = 0, size_t sZ = 0, size_t sH = 0,
-                       const int *active = nullptr);
+                       const int *active = nullptr, const double *Pin = nullptr,
+                       double *hsave = nullptr);
 // hmap/hseq/seq: also publish hout (kcount + 1 values, single instance) to coherent pinned
 // host memory, then seq to *hseq
 void launch_finalize(int is_float, int kcount, const double *h1, const double *h2,

@@ -8,10 +8,11 @@
 //
 // Orthogonalization is classical Gram-Schmidt with one re-orthogonalization pass (CGS2):
 //   h1 = V^T z, z -= V h1, h2 = V^T z, z -= V h2, h = h1 + h2, s = |z|, v_new = z / s
-// (dot_partials, reduce_parts, update_dot, reduce_parts, update_dot, finalize); it spans
-// the same Krylov space as the modified Gram-Schmidt of deal.II's SolverGMRES with two
-// global reductions per step.  Chunk partials are summed once, by one small workgroup
-// (reduce_parts), not redundantly by every chunk of the next pass.
+// (dot_partials, then update_dot twice, then finalize); it spans the same Krylov space as
+// the modified Gram-Schmidt of deal.II's SolverGMRES with two global reductions per step.
+// The chunk partials of a pass are summed by every workgroup of the next kernel, in one
+// fixed order (reduce_partials), which saves a launch per reduction; the batched solver
+// may instead sum them once with reduce_parts.
 #include "wb_internal.h"
 
 namespace wb {
@@ -92,7 +93,40 @@ __global__ __launch_bounds__(KTHREADS) void k_reduce_parts(int kcount, const dou
     }
 }
 
-// z -= V h (h reduced);  then optional partial dots (Pout) and partial |z|^2
+// h[k] = sum over the nparts chunk partials P[p][k] (k < kcount), by one workgroup:
+// with kw = kcount rounded up to a power of two, thread (k, q) = (t % kw, t / kw) sums
+// every (KTHREADS / kw)-th partial from q on, and the KTHREADS / kw sums of each k are then
+// added in q order.  The order depends only on kcount and nparts, so every workgroup that
+// runs it gets the same bits.
+__device__ __forceinline__ void reduce_partials(int kcount, const double *__restrict__ P,
+                                                int nparts, double *red /*LDS, KTHREADS*/,
+                                                double *h /*LDS, KMAX*/)
+{
+    int kw = 1;
+    while (kw < kcount) kw <<= 1;
+    const int nq = KTHREADS / kw;
+    const int k = threadIdx.x % kw, q = threadIdx.x / kw;
+    double s0 = 0.0, s1 = 0.0;
+    if (k < kcount) {
+        int p = q;
+        for (; p + nq < nparts; p += 2 * nq) {
+            s0 += P[(size_t)p * KMAX + k];
+            s1 += P[(size_t)(p + nq) * KMAX + k];
+        }
+        if (p < nparts) s0 += P[(size_t)p * KMAX + k];
+    }
+    red[threadIdx.x] = s0 + s1;
+    __syncthreads();
+    if ((int)threadIdx.x < kcount) {
+        double t = red[threadIdx.x];
+        for (int j = 1; j < nq; ++j) t += red[j * kw + threadIdx.x];
+        h[threadIdx.x] = t;
+    }
+}
+
+// z -= V h;  then optional partial dots (Pout) and partial |z|^2.  h is hin (already
+// reduced), or, with Pin, formed here from the nparts chunk partials of the previous pass
+// by every workgroup (reduce_partials), workgroup 0 also storing it to hsave.
 template <typename T>
 __global__ __launch_bounds__(KTHREADS) void k_update_dot(const T *__restrict__ V, size_t ldv,
                                                          size_t sV, int kcount,
@@ -102,23 +136,33 @@ __global__ __launch_bounds__(KTHREADS) void k_update_dot(const T *__restrict__ V
                                                          double *__restrict__ Pout,
                                                          double *__restrict__ normp,
                                                          int want_norm,
-                                                         const int *__restrict__ active)
+                                                         const int *__restrict__ active,
+                                                         const double *__restrict__ Pin,
+                                                         double *__restrict__ hsave)
 {
     __shared__ double h[KMAX];
     __shared__ T zs[KCHUNK];
     __shared__ double red[KTHREADS / 64];
+    __shared__ double rpart[KTHREADS];
     const int b = blockIdx.y;
     if (active && !active[b]) return;
     const int nparts = gridDim.x;
     V += b * sV;
     z += b * sZ;
-    hin += b * sH;
     if (Pout) Pout += (size_t)b * nparts * KMAX;
     if (normp) normp += (size_t)b * nparts;
     const int base = blockIdx.x * KCHUNK;
     const int m = min(KCHUNK, n - base);
-    for (int k = threadIdx.x; k < kcount; k += KTHREADS) h[k] = hin[k];
-    __syncthreads();
+    if (Pin) {
+        reduce_partials(kcount, Pin + (size_t)b * nparts * KMAX, nparts, rpart, h);
+        __syncthreads();
+        if (blockIdx.x == 0)
+            for (int k = threadIdx.x; k < kcount; k += KTHREADS) hsave[b * sH + k] = h[k];
+    } else {
+        hin += b * sH;
+        for (int k = threadIdx.x; k < kcount; k += KTHREADS) h[k] = hin[k];
+        __syncthreads();
+    }
     double nrm = 0.0;
     for (int e = threadIdx.x; e < KCHUNK; e += KTHREADS) {
         T v = (T)0;
@@ -281,17 +325,18 @@ void launch_reduce_parts(int kcount, const double *P, int n, double *h, hipStrea
 
 void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, const double *h,
                        void *z, int n, double *Pout, double *normp, int want_norm, hipStream_t s,
-                       int batch, size_t sV, size_t sZ, size_t sH, const int *active)
+                       int batch, size_t sV, size_t sZ, size_t sH, const int *active,
+                       const double *Pin, double *hsave)
 {
     dim3 grid(krylov_partials_blocks(n), batch);
     if (is_float)
         hipLaunchKernelGGL(k_update_dot<float>, grid, dim3(KTHREADS), 0, s, (const float *)V,
                            ldv, sV, kcount, h, sH, (float *)z, sZ, n, Pout, normp, want_norm,
-                           active);
+                           active, Pin, hsave);
     else
         hipLaunchKernelGGL(k_update_dot<double>, grid, dim3(KTHREADS), 0, s, (const double *)V,
                            ldv, sV, kcount, h, sH, (double *)z, sZ, n, Pout, normp, want_norm,
-                           active);
+                           active, Pin, hsave);
 }
 
 void launch_finalize(int is_float, int kcount, const double *h1, const double *h2,
