@@ -130,7 +130,10 @@ struct wb_ctx {
     // factors
     HmatPlan hmp;
     DevBuf hm_idx, hm_size, hm_seg, hm_near, hm_rank, hm_scale, hm_Y, hm_Mx, hm_Q, hm_B, hm_D,
-        hm_doff, hm_T, hm_part, hm_iptr, hm_items;
+        hm_doff, hm_T, hm_part, hm_iptr, hm_items, hm_pos;
+    // the FP32 vector whose cluster-order copy hm_T holds (written by the GMRES step that
+    // produced it), or nullptr
+    const void *hm_xc_for = nullptr;
     long hm_assembly_ver = -1, hm_mask_ver = -1;
     int hm_agg = 0;         // aggregate size of the coarse sums written by the build (0: none)
     int float_mode = 1;  // operator mode of the FP32-vector solves: 1 = FP32 hi, 3 = FP16,
@@ -189,7 +192,7 @@ struct wb_ctx {
             &inc_item, &pj_emass, &pj_erhs, &pj_u, &pj_psi, &pj_B, &pj_X, &pj_R, &pj_Z, &pj_P,
             &pj_Q, &pj_dinv, &pj_part, &pj_part2, &pj_state, &A16, &rs16, &hm_idx, &hm_size, &hm_seg,
             &hm_near, &hm_rank, &hm_scale, &hm_Y, &hm_Mx, &hm_Q, &hm_B, &hm_D, &hm_doff, &hm_T,
-            &hm_part, &hm_iptr, &hm_items};
+            &hm_part, &hm_iptr, &hm_items, &hm_pos};
         for (DevBuf *b : mw.all()) v.push_back(b);
         return v;
     }
@@ -375,9 +378,15 @@ static void finalize_publish(wb_ctx *c, int is_float, int kcount, const double *
                              unsigned long long seq)
 {
     double *slot = c->hmap_dev + (seq & 1) * HMAP_STRIDE;
+    // with the compressed inner operator, the new basis vector also goes to the product's
+    // cluster-order copy, so its product needs no gather
+    const bool xc = is_float && c->float_mode == 4 && c->hm_T.bytes >= sizeof(float) *
+                                                                         (size_t)c->hmp.ncl * HM_CL;
     launch_finalize(is_float, kcount, h1, h2, c->Pn.as<double>(), z, n, vout,
                     c->hbuf.as<double>(), c->stream, 1, 0, 0, 0, 0, nullptr, slot,
-                    (unsigned long long *)(slot + HMAP_STRIDE - 1), seq);
+                    (unsigned long long *)(slot + HMAP_STRIDE - 1), seq,
+                    xc ? c->hm_T.as<float>() : nullptr, xc ? c->hm_pos.as<int>() : nullptr);
+    c->hm_xc_for = xc ? vout : nullptr;
 }
 
 // The device's publication of a GMRES column (k_finalize with hmap): waits until the
@@ -425,12 +434,14 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
     if (mode == 4) {
         require(which == 0, WB_E_STATE, "the compressed operator holds the vmult selection only");
         KernelScope ks(c, WB_K_GEMVC, c->n);
+        const bool ready = x == c->hm_xc_for;
+        c->hm_xc_for = nullptr;
         launch_hm_gemv(c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hmp.ncl,
                        c->hm_rank.as<uint8_t>(), c->hm_scale.as<float>(), c->hm_Q.p, c->hm_B.p,
                        c->hm_D.p, c->hm_doff.as<int>(), c->hm_iptr.as<int>(),
                        c->hm_items.as<int>(), c->n, (const float *)x, diag, (const float *)xd,
                        c->hm_T.as<float>(), c->hm_part.as<float>(), c->vlen, (float *)y,
-                       c->stream);
+                       c->stream, ready);
         return;
     }
     if (mode == 3) {
@@ -856,7 +867,11 @@ static void ensure_hm(wb_ctx *c)
     c->hm_Mx.ensure(sizeof(float) * (size_t)c->n * ncl);
     c->hm_Q.ensure(2 * nb * HM_RMAX * HM_CL);
     c->hm_B.ensure(2 * nb * HM_RMAX * HM_CL);
-    c->hm_T.ensure(sizeof(float) * nb * HM_RMAX);
+    // the product's x in cluster order; zero at the padding positions (p >= |J|), which
+    // only the gather writes
+    c->hm_T.ensure(sizeof(float) * ncl * HM_CL);
+    HIPCHK(hipMemsetAsync(c->hm_T.p, 0, sizeof(float) * ncl * HM_CL, c->stream));
+    c->hm_xc_for = nullptr;
     c->hm_part.ensure(sizeof(float) * c->vlen * hm_chunks(c->hmp.ncl));
     c->hm_doff.ensure(sizeof(int) * nb);
     if (agg > 0) c->cs_S.ensure(sizeof(double) * HM_SKETCH_PARTS * (size_t)nc * c->vlen);
@@ -1686,7 +1701,14 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
             upload(c->hm_idx, c->hmp.cl_idx.data(), c->hmp.cl_idx.size(), s);
             upload(c->hm_size, c->hmp.cl_size.data(), c->hmp.cl_size.size(), s);
             upload(c->hm_near, c->hmp.near.data(), c->hmp.near.size(), s);
+            // row -> its position J HM_CL + p in the cluster order
+            std::vector<int> pos(std::max(c->n, 1), -1);  // -1: in no cluster
+            for (int J = 0; J < c->hmp.ncl; ++J)
+                for (int q = 0; q < c->hmp.cl_size[J]; ++q)
+                    pos[c->hmp.cl_idx[(size_t)J * HM_CL + q]] = J * HM_CL + q;
+            upload(c->hm_pos, pos.data(), pos.size(), s);
         }
+        c->hm_xc_for = nullptr;
         c->hm_assembly_ver = c->hm_mask_ver = -1;
         c->hm_agg = 0;
         if (p.n_tasks > 0) {

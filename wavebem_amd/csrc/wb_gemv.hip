@@ -133,7 +133,18 @@ template <int MODE> struct MixedShape;
 template <> struct MixedShape<1> {
     static constexpr int RPW = WB_GEMV32_RPW, WAVES = 16 / WB_GEMV32_RPW;
 };
-template <> struct MixedShape<2> { static constexpr int RPW = 4, WAVES = 4; };
+#ifndef WB_GEMVHL_RPW
+#define WB_GEMVHL_RPW 4
+#endif
+#ifndef WB_GEMVHL_WAVES
+#define WB_GEMVHL_WAVES 4
+#endif
+#ifndef WB_GEMVHL_UNROLL
+#define WB_GEMVHL_UNROLL 2
+#endif
+template <> struct MixedShape<2> {
+    static constexpr int RPW = WB_GEMVHL_RPW, WAVES = WB_GEMVHL_WAVES;
+};
 
 template <int MODE>
 __global__ __launch_bounds__(64 * MixedShape<MODE>::WAVES) void k_gemv_mixed(
@@ -200,11 +211,12 @@ __global__ __launch_bounds__(64 * MixedShape<MODE>::WAVES) void k_gemv_mixed(
                 }
             };
             int g = sg.x + lane;
-            for (; g + 64 < sg.y; g += 128) {
-                step(g);
-                step(g + 64);
+            constexpr int UN = MODE == 2 ? WB_GEMVHL_UNROLL : 2;  // steps with loads in flight
+            for (; g + 64 * (UN - 1) < sg.y; g += 64 * UN) {
+#pragma unroll
+                for (int u = 0; u < UN; ++u) step(g + 64 * u);
             }
-            if (g < sg.y) step(g);
+            for (; g < sg.y; g += 64) step(g);
         } else {
             const unsigned fm = flip ? 0xfu : 0u;
             for (int g = sg.x + lane; g < sg.y; g += 64) {

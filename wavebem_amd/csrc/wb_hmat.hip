@@ -17,8 +17,9 @@
 //   * not at all when it is zero (rank 0).
 // Build per assembly (and mask change): k_hm_sketch (one pass over the FP32 hi rows),
 // k_hm_qr (one wavefront per block), k_hm_fill (a second pass over the rows of each block).
-// Product: k_hm_gemv (one wavefront per 64 rows of a row cluster and a chunk of column
-// clusters) and k_hm_finish (chunk sums, the alpha diagonal).
+// Product: k_hm_gather (x in cluster order), k_hm_gemv (one wavefront per row cluster and
+// chunk of its work items: low-rank blocks and column parts of dense blocks) and
+// k_hm_finish (chunk sums, the alpha diagonal).
 #include <algorithm>
 #include <functional>
 #include <vector>
@@ -508,65 +509,36 @@ __global__ __launch_bounds__(64) void k_hm_fill(const float *__restrict__ G,
         }
 }
 
-// Phase 1 of the product: T[b][k] = sum_j B16[b][k][j] x_J(j) (unscaled) for every
-// low-rank block b = (I, J); one wavefront per (column cluster J, chunk of row clusters),
-// lane l holding x_J at the block columns 4 l .. 4 l + 3.
-__global__ __launch_bounds__(64) void k_hm_gemv_t(const int *__restrict__ cl_idx,
-                                                  const int *__restrict__ cl_size, int ncl,
-                                                  int chunks, const uint8_t *__restrict__ rank,
-                                                  const _Float16 *__restrict__ B16,
-                                                  const float *__restrict__ x,
-                                                  float *__restrict__ T)
+// Phase 1 of the product: x in cluster order, xc[J][p] = x(cl_idx[J][p]) (0 past |J|), so
+// that the work items read x_J with one contiguous load.
+__global__ __launch_bounds__(256) void k_hm_gather(const int *__restrict__ cl_idx,
+                                                   const int *__restrict__ cl_size,
+                                                   const float *__restrict__ x,
+                                                   float *__restrict__ xc)
 {
-    const int lane = threadIdx.x;
-    const int J = blockIdx.x, ch = blockIdx.y;
-    const int szJ = cl_size[J];
-    fvec4 xv;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int p = 4 * lane + c;
-        xv[c] = p < szJ ? x[cl_idx[(size_t)J * HM_CL + p]] : 0.0f;
-    }
-    // row clusters ch, ch + chunks, ...: the blocks of a chunk are spread over the row
-    for (int I = ch; I < ncl; I += chunks) {
-        const size_t b = (size_t)I * ncl + J;
-        const int r = __builtin_amdgcn_readfirstlane(rank[b]);
-        if (r == 0 || r == HM_DENSE) continue;
-        const _Float16 *B = B16 + b * HM_RMAX * HM_CL + 4 * lane;
-        float t[HM_RMAX];
-#pragma unroll
-        for (int k = 0; k < HM_RMAX; ++k)
-            if (k < r) {
-                const hvec4 h = *(const hvec4 *)(B + (size_t)k * HM_CL);
-                t[k] = (float)h[0] * xv[0] + (float)h[1] * xv[1] + (float)h[2] * xv[2] +
-                       (float)h[3] * xv[3];
-            }
-#pragma unroll
-        for (int k = 0; k < HM_RMAX; ++k)
-            if (k < r) t[k] = wave_reduce63<false>(t[k]);
-        if (lane == 63)
-#pragma unroll
-            for (int k = 0; k < HM_RMAX; ++k)
-                if (k < r) T[b * HM_RMAX + k] = t[k];
-    }
+    const int J = blockIdx.x, p = threadIdx.x;
+    xc[(size_t)J * HM_CL + p] = p < cl_size[J] ? x[cl_idx[(size_t)J * HM_CL + p]] : 0.0f;
 }
 
 // Phase 2: one wavefront per (row cluster I, chunk ch), lane l holding the block rows
 // 4 l .. 4 l + 3: ypart[ch][row] = sum over the chunk's work items of (A~_IJ x_J)(row).
 // The items (built on the host from the ranks, hm_plan_items) are the low-rank blocks of
-// row cluster I, applied as fscale Q16 T (8-byte loads of four rows of a Q column, two
-// blocks at a time), and HM_DCOLS-column parts of its dense blocks, streamed column by
-// column in batches of WB_HM_BATCH columns (independent 512-byte loads per wavefront), so
-// no wavefront walks a whole dense block alone.
+// row cluster I and HM_DCOLS-column parts of its dense blocks.  A low-rank block is
+// applied as fscale Q16 (B16 x_J): lane l forms its part of each of the r sums
+// t_k = B16[k] . x_J from the block columns 4 l .. 4 l + 3, the wavefront sums them (DPP)
+// and every lane adds q_k(rows) t_k; the B16 and Q16 loads (8 bytes per lane and column)
+// are issued together.  Dense parts are streamed column by column in batches of
+// WB_HM_BATCH columns (independent 512-byte loads per wavefront), so no wavefront walks a
+// whole dense block alone.
 __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
                                                 const int *__restrict__ cl_size, int ncl,
                                                 int chunks, const uint8_t *__restrict__ rank,
                                                 const float *__restrict__ fscale,
                                                 const _Float16 *__restrict__ Q16,
+                                                const _Float16 *__restrict__ B16,
                                                 const _Float16 *__restrict__ D16,
                                                 const int *__restrict__ doff,
-                                                const float *__restrict__ T,
-                                                const float *__restrict__ x, size_t ldp,
+                                                const float *__restrict__ xc, size_t ldp,
                                                 float *__restrict__ ypart,
                                                 const int *__restrict__ item_ptr,
                                                 const int *__restrict__ items)
@@ -580,36 +552,34 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
     auto lowrank = [&](int J) {
         const size_t b = (size_t)I * ncl + J;
         const int r = __builtin_amdgcn_readfirstlane(rank[b]);
+        const fvec4 xv = *(const fvec4 *)(xc + (size_t)J * HM_CL + 4 * lane);
+        const _Float16 *Bp = B16 + b * HM_RMAX * HM_CL + 4 * lane;
         const _Float16 *Q = Q16 + b * HM_RMAX * HM_CL + 4 * lane;
-        const float *t = T + b * HM_RMAX;
-        hvec4 q[HM_RMAX];
+        hvec4 bk[HM_RMAX], q[HM_RMAX];
 #pragma unroll
         for (int k = 0; k < HM_RMAX; ++k)
-            if (k < r) q[k] = *(const hvec4 *)(Q + (size_t)k * HM_CL);
+            if (k < r) {
+                bk[k] = *(const hvec4 *)(Bp + (size_t)k * HM_CL);
+                q[k] = *(const hvec4 *)(Q + (size_t)k * HM_CL);
+            }
         fvec4 sv = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int k = 0; k < HM_RMAX; ++k)
             if (k < r) {
-                const float tk = t[k];
+                const float part = (float)bk[k][0] * xv[0] + (float)bk[k][1] * xv[1] +
+                                   (float)bk[k][2] * xv[2] + (float)bk[k][3] * xv[3];
+                const float tk = lane63(wave_reduce63<false>(part));
 #pragma unroll
                 for (int c = 0; c < 4; ++c) sv[c] = fmaf((float)q[k][c], tk, sv[c]);
             }
         return sv * fscale[b];
     };
     int p = p0;
-    // low-rank items (listed first), two at a time so that their loads are in flight together
-    for (; p < p1; p += 2) {
-        const int it0 = __builtin_amdgcn_readfirstlane(items[p]);
-        if ((it0 & HM_ITEM_MASK) != HM_ITEM_LOWRANK) break;
-        const int it1 = p + 1 < p1 ? __builtin_amdgcn_readfirstlane(items[p + 1]) : 0;
-        const bool two = p + 1 < p1 && (it1 & HM_ITEM_MASK) == HM_ITEM_LOWRANK;
-        const fvec4 s0 = lowrank(it0 >> HM_ITEM_SHIFT);
-        const fvec4 s1 = two ? lowrank(it1 >> HM_ITEM_SHIFT) : fvec4{0.f, 0.f, 0.f, 0.f};
-        yacc += s0 + s1;
-        if (!two) {
-            ++p;
-            break;
-        }
+    // low-rank items (listed first)
+    for (; p < p1; ++p) {
+        const int it = __builtin_amdgcn_readfirstlane(items[p]);
+        if ((it & HM_ITEM_MASK) != HM_ITEM_LOWRANK) break;
+        yacc += lowrank(it >> HM_ITEM_SHIFT);
     }
     // dense parts: columns HM_DCOLS q .. HM_DCOLS (q + 1) - 1 of block (I, J), column-major
     // storage; lanes 0 .. HM_DCOLS / 4 - 1 hold x_J at the part's columns 4 l .. 4 l + 3
@@ -624,12 +594,8 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
         const float fs = fscale[b];
         const int szJ = __builtin_amdgcn_readfirstlane(cl_size[J]);
         const int cb = HM_DCOLS * q, ce = min(cb + HM_DCOLS, szJ);
-        fvec4 xv;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int pc = cb + 4 * lane + c;
-            xv[c] = (lane < HM_DCOLS / 4 && pc < ce) ? x[cl_idx[(size_t)J * HM_CL + pc]] : 0.0f;
-        }
+        fvec4 xv = {0.f, 0.f, 0.f, 0.f};
+        if (lane < HM_DCOLS / 4) xv = *(const fvec4 *)(xc + (size_t)J * HM_CL + cb + 4 * lane);
         const _Float16 *D = D16 + (size_t)doff[b] * HM_CL * HM_CL + 4 * lane;
         fvec4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0;
         for (int c0 = cb; c0 < ce; c0 += NB) {
@@ -701,7 +667,6 @@ int hm_chunks(int ncl)
     return std::max(1, std::min(128, c));
 }
 
-static int hm_tchunks(int ncl) { return hm_chunks(ncl); }
 
 void launch_hm_rank(const float *G, const float *H, size_t ld, int n, const uint8_t *gmask,
                     const int *cl_idx, const int *cl_size, const int *cl_seg, int ncl,
@@ -793,15 +758,15 @@ void hm_plan_items(int ncl, const uint8_t *rank, const int *cl_size, std::vector
 void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
                     const float *fscale, const void *Q16, const void *B16, const void *D16,
                     const int *doff, const int *item_ptr, const int *items, int n,
-                    const float *x, const double *diag, const float *xd, float *T, float *ypart,
-                    size_t ldp, float *y, hipStream_t s)
+                    const float *x, const double *diag, const float *xd, float *xc, float *ypart,
+                    size_t ldp, float *y, hipStream_t s, bool xc_ready)
 {
-    const int ch = hm_chunks(ncl), tch = hm_tchunks(ncl);
-    hipLaunchKernelGGL(k_hm_gemv_t, dim3(ncl, tch), dim3(64), 0, s, cl_idx, cl_size, ncl, tch,
-                       rank, (const _Float16 *)B16, x, T);
+    const int ch = hm_chunks(ncl);
+    if (!xc_ready)
+        hipLaunchKernelGGL(k_hm_gather, dim3(ncl), dim3(HM_CL), 0, s, cl_idx, cl_size, x, xc);
     hipLaunchKernelGGL(k_hm_gemv, dim3(ncl, ch), dim3(64), 0, s, cl_idx, cl_size, ncl, ch,
-                       rank, fscale, (const _Float16 *)Q16, (const _Float16 *)D16, doff, T, x,
-                       ldp, ypart, item_ptr, items);
+                       rank, fscale, (const _Float16 *)Q16, (const _Float16 *)B16,
+                       (const _Float16 *)D16, doff, xc, ldp, ypart, item_ptr, items);
     hipLaunchKernelGGL(k_hm_finish, dim3((n + 63) / 64), dim3(256), 0, s, ypart, ldp, ch, n,
                        diag, xd, y);
 }

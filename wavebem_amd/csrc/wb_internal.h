@@ -119,8 +119,9 @@ void plan_hmat(const MeshPlan &P, const double *X_internal, HmatPlan &H);
 // fscale (ncl^2), Q16 / B16 (ncl^2 x HM_RMAX x HM_CL halfs) and D16 (ncl^2 x HM_CL x HM_CL
 // halfs) hold the operator.  With S (agg >= HM_SEG; HM_SKETCH_PARTS x nagg x ldS doubles of
 // work space), the build also writes S[a ldS + i] = the sum of row i of the FP32 operator
-// over the columns of aggregate a (a < nagg), the coarse space's A P.  Product: y = A~ x + diag .* xd (FP32 vectors, all n rows); T
-// (ncl^2 x HM_RMAX) and ypart (hm_chunks(ncl) x ldp) floats are work space.
+// over the columns of aggregate a (a < nagg), the coarse space's A P.  Product: y = A~ x + diag .* xd (FP32 vectors, all n rows); xc
+// (ncl x HM_CL, x in cluster order; with xc_ready already holding it) and ypart
+// (hm_chunks(ncl) x ldp) floats are work space.
 int hm_chunks(int ncl);
 // build, in two launches with the host between them: launch_hm_rank (sketch, aggregate
 // sums, rank test, Q) writes rank[]; the caller lays out the dense blocks (doff[b]: the
@@ -144,8 +145,8 @@ void hm_plan_items(int ncl, const uint8_t *rank, const int *cl_size, std::vector
 void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
                     const float *fscale, const void *Q16, const void *B16, const void *D16,
                     const int *doff, const int *item_ptr, const int *items, int n,
-                    const float *x, const double *diag, const float *xd, float *T, float *ypart,
-                    size_t ldp, float *y, hipStream_t s);
+                    const float *x, const double *diag, const float *xd, float *xc, float *ypart,
+                    size_t ldp, float *y, hipStream_t s, bool xc_ready = false);
 
 // Work space of the entry points that allocate nothing (wb_noalloc.hip): a static device
 // array of 2 NOALLOC_MAX_N doubles per GPU.  noalloc_setup looks its address up on a device
@@ -337,12 +338,13 @@ void launch_update_dot(int is_float, const void *V, size_t ldv, int kcount, cons
                        const int *active = nullptr, const double *Pin = nullptr,
                        double *hsave = nullptr);
 // hmap/hseq/seq: also publish hout (kcount + 1 values, single instance) to coherent pinned
-// host memory, then seq to *hseq
+// host memory, then seq to *hseq; xcl/pos (FP32, single instance): also xcl[pos[i]] = vout[i]
 void launch_finalize(int is_float, int kcount, const double *h1, const double *h2,
                      const double *Pn, void *z, int n, void *vout, double *hout, hipStream_t s,
                      int batch = 1, size_t sZ = 0, size_t sV = 0, size_t sHin = 0,
                      size_t sH = 0, const int *active = nullptr, double *hmap = nullptr,
-                     unsigned long long *hseq = nullptr, unsigned long long seq = 0);
+                     unsigned long long *hseq = nullptr, unsigned long long seq = 0,
+                     float *xcl = nullptr, const int *pos = nullptr);
 void launch_axpy_basis(int is_float, const void *V, size_t ldv, int kcount, const double *y,
                        void *x_acc, int x_is_float, int n, hipStream_t s, int batch = 1,
                        size_t sV = 0, size_t sY = 0, size_t sX = 0, const int *active = nullptr);

@@ -194,6 +194,7 @@ __global__ __launch_bounds__(KTHREADS) void k_update_dot(const T *__restrict__ V
 // With hmap (coherent pinned host memory, single instance), the column also goes to the
 // host: every thread of workgroup 0 writes its entries and fences at system scope, then
 // thread 0 publishes seq after them, so a host that reads seq (acquire) sees the column.
+// With xcl (single instance), vout is also written to xcl[pos[i]] (FP32).
 template <typename T>
 __global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double *__restrict__ h1,
                                                        const double *__restrict__ h2,
@@ -205,7 +206,9 @@ __global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double 
                                                        const int *__restrict__ active,
                                                        double *__restrict__ hmap,
                                                        unsigned long long *__restrict__ hseq,
-                                                       unsigned long long seq)
+                                                       unsigned long long seq,
+                                                       float *__restrict__ xcl,
+                                                       const int *__restrict__ pos)
 {
     __shared__ double snorm;
     const int b = blockIdx.y;
@@ -246,7 +249,14 @@ __global__ __launch_bounds__(KTHREADS) void k_finalize(int kcount, const double 
         const double inv = nv > 0 ? 1.0 / nv : 0.0;
         const int base = blockIdx.x * KCHUNK;
         const int m = min(KCHUNK, n - base);
-        for (int e = threadIdx.x; e < m; e += KTHREADS) vout[base + e] = (T)((double)z[base + e] * inv);
+        for (int e = threadIdx.x; e < m; e += KTHREADS) {
+            const T v = (T)((double)z[base + e] * inv);
+            vout[base + e] = v;
+            if (xcl) {
+                const int ps = pos[base + e];
+                if (ps >= 0) xcl[ps] = (float)v;
+            }
+        }
     }
 }
 
@@ -343,17 +353,17 @@ void launch_finalize(int is_float, int kcount, const double *h1, const double *h
                      const double *Pn, void *z, int n, void *vout, double *hout, hipStream_t s,
                      int batch, size_t sZ, size_t sV, size_t sHin, size_t sH,
                      const int *active, double *hmap, unsigned long long *hseq,
-                     unsigned long long seq)
+                     unsigned long long seq, float *xcl, const int *pos)
 {
     dim3 grid(krylov_partials_blocks(n), batch);
     if (is_float)
         hipLaunchKernelGGL(k_finalize<float>, grid, dim3(KTHREADS), 0, s, kcount, h1, h2, sHin,
                            Pn, (const float *)z, sZ, n, (float *)vout, sV, hout, sH, active,
-                           hmap, hseq, seq);
+                           hmap, hseq, seq, xcl, pos);
     else
         hipLaunchKernelGGL(k_finalize<double>, grid, dim3(KTHREADS), 0, s, kcount, h1, h2, sHin,
                            Pn, (const double *)z, sZ, n, (double *)vout, sV, hout, sH, active,
-                           hmap, hseq, seq);
+                           hmap, hseq, seq, (float *)nullptr, pos);
 }
 
 void launch_axpy_basis(int is_float, const void *V, size_t ldv, int kcount, const double *y,
