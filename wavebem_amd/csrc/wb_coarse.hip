@@ -309,7 +309,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_bj_coarse(int n, int bs, const T *__restrict__ binv,
                                                    const T *x, T *y, const T *__restrict__ WT,
                                                    size_t ldw, const double *__restrict__ C,
-                                                   const T *__restrict__ Aci, int nc, int agg)
+                                                   const T *__restrict__ Aci, int nc, int agg,
+                                                   const double *__restrict__ Cpart)
 {
     __shared__ double cs[CS_MAX];
     __shared__ double qs[4][64];
@@ -318,7 +319,19 @@ __global__ __launch_bounds__(256) void k_bj_coarse(int n, int bs, const T *__res
     __shared__ double zs[4];
     const int tid = threadIdx.x, rl = tid & 63, w = tid >> 6;
     const int base = blockIdx.x * bs, m = min(bs, n - base);
-    for (int b = tid; b < nc; b += 256) cs[b] = C[b];
+    for (int b = tid; b < nc; b += 256) {
+        double v;
+        if (Cpart) {
+            // aggregate b from the partial sums of the 64-row blocks it overlaps, in order
+            const int r0 = b * agg, r1 = min(n, (b + 1) * agg);
+            v = 0.0;
+            for (int blk = r0 / 64; blk <= (r1 - 1) / 64; ++blk)
+                v += Cpart[2 * blk + (b - (blk * 64) / agg)];
+        } else {
+            v = C[b];
+        }
+        cs[b] = v;
+    }
     __syncthreads();
     const int i = base + rl;
     const int b0 = w * nc / 4, b1 = (w + 1) * nc / 4;
@@ -454,17 +467,17 @@ bool bj_coarse_fits(int bs, int agg, int nc) { return bs >= 4 && bs <= 64 && agg
 
 void launch_bj_coarse(int is_float, int n, int bs, const void *binv, const void *x, void *y,
                       const void *WT, size_t ldw, const double *C, const void *Aci, int nc,
-                      int agg, hipStream_t s)
+                      int agg, hipStream_t s, const double *Cpart)
 {
     const int nb = (n + bs - 1) / bs;
     if (is_float)
         hipLaunchKernelGGL(k_bj_coarse<float>, dim3(nb), dim3(256), 0, s, n, bs,
                            (const float *)binv, (const float *)x, (float *)y, (const float *)WT,
-                           ldw, C, (const float *)Aci, nc, agg);
+                           ldw, C, (const float *)Aci, nc, agg, Cpart);
     else
         hipLaunchKernelGGL(k_bj_coarse<double>, dim3(nb), dim3(256), 0, s, n, bs,
                            (const double *)binv, (const double *)x, (double *)y,
-                           (const double *)WT, ldw, C, (const double *)Aci, nc, agg);
+                           (const double *)WT, ldw, C, (const double *)Aci, nc, agg, Cpart);
 }
 
 }  // namespace wb

@@ -106,6 +106,8 @@ struct wb_ctx {
     DevBuf surf_d, gmask, diag_op, diag_rhs, segs;
     int n_segs = 0;
     DevBuf con_rows, con_ptr, con_cols, con_w, con_inhom, constrained, free_rows;
+    DevBuf con_line;  // per row: its constraint line, -1 for free rows
+    DevBuf cs_Cpart;  // coarse restriction partials of the compressed product (64-row blocks)
     // transposed constraint lines, gathered by target column: y[tgt] += coef * x[src]
     DevBuf ct_tgt, ct_ptr, ct_src, ct_coef;
     int n_ct = 0;
@@ -184,6 +186,7 @@ struct wb_ctx {
             &iperm_d, &st_task, &st_gap, &st_gcell, &cell_int_s, &orient_s, &cellq_s, &G, &H,
             &Glo, &Hlo, &hsum_part, &hsum_sing, &alpha, &surf_d, &gmask, &segs, &diag_op,
             &diag_rhs, &con_rows, &con_ptr, &con_cols, &con_w, &con_inhom, &constrained,
+            &con_line, &cs_Cpart,
             &free_rows, &binv, &binv32, &cs_AP, &cs_Ac, &cs_Aci, &cs_Aci32, &cs_WT,
             &cs_WT32, &cs_status, &cs_C, &cs_T, &cs_PTA, &cs_WTT, &cs_S, &V, &z, &w, &P1, &P2,
             &Pn, &hbuf, &hred, &vx, &vb, &vr, &vt, &vs, &V32, &z32, &w32, &r32, &d32, &ct_tgt,
@@ -424,6 +427,25 @@ static double device_norm(wb_ctx *c, const void *x, int is_float)
 
 // ------------------------------------------------------------------ operator pieces
 // which: 0 = BEMProblem::vmult selection, 1 = compute_rhs selection
+// The compressed product y = A~ x + alpha .* x (mode 4); with lines, constrained rows get
+// their constraint line's value in the same launch (apply_constrained); with cparts, the
+// last kernel also leaves the coarse restriction partials of y in cs_Cpart.
+static void hm_product(wb_ctx *c, const void *x, void *y, bool lines, bool cparts)
+{
+    KernelScope ks(c, WB_K_GEMVC, c->n);
+    const bool ready = x == c->hm_xc_for;
+    c->hm_xc_for = nullptr;
+    if (cparts) c->cs_Cpart.ensure(sizeof(double) * 2 * (size_t)((c->n + 63) / 64));
+    launch_hm_gemv(c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hmp.ncl,
+                   c->hm_rank.as<uint8_t>(), c->hm_scale.as<float>(), c->hm_Q.p, c->hm_B.p,
+                   c->hm_D.p, c->hm_doff.as<int>(), c->hm_iptr.as<int>(), c->hm_items.as<int>(),
+                   c->n, (const float *)x, c->diag_op.as<double>(), (const float *)x,
+                   c->hm_T.as<float>(), c->hm_part.as<float>(), c->vlen, (float *)y, c->stream,
+                   ready, lines && c->n_lines > 0 ? c->con_line.as<int>() : nullptr,
+                   c->con_ptr.as<int>(), c->con_cols.as<int>(), c->con_w.as<double>(),
+                   cparts ? c->cs_agg : 64, cparts ? c->cs_Cpart.as<double>() : nullptr);
+}
+
 static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, void *y,
                  bool free_rows_only)
 {
@@ -433,15 +455,8 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
     const int nr = free_rows_only ? c->n_free : c->n;
     if (mode == 4) {
         require(which == 0, WB_E_STATE, "the compressed operator holds the vmult selection only");
-        KernelScope ks(c, WB_K_GEMVC, c->n);
-        const bool ready = x == c->hm_xc_for;
-        c->hm_xc_for = nullptr;
-        launch_hm_gemv(c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hmp.ncl,
-                       c->hm_rank.as<uint8_t>(), c->hm_scale.as<float>(), c->hm_Q.p, c->hm_B.p,
-                       c->hm_D.p, c->hm_doff.as<int>(), c->hm_iptr.as<int>(),
-                       c->hm_items.as<int>(), c->n, (const float *)x, diag, (const float *)xd,
-                       c->hm_T.as<float>(), c->hm_part.as<float>(), c->vlen, (float *)y,
-                       c->stream, ready);
+        require(xd == x, WB_E_STATE, "the compressed product takes its diagonal term from x");
+        hm_product(c, x, y, false, false);
         return;
     }
     if (mode == 3) {
@@ -459,7 +474,18 @@ static void gemv(wb_ctx *c, int mode, int which, const void *x, const void *xd, 
 
 // y = ConstrainedOperator(A) x in FP64 (mode 0 or 2) or FP32 vectors (mode 1: FP32
 // operator, mode 3: FP16 operator, mode 4: compressed operator)
-static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
+// whether the block-Jacobi apply can take C = P^T x from the compressed product's partials
+static bool coarse_parts_usable(const wb_ctx *c)
+{
+    return c->cfg.preconditioner == WB_PRECOND_BLOCK_JACOBI && c->cs_on && !c->op_transpose &&
+           bj_coarse_fits(c->pc_bs, c->cs_agg, c->cs_nc);
+}
+
+// y = A_c x.  With coarse_parts and the compressed operator, the product also leaves the
+// coarse restriction partials of y in cs_Cpart; returns whether it did (the caller then
+// hands them to the preconditioner of the same y, apply_precond(..., true)).
+static bool apply_constrained(wb_ctx *c, int mode, const void *x, void *y,
+                              bool coarse_parts = false)
 {
     const int is_float = mode == 1 || mode == 3 || mode == 4;
     if (c->op_transpose) {
@@ -477,7 +503,13 @@ static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
                      c->constrained.as<uint8_t>(), c->gt_part.as<double>(), y, c->stream);
         launch_constraint_T(is_float, c->n_ct, c->ct_tgt.as<int>(), c->ct_ptr.as<int>(),
                             c->ct_src.as<int>(), c->ct_coef.as<double>(), x, y, c->stream);
-        return;
+        return false;
+    }
+    if (mode == 4 && !c->pure_neumann) {
+        // the constraint lines (and the partials) in the product's last kernel
+        const bool parts = coarse_parts && coarse_parts_usable(c);
+        hm_product(c, x, y, true, parts);
+        return parts;
     }
     if (c->pure_neumann) {
         gemv(c, mode, 0, x, x, y, false);
@@ -501,11 +533,14 @@ static void apply_constrained(wb_ctx *c, int mode, const void *x, void *y)
     launch_constraint_rows(is_float, c->n_lines, c->con_rows.as<int>(), c->con_ptr.as<int>(),
                            c->con_cols.as<int>(), c->con_w.as<double>(),
                            c->con_inhom.as<double>(), 0, x, y, c->stream);
+    return false;
 }
 
 // Block Jacobi, with the coarse-space correction (forward operator only):
-// y = P z + B^-1 (x - AP z), z = Ac^-1 P^T x.  x and y may alias.
-static void block_precond(wb_ctx *c, int is_float, int k, const void *x, void *y)
+// y = P z + B^-1 (x - AP z), z = Ac^-1 P^T x.  x and y may alias.  cparts: cs_Cpart holds
+// the restriction partials of x (apply_constrained with coarse_parts returned true).
+static void block_precond(wb_ctx *c, int is_float, int k, const void *x, void *y,
+                          bool cparts = false)
 {
     const void *binv = is_float ? c->binv32.p : c->binv.p;
     if (c->cs_on && c->op_transpose && c->cs_t_on && !is_float) {
@@ -516,12 +551,13 @@ static void block_precond(wb_ctx *c, int is_float, int k, const void *x, void *y
                                   1, c->cs_C.as<double>(), c->cs_Aci.p, c->cs_nc, c->cs_agg);
     } else if (c->cs_on && !c->op_transpose && k == 1 &&
                bj_coarse_fits(c->pc_bs, c->cs_agg, c->cs_nc)) {
-        launch_coarse_restrict(is_float, c->n, c->cs_agg, c->cs_nc, x, c->vlen,
-                               c->cs_C.as<double>(), 1, c->stream);
+        if (!cparts)
+            launch_coarse_restrict(is_float, c->n, c->cs_agg, c->cs_nc, x, c->vlen,
+                                   c->cs_C.as<double>(), 1, c->stream);
         launch_bj_coarse(is_float, c->n, c->pc_bs, binv, x, y,
                          is_float ? c->cs_WT32.p : c->cs_WT.p, c->vlen, c->cs_C.as<double>(),
                          is_float ? c->cs_Aci32.p : c->cs_Aci.p, c->cs_nc, c->cs_agg,
-                         c->stream);
+                         c->stream, cparts ? c->cs_Cpart.as<double>() : nullptr);
     } else if (c->cs_on && !c->op_transpose) {
         launch_coarse_pre(is_float, c->n, c->cs_agg, c->cs_nc, x, c->vlen,
                           is_float ? c->cs_WT32.p : c->cs_WT.p, c->vlen, c->cs_C.as<double>(),
@@ -535,10 +571,10 @@ static void block_precond(wb_ctx *c, int is_float, int k, const void *x, void *y
     }
 }
 
-static void apply_precond(wb_ctx *c, int is_float, const void *x, void *y)
+static void apply_precond(wb_ctx *c, int is_float, const void *x, void *y, bool cparts = false)
 {
     if (c->cfg.preconditioner == WB_PRECOND_BLOCK_JACOBI) {
-        block_precond(c, is_float, 1, x, y);
+        block_precond(c, is_float, 1, x, y, cparts);
     } else {
         HIPCHK(hipMemcpyAsync(y, x, (is_float ? 4 : 8) * (size_t)c->n, hipMemcpyDeviceToDevice,
                               c->stream));
@@ -589,6 +625,11 @@ static void install_constraints(wb_ctx *c, const ConstraintSet &cs)
     upload(c->con_w, cs.w.data(), cs.w.size(), c->stream);
     upload(c->con_inhom, cs.inhom.data(), cs.inhom.size(), c->stream);
     upload(c->constrained, flag.data(), flag.size(), c->stream);
+    {
+        std::vector<int> line(std::max(n, 1), -1);
+        for (int l = 0; l < nl; ++l) line[rows[l]] = l;
+        upload(c->con_line, line.data(), line.size(), c->stream);
+    }
     std::vector<int> fr;
     fr.reserve(n);
     for (int i = 0; i < n; ++i)
@@ -750,8 +791,8 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
         // tolerance: the launches then overlap the wait.  A step queued for a check that
         // ends the cycle only writes v_{k+2} and work vectors, which nothing reads.
         auto enqueue_step = [&](int k) {
-            apply_constrained(c, mode, vec(k), w);
-            apply_precond(c, is_float, w, z);
+            const bool cparts = apply_constrained(c, mode, vec(k), w, true);
+            apply_precond(c, is_float, w, z, cparts);
             double *h1 = c->hred.as<double>(), *h2 = h1 + KMAX;
             launch_dot_partials(is_float, V.p, ldv, k + 1, z, n, c->P1.as<double>(), c->stream);
             launch_update_dot(is_float, V.p, ldv, k + 1, nullptr, z, n, c->P2.as<double>(),
@@ -1769,6 +1810,8 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         upload(c->free_rows, fr.data(), fr.size(), s);
         std::vector<uint8_t> zf(n, 0);
         upload(c->constrained, zf.data(), zf.size(), s);
+        std::vector<int> nol(std::max(n, 1), -1);
+        upload(c->con_line, nol.data(), nol.size(), s);
         c->con_rows.ensure(4);
         c->con_ptr.ensure(4);
         c->con_cols.ensure(4);

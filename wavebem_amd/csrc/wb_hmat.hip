@@ -625,12 +625,22 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
 
 // y[i] = sum over chunks of ypart[c][i] + diag[i] xd[i]: 64 rows per block, the chunks
 // split into four contiguous quarters summed by four threads of a row (in chunk order),
-// the quarters then added in order, so the sum is the same on every call
+// the quarters then added in order, so the sum is the same on every call.
+// With line (the constraint line of each row, -1 for free rows): constrained rows get the
+// line's value instead, x_i - sum_p w_p x_{col_p} (as k_constraint_rows).  With Cpart:
+// the block's sums of y over the (at most two, agg >= 64) aggregates its rows touch,
+// Cpart[2 b + k] for aggregate (64 b) / agg + k, added in a fixed (butterfly) order.
 __global__ __launch_bounds__(256) void k_hm_finish(const float *__restrict__ ypart, size_t ldp,
                                                    int chunks, int n,
                                                    const double *__restrict__ diag,
                                                    const float *__restrict__ xd,
-                                                   float *__restrict__ y)
+                                                   float *__restrict__ y,
+                                                   const int *__restrict__ line,
+                                                   const int *__restrict__ cptr,
+                                                   const int *__restrict__ ccols,
+                                                   const double *__restrict__ cw,
+                                                   const float *__restrict__ x, int agg,
+                                                   double *__restrict__ Cpart)
 {
     __shared__ double qs[4][64];
     const int rl = threadIdx.x & 63, q = threadIdx.x >> 6;
@@ -641,9 +651,34 @@ __global__ __launch_bounds__(256) void k_hm_finish(const float *__restrict__ ypa
         for (int c = c0; c < c1; ++c) s += (double)ypart[(size_t)c * ldp + i];
     qs[q][rl] = s;
     __syncthreads();
-    if (q == 0 && i < n)
-        y[i] = (float)((((qs[0][rl] + qs[1][rl]) + qs[2][rl]) + qs[3][rl]) +
-                       diag[i] * (double)xd[i]);
+    if (q != 0) return;
+    float yv = 0.0f;
+    if (i < n) {
+        yv = (float)((((qs[0][rl] + qs[1][rl]) + qs[2][rl]) + qs[3][rl]) +
+                     diag[i] * (double)xd[i]);
+        if (line) {
+            const int l = line[i];
+            if (l >= 0) {
+                double v = (double)x[i];
+                for (int p = cptr[l]; p < cptr[l + 1]; ++p) v -= cw[p] * (double)x[ccols[p]];
+                yv = (float)v;
+            }
+        }
+        y[i] = yv;
+    }
+    if (Cpart) {
+        const int a0 = (blockIdx.x * 64) / agg, a = i < n ? i / agg : -1;
+        double s0 = a == a0 ? (double)yv : 0.0, s1 = a == a0 + 1 ? (double)yv : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            s0 += __shfl_xor(s0, off, 64);
+            s1 += __shfl_xor(s1, off, 64);
+        }
+        if (rl == 0) {
+            Cpart[2 * blockIdx.x] = s0;
+            Cpart[2 * blockIdx.x + 1] = s1;
+        }
+    }
 }
 
 // S[a][r] += the other sketch parts' sums (parts 1 .. nparts - 1, in order)
@@ -659,7 +694,7 @@ __global__ __launch_bounds__(256) void k_hm_sum_parts(int n, int nagg, int npart
 
 // wavefronts of each product phase (1024 SIMDs: WB_HM_WAVES / 1024 per SIMD)
 #ifndef WB_HM_WAVES
-#define WB_HM_WAVES 16384
+#define WB_HM_WAVES 8192
 #endif
 int hm_chunks(int ncl)
 {
@@ -704,7 +739,7 @@ void hm_plan_items(int ncl, const uint8_t *rank, const int *cl_size, std::vector
     std::vector<int> row;                 // the row cluster's items: low-rank, then dense
     std::vector<int> weight, chunk_of, order, count(chunks + 1);
     std::vector<uint64_t> heap(chunks);  // (load << 8) | chunk, a min-heap
-    const int w_lowrank = 8 + 8 * HM_RMAX, w_dense = 8 * HM_DCOLS;  // weights in 64-byte units
+    const int w_lowrank = 8 + 16 * HM_RMAX, w_dense = 8 * HM_DCOLS;  // weights in 64-byte units
     const int wmax = std::max(w_lowrank, w_dense);
     std::vector<int> cnt(wmax + 2);
     for (int I = 0; I < ncl; ++I) {
@@ -714,7 +749,7 @@ void hm_plan_items(int ncl, const uint8_t *rank, const int *cl_size, std::vector
             const int r = rank[(size_t)I * ncl + J];
             if (r == 0 || r == HM_DENSE) continue;
             row.push_back((J << HM_ITEM_SHIFT) | HM_ITEM_LOWRANK);
-            weight.push_back(8 + 8 * r);  // r columns of Q (512 bytes each) and the r values of T
+            weight.push_back(8 + 16 * r);  // r columns of B and of Q, 512 bytes each
         }
         for (int J = 0; J < ncl; ++J)
             if (rank[(size_t)I * ncl + J] == HM_DENSE)
@@ -759,7 +794,9 @@ void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_
                     const float *fscale, const void *Q16, const void *B16, const void *D16,
                     const int *doff, const int *item_ptr, const int *items, int n,
                     const float *x, const double *diag, const float *xd, float *xc, float *ypart,
-                    size_t ldp, float *y, hipStream_t s, bool xc_ready)
+                    size_t ldp, float *y, hipStream_t s, bool xc_ready, const int *line,
+                    const int *cptr, const int *ccols, const double *cw, int agg,
+                    double *Cpart)
 {
     const int ch = hm_chunks(ncl);
     if (!xc_ready)
@@ -768,7 +805,7 @@ void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_
                        rank, fscale, (const _Float16 *)Q16, (const _Float16 *)B16,
                        (const _Float16 *)D16, doff, xc, ldp, ypart, item_ptr, items);
     hipLaunchKernelGGL(k_hm_finish, dim3((n + 63) / 64), dim3(256), 0, s, ypart, ldp, ch, n,
-                       diag, xd, y);
+                       diag, xd, y, line, cptr, ccols, cw, x, agg, Cpart);
 }
 
 }  // namespace wb

@@ -121,7 +121,10 @@ void plan_hmat(const MeshPlan &P, const double *X_internal, HmatPlan &H);
 // work space), the build also writes S[a ldS + i] = the sum of row i of the FP32 operator
 // over the columns of aggregate a (a < nagg), the coarse space's A P.  Product: y = A~ x + diag .* xd (FP32 vectors, all n rows); xc
 // (ncl x HM_CL, x in cluster order; with xc_ready already holding it) and ypart
-// (hm_chunks(ncl) x ldp) floats are work space.
+// (hm_chunks(ncl) x ldp) floats are work space.  With line (constraint line per row, -1
+// free; cptr/ccols/cw the lines): constrained rows get x_r - sum w x_col instead.  With
+// Cpart (2 ceil(n / 64) doubles): per 64-row block, the sums of y over the two aggregates
+// of agg rows starting at the block's first (for launch_bj_coarse).
 int hm_chunks(int ncl);
 // build, in two launches with the host between them: launch_hm_rank (sketch, aggregate
 // sums, rank test, Q) writes rank[]; the caller lays out the dense blocks (doff[b]: the
@@ -146,7 +149,10 @@ void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_
                     const float *fscale, const void *Q16, const void *B16, const void *D16,
                     const int *doff, const int *item_ptr, const int *items, int n,
                     const float *x, const double *diag, const float *xd, float *xc, float *ypart,
-                    size_t ldp, float *y, hipStream_t s, bool xc_ready = false);
+                    size_t ldp, float *y, hipStream_t s, bool xc_ready = false,
+                    const int *line = nullptr, const int *cptr = nullptr,
+                    const int *ccols = nullptr, const double *cw = nullptr, int agg = 64,
+                    double *Cpart = nullptr);
 
 // Work space of the entry points that allocate nothing (wb_noalloc.hip): a static device
 // array of 2 NOALLOC_MAX_N doubles per GPU.  noalloc_setup looks its address up on a device
@@ -274,10 +280,12 @@ void launch_coarse_restrict(int is_float, int n, int agg, int nc, const void *X,
                             double *C, int k, hipStream_t s);
 // One vector: y = B^-1 (x - W C) + P Aci C in one kernel (x and y may alias); usable when
 // bj_coarse_fits(bs, agg, nc)
+// bj_coarse_fits(bs, agg, nc); with Cpart (see launch_hm_gemv), C = P^T x is summed from
+// those block partials instead of read from C
 bool bj_coarse_fits(int bs, int agg, int nc);
 void launch_bj_coarse(int is_float, int n, int bs, const void *binv, const void *x, void *y,
                       const void *WT, size_t ldw, const double *C, const void *Aci, int nc,
-                      int agg, hipStream_t s);
+                      int agg, hipStream_t s, const double *Cpart = nullptr);
 size_t coarse_invert_lds_bytes(int nc);
 void launch_coarse_build(int n, int agg, int nc, const void *G, const void *H, int precision,
                          size_t ld, const void *A16, const float *rs16, const double *alpha,
