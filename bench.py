@@ -651,7 +651,7 @@ def main():
     if "fp32_ir" not in res:
         gname, esz, xsz, kname = "gemv64", 8, 8, "k_gemv_f64"
     elif args.ir_inner_format == 2:
-        gname, esz, xsz, kname = "gemvc", 0, 4, "k_hm_gemv_t + k_hm_gemv + k_hm_finish"
+        gname, esz, xsz, kname = "gemvc", 0, 4, "k_hm_gemv + k_hm_finish"
     elif args.ir_inner_format == 1:
         gname, esz, xsz, kname = "gemv16", 2, 4, "k_gemv_h16"
     else:
