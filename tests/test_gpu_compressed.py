@@ -79,3 +79,31 @@ def test_compressed_matches_fp64_10k(gpu_available):
     e_dn = np.abs(out["compressed"][1][~N] - out["fp64"][1][~N]).max() / np.abs(out["fp64"][1][~N]).max()
     print(f"  compressed vs FP64: phi {e_phi:.2e} dphi/dn {e_dn:.2e}")
     assert e_phi < 1e-8 and e_dn < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_short_restarts_match_dense(gpu_available, precision):
+    """GMRES restarted every 4 vectors (gmres_max_tmp_vectors = 6): the restart residuals
+    and the first product of every cycle take the inputs that are not fresh Krylov vectors
+    (cluster-order gather, separate coarse restriction), the steps queued ahead of the host
+    end cycles; the solution must still meet the dense-LU bar (SolverGMRES restarts,
+    source/bem_problem.cc:821-895)."""
+    m, ptr, idx, H, G = T.oracle_assembly("wigley_2.5k")
+    bc, _ = M.wigley_bc(m, seed=9)
+    cons = O.compute_constraints(ptr, idx, m.surface_nodes, bc)
+    x = T.dense_solution(H, G, m.surface_nodes, m.other_nodes, cons, bc)
+    phi, dphi = bc * m.surface_nodes, bc * m.other_nodes
+    kw = dict(precision=precision, solver_tolerance=1e-12, gmres_max_tmp_vectors=6,
+              solver_max_steps=2000)
+    if precision == "fp32_ir":
+        kw["ir_inner_format"] = 2
+    with BEMProblem(m, Config(**kw), double_nodes=(ptr, idx)) as p:
+        p.compute_constraints(bc)
+        p.assemble_system()
+        st = p.solve_system(phi, dphi, bc)
+    sol = np.where(m.other_nodes > 0, phi, dphi)
+    e = float(np.abs(sol - x).max() / np.abs(x).max())
+    print(f"  [{precision}] {st['iterations']} it (restart length 4): vs dense LU {e:.2e}",
+          flush=True)
+    assert st["iterations"] > 8
+    assert e < 1e-8
