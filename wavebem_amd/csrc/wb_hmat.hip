@@ -115,7 +115,8 @@ __global__ __launch_bounds__(64) void k_hm_sketch(const float *__restrict__ G,
                                                   int agg, int nagg, double *__restrict__ S,
                                                   size_t ldS)
 {
-    __shared__ float sacc[16][CS_MAX];
+    // aggregate sums of the 16 rows, [a][row] (dynamic: 16 nagg floats with S)
+    extern __shared__ float sacc[];
     const int lane = threadIdx.x, i = lane & 15, kk = lane >> 4;
     const int row0 = blockIdx.x * 16, row = row0 + i;
     const bool rok = row < n;
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(64) void k_hm_sketch(const float *__restrict__ G,
         for (int e = 0; e < 4; ++e)
             sg[g][e] = ((sign_bits((uint32_t)(16 * g + 4 * kk + e)) >> i) & 1u) ? 1.0f : -1.0f;
     if (S)
-        for (int t = lane; t < 16 * CS_MAX; t += 64) sacc[t / CS_MAX][t % CS_MAX] = 0.0f;
+        for (int t = lane; t < 16 * nagg; t += 64) sacc[t] = 0.0f;
     __syncthreads();
     for (int J = J0; J < J1; ++J) {
         fvec4 aG0 = {0.f, 0.f, 0.f, 0.f}, aG1 = aG0, aH0 = aG0, aH1 = aG0;
@@ -226,8 +227,8 @@ __global__ __launch_bounds__(64) void k_hm_sketch(const float *__restrict__ G,
                     hi += __shfl_xor(hi, 16, 64);
                     hi += __shfl_xor(hi, 32, 64);
                     if (kk == 0) {
-                        sacc[i][a0] += lo;
-                        if (a0 + 1 < nagg) sacc[i][a0 + 1] += hi;
+                        sacc[16 * a0 + i] += lo;
+                        if (a0 + 1 < nagg) sacc[16 * (a0 + 1) + i] += hi;
                     }
                 }
             }
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(64) void k_hm_sketch(const float *__restrict__ G,
         __syncthreads();
         for (int t = lane; t < 16 * nagg; t += 64) {
             const int r = row0 + t % 16, a = t / 16;
-            if (r < n) S[(size_t)a * ldS + r] = (double)sacc[t % 16][a];
+            if (r < n) S[(size_t)a * ldS + r] = (double)sacc[t];
         }
     }
 }
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(64) void k_hm_fill(const float *__restrict__ G,
                                                 _Float16 *__restrict__ D16,
                                                 const int *__restrict__ doff)
 {
-    __shared__ float Qs[HM_RMAX][HM_CL];
+    __shared__ _Float16 Qs[HM_RMAX][HM_CL];  // Q in FP16, as stored (6 KB)
     const int b = blockIdx.x, lane = threadIdx.x;
     const int I = b / ncl, J = b % ncl;
     const int r = rank[b];
@@ -453,7 +454,7 @@ __global__ __launch_bounds__(64) void k_hm_fill(const float *__restrict__ G,
     }
     for (int e = lane; e < r * HM_CL; e += 64) {
         const int k = e / HM_CL, i = e % HM_CL;
-        Qs[k][i] = i < szI ? (float)Q16[((size_t)b * HM_RMAX + k) * HM_CL + i] : 0.0f;
+        Qs[k][i] = i < szI ? Q16[((size_t)b * HM_RMAX + k) * HM_CL + i] : (_Float16)0.0f;
     }
     __syncthreads();
     float acc[HM_RMAX][4];
@@ -473,8 +474,9 @@ __global__ __launch_bounds__(64) void k_hm_fill(const float *__restrict__ G,
             for (int k = 0; k < HM_RMAX; ++k)
                 if (k < r) {
                     // Q rows i0 .. i0 + 7 (zero past the block's rows)
-                    const fvec4 q0 = *(const fvec4 *)&Qs[k][i0];
-                    const fvec4 q1 = *(const fvec4 *)&Qs[k][i0 + 4];
+                    const hvec8 qh = *(const hvec8 *)&Qs[k][i0];
+                    const fvec4 q0 = {(float)qh[0], (float)qh[1], (float)qh[2], (float)qh[3]};
+                    const fvec4 q1 = {(float)qh[4], (float)qh[5], (float)qh[6], (float)qh[7]};
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
                         float t = acc[k][c];
@@ -709,7 +711,8 @@ void launch_hm_rank(const float *G, const float *H, size_t ld, int n, const uint
                     float *fscale, void *Q16, int agg, int nagg, double *S, size_t ldS,
                     hipStream_t s)
 {
-    hipLaunchKernelGGL(k_hm_sketch, dim3((n + 15) / 16, HM_SKETCH_PARTS), dim3(64), 0, s, G, H,
+    const size_t lds = S ? sizeof(float) * 16 * (size_t)std::max(nagg, 1) : 0;
+    hipLaunchKernelGGL(k_hm_sketch, dim3((n + 15) / 16, HM_SKETCH_PARTS), dim3(64), lds, s, G, H,
                        ld, n, gmask, (const int4 *)cl_seg, ncl, Y, Mx, agg, nagg, S, ldS);
     if (S && HM_SKETCH_PARTS > 1)
         hipLaunchKernelGGL(k_hm_sum_parts, dim3((n + 255) / 256, nagg), dim3(256), 0, s, n, nagg,
