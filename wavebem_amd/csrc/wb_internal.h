@@ -105,7 +105,11 @@ constexpr int HM_DENSE = 255;
 #endif
 constexpr int HM_DCOLS = WB_HM_DCOLS;
 constexpr int HM_ITEM_SHIFT = 5, HM_ITEM_MASK = 31, HM_ITEM_LOWRANK = 31;
-constexpr int HM_SKETCH_PARTS = 2;  // wavefronts per 16-row block of the sketch (cluster ranges)
+#ifndef WB_HM_SKETCH_PARTS
+#define WB_HM_SKETCH_PARTS 2
+#endif
+// wavefronts per 16-row block of the sketch (cluster ranges)
+constexpr int HM_SKETCH_PARTS = WB_HM_SKETCH_PARTS;
 struct HmatPlan {
     int ncl = 0;
     std::vector<int> cl_idx;    // ncl x HM_CL device indices (-1 past the cluster's size)
