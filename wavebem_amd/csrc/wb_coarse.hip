@@ -305,21 +305,26 @@ __global__ __launch_bounds__(256) void k_coarse_residual(int n, int nc, const T 
 // touch, one wavefront each; then y = B^-1 t + z, the columns of B^-1 split over the four
 // wavefronts and the partial products added in wavefront order.  x and y may alias: every
 // row is read and written by its own workgroup only, the read before the write.
+#ifndef WB_BJC_WAVES
+#define WB_BJC_WAVES 4
+#endif
+constexpr int BJC_W = WB_BJC_WAVES;  // wavefronts per 64-row block
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_bj_coarse(int n, int bs, const T *__restrict__ binv,
+__global__ __launch_bounds__(64 * BJC_W) void k_bj_coarse(int n, int bs, const T *__restrict__ binv,
                                                    const T *x, T *y, const T *__restrict__ WT,
                                                    size_t ldw, const double *__restrict__ C,
                                                    const T *__restrict__ Aci, int nc, int agg,
                                                    const double *__restrict__ Cpart)
 {
     __shared__ double cs[CS_MAX];
-    __shared__ double qs[4][64];
+    __shared__ double qs[BJC_W][64];
     __shared__ T ts[64];
-    __shared__ T ps[4][64];
+    __shared__ T ps[BJC_W][64];
     __shared__ double zs[4];
     const int tid = threadIdx.x, rl = tid & 63, w = tid >> 6;
     const int base = blockIdx.x * bs, m = min(bs, n - base);
-    for (int b = tid; b < nc; b += 256) {
+    for (int b = tid; b < nc; b += 64 * BJC_W) {
         double v;
         if (Cpart) {
             // aggregate b from the partial sums of the 64-row blocks it overlaps, in order
@@ -334,7 +339,7 @@ __global__ __launch_bounds__(256) void k_bj_coarse(int n, int bs, const T *__res
     }
     __syncthreads();
     const int i = base + rl;
-    const int b0 = w * nc / 4, b1 = (w + 1) * nc / 4;
+    const int b0 = w * nc / BJC_W, b1 = (w + 1) * nc / BJC_W;
     // loads in batches of 16 (independent, so one wait per batch), two accumulators
     double a0 = 0.0, a1 = 0.0;
     if (rl < m) {
@@ -362,29 +367,36 @@ __global__ __launch_bounds__(256) void k_bj_coarse(int n, int bs, const T *__res
         if (rl == 0) zs[w] = p;
     }
     __syncthreads();
-    if (w == 0)
-        ts[rl] = rl < m ? (T)((double)x[i] - (((qs[0][rl] + qs[1][rl]) + qs[2][rl]) + qs[3][rl]))
-                        : (T)0;
+    if (w == 0) {
+        double q = qs[0][rl];
+#pragma unroll
+        for (int u = 1; u < BJC_W; ++u) q += qs[u][rl];
+        ts[rl] = rl < m ? (T)((double)x[i] - q) : (T)0;
+    }
     __syncthreads();
-    const int c0 = w * bs / 4, c1 = min(m, (w + 1) * bs / 4);
+    const int c0 = w * bs / BJC_W, c1 = min(m, (w + 1) * bs / BJC_W);
     const T *B = binv + (size_t)blockIdx.x * bs * bs;
     T acc = 0;
     if (rl < m) {
         int c = c0;
-        for (; c + 16 <= c1; c += 16) {
-            T bv[16];
+        constexpr int CB = 64 / BJC_W;  // columns per wavefront at bs = 64, loaded together
+        for (; c + CB <= c1; c += CB) {
+            T bv[CB];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) bv[u] = B[(size_t)(c + u) * bs + rl];
+            for (int u = 0; u < CB; ++u) bv[u] = B[(size_t)(c + u) * bs + rl];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) acc += bv[u] * ts[c + u];
+            for (int u = 0; u < CB; ++u) acc += bv[u] * ts[c + u];
         }
         for (; c < c1; ++c) acc += B[(size_t)c * bs + rl] * ts[c];
     }
     ps[w][rl] = acc;
     __syncthreads();
-    if (w == 0 && rl < m)
-        y[i] = (T)((double)(((ps[0][rl] + ps[1][rl]) + ps[2][rl]) + ps[3][rl]) +
-                   zs[i / agg - ag0]);
+    if (w == 0 && rl < m) {
+        T p = ps[0][rl];
+#pragma unroll
+        for (int u = 1; u < BJC_W; ++u) p += ps[u][rl];
+        y[i] = (T)((double)p + zs[i / agg - ag0]);
+    }
 }
 
 }  // namespace
@@ -471,11 +483,11 @@ void launch_bj_coarse(int is_float, int n, int bs, const void *binv, const void 
 {
     const int nb = (n + bs - 1) / bs;
     if (is_float)
-        hipLaunchKernelGGL(k_bj_coarse<float>, dim3(nb), dim3(256), 0, s, n, bs,
+        hipLaunchKernelGGL(k_bj_coarse<float>, dim3(nb), dim3(64 * BJC_W), 0, s, n, bs,
                            (const float *)binv, (const float *)x, (float *)y, (const float *)WT,
                            ldw, C, (const float *)Aci, nc, agg, Cpart);
     else
-        hipLaunchKernelGGL(k_bj_coarse<double>, dim3(nb), dim3(256), 0, s, n, bs,
+        hipLaunchKernelGGL(k_bj_coarse<double>, dim3(nb), dim3(64 * BJC_W), 0, s, n, bs,
                            (const double *)binv, (const double *)x, (double *)y,
                            (const double *)WT, ldw, C, (const double *)Aci, nc, agg, Cpart);
 }

@@ -742,6 +742,9 @@ static GmresResult gmres(wb_ctx *c, int is_float, const void *b, void *x, double
     const int m = std::max(1, c->cfg.gmres_max_tmp_vectors - 2);
     const size_t es = is_float ? 4 : 8;
     DevBuf &V = is_float ? c->V32 : c->V;
+    // the cluster-order copy is only trusted for vectors this call's steps produce (the
+    // basis may be reallocated below)
+    c->hm_xc_for = nullptr;
     if (V.bytes < es * ldv * (m + 1)) {
         V.ensure(es * ldv * (m + 1));
         HIPCHK(hipMemsetAsync(V.p, 0, V.bytes, c->stream));
