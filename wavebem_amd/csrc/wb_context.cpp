@@ -9,6 +9,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "wb_internal.h"
@@ -65,6 +66,10 @@ double now_ms()
 struct wb_ctx {
     wb_config cfg{};
     hipStream_t stream = nullptr;
+    // second stream for preconditioner builds that overlap the compressed operator's build
+    // (default priority), and the events that order it against the main stream
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_main = nullptr, ev_side = nullptr;
     std::string err;
     Quadrature quad;
     MeshPlan plan;
@@ -107,6 +112,7 @@ struct wb_ctx {
     int n_segs = 0;
     DevBuf con_rows, con_ptr, con_cols, con_w, con_inhom, constrained, free_rows;
     DevBuf con_line;  // per row: its constraint line, -1 for free rows
+    DevBuf surf_c;    // surface mask in caller order (uint8)
     DevBuf cs_Cpart;  // coarse restriction partials of the compressed product (64-row blocks)
     // transposed constraint lines, gathered by target column: y[tgt] += coef * x[src]
     DevBuf ct_tgt, ct_ptr, ct_src, ct_coef;
@@ -186,7 +192,7 @@ struct wb_ctx {
             &iperm_d, &st_task, &st_gap, &st_gcell, &cell_int_s, &orient_s, &cellq_s, &G, &H,
             &Glo, &Hlo, &hsum_part, &hsum_sing, &alpha, &surf_d, &gmask, &segs, &diag_op,
             &diag_rhs, &con_rows, &con_ptr, &con_cols, &con_w, &con_inhom, &constrained,
-            &con_line, &cs_Cpart,
+            &con_line, &cs_Cpart, &surf_c,
             &free_rows, &binv, &binv32, &cs_AP, &cs_Ac, &cs_Aci, &cs_Aci32, &cs_WT,
             &cs_WT32, &cs_status, &cs_C, &cs_T, &cs_PTA, &cs_WTT, &cs_S, &V, &z, &w, &P1, &P2,
             &Pn, &hbuf, &hred, &vx, &vb, &vr, &vt, &vs, &V32, &z32, &w32, &r32, &d32, &ct_tgt,
@@ -308,7 +314,7 @@ static void cell_records(wb_ctx *c)
 }
 static void alloc_matrices(wb_ctx *c);
 static void ensure_h16(wb_ctx *c);
-static void ensure_hm(wb_ctx *c);
+static void ensure_hm(wb_ctx *c, const std::function<void()> &after_ranks = {});
 
 // coarse aggregates: a multiple of 8 dofs, at least 64, at most precond_coarse of them
 static void coarse_shape(const wb_ctx *c, int &agg, int &nc)
@@ -602,6 +608,17 @@ static void install_constraints(wb_ctx *c, const ConstraintSet &cs)
 {
     const int n = c->n;
     const int nl = (int)cs.rows.size();
+    if (c->have_cons && c->cons.rows == cs.rows && c->cons.ptr == cs.ptr &&
+        c->cons.cols == cs.cols && c->cons.w == cs.w) {
+        // the installed lines (already checked) with, at most, new inhomogeneities: the
+        // device copies of everything else stay
+        if (c->cons.inhom != cs.inhom) {
+            c->cons.inhom = cs.inhom;
+            upload(c->con_inhom, c->cons.inhom.data(), c->cons.inhom.size(), c->stream);
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        return;
+    }
     std::vector<int> rows(nl), ptr(cs.ptr), cols(cs.cols.size());
     std::vector<uint8_t> flag(n, 0);
     for (int l = 0; l < nl; ++l) {
@@ -674,10 +691,22 @@ static void ensure_precond(wb_ctx *c)
     const int nb = (c->n + bs - 1) / bs;
     c->binv.ensure(sizeof(double) * (size_t)nb * bs * bs);
     c->binv32.ensure(sizeof(float) * (size_t)nb * bs * bs);
+    // With the compressed inner operator, the blocks and the coarse space are built on the
+    // second stream, beside the operator's sketch and fill on the main stream (they read
+    // the matrices, the masks and the sketch's aggregate sums, and write only their own
+    // buffers); the main stream then waits for them.
+    const bool side = c->cfg.precision == WB_PREC_FP32_IR && c->cfg.ir_inner_format == 2 &&
+                      c->cfg.precond_coarse > 0;
+    hipStream_t bs_stream = c->stream;
+    if (side) {
+        HIPCHK(hipEventRecord(c->ev_main, c->stream));
+        HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_main, 0));
+        bs_stream = c->stream2;
+    }
     launch_block_jacobi_build(c->n, bs, c->G.p, c->H.p, c->Glo.p, c->Hlo.p, c->cfg.precision,
                               c->ld, c->alpha.as<double>(), c->surf_d.as<double>(),
                               c->constrained.as<uint8_t>(), c->binv.as<double>(),
-                              c->binv32.as<float>(), c->stream);
+                              c->binv32.as<float>(), bs_stream);
     HIPCHK(hipGetLastError());
     c->cs_on = false;
     c->cs_t_on = false;
@@ -700,21 +729,36 @@ static void ensure_precond(wb_ctx *c)
         const bool from16 = c->cfg.precision == WB_PREC_FP32_IR && c->cfg.ir_inner_format == 1;
         const bool fromhm = c->cfg.precision == WB_PREC_FP32_IR && c->cfg.ir_inner_format == 2;
         if (from16) ensure_h16(c);
-        if (fromhm) ensure_hm(c);
-        launch_coarse_build(c->n, agg, nc, c->G.p, c->H.p, c->cfg.precision, c->ld,
-                            from16 ? c->A16.p : nullptr,
-                            from16 ? c->rs16.as<float>() : nullptr, c->alpha.as<double>(),
-                            c->gmask.as<uint8_t>(),
-                            c->constrained.as<uint8_t>(), c->cs_AP.as<double>(), c->vlen,
-                            (from16 && c->h16_sums) || fromhm ? c->cs_S.as<double>() : nullptr,
-                            c->cs_Ac.as<double>(),
-                            c->cs_Aci.as<double>(), c->cs_Aci32.as<float>(), c->cs_WT.as<double>(),
-                            c->cs_WT32.as<float>(), c->cs_status.as<int>(), c->stream);
-        HIPCHK(hipGetLastError());
+        auto coarse = [&] {
+            launch_coarse_build(c->n, agg, nc, c->G.p, c->H.p, c->cfg.precision, c->ld,
+                                from16 ? c->A16.p : nullptr,
+                                from16 ? c->rs16.as<float>() : nullptr, c->alpha.as<double>(),
+                                c->gmask.as<uint8_t>(), c->constrained.as<uint8_t>(),
+                                c->cs_AP.as<double>(), c->vlen,
+                                (from16 && c->h16_sums) || fromhm ? c->cs_S.as<double>()
+                                                                  : nullptr,
+                                c->cs_Ac.as<double>(), c->cs_Aci.as<double>(),
+                                c->cs_Aci32.as<float>(), c->cs_WT.as<double>(),
+                                c->cs_WT32.as<float>(), c->cs_status.as<int>(), bs_stream);
+            HIPCHK(hipGetLastError());
+        };
+        bool coarse_done = false;
+        if (fromhm) {
+            // the coarse build starts as soon as the sketch's sums exist, beside the fill
+            ensure_hm(c, [&] {
+                coarse();
+                coarse_done = true;
+            });
+        }
+        if (!coarse_done) coarse();
         int st = 1;
         HIPCHK(hipMemcpyAsync(&st, c->cs_status.p, sizeof(int), hipMemcpyDeviceToHost,
-                              c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
+                              bs_stream));
+        HIPCHK(hipStreamSynchronize(bs_stream));
+        if (side) {
+            HIPCHK(hipEventRecord(c->ev_side, c->stream2));
+            HIPCHK(hipStreamWaitEvent(c->stream, c->ev_side, 0));
+        }
         // a singular or non-finite coarse inverse leaves the blocks alone
         c->cs_on = st == 0;
     }
@@ -896,7 +940,9 @@ static void ensure_h16(wb_ctx *c)
 }
 
 // The compressed copy of the FP32 vmult operator for the current assembly and masks.
-static void ensure_hm(wb_ctx *c)
+// after_ranks (optional) runs once the sketch, its aggregate sums and the rank test are
+// complete on the device (the host has synchronised), before the fill is launched
+static void ensure_hm(wb_ctx *c, const std::function<void()> &after_ranks)
 {
     int agg = 0, nc = 0;
     if (c->cfg.precond_coarse > 0) coarse_shape(c, agg, nc);
@@ -932,6 +978,7 @@ static void ensure_hm(wb_ctx *c)
         std::vector<uint8_t> rk(nb);
         HIPCHK(hipMemcpyAsync(rk.data(), c->hm_rank.p, nb, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        if (after_ranks) after_ranks();
         std::vector<int> doff(nb);
         int nd = 0;
         for (size_t b = 0; b < nb; ++b) doff[b] = rk[b] == HM_DENSE ? nd++ : -1;
@@ -1660,6 +1707,9 @@ wb_status wb_create(wb_ctx **out, const wb_config *cfg)
     wb_status st = guarded(c, [&] {
         HIPCHK(hipSetDevice(k.device));
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_main, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_side, hipEventDisableTiming));
         // the no-allocation entry point's work space on this device
         HIPCHK(noalloc_setup(k.device));
         build_quadrature(k.quadrature_order, k.singular_quadrature_order, c->quad);
@@ -1678,12 +1728,16 @@ void wb_destroy(wb_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     for (DevBuf *b : c->all_bufs()) b->release();
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->hpin) (void)hipHostFree(c->hpin);
     if (c->hmap) (void)hipHostFree(c->hmap);
     if (c->mw.hpin) (void)hipHostFree(c->mw.hpin);
     if (c->mw.apin) (void)hipHostFree(c->mw.apin);
+    if (c->ev_main) (void)hipEventDestroy(c->ev_main);
+    if (c->ev_side) (void)hipEventDestroy(c->ev_side);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1807,6 +1861,7 @@ wb_status wb_set_mesh(wb_ctx *c, int n, int nc, const double *support, const int
         c->gmask.ensure((c->vlen + 3) / 4);
         // default: no constraints
         c->n_lines = 0;
+        c->n_ct = 0;  // no transposed lines either (the old ones index the old mesh)
         std::vector<int> fr(n);
         for (int i = 0; i < n; ++i) fr[i] = i;
         c->n_free = n;
@@ -1927,6 +1982,11 @@ static void install_masks(wb_ctx *c, const double *surface, const double *other)
     std::vector<double> si;
     to_internal(c, surface, si);
     upload(c->surf_d, si.data(), c->vlen, c->stream);
+    {
+        std::vector<uint8_t> sc(std::max(n, 1), 0);  // caller order, for the solution scatter
+        for (int i = 0; i < n; ++i) sc[i] = surface[i] != 0.0;
+        upload(c->surf_c, sc.data(), sc.size(), c->stream);
+    }
     // column selection: bit set -> column comes from H in vmult (other_nodes != 0)
     std::vector<double> oi;
     to_internal(c, other, oi);
@@ -2415,28 +2475,13 @@ wb_status wb_solve_system_dev(wb_ctx *c, const double *tmp_dev, double *phi_dev,
             refresh_auto_constraints(c, h.data());
         }
         solve_system_dev(c, c->vs.as<double>(), st);
-        // scatter by mask: internal sol -> caller phi / dphi_dn
+        // scatter by mask (source/bem_problem.cc:869-878): internal sol -> caller phi where
+        // the surface mask is 0, dphi_dn where it is 1
         launch_permute(c->n, c->perm_d.as<int>(), c->vx.as<double>(), c->vt.as<double>(), 1,
                        c->stream);  // vt[d] = sol[perm[d]] (caller order)
-        std::vector<double> sol(c->n);
-        HIPCHK(hipMemcpyAsync(sol.data(), c->vt.p, sizeof(double) * c->n, hipMemcpyDeviceToHost,
-                              c->stream));
-        std::vector<double> ph(c->n), dp(c->n);
-        HIPCHK(hipMemcpyAsync(ph.data(), phi_dev, sizeof(double) * c->n, hipMemcpyDeviceToHost,
-                              c->stream));
-        HIPCHK(hipMemcpyAsync(dp.data(), dphi_dev, sizeof(double) * c->n, hipMemcpyDeviceToHost,
-                              c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        for (int i = 0; i < c->n; ++i) {
-            if (c->surface[i] == 0)
-                ph[i] = sol[i];
-            else
-                dp[i] = sol[i];
-        }
-        HIPCHK(hipMemcpyAsync(phi_dev, ph.data(), sizeof(double) * c->n, hipMemcpyHostToDevice,
-                              c->stream));
-        HIPCHK(hipMemcpyAsync(dphi_dev, dp.data(), sizeof(double) * c->n, hipMemcpyHostToDevice,
-                              c->stream));
+        launch_scatter_mask(c->n, c->surf_c.as<uint8_t>(), c->vt.as<double>(), phi_dev, dphi_dev,
+                            c->stream);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
     });
 }

@@ -253,6 +253,9 @@ void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *G
                            const int *rows, int n_rows, void *Y, size_t ldy, double *part,
                            int n_groups, hipStream_t s);
 // constrained rows: y[row] = x[row] - sum w x[col]  (mode_inhom=0)  or y[row] = inhom (1)
+// a[i] = v[i] where mask[i] == 0, b[i] = v[i] elsewhere (n entries)
+void launch_scatter_mask(int n, const uint8_t *mask, const double *v, double *a, double *b,
+                         hipStream_t s);
 void launch_constraint_rows(int is_float, int n_lines, const int *rows, const int *ptr,
                             const int *cols, const double *w, const double *inhom, int mode_inhom,
                             const void *x, void *y, hipStream_t s);

@@ -57,6 +57,24 @@ __global__ void k_constraint_rows(int n_lines, const int *__restrict__ rows,
     y[r] = (T)v;
 }
 
+// a[i] = v[i] where mask[i] == 0, b[i] = v[i] where mask[i] != 0
+__global__ void k_scatter_mask(int n, const uint8_t *__restrict__ mask,
+                               const double *__restrict__ v, double *__restrict__ a,
+                               double *__restrict__ b)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mask[i]) b[i] = v[i];
+    else a[i] = v[i];
+}
+
+void launch_scatter_mask(int n, const uint8_t *mask, const double *v, double *a, double *b,
+                         hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scatter_mask, dim3((n + 255) / 256), dim3(256), 0, s, n, mask, v, a, b);
+}
+
 void launch_constraint_rows(int is_float, int n_lines, const int *rows, const int *ptr,
                             const int *cols, const double *w, const double *inhom, int mode_inhom,
                             const void *x, void *y, hipStream_t s)
