@@ -589,18 +589,13 @@ static void apply_precond(wb_ctx *c, int is_float, const void *x, void *y, bool 
 
 static void refresh_diag(wb_ctx *c)
 {
-    // diag_op = alpha .* other, diag_rhs = -alpha .* surface (internal order)
-    std::vector<double> so, ss;
-    to_internal(c, c->other.data(), so);
-    to_internal(c, c->surface.data(), ss);
-    for (auto &v : ss) v = -v;
-    c->vt.ensure(sizeof(double) * c->vlen);
-    upload(c->vt, so.data(), c->vlen, c->stream);
+    // diag_op = alpha .* other, diag_rhs = -alpha .* surface (internal order; other is
+    // 1 - surface, surf_d the surface mask in internal order)
     c->diag_op.ensure(sizeof(double) * c->vlen);
     c->diag_rhs.ensure(sizeof(double) * c->vlen);
-    launch_vec_ops(4, c->n, c->alpha.p, c->vt.p, c->diag_op.p, 0, c->stream);
-    upload(c->vt, ss.data(), c->vlen, c->stream);
-    launch_vec_ops(4, c->n, c->alpha.p, c->vt.p, c->diag_rhs.p, 0, c->stream);
+    launch_diag_from_mask(c->n, c->alpha.as<double>(), c->surf_d.as<double>(),
+                          c->diag_op.as<double>(), c->diag_rhs.as<double>(), c->stream);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
 }
 

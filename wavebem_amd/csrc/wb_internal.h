@@ -253,6 +253,9 @@ void launch_gemv_sel_multi(int mode, const void *G, const void *H, const void *G
                            const int *rows, int n_rows, void *Y, size_t ldy, double *part,
                            int n_groups, hipStream_t s);
 // constrained rows: y[row] = x[row] - sum w x[col]  (mode_inhom=0)  or y[row] = inhom (1)
+// op = alpha .* (1 - s), rhs = -alpha .* s (n entries, s a 0/1 mask)
+void launch_diag_from_mask(int n, const double *alpha, const double *s, double *op, double *rhs,
+                           hipStream_t st);
 // a[i] = v[i] where mask[i] == 0, b[i] = v[i] elsewhere (n entries)
 void launch_scatter_mask(int n, const uint8_t *mask, const double *v, double *a, double *b,
                          hipStream_t s);

@@ -57,6 +57,25 @@ __global__ void k_constraint_rows(int n_lines, const int *__restrict__ rows,
     y[r] = (T)v;
 }
 
+// op[i] = alpha[i] (1 - s[i]), rhs[i] = -alpha[i] s[i] for a 0/1 mask s
+__global__ void k_diag_from_mask(int n, const double *__restrict__ alpha,
+                                 const double *__restrict__ s, double *__restrict__ op,
+                                 double *__restrict__ rhs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    op[i] = alpha[i] * (1.0 - s[i]);
+    rhs[i] = alpha[i] * -s[i];
+}
+
+void launch_diag_from_mask(int n, const double *alpha, const double *s, double *op, double *rhs,
+                           hipStream_t st)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_diag_from_mask, dim3((n + 255) / 256), dim3(256), 0, st, n, alpha, s, op,
+                       rhs);
+}
+
 // a[i] = v[i] where mask[i] == 0, b[i] = v[i] where mask[i] != 0
 __global__ void k_scatter_mask(int n, const uint8_t *__restrict__ mask,
                                const double *__restrict__ v, double *__restrict__ a,
