@@ -406,6 +406,17 @@ int wb_host_cluster_plan(int n_dofs, int n_cells, const double *support_points,
                          int32_t *n_clusters, int32_t *cl_size, int32_t *cl_idx, uint8_t *near,
                          int32_t *perm);
 
+/* No counterpart in the reference.  The work lists of the compressed operator's product
+ * for inspection, from block ranks (n_clusters^2 bytes, block (I, J) at I n_clusters + J:
+ * 0 zero, 255 dense, else the rank) and cluster sizes: sizes[0..4] = chunks per row cluster,
+ * items, columns per dense item, item shift, low-rank kind.  With non-NULL buffers (of
+ * n_clusters chunks + 1 and items entries), item_ptr and items: the items of row cluster I
+ * and chunk k are items[item_ptr[I chunks + k] .. item_ptr[I chunks + k + 1]), each
+ * (J << shift) | kind, kind = low-rank kind for block (I, J) or q for its dense columns
+ * q cols .. (q + 1) cols - 1; low-rank items first in each chunk. */
+int wb_host_hm_work_items(int n_clusters, const uint8_t *rank, const int32_t *cl_size,
+                          int32_t *sizes, int32_t *item_ptr, int32_t *items);
+
 /* BEMProblem::compute_constraints (source/bem_problem.cc:990-1105) rules, closed as by
  * ConstraintMatrix::close; returns the number of lines, or minus a wb_status. */
 int wb_host_compute_constraints(int n, const int32_t *dn_ptr, const int32_t *dn_idx,

@@ -47,3 +47,55 @@ def test_most_blocks_far_at_40k():
     m = T.wigley_preset("40k")
     near = B.host_cluster_plan(m)["near"]
     assert near.mean() < 0.05, near.mean()
+
+
+@pytest.mark.parametrize("ncl,seed", [(1, 0), (7, 1), (97, 2), (175, 3)])
+def test_product_work_items_cover_every_block_once(ncl, seed):
+    """The compressed product's work lists (hm_plan_items): every low-rank block is one
+    item and every dense block splits into column parts covering its columns, each listed
+    exactly once, in the chunks of its row cluster, low-rank items first; chunk loads are
+    balanced to within one item's weight of the mean (largest-first packing)."""
+    rng = np.random.default_rng(seed)
+    size = rng.integers(1, 257, ncl).astype(np.int32)
+    u = rng.random((ncl, ncl))
+    rank = np.where(u < 0.05, 0, np.where(u < 0.15, 255, rng.integers(1, 13, (ncl, ncl))))
+    rank = rank.astype(np.uint8)
+    w = B.host_hm_work_items(rank, size)
+    ch, ptr, items, dc, sh, lr = (w["chunks"], w["item_ptr"], w["items"], w["dcols"],
+                                  w["shift"], w["lowrank"])
+    assert ptr[0] == 0 and np.all(np.diff(ptr) >= 0) and ptr[-1] == items.size
+    expect = 0
+    for I in range(ncl):
+        seen = {}
+        loads = []
+        for k in range(ch):
+            its = items[ptr[I * ch + k]:ptr[I * ch + k + 1]]
+            kinds = its & ((1 << sh) - 1)
+            J = its >> sh
+            low = kinds == lr
+            # low-rank first, then dense parts
+            assert not np.any(low[1:] & ~low[:-1])
+            load = 0
+            for j, q in zip(J, kinds):
+                key = (int(j), int(q))
+                seen[key] = seen.get(key, 0) + 1
+                r = int(rank[I, j])
+                if q == lr:
+                    assert 0 < r < 255
+                    load += 8 + 16 * r
+                else:
+                    assert r == 255 and q * dc < size[j]
+                    load += 8 * dc
+            loads.append(load)
+        for j in range(ncl):
+            r = int(rank[I, j])
+            if r == 255:
+                for q in range(-(-int(size[j]) // dc)):
+                    assert seen.pop((j, q)) == 1
+                    expect += 1
+            elif r > 0:
+                assert seen.pop((j, lr)) == 1
+                expect += 1
+        assert not seen  # nothing else listed
+        assert max(loads) <= np.mean(loads) + max(8 * dc, 8 + 16 * 12)
+    assert items.size == expect

@@ -109,6 +109,8 @@ _SIGS = {
     "wb_host_cluster_plan": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _i32p, _i32p, _i32p,
                                             _i32p, _i32p, ctypes.POINTER(ctypes.c_uint8),
                                             _i32p]),
+    "wb_host_hm_work_items": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_uint8),
+                                             _i32p, _i32p, _i32p, _i32p]),
     "wb_get_coarse_matrix": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
                                             ctypes.POINTER(ctypes.c_int), _dp]),
     "wb_solve_adjoint": (ctypes.c_int, [_vp, _dp, _dp, ctypes.c_void_p]),

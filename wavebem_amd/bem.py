@@ -495,6 +495,27 @@ def host_cluster_plan(mesh: SurfaceMesh):
     return dict(size=size, idx=idx, near=near.astype(bool), perm=perm)
 
 
+def host_hm_work_items(rank, cl_size):
+    """The compressed product's work lists for block ranks (ncl x ncl uint8: 0 zero, 255
+    dense, else the rank) and cluster sizes (wb_host_hm_work_items): dict with chunks,
+    item_ptr, items, dcols, shift, lowrank."""
+    r = np.ascontiguousarray(rank, dtype=np.uint8)
+    ncl = r.shape[0]
+    sz = np.ascontiguousarray(cl_size, dtype=np.int32)
+    sizes = np.zeros(5, dtype=np.int32)
+    rp = r.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    st = L.lib().wb_host_hm_work_items(ncl, rp, _i(sz), _i(sizes), None, None)
+    if st != L.WB_OK:
+        raise WaveBemError(st, "wb_host_hm_work_items")
+    ptr = np.zeros(ncl * int(sizes[0]) + 1, dtype=np.int32)
+    items = np.zeros(max(int(sizes[1]), 1), dtype=np.int32)
+    st = L.lib().wb_host_hm_work_items(ncl, rp, _i(sz), _i(sizes), _i(ptr), _i(items))
+    if st != L.WB_OK:
+        raise WaveBemError(st, "wb_host_hm_work_items")
+    return dict(chunks=int(sizes[0]), item_ptr=ptr, items=items[:int(sizes[1])],
+                dcols=int(sizes[2]), shift=int(sizes[3]), lowrank=int(sizes[4]))
+
+
 def host_mesh_plan(mesh: SurfaceMesh):
     n, nc = mesh.n_dofs, mesh.n_cells
     X = _f64(mesh.support_points).reshape(-1)

@@ -89,6 +89,27 @@ int wb_host_cluster_plan(int n, int nc, const double *support, const int32_t *ce
     return WB_OK;
 }
 
+int wb_host_hm_work_items(int ncl, const uint8_t *rank, const int32_t *cl_size,
+                          int32_t *sizes, int32_t *item_ptr, int32_t *items)
+{
+    if (ncl <= 0 || !rank || !cl_size || !sizes) return WB_E_ARG;
+    for (int J = 0; J < ncl; ++J)
+        if (cl_size[J] < 0 || cl_size[J] > wb::HM_CL) return WB_E_ARG;
+    for (long b = 0; b < (long)ncl * ncl; ++b)
+        if (rank[b] != wb::HM_DENSE && rank[b] > wb::HM_RMAX) return WB_E_ARG;
+    std::vector<int> ptr, it;
+    wb::hm_plan_items(ncl, rank, cl_size, ptr, it);
+    const int nitems = ptr.back();
+    sizes[0] = wb::hm_chunks(ncl);
+    sizes[1] = nitems;
+    sizes[2] = wb::HM_DCOLS;
+    sizes[3] = wb::HM_ITEM_SHIFT;
+    sizes[4] = wb::HM_ITEM_LOWRANK;
+    if (item_ptr) std::copy(ptr.begin(), ptr.end(), item_ptr);
+    if (items) std::copy(it.begin(), it.begin() + nitems, items);
+    return WB_OK;
+}
+
 int wb_host_compute_constraints(int n, const int32_t *dn_ptr, const int32_t *dn_idx,
                                 const double *surface, const double *normals,
                                 const double *grads, const double *tmp_rhs, int cap_lines,
