@@ -276,18 +276,23 @@ def test_pure_neumann_vmult(gpu_available):
     assert np.abs(d - o).max() / np.abs(o).max() < 1e-12
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir", "fp32_ir_fp16"])
 @pytest.mark.parametrize("k", [5, 17])
 def test_solve_system_multi_matches_dense_oracle(gpu_available, precision, k):
-    """k seeded right-hand sides solved together (batched matrix-core product, per-column
-    GMRES state, batches of at most 16); every column against the dense solution of its own
-    constrained system, to the same 1e-8 bar as a single solve."""
+    """k seeded right-hand sides solved together (per-column GMRES state, batches of at most
+    16; the FP32-IR inner solves on the compressed operator, read once per group of 8
+    vectors, or with fp32_ir_fp16 on the FP16 copy's batched matrix-core product); every
+    column against the dense solution of its own constrained system, to the same 1e-8 bar
+    as a single solve."""
+    kw = dict(precision=precision)
+    if precision == "fp32_ir_fp16":
+        kw = dict(precision="fp32_ir", ir_inner_format=1)
     m, ptr, idx, H, G = T.oracle_assembly("wigley_small")
     bcs = [M.wigley_bc(m, seed=s)[0] * (1.0 + 0.25 * s) for s in range(k)]
     tmp = np.stack(bcs)
     phi = tmp * m.surface_nodes
     dphi = tmp * m.other_nodes
-    with _problem(m, ptr, idx, precision=precision, solver_tolerance=1e-13) as p:
+    with _problem(m, ptr, idx, solver_tolerance=1e-13, **kw) as p:
         p.compute_constraints(bcs[0])
         st = p.solve_system_multi(phi, dphi, tmp)
     print(f"  [{precision} k={k}] stats {st}")

@@ -173,13 +173,13 @@ struct wb_ctx {
     struct Multi {
         int k = 0;
         DevBuf V, V32, Z, W, Z32, W32, P1, P2, Pn, H, hred, X, B, R, T, D32, R32, act, inh, part,
-            xs;
+            xs, hxc, hpart;
         double *hpin = nullptr;  // pinned k x 132 coefficients
         int *apin = nullptr;     // pinned k active flags
         std::vector<DevBuf *> all()
         {
             return {&V, &V32, &Z, &W, &Z32, &W32, &P1, &P2, &Pn, &H, &hred,
-                    &X, &B, &R, &T, &D32, &R32, &act, &inh, &part, &xs};
+                    &X, &B, &R, &T, &D32, &R32, &act, &inh, &part, &xs, &hxc, &hpart};
         }
     } mw;
 
@@ -1283,6 +1283,25 @@ static void apply_constrained_multi(wb_ctx *c, int mode, int k, const void *X, s
 {
     require(!c->pure_neumann, WB_E_UNSUPPORTED,
             "several right-hand sides with all-Neumann masks are not supported");
+    if (mode == 4) {
+        // the compressed operator, read once per group of up to 8 vectors, with the
+        // constraint lines in its last kernel
+        KernelScope ks(c, WB_K_GEMVC, (long long)c->n * k);
+        size_t xcf = 0;
+        const size_t pf = hm_multi_work_floats(c->hmp.ncl, c->vlen, &xcf);
+        c->mw.hxc.ensure(sizeof(float) * xcf);
+        c->mw.hpart.ensure(sizeof(float) * pf);
+        launch_hm_gemv_multi(c->hm_idx.as<int>(), c->hm_size.as<int>(), c->hmp.ncl,
+                             c->hm_rank.as<uint8_t>(), c->hm_scale.as<float>(), c->hm_Q.p,
+                             c->hm_B.p, c->hm_D.p, c->hm_doff.as<int>(), c->hm_iptr.as<int>(),
+                             c->hm_items.as<int>(), c->n, k, (const float *)X, ldx,
+                             c->diag_op.as<double>(), c->mw.hxc.as<float>(),
+                             c->mw.hpart.as<float>(), c->vlen, (float *)Y, ldy,
+                             c->n_lines > 0 ? c->con_line.as<int>() : nullptr,
+                             c->con_ptr.as<int>(), c->con_cols.as<int>(), c->con_w.as<double>(),
+                             c->stream);
+        return;
+    }
     const int is_float = mode == 1 || mode == 3;
     const size_t es = is_float ? 4 : 8;
     {
@@ -1565,10 +1584,11 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
         HIPCHK(hipMemcpyAsync(M.R.p, M.B.p, d8, hipMemcpyDeviceToDevice, c->stream));
         int sweeps = 0, total = 0;
         std::vector<int> done(k, 0);
-        // inner solves on the FP16 operator (ir_inner_format 1; also for 2, whose compressed
-        // copy has no several-vector product) or the FP32 one
-        const int fmode = c->cfg.ir_inner_format >= 1 ? 3 : 1;
+        // inner solves on the compressed operator (ir_inner_format 2), the FP16 one (1) or
+        // the FP32 one (0)
+        const int fmode = c->cfg.ir_inner_format == 2 ? 4 : (c->cfg.ir_inner_format == 1 ? 3 : 1);
         if (fmode == 3) ensure_h16(c);
+        if (fmode == 4) ensure_hm(c);
         FloatMode fm(c, fmode);
         std::vector<double> prev(k, 0.0);
         for (;;) {
@@ -1590,9 +1610,9 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
             if (all || bad || sweeps >= c->cfg.ir_max_sweeps ||
                 total >= c->cfg.solver_max_steps)
                 break;
-            // an FP16 sweep that did not halve some residual hands the remaining sweeps of
-            // every right-hand side to the FP32 operator
-            if (c->float_mode == 3 && sweeps > 0)
+            // an FP16 or compressed sweep that did not halve some residual hands the
+            // remaining sweeps of every right-hand side to the FP32 operator
+            if (c->float_mode >= 3 && sweeps > 0)
                 for (int v = 0; v < k; ++v)
                     if (!done[v] && !(resid[v] < 0.5 * prev[v])) c->float_mode = 1;
             prev = resid;

@@ -625,6 +625,156 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
     }
 }
 
+// ------------------------------------------------------------------ several vectors
+// The product for up to HM_KV vectors at once, over the same work items: the operator is
+// read once per launch.  xcm holds the vectors in cluster order ([v][J HM_CL + p]); a
+// low-rank item forms t_{k,v} = B_k . x_J^v for every vector (lane partials, DPP wavefront
+// sums) and adds Q t; a dense part multiplies every streamed column into every vector.
+// ypart is [ch][v][row].
+constexpr int HM_KV = 8;
+
+__global__ __launch_bounds__(HM_CL) void k_hm_gather_multi(const int *__restrict__ cl_idx,
+                                                           const int *__restrict__ cl_size,
+                                                           int ncl, const float *__restrict__ X,
+                                                           size_t ldx, float *__restrict__ xcm)
+{
+    const int J = blockIdx.x, v = blockIdx.y, p = threadIdx.x;
+    xcm[((size_t)v * ncl + J) * HM_CL + p] =
+        p < cl_size[J] ? X[(size_t)v * ldx + cl_idx[(size_t)J * HM_CL + p]] : 0.0f;
+}
+
+__global__ __launch_bounds__(64) void k_hm_gemv_multi(
+    const int *__restrict__ cl_idx, const int *__restrict__ cl_size, int ncl, int chunks,
+    const uint8_t *__restrict__ rank, const float *__restrict__ fscale,
+    const _Float16 *__restrict__ Q16, const _Float16 *__restrict__ B16,
+    const _Float16 *__restrict__ D16, const int *__restrict__ doff,
+    const float *__restrict__ xcm, int k, size_t ldp, float *__restrict__ ypart,
+    const int *__restrict__ item_ptr, const int *__restrict__ items)
+{
+    const int lane = threadIdx.x;
+    const int I = blockIdx.x, ch = blockIdx.y;
+    const int szI = cl_size[I];
+    const size_t vs = (size_t)ncl * HM_CL;  // vector stride of xcm
+    const int p0 = __builtin_amdgcn_readfirstlane(item_ptr[I * chunks + ch]);
+    const int p1 = __builtin_amdgcn_readfirstlane(item_ptr[I * chunks + ch + 1]);
+    fvec4 yacc[HM_KV];
+#pragma unroll
+    for (int v = 0; v < HM_KV; ++v) yacc[v] = fvec4{0.f, 0.f, 0.f, 0.f};
+    constexpr int NB = WB_HM_BATCH;
+    for (int p = p0; p < p1; ++p) {
+        const int it = __builtin_amdgcn_readfirstlane(items[p]);
+        const int J = it >> HM_ITEM_SHIFT, kind = it & HM_ITEM_MASK;
+        const size_t b = (size_t)I * ncl + J;
+        const float fs = fscale[b];
+        if (kind == HM_ITEM_LOWRANK) {
+            const int r = __builtin_amdgcn_readfirstlane(rank[b]);
+            const _Float16 *Bp = B16 + b * HM_RMAX * HM_CL + 4 * lane;
+            const _Float16 *Q = Q16 + b * HM_RMAX * HM_CL + 4 * lane;
+            hvec4 bk[HM_RMAX], q[HM_RMAX];
+#pragma unroll
+            for (int kr = 0; kr < HM_RMAX; ++kr)
+                if (kr < r) {
+                    bk[kr] = *(const hvec4 *)(Bp + (size_t)kr * HM_CL);
+                    q[kr] = *(const hvec4 *)(Q + (size_t)kr * HM_CL);
+                }
+            fvec4 xv[HM_KV];
+#pragma unroll
+            for (int v = 0; v < HM_KV; ++v)
+                xv[v] = v < k ? *(const fvec4 *)(xcm + v * vs + (size_t)J * HM_CL + 4 * lane)
+                              : fvec4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kr = 0; kr < HM_RMAX; ++kr)
+                if (kr < r)
+#pragma unroll
+                    for (int v = 0; v < HM_KV; ++v)
+                        if (v < k) {
+                            const float part = (float)bk[kr][0] * xv[v][0] +
+                                               (float)bk[kr][1] * xv[v][1] +
+                                               (float)bk[kr][2] * xv[v][2] +
+                                               (float)bk[kr][3] * xv[v][3];
+                            const float t = lane63(wave_reduce63<false>(part)) * fs;
+#pragma unroll
+                            for (int c = 0; c < 4; ++c)
+                                yacc[v][c] = fmaf((float)q[kr][c], t, yacc[v][c]);
+                        }
+        } else {
+            const int szJ = __builtin_amdgcn_readfirstlane(cl_size[J]);
+            const int cb = HM_DCOLS * kind, ce = min(cb + HM_DCOLS, szJ);
+            fvec4 xv[HM_KV];
+#pragma unroll
+            for (int v = 0; v < HM_KV; ++v)
+                xv[v] = (v < k && lane < HM_DCOLS / 4)
+                            ? *(const fvec4 *)(xcm + v * vs + (size_t)J * HM_CL + cb + 4 * lane)
+                            : fvec4{0.f, 0.f, 0.f, 0.f};
+            const _Float16 *D = D16 + (size_t)doff[b] * HM_CL * HM_CL + 4 * lane;
+            fvec4 a[HM_KV];
+#pragma unroll
+            for (int v = 0; v < HM_KV; ++v) a[v] = fvec4{0.f, 0.f, 0.f, 0.f};
+            for (int c0 = cb; c0 < ce; c0 += NB) {
+                hvec4 h[NB];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) h[u] = *(const hvec4 *)(D + (size_t)(c0 + u) * HM_CL);
+                const int src = (c0 - cb) >> 2;
+#pragma unroll
+                for (int u = 0; u < NB; ++u)
+#pragma unroll
+                    for (int v = 0; v < HM_KV; ++v)
+                        if (v < k) {
+                            const float xc = __int_as_float(__builtin_amdgcn_readlane(
+                                __float_as_int(xv[v][u & 3]), src + (u >> 2)));
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) a[v][e] = fmaf((float)h[u][e], xc, a[v][e]);
+                        }
+            }
+#pragma unroll
+            for (int v = 0; v < HM_KV; ++v)
+                if (v < k) yacc[v] += a[v] * fs;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < HM_KV; ++v)
+        if (v < k)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int li = 4 * lane + c;
+                if (li < szI)
+                    ypart[((size_t)ch * k + v) * ldp + cl_idx[(size_t)I * HM_CL + li]] =
+                        yacc[v][c];
+            }
+}
+
+// Y[v][i] = sum over chunks of ypart[c][v][i] (as k_hm_finish) + diag[i] X[v][i]; with
+// line, constrained rows get X[v][r] - sum w X[v][col] instead
+__global__ __launch_bounds__(256) void k_hm_finish_multi(
+    const float *__restrict__ ypart, size_t ldp, int chunks, int n, int k,
+    const double *__restrict__ diag, const float *__restrict__ X, size_t ldx,
+    float *__restrict__ Y, size_t ldy, const int *__restrict__ line,
+    const int *__restrict__ cptr, const int *__restrict__ ccols, const double *__restrict__ cw)
+{
+    __shared__ double qs[4][64];
+    const int rl = threadIdx.x & 63, q = threadIdx.x >> 6, v = blockIdx.y;
+    const int i = blockIdx.x * 64 + rl;
+    const int c0 = (int)((long)q * chunks / 4), c1 = (int)((long)(q + 1) * chunks / 4);
+    double s = 0.0;
+    if (i < n)
+        for (int c = c0; c < c1; ++c) s += (double)ypart[((size_t)c * k + v) * ldp + i];
+    qs[q][rl] = s;
+    __syncthreads();
+    if (q != 0 || i >= n) return;
+    const float *x = X + (size_t)v * ldx;
+    float yv = (float)((((qs[0][rl] + qs[1][rl]) + qs[2][rl]) + qs[3][rl]) +
+                       diag[i] * (double)x[i]);
+    if (line) {
+        const int l = line[i];
+        if (l >= 0) {
+            double t = (double)x[i];
+            for (int p = cptr[l]; p < cptr[l + 1]; ++p) t -= cw[p] * (double)x[ccols[p]];
+            yv = (float)t;
+        }
+    }
+    Y[(size_t)v * ldy + i] = yv;
+}
+
 // y[i] = sum over chunks of ypart[c][i] + diag[i] xd[i]: 64 rows per block, the chunks
 // split into four contiguous quarters summed by four threads of a row (in chunk order),
 // the quarters then added in order, so the sum is the same on every call.
@@ -809,6 +959,35 @@ void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_
                        (const _Float16 *)D16, doff, xc, ldp, ypart, item_ptr, items);
     hipLaunchKernelGGL(k_hm_finish, dim3((n + 63) / 64), dim3(256), 0, s, ypart, ldp, ch, n,
                        diag, xd, y, line, cptr, ccols, cw, x, agg, Cpart);
+}
+
+size_t hm_multi_work_floats(int ncl, size_t ldp, size_t *xc_floats)
+{
+    if (xc_floats) *xc_floats = (size_t)HM_KV * ncl * HM_CL;
+    return (size_t)hm_chunks(ncl) * HM_KV * ldp;
+}
+
+void launch_hm_gemv_multi(const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
+                          const float *fscale, const void *Q16, const void *B16,
+                          const void *D16, const int *doff, const int *item_ptr,
+                          const int *items, int n, int k, const float *X, size_t ldx,
+                          const double *diag, float *xcm, float *ypart, size_t ldp, float *Y,
+                          size_t ldy, const int *line, const int *cptr, const int *ccols,
+                          const double *cw, hipStream_t s)
+{
+    const int ch = hm_chunks(ncl);
+    for (int g = 0; g < k; g += HM_KV) {
+        const int kg = std::min(HM_KV, k - g);
+        const float *Xg = X + (size_t)g * ldx;
+        hipLaunchKernelGGL(k_hm_gather_multi, dim3(ncl, kg), dim3(HM_CL), 0, s, cl_idx, cl_size,
+                           ncl, Xg, ldx, xcm);
+        hipLaunchKernelGGL(k_hm_gemv_multi, dim3(ncl, ch), dim3(64), 0, s, cl_idx, cl_size, ncl,
+                           ch, rank, fscale, (const _Float16 *)Q16, (const _Float16 *)B16,
+                           (const _Float16 *)D16, doff, xcm, kg, ldp, ypart, item_ptr, items);
+        hipLaunchKernelGGL(k_hm_finish_multi, dim3((n + 63) / 64, kg), dim3(256), 0, s, ypart,
+                           ldp, ch, n, kg, diag, Xg, ldx, Y + (size_t)g * ldy, ldy, line, cptr,
+                           ccols, cw);
+    }
 }
 
 }  // namespace wb

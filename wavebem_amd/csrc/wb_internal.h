@@ -158,6 +158,19 @@ void launch_hm_gemv(const int *cl_idx, const int *cl_size, int ncl, const uint8_
                     const int *ccols = nullptr, const double *cw = nullptr, int agg = 64,
                     double *Cpart = nullptr);
 
+// The compressed product for k vectors (columns of X and Y at strides ldx / ldy, FP32),
+// read once per group of up to 8 vectors: Y = A~ X + diag .* X, with the constraint lines
+// (line / cptr / ccols / cw as launch_hm_gemv, or line = nullptr).  Work space: xcm and
+// ypart of the sizes hm_multi_work_floats returns (floats).
+size_t hm_multi_work_floats(int ncl, size_t ldp, size_t *xc_floats);
+void launch_hm_gemv_multi(const int *cl_idx, const int *cl_size, int ncl, const uint8_t *rank,
+                          const float *fscale, const void *Q16, const void *B16,
+                          const void *D16, const int *doff, const int *item_ptr,
+                          const int *items, int n, int k, const float *X, size_t ldx,
+                          const double *diag, float *xcm, float *ypart, size_t ldp, float *Y,
+                          size_t ldy, const int *line, const int *cptr, const int *ccols,
+                          const double *cw, hipStream_t s);
+
 // Work space of the entry points that allocate nothing (wb_noalloc.hip): a static device
 // array of 2 NOALLOC_MAX_N doubles per GPU.  noalloc_setup looks its address up on a device
 // (once; called when a context is created there); noalloc_scratch returns it (nullptr
