@@ -596,24 +596,22 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
         const float fs = fscale[b];
         const int szJ = __builtin_amdgcn_readfirstlane(cl_size[J]);
         const int cb = HM_DCOLS * q, ce = min(cb + HM_DCOLS, szJ);
-        fvec4 xv = {0.f, 0.f, 0.f, 0.f};
-        if (lane < HM_DCOLS / 4) xv = *(const fvec4 *)(xc + (size_t)J * HM_CL + cb + 4 * lane);
+        // x of the part's columns: wave-uniform addresses, so scalar loads
+        const float *xp = xc + (size_t)J * HM_CL;
         const _Float16 *D = D16 + (size_t)doff[b] * HM_CL * HM_CL + 4 * lane;
         fvec4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = a0;
         for (int c0 = cb; c0 < ce; c0 += NB) {
             // columns c0 .. c0 + NB - 1 (inside the part and the 256 stored columns, zero past
-            // |J|), their x from lanes (c0 - cb) / 4 ..
+            // |J|; x is zero there too)
             hvec4 h[NB];
 #pragma unroll
             for (int u = 0; u < NB; ++u) h[u] = *(const hvec4 *)(D + (size_t)(c0 + u) * HM_CL);
-            const int src = (c0 - cb) >> 2;
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
-                const float xc = __int_as_float(
-                    __builtin_amdgcn_readlane(__float_as_int(xv[u & 3]), src + (u >> 2)));
+                const float xcol = xp[c0 + u];
                 fvec4 &a = (u & 1) ? a1 : a0;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) a[e] = fmaf((float)h[u][e], xc, a[e]);
+                for (int e = 0; e < 4; ++e) a[e] = fmaf((float)h[u][e], xcol, a[e]);
             }
         }
         yacc += (a0 + a1) * fs;
@@ -700,12 +698,8 @@ __global__ __launch_bounds__(64) void k_hm_gemv_multi(
         } else {
             const int szJ = __builtin_amdgcn_readfirstlane(cl_size[J]);
             const int cb = HM_DCOLS * kind, ce = min(cb + HM_DCOLS, szJ);
-            fvec4 xv[HM_KV];
-#pragma unroll
-            for (int v = 0; v < HM_KV; ++v)
-                xv[v] = (v < k && lane < HM_DCOLS / 4)
-                            ? *(const fvec4 *)(xcm + v * vs + (size_t)J * HM_CL + cb + 4 * lane)
-                            : fvec4{0.f, 0.f, 0.f, 0.f};
+            // x of the part's columns: wave-uniform addresses, so scalar loads
+            const float *xp = xcm + (size_t)J * HM_CL;
             const _Float16 *D = D16 + (size_t)doff[b] * HM_CL * HM_CL + 4 * lane;
             fvec4 a[HM_KV];
 #pragma unroll
@@ -714,16 +708,19 @@ __global__ __launch_bounds__(64) void k_hm_gemv_multi(
                 hvec4 h[NB];
 #pragma unroll
                 for (int u = 0; u < NB; ++u) h[u] = *(const hvec4 *)(D + (size_t)(c0 + u) * HM_CL);
-                const int src = (c0 - cb) >> 2;
+                fvec4 hf[NB];
 #pragma unroll
                 for (int u = 0; u < NB; ++u)
 #pragma unroll
-                    for (int v = 0; v < HM_KV; ++v)
-                        if (v < k) {
-                            const float xc = __int_as_float(__builtin_amdgcn_readlane(
-                                __float_as_int(xv[v][u & 3]), src + (u >> 2)));
+                    for (int e = 0; e < 4; ++e) hf[u][e] = (float)h[u][e];
 #pragma unroll
-                            for (int e = 0; e < 4; ++e) a[v][e] = fmaf((float)h[u][e], xc, a[v][e]);
+                for (int v = 0; v < HM_KV; ++v)
+                    if (v < k)
+#pragma unroll
+                        for (int u = 0; u < NB; ++u) {
+                            const float xcol = xp[v * vs + c0 + u];
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) a[v][e] = fmaf(hf[u][e], xcol, a[v][e]);
                         }
             }
 #pragma unroll
