@@ -73,6 +73,31 @@ __device__ __forceinline__ float lane63(float v)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Wavefront sums of 8 values at once (a transposing reduction): each step pairs a lane
+// with a partner that holds the same set of value indices, keeps half of them and adds
+// the partner's copy of that half (row_mirror splits on lane bit 3, row_half_mirror on
+// bit 2, quad_perm [2,3,0,1] on bit 1); then the lane pairs, the rows and the row pairs
+// are summed.  On return lane l holds the total of value ((l >> 1) & 7): 7 + 3 + 1 DPP
+// adds and 2 cross-row shuffles where 8 separate sums take 8 x 6.
+__device__ __forceinline__ float wave_sum8(const float (&p)[8])
+{
+    const unsigned l = threadIdx.x & 63u;
+    const bool b3 = l & 8u, b2 = l & 4u, b1 = l & 2u;
+    float a[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        a[j] = (b3 ? p[j + 4] : p[j]) + WB_DPP_MOV(b3 ? p[j] : p[j + 4], 0x140);
+    float c[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        c[j] = (b2 ? a[j + 2] : a[j]) + WB_DPP_MOV(b2 ? a[j] : a[j + 2], 0x141);
+    float t = (b1 ? c[1] : c[0]) + WB_DPP_MOV(b1 ? c[0] : c[1], 0x4e);
+    t += WB_DPP_MOV(t, 0xb1);
+    t += __shfl_xor(t, 16);
+    t += __shfl_xor(t, 32);
+    return t;
+}
+
 // sign of sketch vector k at cluster-local position p: bit k of a 32-bit integer hash of p
 __device__ __forceinline__ uint32_t sign_bits(uint32_t p)
 {
@@ -626,10 +651,12 @@ __global__ __launch_bounds__(64) void k_hm_gemv(const int *__restrict__ cl_idx,
 // ------------------------------------------------------------------ several vectors
 // The product for up to HM_KV vectors at once, over the same work items: the operator is
 // read once per launch.  xcm holds the vectors in cluster order ([v][J HM_CL + p]); a
-// low-rank item forms t_{k,v} = B_k . x_J^v for every vector (lane partials, DPP wavefront
-// sums) and adds Q t; a dense part multiplies every streamed column into every vector.
+// low-rank item forms t_{k,v} = B_k . x_J^v for every vector (lane partials, one
+// transposing wavefront sum per k for the 8 vectors) and adds Q t; a dense part multiplies
+// every streamed column into every vector, x of its columns read with scalar loads.
 // ypart is [ch][v][row].
 constexpr int HM_KV = 8;
+static_assert(HM_KV == 8, "the low-rank sums use wave_sum8");
 
 __global__ __launch_bounds__(HM_CL) void k_hm_gather_multi(const int *__restrict__ cl_idx,
                                                            const int *__restrict__ cl_size,
@@ -680,21 +707,27 @@ __global__ __launch_bounds__(64) void k_hm_gemv_multi(
             for (int v = 0; v < HM_KV; ++v)
                 xv[v] = v < k ? *(const fvec4 *)(xcm + v * vs + (size_t)J * HM_CL + 4 * lane)
                               : fvec4{0.f, 0.f, 0.f, 0.f};
+            // t_{kr,v} for the 8 vectors in one transposing sum (x is zero past k)
 #pragma unroll
             for (int kr = 0; kr < HM_RMAX; ++kr)
-                if (kr < r)
+                if (kr < r) {
+                    float part[HM_KV];
+#pragma unroll
+                    for (int v = 0; v < HM_KV; ++v)
+                        part[v] = (float)bk[kr][0] * xv[v][0] + (float)bk[kr][1] * xv[v][1] +
+                                  (float)bk[kr][2] * xv[v][2] + (float)bk[kr][3] * xv[v][3];
+                    const float ts = wave_sum8(part);
 #pragma unroll
                     for (int v = 0; v < HM_KV; ++v)
                         if (v < k) {
-                            const float part = (float)bk[kr][0] * xv[v][0] +
-                                               (float)bk[kr][1] * xv[v][1] +
-                                               (float)bk[kr][2] * xv[v][2] +
-                                               (float)bk[kr][3] * xv[v][3];
-                            const float t = lane63(wave_reduce63<false>(part)) * fs;
+                            const float t =
+                                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ts), 2 * v)) *
+                                fs;
 #pragma unroll
                             for (int c = 0; c < 4; ++c)
                                 yacc[v][c] = fmaf((float)q[kr][c], t, yacc[v][c]);
                         }
+                }
         } else {
             const int szJ = __builtin_amdgcn_readfirstlane(cl_size[J]);
             const int cb = HM_DCOLS * kind, ce = min(cb + HM_DCOLS, szJ);
@@ -708,11 +741,6 @@ __global__ __launch_bounds__(64) void k_hm_gemv_multi(
                 hvec4 h[NB];
 #pragma unroll
                 for (int u = 0; u < NB; ++u) h[u] = *(const hvec4 *)(D + (size_t)(c0 + u) * HM_CL);
-                fvec4 hf[NB];
-#pragma unroll
-                for (int u = 0; u < NB; ++u)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) hf[u][e] = (float)h[u][e];
 #pragma unroll
                 for (int v = 0; v < HM_KV; ++v)
                     if (v < k)
@@ -720,7 +748,8 @@ __global__ __launch_bounds__(64) void k_hm_gemv_multi(
                         for (int u = 0; u < NB; ++u) {
                             const float xcol = xp[v * vs + c0 + u];
 #pragma unroll
-                            for (int e = 0; e < 4; ++e) a[v][e] = fmaf(hf[u][e], xcol, a[v][e]);
+                            for (int e = 0; e < 4; ++e)
+                                a[v][e] = fmaf((float)h[u][e], xcol, a[v][e]);
                         }
             }
 #pragma unroll
