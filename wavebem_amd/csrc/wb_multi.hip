@@ -482,7 +482,12 @@ constexpr int HL_U = HL_P / 4;      // 4-column groups per panel row
 constexpr int HL_SH = HL_U + 1;     // hi row stride (16-byte units)
 constexpr int HL_SL = HL_U + 2;     // lo row stride (8-byte units)
 
-__global__ __launch_bounds__(64 * MW) void k_gemm_hl_lds(
+#ifndef WB_MULTI_HL_WAVES
+#define WB_MULTI_HL_WAVES 4  // wavefronts per SIMD the hi + lo kernel is built for (4 blocks
+                             // per CU: what its LDS allows)
+#endif
+
+__global__ __launch_bounds__(64 * MW, WB_MULTI_HL_WAVES) void k_gemm_hl_lds(
     const float *__restrict__ G, const float *__restrict__ H, const uint16_t *__restrict__ Glo,
     const uint16_t *__restrict__ Hlo, size_t ld, int n, const uint8_t *__restrict__ gmask,
     int flip, double s_sel, double s_uns, int k, const double *__restrict__ X, size_t ldx,
@@ -498,9 +503,11 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_hl_lds(
     constexpr int SR = 64 * MW / HL_U;  // rows per staging pass
     constexpr int SJ = RPB / SR;        // staging passes
     const int su = tid % HL_U, sr = tid / HL_U;
-    size_t soff[SJ];
+    // staged rows as 32-bit indices (their 64-bit offsets are formed at each load, which
+    // keeps 4 registers free)
+    int srow[SJ];
 #pragma unroll
-    for (int j = 0; j < SJ; ++j) soff[j] = (size_t)tile_row(brow + sr + SR * j, rows, n_rows) * ld;
+    for (int j = 0; j < SJ; ++j) srow[j] = tile_row(brow + sr + SR * j, rows, n_rows);
     const int i = lane & 15, q = lane >> 4;
     const int v = lane & 15;
     const bool vok = v < k;
@@ -512,6 +519,10 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_hl_lds(
     uvec4 sl[SJ];
     // stage: group su of the panel at pc for rows sr + SR j
     auto load_panel = [&](int pc) {
+        // an empty asm that "changes" the rows keeps their 64-bit offsets from being
+        // hoisted out of the panel loop into registers
+#pragma unroll
+        for (int j = 0; j < SJ; ++j) asm volatile("" : "+v"(srow[j]));
         const int scol = pc + 4 * su;
         const fvec4 z4 = {0.f, 0.f, 0.f, 0.f};
         const uvec4 zl = {0, 0, 0, 0};
@@ -522,16 +533,16 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_hl_lds(
                 const uint16_t *L = m ? Hlo : Glo;
 #pragma unroll
                 for (int j = 0; j < SJ; ++j) {
-                    sh[j] = __builtin_nontemporal_load((const fvec4 *)(M + soff[j] + scol));
-                    sl[j] = __builtin_nontemporal_load((const uvec4 *)(L + soff[j] + scol));
+                    sh[j] = __builtin_nontemporal_load((const fvec4 *)(M + (size_t)srow[j] * ld + scol));
+                    sl[j] = __builtin_nontemporal_load((const uvec4 *)(L + (size_t)srow[j] * ld + scol));
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < SJ; ++j) {
-                    const fvec4 g = *(const fvec4 *)(G + soff[j] + scol);
-                    const fvec4 h = *(const fvec4 *)(H + soff[j] + scol);
-                    const uvec4 gl = *(const uvec4 *)(Glo + soff[j] + scol);
-                    const uvec4 hl = *(const uvec4 *)(Hlo + soff[j] + scol);
+                    const fvec4 g = *(const fvec4 *)(G + (size_t)srow[j] * ld + scol);
+                    const fvec4 h = *(const fvec4 *)(H + (size_t)srow[j] * ld + scol);
+                    const uvec4 gl = *(const uvec4 *)(Glo + (size_t)srow[j] * ld + scol);
+                    const uvec4 hl = *(const uvec4 *)(Hlo + (size_t)srow[j] * ld + scol);
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         const bool he = (m >> e) & 1u;
