@@ -1639,11 +1639,16 @@ static void solve_multi(wb_ctx *c, int k, const double *tmp_rhs, double *phi, do
     HIPCHK(hipStreamSynchronize(c->stream));
     double t3 = now_ms();
     for (int v = 0; v < k; ++v) s.residual = std::max(s.residual, resid[v]);
-    // true relative residuals with the FP64-accurate operator
-    apply_constrained_multi(c, mode64, k, M.X.p, vl, M.W.p, vl);
-    launch_vec_ops(8, (int)(vl * k), M.W.p, M.B.p, M.T.p, 0, c->stream);
+    // true relative residuals with the FP64-accurate operator (the refinement loop already
+    // holds them in R: its last product used the same operator on the final X)
+    const void *Rt = M.R.p;
+    if (mode64 == 0) {
+        apply_constrained_multi(c, mode64, k, M.X.p, vl, M.W.p, vl);
+        launch_vec_ops(8, (int)(vl * k), M.W.p, M.B.p, M.T.p, 0, c->stream);
+        Rt = M.T.p;
+    }
     for (int v = 0; v < k; ++v) {
-        const double rn = device_norm(c, (double *)M.T.p + vl * v, 0);
+        const double rn = device_norm(c, (const double *)Rt + vl * v, 0);
         const double bn = device_norm(c, (double *)M.B.p + vl * v, 0);
         s.true_rel_residual = std::max(s.true_rel_residual, bn > 0 ? rn / bn : rn);
     }
