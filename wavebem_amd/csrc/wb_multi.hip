@@ -472,10 +472,11 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_d(
 // thread owns one 4-column group of the panel: gmask says whether its columns read G or H
 // (both, column by column, for a mixed group); the signs go to the vector operand.
 // LDS row strides of HL_U + 1 (hi, 16-byte units) and HL_U + 2 (lo, 8-byte units) put the
-// 16 rows of a fragment read in distinct banks.  (64-column panels without the vector staging:
-// 5.1-5.9 ms at 61k, k = 8; 32 columns with it: 5.35 ms on the box that measured 5.9.)
+// 16 rows of a fragment read in distinct banks.  At 61k, k = 8: 64-column panels, 2 blocks
+// per CU (61.5 KB of LDS each), 4.68 ms; 32 columns, 4 blocks per CU, 4.96 ms; 16 columns
+// 8.87 ms.
 #ifndef WB_MULTI_HL_PANEL
-#define WB_MULTI_HL_PANEL 32
+#define WB_MULTI_HL_PANEL 64
 #endif
 constexpr int HL_P = WB_MULTI_HL_PANEL;  // columns per panel (a multiple of 16)
 constexpr int HL_U = HL_P / 4;      // 4-column groups per panel row
@@ -483,8 +484,8 @@ constexpr int HL_SH = HL_U + 1;     // hi row stride (16-byte units)
 constexpr int HL_SL = HL_U + 2;     // lo row stride (8-byte units)
 
 #ifndef WB_MULTI_HL_WAVES
-#define WB_MULTI_HL_WAVES 4  // wavefronts per SIMD the hi + lo kernel is built for (4 blocks
-                             // per CU: what its LDS allows)
+#define WB_MULTI_HL_WAVES 2  // wavefronts per SIMD the hi + lo kernel is built for (the
+                             // blocks per CU its LDS allows)
 #endif
 
 __global__ __launch_bounds__(64 * MW, WB_MULTI_HL_WAVES) void k_gemm_hl_lds(
