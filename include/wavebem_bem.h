@@ -351,7 +351,9 @@ typedef enum {
     WB_K_GEMV32 = 3,    /* column-selected GEMV, FP32 (hi) matrices, inner solver */
     WB_K_GEMVHL = 4,    /* column-selected GEMV, FP32 hi + bf16 lo, FP64 accumulation */
     WB_K_GEMV16 = 5,    /* GEMV of the row-scaled FP16 operator, inner solver */
-    WB_K_GEMVC = 6,     /* product of the compressed operator (ir_inner_format 2) */
+    WB_K_GEMVC = 6,     /* product of the compressed operator (ir_inner_format 2); timed on
+                           one launch in 8 (launches counts the timed ones), since each
+                           event pair idles the GPU for about 11 us of a 70 us product */
     WB_K_INNER = 7,     /* build of the inner operator (FP16 copy or compressed copy) */
     WB_K_COUNT = 8
 } wb_kernel_kind;
