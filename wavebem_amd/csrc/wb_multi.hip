@@ -466,7 +466,7 @@ __global__ __launch_bounds__(64 * MW) void k_gemm_sel_d(
 
 // The hi + bf16 lo operator (FP64 vectors) with LDS-staged panels: the block's 128 rows x
 // HL_P columns of the selected hi (FP32) and lo (bf16) entries are staged with whole-row
-// loads (at 32 columns a wavefront reads 8 rows x 128 bytes of hi and 8 rows x 64 bytes of
+// loads (at 64 columns a wavefront reads 4 rows x 256 bytes of hi and 4 rows x 128 bytes of
 // lo per instruction), the panel's vector entries once per block with their signs, and
 // contracted on the FP64 matrix cores as in k_gemm_sel_d.  A staging
 // thread owns one 4-column group of the panel: gmask says whether its columns read G or H
