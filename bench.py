@@ -668,26 +668,22 @@ def main():
         if esz == 2:
             bytes_per_launch += rows_per_launch * 4
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    # launches per step: the compressed product is timed on one launch in 8 (the library's
-    # event pairs idle the GPU), so its count per step is one per inner GMRES step and
-    # restart of the last solve
-    if gname == "gemvc":
-        per_step = head["stats"]["iterations"] + head["stats"]["restarts"]
-    else:
-        per_step = g["launches"] / max(args.steps, 1)
+    # launches per step: the per-GMRES-step products are timed on one launch in 8 (the
+    # library's event pairs idle the GPU); calls counts every launch
+    per_step = g["calls"] / max(args.steps, 1)
     line["roofline_gemv"] = {"kernel": kname, "bound": "hbm",
                              "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                              "traffic": profiled_traffic(args.workload, kname),
                              "avg_launch_ms": round(avg_ms, 4), "launches": g["launches"],
-                             "timed_one_launch_in": 8 if gname == "gemvc" else 1,
+                             "calls": g["calls"], "timed_one_launch_in": 8,
                              "algorithmic_bytes_per_launch": int(bytes_per_launch),
                              "share_of_step": round(avg_ms * per_step / head["ms_per_step"], 3)}
     kern = {}
     for prec, r in res.items():
         for name, k in r["kernels"].items():
             if k["launches"]:
-                kern[f"{prec}:{name}"] = {"launches": k["launches"],
+                kern[f"{prec}:{name}"] = {"launches": k["launches"], "calls": k["calls"],
                                           "avg_ms": round(k["total_ms"] / k["launches"], 4)}
         a = r["kernels"]["assemble"]
         if a["launches"]:

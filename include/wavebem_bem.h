@@ -351,17 +351,19 @@ typedef enum {
     WB_K_GEMV32 = 3,    /* column-selected GEMV, FP32 (hi) matrices, inner solver */
     WB_K_GEMVHL = 4,    /* column-selected GEMV, FP32 hi + bf16 lo, FP64 accumulation */
     WB_K_GEMV16 = 5,    /* GEMV of the row-scaled FP16 operator, inner solver */
-    WB_K_GEMVC = 6,     /* product of the compressed operator (ir_inner_format 2); timed on
-                           one launch in 8 (launches counts the timed ones), since each
-                           event pair idles the GPU for about 11 us of a 70 us product */
+    WB_K_GEMVC = 6,     /* product of the compressed operator (ir_inner_format 2) */
     WB_K_INNER = 7,     /* build of the inner operator (FP16 copy or compressed copy) */
     WB_K_COUNT = 8
 } wb_kernel_kind;
 
 typedef struct {
-    int launches[WB_K_COUNT];
+    int launches[WB_K_COUNT];   /* timed launches */
     double total_ms[WB_K_COUNT];
-    long long rows[WB_K_COUNT]; /* GEMV: matrix rows read, summed over launches */
+    long long rows[WB_K_COUNT]; /* GEMV: matrix rows read, summed over timed launches */
+    int calls[WB_K_COUNT];      /* all launches.  GEMV64, GEMV32, GEMV16 and GEMVC (one per
+                                   GMRES step) are timed on one launch in 8: each event pair
+                                   idles the GPU for about 11 us, against 70 us for the
+                                   compressed product; the others on every launch */
 } wb_kernel_stats;
 
 /* enable != 0 starts (and resets) event timing of the hot kernels; 0 stops it. */

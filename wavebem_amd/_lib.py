@@ -74,11 +74,12 @@ class WbKernelStats(ctypes.Structure):
         ("launches", ctypes.c_int * len(WB_K_NAMES)),
         ("total_ms", ctypes.c_double * len(WB_K_NAMES)),
         ("rows", ctypes.c_longlong * len(WB_K_NAMES)),
+        ("calls", ctypes.c_int * len(WB_K_NAMES)),
     ]
 
     def as_dict(self):
         return {name: dict(launches=self.launches[k], total_ms=self.total_ms[k],
-                           rows=self.rows[k])
+                           rows=self.rows[k], calls=self.calls[k])
                 for k, name in enumerate(WB_K_NAMES)}
 
 

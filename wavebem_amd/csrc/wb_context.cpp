@@ -161,7 +161,7 @@ struct wb_ctx {
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
-    unsigned long long gemvc_calls = 0;  // compressed products seen while profiling
+    unsigned long long kcalls[WB_K_COUNT] = {};  // launches seen per kind while profiling
     struct Pending {
         int kind;
         size_t ev;
@@ -208,10 +208,16 @@ struct wb_ctx {
     }
 };
 
-// Brackets one launch with a pair of events when profiling is on.  The compressed product,
-// launched once per inner GMRES step, is timed on one launch in GEMVC_SAMPLE: each timing
-// event in the stream costs about 5.6 us of idle GPU, against 70 us of product.
-constexpr unsigned GEMVC_SAMPLE = 8;
+// Brackets one launch with a pair of events when profiling is on.  The products launched
+// once per GMRES step (FP64, FP32 and FP16 matrices, compressed operator) are timed on one
+// launch in STEP_SAMPLE, in both precisions alike: each timing event in the stream costs
+// about 5.6 us of idle GPU, against 70 us for the compressed product.  calls counts them all.
+constexpr unsigned STEP_SAMPLE = 8;
+static bool step_product(int kind)
+{
+    return kind == WB_K_GEMV64 || kind == WB_K_GEMV32 || kind == WB_K_GEMV16 ||
+           kind == WB_K_GEMVC;
+}
 struct KernelScope {
     wb_ctx *c;
     int kind;
@@ -221,7 +227,8 @@ struct KernelScope {
     KernelScope(wb_ctx *c_, int kind_, long long rows_ = 0) : c(c_), kind(kind_), rows(rows_)
     {
         if (!c->profiling) return;
-        if (kind == WB_K_GEMVC && c->gemvc_calls++ % GEMVC_SAMPLE != 0) return;
+        c->kstats.calls[kind] += 1;
+        if (step_product(kind) && c->kcalls[kind]++ % STEP_SAMPLE != 0) return;
         on = true;
         while (c->ev_pool.size() < c->ev_used + 2) {
             hipEvent_t e;
@@ -2798,7 +2805,7 @@ wb_status wb_set_profiling(wb_ctx *c, int enable)
         fold_kernel_stats(c);
         c->kstats = wb_kernel_stats{};
         c->profiling = enable != 0;
-        c->gemvc_calls = 0;
+        for (auto &k : c->kcalls) k = 0;
     });
 }
 
